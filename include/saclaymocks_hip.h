@@ -1,0 +1,121 @@
+/*
+ * saclaymocks_hip.h - C ABI of libsaclay_hip.so, the MI355X (gfx950) kernels
+ * behind the SaclayMocks box stage (make_boxes.py) and forest stage
+ * (make_spectra.py + merge_spectra.py) and the FGPA fit (fit_az.py).
+ *
+ * Conventions
+ *   - every entry point returns 0 (SMK_OK) or a negative SMK_ERR_* code and
+ *     leaves a message readable with smk_last_error() (per host thread);
+ *   - every array argument is a DEVICE pointer (e.g. torch tensor data_ptr()),
+ *     except where a comment says "host";
+ *   - `stream` is a hipStream_t passed as void* (0 = legacy default stream);
+ *     calls are stream ordered and asynchronous;
+ *   - compute entry points never allocate: scratch memory is a caller-given
+ *     workspace whose size the library reports;
+ *   - a plan must not be shared by host threads running calls concurrently.
+ *
+ * Layouts follow the reference's numpy arrays (C order):
+ *   real box       float32 [nx][ny][nz]        (make_boxes.py:86)
+ *   k-space box    complex64 [nx][ny][nz/2+1]  (make_boxes.py:52), stored as
+ *                  interleaved float pairs.
+ */
+#ifndef SACLAYMOCKS_HIP_H
+#define SACLAYMOCKS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SMK_ABI_VERSION 1
+
+#define SMK_OK 0
+#define SMK_ERR_ARG (-1)         /* bad argument (null pointer, size, enum) */
+#define SMK_ERR_OOM (-2)         /* device allocation failed (plan tables only) */
+#define SMK_ERR_NULLBOX (-3)     /* output box all-zero or holds NaN */
+#define SMK_ERR_HIP (-4)         /* HIP runtime error */
+#define SMK_ERR_UNSUPPORTED (-5) /* FFT length without a compiled kernel */
+#define SMK_ERR_RANGE (-6)       /* value outside a table or the box */
+
+/* auxiliary field produced beside delta by the fused k-space pass */
+#define SMK_AUX_NONE 0
+#define SMK_AUX_ETA_XX 1
+#define SMK_AUX_ETA_YY 2
+#define SMK_AUX_ETA_ZZ 3 /* eta_par for a line of sight along z */
+#define SMK_AUX_ETA_XY 4
+#define SMK_AUX_ETA_XZ 5
+#define SMK_AUX_ETA_YZ 6
+#define SMK_AUX_VX 7
+#define SMK_AUX_VY 8
+#define SMK_AUX_VZ 9
+
+int smk_abi_version(void);
+const char* smk_last_error(void);
+
+/* 1 if an FFT line of n complex points has a compiled kernel, else 0.
+ * Box sizes need nx, ny and nz/2 to be supported (nz even). */
+int smk_fft_length_supported(int n);
+
+/* ------------------------------------------------------------------ box */
+typedef struct smk_box_plan smk_box_plan;
+
+/* Plan for an nx*ny*nz box of cubic cells of side `cell` (Mpc/h).  Owns the
+ * FFT twiddle tables (FP64-computed, stored FP32) and the wavenumber axes
+ * kx = fftfreq(nx)*2*k_ny, ky likewise, kz = rfftfreq(nz)*2*k_ny, cast to
+ * float32 as in interpolate_pk.py:63-77 / make_boxes.py:299-305.
+ * Replaces the FFTW plans built at make_boxes.py:53 and :87. */
+int smk_box_plan_create(smk_box_plan** plan, int nx, int ny, int nz, double cell);
+int smk_box_plan_destroy(smk_box_plan* plan);
+
+/* Bytes of device workspace smk_grf_box needs (k-space box, auxiliary
+ * k-space box, per-block FP64 partial sums). */
+size_t smk_box_workspace_bytes(const smk_box_plan* plan);
+
+/* Fused GRF box: noise -> R2C -> x sqrt(P(k)/Vcell) -> C2R giving delta, and
+ * the auxiliary field (eta_zz for SMK_AUX_ETA_ZZ) out of the same k-space
+ * pass.  Replaces make_boxes.py:40-72 (DrawGRF_boxk), :289-292 (x P0 and
+ * FFTandStore of the density) and :341-352 (eta_zz), with the power spectrum
+ * evaluated per mode instead of read from the P0 table of
+ * interpolate_pk.py:146-150.
+ *   seed        Philox key for on-device N(0,1) noise (used if noise == NULL)
+ *   noise       NULL, or float32 [nx][ny][nz] white noise (parity mode)
+ *   pk_params   host double[4] {A, k0, n, R}:
+ *               P(k) = A (k/k0) / (1 + (k/k0)^n) exp(-(k R)^2)
+ *   aux_kind    SMK_AUX_*; aux_coef multiplies velocity kinds
+ *               (H0 * dD/dz(z=0), make_boxes.py:403)
+ *   delta_out   float32 [nx][ny][nz]
+ *   aux_out     float32 [nx][ny][nz] or NULL when aux_kind == SMK_AUX_NONE
+ *   workspace   smk_box_workspace_bytes(plan) bytes, 256-byte aligned
+ *   var_out     device double[1]: variance of delta from k-space, FP64
+ *   status_out  device int[2]: [0] bit0 = NaN seen, bit1 = non-zero seen in
+ *               delta; [1] same for aux.  The null/NaN guard of
+ *               make_boxes.py:100-105 is the host's check of these bits. */
+int smk_grf_box(smk_box_plan* plan, uint64_t seed, const float* noise, const double* pk_params,
+                int aux_kind, double aux_coef, float* delta_out, float* aux_out, void* workspace,
+                double* var_out, int* status_out, void* stream);
+
+/* DrawGRF_boxk (make_boxes.py:40-72): unnormalised forward R2C FFT of the
+ * white noise into boxk, complex64 [nx][ny][nz/2+1].  noise == NULL draws
+ * the Philox noise of `seed` on the device. */
+int smk_draw_grf_boxk(smk_box_plan* plan, uint64_t seed, const float* noise, float* boxk_out,
+                      void* stream);
+
+/* FFTandStore's transform (make_boxes.py:86-93): box = irfftn(boxk) / N.
+ * boxk is destroyed (FFTW_DESTROY_INPUT at make_boxes.py:87).
+ *   workspace   smk_fft_and_store_workspace_bytes(plan) bytes, or NULL when
+ *               sumsq_out is NULL
+ *   sumsq_out   NULL or device double[1]: sum over the FULL spectrum of
+ *               |boxk|^2, FP64, taken before the transform (Parseval:
+ *               sum(box^2) = sumsq / N); gives np.std(box) of make_boxes.py:92
+ *               once the mean (Re boxk[0,0,0] / N) is removed
+ *   status_out  NULL or device int[1], same bits as smk_grf_box. */
+int smk_fft_and_store(smk_box_plan* plan, float* boxk_inout, float* box_out, void* workspace,
+                      double* sumsq_out, int* status_out, void* stream);
+size_t smk_fft_and_store_workspace_bytes(const smk_box_plan* plan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SACLAYMOCKS_HIP_H */

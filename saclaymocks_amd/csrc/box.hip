@@ -1,0 +1,638 @@
+// GRF box stage on gfx950: white noise -> R2C -> sqrt(P(k)/Vcell) -> C2R.
+//
+// Reference: bin/make_boxes.py:40-125,289-352 and bin/interpolate_pk.py:63-77.
+//
+// The k-space box K lives in the numpy layout complex64 [nx][ny][nzh]
+// (nzh = nz/2 + 1) and every pass works IN PLACE on it:
+//   Z1  z R2C     pencils (x,y) contiguous; noise drawn in-kernel (Philox)
+//   Y   y FFT     lines along y (stride nzh); a block owns T consecutive kz,
+//                 so every HBM access is a run of T complex values
+//   X   x FFT     lines along x (stride ny*nzh), same tiling; in the fused
+//                 job the forward x-FFT, the sqrt(P/Vcell) multiply, the
+//                 aux factor (eta_zz = kz^2/k^2) and both inverse x-FFTs are
+//                 one kernel, writing delta_k in place and aux_k to a 2nd box
+//   Y^-1          on both boxes
+//   Z^-1 C2R      both boxes -> float32 [nx][ny][nz]
+// HBM traffic of the fused job: 4 (Z1 write) + 8 (Y) + 12 (X, one read two
+// writes) + 16 (Y^-1 x2) + 16 (Z^-1 x2) = 56 bytes per real cell.
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <mutex>
+#include <vector>
+
+#include "fft_core.h"
+#include "smk_internal.h"
+
+using namespace smk;
+
+namespace {
+
+// --------------------------------------------------------------- launch shapes
+// Threads per line P = N / E.  Lines per block T: strided passes use runs of
+// T consecutive kz (T*8 bytes per HBM access run); pencil passes just batch.
+constexpr int threads_per_line(int n) { return n / elems_per_thread(n); }
+
+constexpr int strided_T(int n) {
+  int T = 8;
+  while (T > 1 && (T * threads_per_line(n) > 512 || T * lds_line_len(n) * 8 > 81920)) T >>= 1;
+  return T;
+}
+
+constexpr int pencil_T(int m) {
+  int T = 16;
+  while (T > 1 && (T * threads_per_line(m) > 512 || T * lds_line_len(m) * 8 > 65536)) T >>= 1;
+  return T;
+}
+
+enum { MODE_FWD = 0, MODE_INV = 1, MODE_SPECTRAL = 2 };
+
+struct StridedArgs {
+  float2* data;          // in-place box
+  float2* aux;           // second box (MODE_SPECTRAL with aux_kind != NONE)
+  const float2* tw;      // W_N^m, m < N
+  uint64_t s_outer;      // stride (float2) between outer lines
+  uint64_t s_elem;       // stride (float2) between points of a line
+  uint32_t n_inner;      // inner (run) extent; the inner stride is 1
+  uint32_t n_iblk;       // ceil(n_inner / T)
+  // ---- MODE_SPECTRAL only (line axis = x, outer = ky, inner = kz)
+  const float* kxv;
+  const float* kyv;
+  const float* kzv;
+  float pkA, pk_k0inv, pk_n, pk_R;
+  float vcell;           // float32(cell^3), as make_boxes.py:181
+  float norm;            // 1 / (nx*ny*nz) folded into the spectrum
+  float aux_coef;
+  int aux_kind;
+  int write_delta;
+  uint32_t nz_nyq;       // kz index of the Nyquist plane (nz/2)
+  double* partial;       // per-block sum of w(kz)|delta_k|^2, FP64
+};
+
+__device__ __forceinline__ float pk_analytic(float k, const StridedArgs& a) {
+  float x = k * a.pk_k0inv;
+  float kr = k * a.pk_R;
+  return a.pkA * x / (1.0f + powf(x, a.pk_n)) * expf(-kr * kr);
+}
+
+// Wave reduction then block reduction in a fixed order (deterministic).
+// Every thread must call it; `red` may alias LDS used by other phases: the
+// function is bracketed by barriers.  The result is valid on thread 0.
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  // NT is a power of two; blocks smaller than a wave shuffle only among
+  // their own (active) lanes
+  constexpr int W = NT < 64 ? NT : 64;
+  static_assert((NT & (NT - 1)) == 0, "block size must be a power of two");
+  for (int off = W / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, W);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wv] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < (NT + 63) / 64; ++i) s += red[i];
+  }
+  __syncthreads();
+  return s;
+}
+
+template <int N, int T, int MODE>
+__global__ __launch_bounds__(T* threads_per_line(N)) void k_strided(StridedArgs a) {
+  constexpr int E = elems_per_thread(N);
+  constexpr int P = N / E;
+  constexpr int LL = lds_line_len(N);
+  constexpr int NT = T * P;
+  __shared__ float2 lds[T * LL];
+
+  const int tid = threadIdx.x;
+  const int t = tid % T, p = tid / T;
+  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t o = tile / a.n_iblk;
+  const uint32_t inner = (tile % a.n_iblk) * T + t;
+  const bool valid = inner < a.n_inner;
+  float2* base = a.data + (uint64_t)o * a.s_outer + inner;
+
+  float2 v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e)
+    v[e] = valid ? base[(uint64_t)(p + P * e) * a.s_elem] : make_float2(0.f, 0.f);
+
+  float2* line = lds + t * LL;
+  if constexpr (MODE == MODE_INV) {
+    // optional FP64 Parseval sum of the input spectrum (x axis only: inner = kz)
+    if (a.partial != nullptr) {
+      const double wz = (inner == 0 || inner == a.nz_nyq) ? 1.0 : 2.0;
+      double acc = 0.0;
+#pragma unroll
+      for (int e = 0; e < E; ++e) acc += (double)v[e].x * (double)v[e].x + (double)v[e].y * (double)v[e].y;
+      acc = valid ? wz * acc : 0.0;
+      const double bs = block_sum<NT>(acc, reinterpret_cast<double*>(lds));
+      if (threadIdx.x == 0) a.partial[tile] = bs;
+    }
+  }
+  fft_line<N, MODE == MODE_INV>(v, line, p, a.tw);
+
+  if constexpr (MODE != MODE_SPECTRAL) {
+    if (valid) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) base[(uint64_t)(p + P * e) * a.s_elem] = v[e];
+    }
+  } else {
+    // v[e] holds X(kx = p + P e, ky = o, kz = inner)
+    const float ky = a.kyv[o];
+    const float kz = valid ? a.kzv[inner] : 0.0f;
+    const double wz = (inner == 0 || inner == a.nz_nyq) ? 1.0 : 2.0;
+    float2 w[E];
+    double acc = 0.0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int i = p + P * e;
+      const float kx = a.kxv[i];
+      // kk = kx*kx + ky*ky + kz*kz in float32, numpy evaluation order
+      const float kk = __fadd_rn(__fadd_rn(__fmul_rn(kx, kx), __fmul_rn(ky, ky)), __fmul_rn(kz, kz));
+      const float k = __fsqrt_rn(kk);
+      const float s = __fsqrt_rn(__fdiv_rn(pk_analytic(k, a), a.vcell));
+      const float2 d = cscale(v[e], s * a.norm);
+      acc += wz * ((double)d.x * (double)d.x + (double)d.y * (double)d.y);
+      const float kkp = (i == 0 && o == 0 && inner == 0) ? 1.0f : kk;
+      float2 x;
+      switch (a.aux_kind) {
+        case SMK_AUX_ETA_XX: x = cscale(d, __fdiv_rn(__fmul_rn(kx, kx), kkp)); break;
+        case SMK_AUX_ETA_YY: x = cscale(d, __fdiv_rn(__fmul_rn(ky, ky), kkp)); break;
+        case SMK_AUX_ETA_ZZ: x = cscale(d, __fdiv_rn(__fmul_rn(kz, kz), kkp)); break;
+        case SMK_AUX_ETA_XY: x = cscale(d, __fdiv_rn(__fmul_rn(kx, ky), kkp)); break;
+        case SMK_AUX_ETA_XZ: x = cscale(d, __fdiv_rn(__fmul_rn(kx, kz), kkp)); break;
+        case SMK_AUX_ETA_YZ: x = cscale(d, __fdiv_rn(__fmul_rn(ky, kz), kkp)); break;
+        case SMK_AUX_VX: x = mul_mi<false>(cscale(d, __fdiv_rn(kx, kkp) * a.aux_coef)); break;
+        case SMK_AUX_VY: x = mul_mi<false>(cscale(d, __fdiv_rn(ky, kkp) * a.aux_coef)); break;
+        case SMK_AUX_VZ: x = mul_mi<false>(cscale(d, __fdiv_rn(kz, kkp) * a.aux_coef)); break;
+        default: x = make_float2(0.f, 0.f); break;
+      }
+      w[e] = x;
+      v[e] = d;
+    }
+    if (!valid) acc = 0.0;
+    const double bs = block_sum<NT>(acc, reinterpret_cast<double*>(lds));
+    if (threadIdx.x == 0) a.partial[tile] = bs;
+
+    if (a.write_delta) {
+      fft_line<N, true>(v, line, p, a.tw);
+      if (valid) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) base[(uint64_t)(p + P * e) * a.s_elem] = v[e];
+      }
+    }
+    if (a.aux_kind != SMK_AUX_NONE) {
+      __syncthreads();
+      fft_line<N, true>(w, line, p, a.tw);
+      if (valid) {
+        float2* abase = a.aux + (uint64_t)o * a.s_outer + inner;
+#pragma unroll
+        for (int e = 0; e < E; ++e) abase[(uint64_t)(p + P * e) * a.s_elem] = w[e];
+      }
+    }
+  }
+}
+
+// z pencils, R2C.  M = nz/2 complex points z_n = x_{2n} + i x_{2n+1}; after
+// the M-point FFT, X_k = Ze_k + W_{2M}^k Zo_k with
+// Ze_k = (Z_k + conj Z_{M-k})/2, Zo_k = -i (Z_k - conj Z_{M-k})/2.
+template <int M, int T>
+__global__ __launch_bounds__(T* threads_per_line(M)) void k_z_r2c(const float* __restrict__ noise,
+                                                                 uint64_t seed, float2* __restrict__ K,
+                                                                 const float2* __restrict__ tw,
+                                                                 const float2* __restrict__ twr,
+                                                                 uint64_t n_pencils) {
+  constexpr int E = elems_per_thread(M);
+  constexpr int P = M / E;
+  constexpr int LL = lds_line_len(M);
+  constexpr int NT = T * P;
+  __shared__ float2 lds[T * LL];
+  const int tid = threadIdx.x;
+  const int p = tid % P, t = tid / P;
+  const uint64_t L0 = (uint64_t)blockIdx.x * T;
+  const uint64_t L = L0 + t;
+  const bool valid = L < n_pencils;
+  float2* line = lds + t * LL;
+  float2 v[E];
+
+  if (noise == nullptr) {
+    // four cells per Philox call: cells 4q..4q+3 of the whole box
+    constexpr int Q = M / 2;
+    for (int q = tid; q < T * Q; q += NT) {
+      const int tt = q / Q, qq = q % Q;
+      const uint64_t LL_ = L0 + tt;
+      if (LL_ < n_pencils) {
+        U4 u = philox4x32_10(LL_ * (uint64_t)Q + qq, seed);
+        float n0, n1, n2, n3;
+        box_muller(u.x, u.y, n0, n1);
+        box_muller(u.z, u.w, n2, n3);
+        float2* ln = lds + tt * LL;
+        ln[lds_pos<M>(2 * qq)] = make_float2(n0, n1);
+        ln[lds_pos<M>(2 * qq + 1)] = make_float2(n2, n3);
+      }
+    }
+    __syncthreads();
+    lds_gather<M, P, E>(v, line, p);
+    __syncthreads();
+  } else {
+    const float2* src = reinterpret_cast<const float2*>(noise) + L * M;
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = valid ? src[p + P * e] : make_float2(0.f, 0.f);
+  }
+
+  fft_line<M, false>(v, line, p, tw);
+
+  lds_scatter<M, P, E>(v, line, p);
+  __syncthreads();
+  float2* dst = K + L * (uint64_t)(M + 1);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int k = p + P * e;
+    const float2 zk = line[lds_pos<M>(k)];
+    const float2 zc = cconj(line[lds_pos<M>((M - k) % M)]);
+    const float2 ze = cscale(cadd(zk, zc), 0.5f);
+    const float2 zo = mul_mi<false>(cscale(csub(zk, zc), 0.5f));
+    const float2 x = cadd(ze, cmul(twr[k], zo));
+    if (valid) dst[k] = x;
+  }
+  if (p == 0 && valid) {
+    const float2 z0 = line[lds_pos<M>(0)];
+    dst[M] = make_float2(z0.x - z0.y, 0.0f);
+  }
+}
+
+// z pencils, C2R (unnormalised, times `scale`).  Z'_k = (X_k + conj X_{M-k})
+// + i W_{2M}^{-k} (X_k - conj X_{M-k}); the inverse M-point FFT of Z' is
+// x_{2n} + i x_{2n+1}.  Imaginary parts of X_0 and X_M are ignored, as a C2R
+// transform does.
+template <int M, int T>
+__global__ __launch_bounds__(T* threads_per_line(M)) void k_z_c2r(const float2* __restrict__ K,
+                                                                 float* __restrict__ out,
+                                                                 const float2* __restrict__ tw,
+                                                                 const float2* __restrict__ twr,
+                                                                 uint64_t n_pencils, float scale,
+                                                                 int* __restrict__ status) {
+  constexpr int E = elems_per_thread(M);
+  constexpr int P = M / E;
+  constexpr int LL = lds_line_len(M);
+  __shared__ float2 lds[T * LL];
+  const int tid = threadIdx.x;
+  const int p = tid % P, t = tid / P;
+  const uint64_t L = (uint64_t)blockIdx.x * T + t;
+  const bool valid = L < n_pencils;
+  float2* line = lds + t * LL;
+  const float2* src = K + L * (uint64_t)(M + 1);
+  float2 v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int k = p + P * e;
+    float2 xk = valid ? src[k] : make_float2(0.f, 0.f);
+    float2 xc = valid ? src[M - k] : make_float2(0.f, 0.f);
+    if (k == 0) { xk.y = 0.0f; xc.y = 0.0f; }
+    const float2 s = cadd(xk, cconj(xc));
+    const float2 d = csub(xk, cconj(xc));
+    const float2 wd = cmul(cconj(twr[k]), d);
+    v[e] = make_float2(s.x - wd.y, s.y + wd.x);  // s + i*wd
+  }
+  fft_line<M, true>(v, line, p, tw);
+  int flags = 0;
+  if (valid) {
+    float2* dst = reinterpret_cast<float2*>(out + L * (uint64_t)(2 * M));
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const float2 y = cscale(v[e], scale);
+      dst[p + P * e] = y;
+      if (isnan(y.x) || isnan(y.y)) flags |= 1;
+      if (y.x != 0.0f || y.y != 0.0f) flags |= 2;
+    }
+  }
+  const int any_nan = __syncthreads_or(flags & 1);
+  const int any_nz = __syncthreads_or(flags & 2);
+  if (tid == 0 && status != nullptr) {
+    const int f = (any_nan ? 1 : 0) | (any_nz ? 2 : 0);
+    if (f) atomicOr(status, f);
+  }
+}
+
+__global__ void k_reduce_partials(const double* __restrict__ partial, uint64_t n, double* __restrict__ out) {
+  __shared__ double red[16];
+  double s = 0.0;
+  for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) s += partial[i];
+  const double tot = block_sum<1024>(s, red);
+  if (threadIdx.x == 0) *out = tot;
+}
+
+__global__ void k_clear_ints(int* p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+// --------------------------------------------------------------- dispatch
+#define SMK_FFT_LENGTHS(X) \
+  X(8) X(12) X(16) X(20) X(24) X(32) X(40) X(48) X(64) X(80) X(96) X(128) X(160) X(192) X(256) X(320) X(384) \
+  X(512) X(640) X(768) X(1024) X(1280) X(1536) X(2048) X(2560)
+
+bool length_supported(int n) {
+  switch (n) {
+#define SMK_CASE(N) case N:
+    SMK_FFT_LENGTHS(SMK_CASE)
+#undef SMK_CASE
+    return true;
+    default: return false;
+  }
+}
+
+template <int N, int MODE>
+hipError_t launch_strided_n(const StridedArgs& a, uint64_t n_outer, hipStream_t s) {
+  constexpr int T = strided_T(N);
+  StridedArgs b = a;
+  b.n_iblk = (a.n_inner + T - 1) / T;
+  const uint64_t nblk = n_outer * b.n_iblk;
+  if (nblk > 0x7fffffffull) return hipErrorInvalidConfiguration;
+  hipLaunchKernelGGL((k_strided<N, T, MODE>), dim3((uint32_t)nblk), dim3(T * threads_per_line(N)), 0, s, b);
+  return hipGetLastError();
+}
+
+template <int MODE>
+hipError_t launch_strided(int n, const StridedArgs& a, uint64_t n_outer, hipStream_t s) {
+  switch (n) {
+#define SMK_CASE(N) \
+  case N: return launch_strided_n<N, MODE>(a, n_outer, s);
+    SMK_FFT_LENGTHS(SMK_CASE)
+#undef SMK_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+uint64_t strided_blocks(int n, uint64_t n_outer, uint32_t n_inner) {
+  switch (n) {
+#define SMK_CASE(N) \
+  case N: return n_outer * ((n_inner + strided_T(N) - 1) / strided_T(N));
+    SMK_FFT_LENGTHS(SMK_CASE)
+#undef SMK_CASE
+    default: return 0;
+  }
+}
+
+hipError_t launch_z_r2c(int m, const float* noise, uint64_t seed, float2* K, const float2* tw,
+                        const float2* twr, uint64_t np, hipStream_t s) {
+  switch (m) {
+#define SMK_CASE(N)                                                                                    \
+  case N: {                                                                                            \
+    constexpr int T = pencil_T(N);                                                                     \
+    hipLaunchKernelGGL((k_z_r2c<N, T>), dim3((uint32_t)((np + T - 1) / T)), dim3(T * threads_per_line(N)), 0, \
+                       s, noise, seed, K, tw, twr, np);                                                 \
+    return hipGetLastError();                                                                          \
+  }
+    SMK_FFT_LENGTHS(SMK_CASE)
+#undef SMK_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_z_c2r(int m, const float2* K, float* out, const float2* tw, const float2* twr, uint64_t np,
+                        float scale, int* status, hipStream_t s) {
+  switch (m) {
+#define SMK_CASE(N)                                                                                    \
+  case N: {                                                                                            \
+    constexpr int T = pencil_T(N);                                                                     \
+    hipLaunchKernelGGL((k_z_c2r<N, T>), dim3((uint32_t)((np + T - 1) / T)), dim3(T * threads_per_line(N)), 0, \
+                       s, K, out, tw, twr, np, scale, status);                                          \
+    return hipGetLastError();                                                                          \
+  }
+    SMK_FFT_LENGTHS(SMK_CASE)
+#undef SMK_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+#define SMK_HIP_RET(expr)                                                                        \
+  do {                                                                                           \
+    hipError_t smk_e_ = (expr);                                                                  \
+    if (smk_e_ != hipSuccess)                                                                    \
+      return smk::set_error(SMK_ERR_HIP, "%s: %s", #expr, hipGetErrorString(smk_e_));            \
+  } while (0)
+
+std::vector<float2> twiddles(int n, double sign_angle_scale) {
+  // W^m = exp(-2 pi i m / (n * sign_angle_scale)), computed in double
+  std::vector<float2> t(n);
+  for (int m = 0; m < n; ++m) {
+    const double ang = -2.0 * M_PI * (double)m / ((double)n * sign_angle_scale);
+    t[m] = make_float2((float)cos(ang), (float)sin(ang));
+  }
+  return t;
+}
+
+// numpy.fft.fftfreq(n) * 2 * k_ny, then float32 (make_boxes.py:299-304)
+std::vector<float> fftfreq_k(int n, double cell, bool half) {
+  const double k_ny = M_PI / cell;
+  const double val = 1.0 / (double)n;
+  const int len = half ? n / 2 + 1 : n;
+  std::vector<float> k(len);
+  for (int i = 0; i < len; ++i) {
+    int ii = i;
+    if (!half && i >= (n + 1) / 2) ii = i - n;
+    k[i] = (float)(((double)ii * val) * 2.0 * k_ny);
+  }
+  return k;
+}
+
+}  // namespace
+
+struct smk_box_plan {
+  int nx, ny, nz, m, nzh;
+  double cell;
+  float2* tw_x;
+  float2* tw_y;
+  float2* tw_z;
+  float2* tw_r;
+  float* kx;
+  float* ky;
+  float* kz;
+  uint64_t n_partial;
+};
+
+extern "C" int smk_fft_length_supported(int n) { return length_supported(n) ? 1 : 0; }
+
+extern "C" int smk_box_plan_create(smk_box_plan** plan, int nx, int ny, int nz, double cell) {
+  SMK_REQUIRE(plan != nullptr, SMK_ERR_ARG, "plan pointer is NULL");
+  *plan = nullptr;
+  SMK_REQUIRE(nx > 0 && ny > 0 && nz > 0, SMK_ERR_ARG, "box sizes must be positive (%d, %d, %d)", nx, ny, nz);
+  SMK_REQUIRE(nz % 2 == 0, SMK_ERR_ARG, "nz must be even (got %d)", nz);
+  SMK_REQUIRE(cell > 0.0, SMK_ERR_ARG, "cell size must be positive");
+  SMK_REQUIRE(length_supported(nx), SMK_ERR_UNSUPPORTED, "no FFT kernel for nx = %d", nx);
+  SMK_REQUIRE(length_supported(ny), SMK_ERR_UNSUPPORTED, "no FFT kernel for ny = %d", ny);
+  SMK_REQUIRE(length_supported(nz / 2), SMK_ERR_UNSUPPORTED, "no FFT kernel for nz/2 = %d", nz / 2);
+  smk_box_plan* p = (smk_box_plan*)calloc(1, sizeof(smk_box_plan));
+  SMK_REQUIRE(p != nullptr, SMK_ERR_OOM, "host allocation failed");
+  p->nx = nx; p->ny = ny; p->nz = nz; p->m = nz / 2; p->nzh = nz / 2 + 1; p->cell = cell;
+  p->n_partial = strided_blocks(nx, (uint64_t)ny, (uint32_t)p->nzh);
+
+  auto upload = [](void** dst, const void* src, size_t bytes) -> hipError_t {
+    hipError_t e = hipMalloc(dst, bytes);
+    if (e != hipSuccess) return e;
+    return hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+  };
+  auto twx = twiddles(nx, 1.0), twy = twiddles(ny, 1.0), twz = twiddles(p->m, 1.0), twr = twiddles(p->m, 2.0);
+  auto kx = fftfreq_k(nx, cell, false), ky = fftfreq_k(ny, cell, false), kz = fftfreq_k(nz, cell, true);
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = upload((void**)&p->tw_x, twx.data(), twx.size() * sizeof(float2));
+  if (e == hipSuccess) e = upload((void**)&p->tw_y, twy.data(), twy.size() * sizeof(float2));
+  if (e == hipSuccess) e = upload((void**)&p->tw_z, twz.data(), twz.size() * sizeof(float2));
+  if (e == hipSuccess) e = upload((void**)&p->tw_r, twr.data(), twr.size() * sizeof(float2));
+  if (e == hipSuccess) e = upload((void**)&p->kx, kx.data(), kx.size() * sizeof(float));
+  if (e == hipSuccess) e = upload((void**)&p->ky, ky.data(), ky.size() * sizeof(float));
+  if (e == hipSuccess) e = upload((void**)&p->kz, kz.data(), kz.size() * sizeof(float));
+  if (e != hipSuccess) {
+    smk_box_plan_destroy(p);
+    return set_error(e == hipErrorOutOfMemory ? SMK_ERR_OOM : SMK_ERR_HIP, "plan tables: %s", hipGetErrorString(e));
+  }
+  *plan = p;
+  return SMK_OK;
+}
+
+extern "C" int smk_box_plan_destroy(smk_box_plan* p) {
+  if (p == nullptr) return SMK_OK;
+  void* bufs[] = {p->tw_x, p->tw_y, p->tw_z, p->tw_r, p->kx, p->ky, p->kz};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  free(p);
+  return SMK_OK;
+}
+
+static uint64_t kbox_elems(const smk_box_plan* p) { return (uint64_t)p->nx * p->ny * p->nzh; }
+static uint64_t align256(uint64_t b) { return (b + 255) & ~(uint64_t)255; }
+
+extern "C" size_t smk_box_workspace_bytes(const smk_box_plan* p) {
+  if (p == nullptr) return 0;
+  const uint64_t kb = align256(kbox_elems(p) * sizeof(float2));
+  return (size_t)(2 * kb + align256(p->n_partial * sizeof(double)));
+}
+
+namespace {
+
+StridedArgs y_args(const smk_box_plan* p, float2* K) {
+  StridedArgs a{};
+  a.data = K;
+  a.tw = p->tw_y;
+  a.s_outer = (uint64_t)p->ny * p->nzh;  // outer = x
+  a.s_elem = (uint64_t)p->nzh;           // along y
+  a.n_inner = (uint32_t)p->nzh;          // runs along kz
+  return a;
+}
+
+StridedArgs x_args(const smk_box_plan* p, float2* K) {
+  StridedArgs a{};
+  a.data = K;
+  a.tw = p->tw_x;
+  a.s_outer = (uint64_t)p->nzh;              // outer = ky
+  a.s_elem = (uint64_t)p->ny * p->nzh;       // along x
+  a.n_inner = (uint32_t)p->nzh;
+  return a;
+}
+
+int forward_to_k(smk_box_plan* p, uint64_t seed, const float* noise, float2* K, hipStream_t s) {
+  const uint64_t np = (uint64_t)p->nx * p->ny;
+  SMK_HIP_RET(launch_z_r2c(p->m, noise, seed, K, p->tw_z, p->tw_r, np, s));
+  SMK_HIP_RET(launch_strided<MODE_FWD>(p->ny, y_args(p, K), (uint64_t)p->nx, s));
+  return SMK_OK;
+}
+
+int k_to_real(smk_box_plan* p, float2* K, float* out, float scale, int* status, hipStream_t s) {
+  const uint64_t np = (uint64_t)p->nx * p->ny;
+  SMK_HIP_RET(launch_strided<MODE_INV>(p->ny, y_args(p, K), (uint64_t)p->nx, s));
+  SMK_HIP_RET(launch_z_c2r(p->m, K, out, p->tw_z, p->tw_r, np, scale, status, s));
+  return SMK_OK;
+}
+
+}  // namespace
+
+extern "C" int smk_grf_box(smk_box_plan* p, uint64_t seed, const float* noise, const double* pk_params, int aux_kind,
+                           double aux_coef, float* delta_out, float* aux_out, void* workspace, double* var_out,
+                           int* status_out, void* stream) {
+  SMK_REQUIRE(p != nullptr, SMK_ERR_ARG, "plan is NULL");
+  SMK_REQUIRE(pk_params != nullptr, SMK_ERR_ARG, "pk_params is NULL");
+  SMK_REQUIRE(delta_out != nullptr && workspace != nullptr, SMK_ERR_ARG, "delta_out/workspace is NULL");
+  SMK_REQUIRE(aux_kind >= SMK_AUX_NONE && aux_kind <= SMK_AUX_VZ, SMK_ERR_ARG, "bad aux_kind %d", aux_kind);
+  SMK_REQUIRE(aux_kind == SMK_AUX_NONE || aux_out != nullptr, SMK_ERR_ARG, "aux_out is NULL");
+  SMK_REQUIRE(((uintptr_t)workspace & 255) == 0, SMK_ERR_ARG, "workspace must be 256-byte aligned");
+  SMK_REQUIRE(pk_params[1] > 0.0, SMK_ERR_ARG, "pk k0 must be positive");
+  hipStream_t s = (hipStream_t)stream;
+  const uint64_t kb = align256(kbox_elems(p) * sizeof(float2));
+  float2* K = (float2*)workspace;
+  float2* A = (float2*)((char*)workspace + kb);
+  double* partial = (double*)((char*)workspace + 2 * kb);
+
+  if (status_out) {
+    hipLaunchKernelGGL(k_clear_ints, dim3(1), dim3(64), 0, s, status_out, 2);
+    SMK_LAUNCH_CHECK();
+  }
+  int rc = forward_to_k(p, seed, noise, K, s);
+  if (rc) return rc;
+
+  StridedArgs a = x_args(p, K);
+  a.aux = A;
+  a.kxv = p->kx; a.kyv = p->ky; a.kzv = p->kz;
+  a.pkA = (float)pk_params[0];
+  a.pk_k0inv = (float)(1.0 / pk_params[1]);
+  a.pk_n = (float)pk_params[2];
+  a.pk_R = (float)pk_params[3];
+  a.vcell = (float)(p->cell * p->cell * p->cell);
+  a.norm = (float)(1.0 / ((double)p->nx * p->ny * p->nz));
+  a.aux_coef = (float)aux_coef;
+  a.aux_kind = aux_kind;
+  a.write_delta = 1;
+  a.nz_nyq = (uint32_t)p->m;
+  a.partial = partial;
+  SMK_HIP_RET(launch_strided<MODE_SPECTRAL>(p->nx, a, (uint64_t)p->ny, s));
+  if (var_out) {
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, s, partial, p->n_partial, var_out);
+    SMK_LAUNCH_CHECK();
+  }
+  rc = k_to_real(p, K, delta_out, 1.0f, status_out, s);
+  if (rc) return rc;
+  if (aux_kind != SMK_AUX_NONE) {
+    rc = k_to_real(p, A, aux_out, 1.0f, status_out ? status_out + 1 : nullptr, s);
+    if (rc) return rc;
+  }
+  return SMK_OK;
+}
+
+extern "C" int smk_draw_grf_boxk(smk_box_plan* p, uint64_t seed, const float* noise, float* boxk_out, void* stream) {
+  SMK_REQUIRE(p != nullptr && boxk_out != nullptr, SMK_ERR_ARG, "plan/boxk_out is NULL");
+  hipStream_t s = (hipStream_t)stream;
+  float2* K = (float2*)boxk_out;
+  int rc = forward_to_k(p, seed, noise, K, s);
+  if (rc) return rc;
+  SMK_HIP_RET(launch_strided<MODE_FWD>(p->nx, x_args(p, K), (uint64_t)p->ny, s));
+  return SMK_OK;
+}
+
+extern "C" int smk_fft_and_store(smk_box_plan* p, float* boxk_inout, float* box_out, void* workspace,
+                                 double* sumsq_out, int* status_out, void* stream) {
+  SMK_REQUIRE(p != nullptr && boxk_inout != nullptr && box_out != nullptr, SMK_ERR_ARG, "NULL argument");
+  SMK_REQUIRE(sumsq_out == nullptr || workspace != nullptr, SMK_ERR_ARG, "sumsq_out needs a workspace");
+  hipStream_t s = (hipStream_t)stream;
+  float2* K = (float2*)boxk_inout;
+  if (status_out) {
+    hipLaunchKernelGGL(k_clear_ints, dim3(1), dim3(64), 0, s, status_out, 1);
+    SMK_LAUNCH_CHECK();
+  }
+  StridedArgs a = x_args(p, K);
+  a.nz_nyq = (uint32_t)p->m;
+  a.partial = sumsq_out ? (double*)workspace : nullptr;
+  SMK_HIP_RET(launch_strided<MODE_INV>(p->nx, a, (uint64_t)p->ny, s));
+  if (sumsq_out) {
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, s, a.partial, p->n_partial, sumsq_out);
+    SMK_LAUNCH_CHECK();
+  }
+  const float scale = (float)(1.0 / ((double)p->nx * p->ny * p->nz));
+  return k_to_real(p, K, box_out, scale, status_out, s);
+}
+
+extern "C" size_t smk_fft_and_store_workspace_bytes(const smk_box_plan* p) {
+  if (p == nullptr) return 0;
+  return (size_t)align256(p->n_partial * sizeof(double));
+}

@@ -1,0 +1,316 @@
+// Register-resident Stockham FFT engine for gfx950 (64-wide wavefronts).
+//
+// One line of N complex points is owned by P threads; each thread keeps
+// E = N / P points in VGPRs.  Every radix stage is done in registers and the
+// permutation between stages goes through LDS (one write, one read per point).
+// Thread p always holds, before the first stage and after the last one, the
+// points n = p + P*e (e = 0..E-1): the global load of the first stage and the
+// global store of the last stage are therefore unit-stride across p, which is
+// what makes the HBM side coalesced whatever the line stride is.
+//
+// Twiddles come from a per-length table W_N^m = exp(-2 pi i m / N), m < N,
+// computed in FP64 on the host and rounded once to FP32.
+//
+// This file is device-only; it is included by the box and skewer kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace smk {
+
+// ---------------------------------------------------------------- complex ops
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+// multiply by -i (forward transform) or +i (inverse transform)
+template <bool INV> __device__ __forceinline__ float2 mul_mi(float2 a) {
+  return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+// ------------------------------------------------------------ radix kernels
+template <int R, bool INV> struct DFT;
+
+template <bool INV> struct DFT<2, INV> {
+  static __device__ __forceinline__ void run(float2* a) {
+    float2 t = a[0];
+    a[0] = cadd(t, a[1]);
+    a[1] = csub(t, a[1]);
+  }
+};
+
+template <bool INV> struct DFT<3, INV> {
+  static __device__ __forceinline__ void run(float2* a) {
+    const float s = 0.866025403784438646763723f;  // sin(2 pi / 3)
+    float2 b = cadd(a[1], a[2]);
+    float2 d = csub(a[1], a[2]);
+    float2 y0 = cadd(a[0], b);
+    float2 t = make_float2(a[0].x - 0.5f * b.x, a[0].y - 0.5f * b.y);
+    float2 u = mul_mi<INV>(cscale(d, s));
+    a[0] = y0;
+    a[1] = cadd(t, u);
+    a[2] = csub(t, u);
+  }
+};
+
+template <bool INV> struct DFT<4, INV> {
+  static __device__ __forceinline__ void run(float2* a) {
+    float2 t0 = cadd(a[0], a[2]);
+    float2 t1 = csub(a[0], a[2]);
+    float2 t2 = cadd(a[1], a[3]);
+    float2 t3 = mul_mi<INV>(csub(a[1], a[3]));
+    a[0] = cadd(t0, t2);
+    a[2] = csub(t0, t2);
+    a[1] = cadd(t1, t3);
+    a[3] = csub(t1, t3);
+  }
+};
+
+template <bool INV> struct DFT<5, INV> {
+  static __device__ __forceinline__ void run(float2* a) {
+    const float c1 = 0.309016994374947424102293f;   // cos(2 pi / 5)
+    const float c2 = -0.809016994374947424102293f;  // cos(4 pi / 5)
+    const float s1 = 0.951056516295153572116439f;   // sin(2 pi / 5)
+    const float s2 = 0.587785252292473129168706f;   // sin(4 pi / 5)
+    float2 b1 = cadd(a[1], a[4]);
+    float2 b2 = cadd(a[2], a[3]);
+    float2 d1 = csub(a[1], a[4]);
+    float2 d2 = csub(a[2], a[3]);
+    float2 y0 = cadd(a[0], cadd(b1, b2));
+    float2 t1 = make_float2(a[0].x + c1 * b1.x + c2 * b2.x, a[0].y + c1 * b1.y + c2 * b2.y);
+    float2 t2 = make_float2(a[0].x + c2 * b1.x + c1 * b2.x, a[0].y + c2 * b1.y + c1 * b2.y);
+    float2 u1 = mul_mi<INV>(make_float2(s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y));
+    float2 u2 = mul_mi<INV>(make_float2(s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y));
+    a[0] = y0;
+    a[1] = cadd(t1, u1);
+    a[4] = csub(t1, u1);
+    a[2] = cadd(t2, u2);
+    a[3] = csub(t2, u2);
+  }
+};
+
+template <bool INV> struct DFT<8, INV> {
+  static __device__ __forceinline__ void run(float2* a) {
+    const float r = 0.707106781186547524400844f;
+    float2 e[4] = {a[0], a[2], a[4], a[6]};
+    float2 o[4] = {a[1], a[3], a[5], a[7]};
+    DFT<4, INV>::run(e);
+    DFT<4, INV>::run(o);
+    // o[k] *= W8^{+-k}
+    float2 o1, o3;
+    if (INV) {
+      o1 = make_float2(r * (o[1].x - o[1].y), r * (o[1].x + o[1].y));
+      o3 = make_float2(-r * (o[3].x + o[3].y), r * (o[3].x - o[3].y));
+    } else {
+      o1 = make_float2(r * (o[1].x + o[1].y), r * (o[1].y - o[1].x));
+      o3 = make_float2(r * (o[3].y - o[3].x), -r * (o[3].x + o[3].y));
+    }
+    float2 o2 = mul_mi<INV>(o[2]);
+    a[0] = cadd(e[0], o[0]);
+    a[4] = csub(e[0], o[0]);
+    a[1] = cadd(e[1], o1);
+    a[5] = csub(e[1], o1);
+    a[2] = cadd(e[2], o2);
+    a[6] = csub(e[2], o2);
+    a[3] = cadd(e[3], o3);
+    a[7] = csub(e[3], o3);
+  }
+};
+
+// --------------------------------------------------------- compile-time plan
+// Radices are picked greedily from {8,5,4,3,2}; every radix must divide E so
+// that each thread owns a whole number of butterflies in every stage.
+struct RadixPlan {
+  int n, e, p, nstage;
+  int r[8];
+};
+
+__host__ __device__ constexpr RadixPlan make_plan(int n, int e) {
+  RadixPlan pl{n, e, n / e, 0, {0, 0, 0, 0, 0, 0, 0, 0}};
+  int rem = n;
+  const int cand[5] = {8, 5, 4, 3, 2};
+  while (rem > 1 && pl.nstage < 8) {
+    int pick = 0;
+    for (int i = 0; i < 5; ++i) {
+      if (e % cand[i] == 0 && rem % cand[i] == 0) { pick = cand[i]; break; }
+    }
+    if (pick == 0) { pl.nstage = -1; return pl; }
+    pl.r[pl.nstage++] = pick;
+    rem /= pick;
+  }
+  return pl;
+}
+
+// Elements per thread for each supported line length (chosen to keep the
+// number of LDS exchanges at 2-4 and the register tile at <= 40 points).
+__host__ __device__ constexpr int elems_per_thread(int n) {
+  switch (n) {
+    case 8: return 8;
+    case 12: return 12;
+    case 16: return 4;
+    case 20: return 20;
+    case 24: return 12;
+    case 32: return 8;
+    case 40: return 20;
+    case 48: return 12;
+    case 64: return 8;
+    case 80: return 20;
+    case 96: return 24;
+    case 128: return 8;
+    case 160: return 20;
+    case 192: return 24;
+    case 256: return 8;
+    case 320: return 40;
+    case 384: return 24;
+    case 512: return 8;
+    case 640: return 40;
+    case 768: return 24;
+    case 1024: return 8;
+    case 1280: return 40;
+    case 1536: return 24;
+    case 2048: return 8;
+    case 2560: return 40;
+    default: return 0;
+  }
+}
+
+// LDS line stride (in float2): one pad slot every 32 points breaks the
+// power-of-two strides of the early Stockham stages across banks.  The
+// 2560-point line is left unpadded so that four lines fill exactly 80 KiB and
+// two blocks share one CU's 160 KiB.
+__host__ __device__ constexpr bool lds_padded(int n) { return n != 2560; }
+template <int N> __host__ __device__ constexpr int lds_pos(int pos) {
+  return lds_padded(N) ? pos + (pos >> 5) : pos;
+}
+__host__ __device__ constexpr int lds_line_len(int n) { return lds_padded(n) ? n + (n >> 5) + 1 : n; }
+
+// ------------------------------------------------------------- stage driver
+// One radix-R stage of a Stockham autosort FFT.  Inputs of butterfly j are
+// the points j + r*N/R (r = 0..R-1); outputs go to idxD + r*Ns with
+// idxD = (j/Ns)*Ns*R + j%Ns.  For the last stage that is j + r*N/R again, so
+// the result stays in the registers it was computed in.
+template <int N, int P, int R, int Ns, bool INV, bool LAST, int E>
+__device__ __forceinline__ void fft_stage(float2 (&v)[E], float2* __restrict__ line, int p,
+                                          const float2* __restrict__ tw) {
+  constexpr int B = E / R;
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
+    const int j = p + P * b;
+    float2 a[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) a[r] = v[b + r * B];
+    const int jj = (Ns > 1) ? (j % Ns) : 0;
+    if constexpr (Ns > 1) {
+      constexpr int step = N / (Ns * R);
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        float2 w = tw[r * jj * step];
+        if (INV) w.y = -w.y;
+        a[r] = cmul(a[r], w);
+      }
+    }
+    DFT<R, INV>::run(a);
+    if constexpr (LAST) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b + r * B] = a[r];
+    } else {
+      const int idxD = (j / Ns) * Ns * R + jj;
+#pragma unroll
+      for (int r = 0; r < R; ++r) line[lds_pos<N>(idxD + r * Ns)] = a[r];
+    }
+  }
+}
+
+template <int N, int P, int E>
+__device__ __forceinline__ void lds_gather(float2 (&v)[E], const float2* __restrict__ line, int p) {
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = line[lds_pos<N>(p + P * e)];
+}
+
+template <int N, int P, int E>
+__device__ __forceinline__ void lds_scatter(const float2 (&v)[E], float2* __restrict__ line, int p) {
+#pragma unroll
+  for (int e = 0; e < E; ++e) line[lds_pos<N>(p + P * e)] = v[e];
+}
+
+template <int N, int E, bool INV, int S, int Ns>
+__device__ __forceinline__ void fft_stages(float2 (&v)[E], float2* __restrict__ line, int p,
+                                           const float2* __restrict__ tw) {
+  constexpr RadixPlan pl = make_plan(N, E);
+  static_assert(pl.nstage > 0, "unsupported FFT length");
+  constexpr int P = N / E;
+  if constexpr (S < pl.nstage) {
+    constexpr int R = pl.r[S];
+    constexpr bool LAST = (S + 1 == pl.nstage);
+    fft_stage<N, P, R, Ns, INV, LAST, E>(v, line, p, tw);
+    if constexpr (!LAST) {
+      __syncthreads();
+      lds_gather<N, P, E>(v, line, p);
+      __syncthreads();
+      fft_stages<N, E, INV, S + 1, Ns * R>(v, line, p, tw);
+    }
+  }
+}
+
+// Full in-register FFT of one line.  All threads of the block must call it
+// (it contains block barriers).  On entry and exit thread p holds points
+// p + P*e.  Forward: X_k = sum_n x_n exp(-2 pi i n k / N); inverse uses
+// exp(+...) and is unnormalised.
+template <int N, bool INV>
+__device__ __forceinline__ void fft_line(float2 (&v)[elems_per_thread(N)], float2* __restrict__ line, int p,
+                                         const float2* __restrict__ tw) {
+  fft_stages<N, elems_per_thread(N), INV, 0, 1>(v, line, p, tw);
+}
+
+// ------------------------------------------------------ counter-based noise
+// Philox4x32-10 (Salmon et al., SC'11), keyed by the 64-bit seed, one counter
+// per group of four consecutive cells; Box-Muller turns the four uniforms
+// into four N(0,1) values.  The value at a cell depends only on (seed, cell
+// index), never on the launch geometry.
+struct U4 { uint32_t x, y, z, w; };
+
+__host__ __device__ __forceinline__ U4 philox4x32_10(uint64_t ctr, uint64_t seed) {
+  uint32_t c0 = (uint32_t)ctr, c1 = (uint32_t)(ctr >> 32), c2 = 0u, c3 = 0u;
+  uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+  for (int i = 0; i < 10; ++i) {
+    uint64_t p0 = (uint64_t)M0 * c0;
+    uint64_t p1 = (uint64_t)M1 * c2;
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    uint32_t n0 = hi1 ^ c1 ^ k0;
+    uint32_t n2 = hi0 ^ c3 ^ k1;
+    c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    k0 += W0; k1 += W1;
+  }
+  return U4{c0, c1, c2, c3};
+}
+
+// uniform in (0, 1): 24 high bits plus half an ulp, exactly representable
+__device__ __forceinline__ float u01(uint32_t x) {
+  return (float)(x >> 8) * 5.9604644775390625e-8f + 2.98023223876953125e-8f;
+}
+
+__device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, float& n0, float& n1) {
+  float r = sqrtf(-2.0f * logf(u01(a)));
+  float s, c;
+  sincospif(2.0f * u01(b), &s, &c);
+  n0 = r * c;
+  n1 = r * s;
+}
+
+// XCD-aware tile order: consecutive hardware block ids land on different
+// XCDs (round robin over 8); remap so that each XCD walks a contiguous range
+// of tiles and neighbouring tiles share that XCD's L2.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nblk) {
+  const uint32_t nx = 8;
+  uint32_t q = nblk / nx, r = nblk % nx;
+  uint32_t x = b % nx, i = b / nx;
+  return x * q + (x < r ? x : r) + i;
+}
+
+}  // namespace smk
