@@ -69,8 +69,8 @@ typedef struct smk_box_plan smk_box_plan;
 int smk_box_plan_create(smk_box_plan** plan, int nx, int ny, int nz, double cell);
 int smk_box_plan_destroy(smk_box_plan* plan);
 
-/* Bytes of device workspace smk_grf_box needs (k-space box, auxiliary
- * k-space box, per-block FP64 partial sums). */
+/* Bytes of device workspace smk_grf_box needs (three k-space boxes and
+ * per-block FP64 partial sums: 3 * nx*ny*(nz/2+1)*8 bytes + small). */
 size_t smk_box_workspace_bytes(const smk_box_plan* plan);
 
 /* Fused GRF box: noise -> R2C -> x sqrt(P(k)/Vcell) -> C2R giving delta, and

@@ -2,19 +2,24 @@
 //
 // Reference: bin/make_boxes.py:40-125,289-352 and bin/interpolate_pk.py:63-77.
 //
-// The k-space box K lives in the numpy layout complex64 [nx][ny][nzh]
-// (nzh = nz/2 + 1) and every pass works IN PLACE on it:
-//   Z1  z R2C     pencils (x,y) contiguous; noise drawn in-kernel (Philox)
-//   Y   y FFT     lines along y (stride nzh); a block owns T consecutive kz,
-//                 so every HBM access is a run of T complex values
-//   X   x FFT     lines along x (stride ny*nzh), same tiling; in the fused
-//                 job the forward x-FFT, the sqrt(P/Vcell) multiply, the
-//                 aux factor (eta_zz = kz^2/k^2) and both inverse x-FFTs are
-//                 one kernel, writing delta_k in place and aux_k to a 2nd box
-//   Y^-1          on both boxes
-//   Z^-1 C2R      both boxes -> float32 [nx][ny][nz]
-// HBM traffic of the fused job: 4 (Z1 write) + 8 (Y) + 12 (X, one read two
-// writes) + 16 (Y^-1 x2) + 16 (Z^-1 x2) = 56 bytes per real cell.
+// Two pass chains share one set of line-FFT kernels.  Every kernel FFTs T
+// lines per block; lanes are ordered line-fastest so that an access to the
+// "run" axis of a tile is T consecutive complex values.
+//
+// Fused job (smk_grf_box), transposing chain, 3 complex work boxes A1..A3:
+//   Z    z R2C of Philox noise       -> A1 [x][kz][y]
+//   Y    y FFT                A1     -> A2 [kz][ky][x]
+//   X    x FFT, * sqrt(P/Vcell)/N, aux factor, x IFFT twice
+//                             A2     -> A1 (delta_k) and A3 (aux_k), [kz][x][ky]
+//   Y^-1 y IFFT  A1 -> A2, A3 -> A1  [x][y][kz]
+//   Z^-1 z C2R   A2 -> delta, A1 -> aux, float32 [x][y][z]
+// Each pass reads whole contiguous lines and writes runs of T values whose
+// stride stays within one (y,kz) or (x,ky) plane, so no tile walks the full
+// 40 GB box (the x axis of the numpy layout has a 15.7 MB stride at
+// 2560x2560x1536).  HBM traffic: 4 + 8 + 12 + 16 + 16 = 56 bytes per cell.
+//
+// Drop-in steps (smk_draw_grf_boxk, smk_fft_and_store) keep boxk in the
+// numpy layout complex64 [nx][ny][nz/2+1] and work in place on it.
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -34,11 +39,14 @@ namespace {
 // T consecutive kz (T*8 bytes per HBM access run); pencil passes just batch.
 constexpr int threads_per_line(int n) { return n / elems_per_thread(n); }
 
-constexpr int strided_T(int n) {
+constexpr int mode_E(int n, int mode) { return elems_per_thread(n); }
+
+constexpr int strided_T(int n, int e) {
   int T = 8;
-  while (T > 1 && (T * threads_per_line(n) > 512 || T * lds_line_len(n) * 8 > 81920)) T >>= 1;
+  while (T > 1 && (T * (n / e) > 512 || T * lds_line_len(n) * 8 > 81920)) T >>= 1;
   return T;
 }
+constexpr int strided_T(int n) { return strided_T(n, elems_per_thread(n)); }
 
 constexpr int pencil_T(int m) {
   int T = 16;
@@ -49,14 +57,20 @@ constexpr int pencil_T(int m) {
 enum { MODE_FWD = 0, MODE_INV = 1, MODE_SPECTRAL = 2 };
 
 struct StridedArgs {
-  float2* data;          // in-place box
-  float2* aux;           // second box (MODE_SPECTRAL with aux_kind != NONE)
+  // line (o, i) point n is read at  in[o*in_so + i*in_si + n*in_sn]  and
+  // written at out[o*out_so + i*out_si + n*out_sk]; in == out is allowed
+  // (each block reads its whole tile before it writes it)
+  const float2* in;
+  float2* out;
+  float2* aux;           // second output (MODE_SPECTRAL with aux_kind != NONE)
   const float2* tw;      // W_N^m, m < N
-  uint64_t s_outer;      // stride (float2) between outer lines
-  uint64_t s_elem;       // stride (float2) between points of a line
-  uint32_t n_inner;      // inner (run) extent; the inner stride is 1
+  uint64_t in_so, in_si, in_sn;
+  uint64_t out_so, out_si, out_sk;
+  uint32_t n_inner;      // extent of the inner (run) index
   uint32_t n_iblk;       // ceil(n_inner / T)
-  // ---- MODE_SPECTRAL only (line axis = x, outer = ky, inner = kz)
+  // ---- MODE_SPECTRAL only: the line axis is x; kz_outer = 1 when the outer
+  // index is kz and the inner one ky, 0 for the opposite
+  int kz_outer;
   const float* kxv;
   const float* kyv;
   const float* kzv;
@@ -67,13 +81,17 @@ struct StridedArgs {
   int aux_kind;
   int write_delta;
   uint32_t nz_nyq;       // kz index of the Nyquist plane (nz/2)
-  double* partial;       // per-block sum of w(kz)|delta_k|^2, FP64
+  double* partial;       // per-block FP64 sums (spectral: w(kz)|delta_k|^2)
 };
 
+// P(k) = A x / (1 + x^n) exp(-(k R)^2), x = k / k0, with the hardware
+// transcendentals (about 1 ulp each); x = 0 gives P = 0.
 __device__ __forceinline__ float pk_analytic(float k, const StridedArgs& a) {
-  float x = k * a.pk_k0inv;
-  float kr = k * a.pk_R;
-  return a.pkA * x / (1.0f + powf(x, a.pk_n)) * expf(-kr * kr);
+  const float x = k * a.pk_k0inv;
+  const float xn = __builtin_amdgcn_exp2f(a.pk_n * __builtin_amdgcn_logf(x));
+  const float kr = k * a.pk_R;
+  const float g = __builtin_amdgcn_exp2f(-1.4426950408889634f * kr * kr);
+  return a.pkA * x * __builtin_amdgcn_rcpf(1.0f + xn) * g;
 }
 
 // Wave reduction then block reduction in a fixed order (deterministic).
@@ -98,9 +116,8 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   return s;
 }
 
-template <int N, int T, int MODE>
-__global__ __launch_bounds__(T* threads_per_line(N)) void k_strided(StridedArgs a) {
-  constexpr int E = elems_per_thread(N);
+template <int N, int E, int T, int MODE>
+__global__ __launch_bounds__(T*(N / E)) void k_strided(StridedArgs a) {
   constexpr int P = N / E;
   constexpr int LL = lds_line_len(N);
   constexpr int NT = T * P;
@@ -112,16 +129,18 @@ __global__ __launch_bounds__(T* threads_per_line(N)) void k_strided(StridedArgs 
   const uint32_t o = tile / a.n_iblk;
   const uint32_t inner = (tile % a.n_iblk) * T + t;
   const bool valid = inner < a.n_inner;
-  float2* base = a.data + (uint64_t)o * a.s_outer + inner;
+  const float2* src = a.in + (uint64_t)o * a.in_so + (uint64_t)inner * a.in_si;
+  float2* dst = a.out + (uint64_t)o * a.out_so + (uint64_t)inner * a.out_si;
 
   float2 v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e)
-    v[e] = valid ? base[(uint64_t)(p + P * e) * a.s_elem] : make_float2(0.f, 0.f);
+    v[e] = valid ? src[(uint64_t)(p + P * e) * a.in_sn] : make_float2(0.f, 0.f);
 
   float2* line = lds + t * LL;
   if constexpr (MODE == MODE_INV) {
-    // optional FP64 Parseval sum of the input spectrum (x axis only: inner = kz)
+    // optional FP64 Parseval sum of the input spectrum (natural x pass only:
+    // the inner index is kz)
     if (a.partial != nullptr) {
       const double wz = (inner == 0 || inner == a.nz_nyq) ? 1.0 : 2.0;
       double acc = 0.0;
@@ -132,65 +151,86 @@ __global__ __launch_bounds__(T* threads_per_line(N)) void k_strided(StridedArgs 
       if (threadIdx.x == 0) a.partial[tile] = bs;
     }
   }
-  fft_line<N, MODE == MODE_INV>(v, line, p, a.tw);
+  fft_line_e<N, E, MODE == MODE_INV>(v, line, p, a.tw);
 
   if constexpr (MODE != MODE_SPECTRAL) {
     if (valid) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) base[(uint64_t)(p + P * e) * a.s_elem] = v[e];
+      for (int e = 0; e < E; ++e) dst[(uint64_t)(p + P * e) * a.out_sk] = v[e];
     }
   } else {
-    // v[e] holds X(kx = p + P e, ky = o, kz = inner)
-    const float ky = a.kyv[o];
-    const float kz = valid ? a.kzv[inner] : 0.0f;
-    const double wz = (inner == 0 || inner == a.nz_nyq) ? 1.0 : 2.0;
-    float2 w[E];
+    // v[e] holds X(kx = p + P e, ky, kz).  One register tile only: delta_k
+    // is inverse transformed and stored; for the aux field the same tile is
+    // forward transformed again (FFT(IFFT(d)) = N d, exact up to rounding),
+    // scaled by f/N and inverse transformed.
+    const uint32_t iky = a.kz_outer ? inner : o;
+    const uint32_t ikz = a.kz_outer ? o : inner;
+    const float ky = valid ? a.kyv[iky] : 0.0f;
+    const float kz = valid ? a.kzv[ikz] : 0.0f;
+    const double wz = (ikz == 0 || ikz == a.nz_nyq) ? 1.0 : 2.0;
+    const float inv_vcell = 1.0f / a.vcell;
     double acc = 0.0;
+    const int z1 = opaque_zero();
+    const float* kxv = a.kxv + z1;
+    const int p1 = p + z1;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      const int i = p + P * e;
-      const float kx = a.kxv[i];
+      const int i = p1 + P * e;
+      const float kx = kxv[i];
       // kk = kx*kx + ky*ky + kz*kz in float32, numpy evaluation order
       const float kk = __fadd_rn(__fadd_rn(__fmul_rn(kx, kx), __fmul_rn(ky, ky)), __fmul_rn(kz, kz));
-      const float k = __fsqrt_rn(kk);
-      const float s = __fsqrt_rn(__fdiv_rn(pk_analytic(k, a), a.vcell));
-      const float2 d = cscale(v[e], s * a.norm);
-      acc += wz * ((double)d.x * (double)d.x + (double)d.y * (double)d.y);
-      const float kkp = (i == 0 && o == 0 && inner == 0) ? 1.0f : kk;
-      float2 x;
-      switch (a.aux_kind) {
-        case SMK_AUX_ETA_XX: x = cscale(d, __fdiv_rn(__fmul_rn(kx, kx), kkp)); break;
-        case SMK_AUX_ETA_YY: x = cscale(d, __fdiv_rn(__fmul_rn(ky, ky), kkp)); break;
-        case SMK_AUX_ETA_ZZ: x = cscale(d, __fdiv_rn(__fmul_rn(kz, kz), kkp)); break;
-        case SMK_AUX_ETA_XY: x = cscale(d, __fdiv_rn(__fmul_rn(kx, ky), kkp)); break;
-        case SMK_AUX_ETA_XZ: x = cscale(d, __fdiv_rn(__fmul_rn(kx, kz), kkp)); break;
-        case SMK_AUX_ETA_YZ: x = cscale(d, __fdiv_rn(__fmul_rn(ky, kz), kkp)); break;
-        case SMK_AUX_VX: x = mul_mi<false>(cscale(d, __fdiv_rn(kx, kkp) * a.aux_coef)); break;
-        case SMK_AUX_VY: x = mul_mi<false>(cscale(d, __fdiv_rn(ky, kkp) * a.aux_coef)); break;
-        case SMK_AUX_VZ: x = mul_mi<false>(cscale(d, __fdiv_rn(kz, kkp) * a.aux_coef)); break;
-        default: x = make_float2(0.f, 0.f); break;
-      }
-      w[e] = x;
+      const float k = __builtin_amdgcn_sqrtf(kk);
+      const float sp = __builtin_amdgcn_sqrtf(pk_analytic(k, a) * inv_vcell);
+      const float2 d = cscale(v[e], sp * a.norm);
+      acc += (double)d.x * (double)d.x + (double)d.y * (double)d.y;
       v[e] = d;
     }
-    if (!valid) acc = 0.0;
+    acc = valid ? wz * acc : 0.0;
     const double bs = block_sum<NT>(acc, reinterpret_cast<double*>(lds));
     if (threadIdx.x == 0) a.partial[tile] = bs;
 
-    if (a.write_delta) {
-      fft_line<N, true>(v, line, p, a.tw);
-      if (valid) {
+    fft_line_e<N, E, true>(v, line, p, a.tw);
+    if (valid && a.write_delta) {
+      const int z2 = opaque_zero();
+      const uint64_t sk = a.out_sk + z2;
+      const int p2 = p + z2;
 #pragma unroll
-        for (int e = 0; e < E; ++e) base[(uint64_t)(p + P * e) * a.s_elem] = v[e];
-      }
+      for (int e = 0; e < E; ++e) dst[(uint64_t)(p2 + P * e) * sk] = v[e];
     }
     if (a.aux_kind != SMK_AUX_NONE) {
+      // aux factor: eta_ij = k_i k_j / kk, v_i = -i k_i / kk * coef; operand
+      // selectors 0 = kx, 1 = ky, 2 = kz, 3 = 1
+      const int ak = a.aux_kind;
+      const int sa = (ak == SMK_AUX_ETA_XX || ak == SMK_AUX_ETA_XY || ak == SMK_AUX_ETA_XZ || ak == SMK_AUX_VX) ? 0
+                     : (ak == SMK_AUX_ETA_YY || ak == SMK_AUX_ETA_YZ || ak == SMK_AUX_VY) ? 1 : 2;
+      const int sb = (ak == SMK_AUX_ETA_XX) ? 0 : (ak == SMK_AUX_ETA_YY || ak == SMK_AUX_ETA_XY) ? 1
+                     : (ak >= SMK_AUX_VX) ? 3 : 2;
+      const bool vel = ak >= SMK_AUX_VX;
+      const float coef = (vel ? a.aux_coef : 1.0f) / (float)N;
       __syncthreads();
-      fft_line<N, true>(w, line, p, a.tw);
-      if (valid) {
-        float2* abase = a.aux + (uint64_t)o * a.s_outer + inner;
+      fft_line_e<N, E, false>(v, line, p, a.tw);
+      const int z3 = opaque_zero();
+      const float* kxv2 = a.kxv + z3;
+      const int p3 = p + z3;
 #pragma unroll
-        for (int e = 0; e < E; ++e) abase[(uint64_t)(p + P * e) * a.s_elem] = w[e];
+      for (int e = 0; e < E; ++e) {
+        const int i = p3 + P * e;
+        const float kx = kxv2[i];
+        const float kk = __fadd_rn(__fadd_rn(__fmul_rn(kx, kx), __fmul_rn(ky, ky)), __fmul_rn(kz, kz));
+        const float kkp = (i == 0 && iky == 0 && ikz == 0) ? 1.0f : kk;
+        const float ka = sa == 0 ? kx : (sa == 1 ? ky : kz);
+        const float kb = sb == 0 ? kx : (sb == 1 ? ky : (sb == 2 ? kz : 1.0f));
+        const float2 x = cscale(v[e], ka * kb * __builtin_amdgcn_rcpf(kkp) * coef);
+        v[e] = vel ? mul_mi<false>(x) : x;
+      }
+      fft_line_e<N, E, true>(v, line, p, a.tw);
+      if (valid) {
+        const int z4 = opaque_zero();
+        float2* adst = a.aux + (uint64_t)o * a.out_so + (uint64_t)inner * a.out_si;
+        const uint64_t sk = a.out_sk + z4;
+        const int p4 = p + z4;
+#pragma unroll
+        for (int e = 0; e < E; ++e) adst[(uint64_t)(p4 + P * e) * sk] = v[e];
       }
     }
   }
@@ -199,12 +239,14 @@ __global__ __launch_bounds__(T* threads_per_line(N)) void k_strided(StridedArgs 
 // z pencils, R2C.  M = nz/2 complex points z_n = x_{2n} + i x_{2n+1}; after
 // the M-point FFT, X_k = Ze_k + W_{2M}^k Zo_k with
 // Ze_k = (Z_k + conj Z_{M-k})/2, Zo_k = -i (Z_k - conj Z_{M-k})/2.
+// Output: trans == 0 -> numpy layout K[x][y][kz]; trans == 1 -> K[x][kz][y]
+// (the store phase then runs with lanes ordered pencil-fastest, i.e. along y).
 template <int M, int T>
 __global__ __launch_bounds__(T* threads_per_line(M)) void k_z_r2c(const float* __restrict__ noise,
                                                                  uint64_t seed, float2* __restrict__ K,
                                                                  const float2* __restrict__ tw,
                                                                  const float2* __restrict__ twr,
-                                                                 uint64_t n_pencils) {
+                                                                 uint64_t n_pencils, int trans, uint32_t ny) {
   constexpr int E = elems_per_thread(M);
   constexpr int P = M / E;
   constexpr int LL = lds_line_len(M);
@@ -247,20 +289,35 @@ __global__ __launch_bounds__(T* threads_per_line(M)) void k_z_r2c(const float* _
 
   lds_scatter<M, P, E>(v, line, p);
   __syncthreads();
-  float2* dst = K + L * (uint64_t)(M + 1);
+  // store phase: (ts, ps) is the lane's (pencil, point) role
+  const int ts = trans ? tid % T : t;
+  const int ps = trans ? tid / T : p;
+  const uint64_t Ls = L0 + ts;
+  const bool vs = Ls < n_pencils;
+  const float2* sl = lds + ts * LL;
+  float2* dst;
+  uint64_t sk;
+  if (trans) {
+    const uint64_t x = Ls / ny, y = Ls % ny;
+    dst = K + x * (uint64_t)(M + 1) * ny + y;
+    sk = ny;
+  } else {
+    dst = K + Ls * (uint64_t)(M + 1);
+    sk = 1;
+  }
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    const int k = p + P * e;
-    const float2 zk = line[lds_pos<M>(k)];
-    const float2 zc = cconj(line[lds_pos<M>((M - k) % M)]);
+    const int k = ps + P * e;
+    const float2 zk = sl[lds_pos<M>(k)];
+    const float2 zc = cconj(sl[lds_pos<M>((M - k) % M)]);
     const float2 ze = cscale(cadd(zk, zc), 0.5f);
     const float2 zo = mul_mi<false>(cscale(csub(zk, zc), 0.5f));
     const float2 x = cadd(ze, cmul(twr[k], zo));
-    if (valid) dst[k] = x;
+    if (vs) dst[k * sk] = x;
   }
-  if (p == 0 && valid) {
-    const float2 z0 = line[lds_pos<M>(0)];
-    dst[M] = make_float2(z0.x - z0.y, 0.0f);
+  if (ps == 0 && vs) {
+    const float2 z0 = sl[lds_pos<M>(0)];
+    dst[(uint64_t)M * sk] = make_float2(z0.x - z0.y, 0.0f);
   }
 }
 
@@ -279,6 +336,7 @@ __global__ __launch_bounds__(T* threads_per_line(M)) void k_z_c2r(const float2* 
   constexpr int P = M / E;
   constexpr int LL = lds_line_len(M);
   __shared__ float2 lds[T * LL];
+  __shared__ float2 xnyq[T];
   const int tid = threadIdx.x;
   const int p = tid % P, t = tid / P;
   const uint64_t L = (uint64_t)blockIdx.x * T + t;
@@ -286,25 +344,36 @@ __global__ __launch_bounds__(T* threads_per_line(M)) void k_z_c2r(const float2* 
   float2* line = lds + t * LL;
   const float2* src = K + L * (uint64_t)(M + 1);
   float2 v[E];
+  // one coalesced load per point; the mirror point X_{M-k} comes from LDS
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = valid ? src[p + P * e] : make_float2(0.f, 0.f);
+  if (p == 0) xnyq[t] = valid ? src[M] : make_float2(0.f, 0.f);
+  lds_scatter<M, P, E>(v, line, p);
+  __syncthreads();
+  const int z1 = opaque_zero();
+  const int p1 = p + z1;
+  const float2* twr1 = twr + z1;
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    const int k = p + P * e;
-    float2 xk = valid ? src[k] : make_float2(0.f, 0.f);
-    float2 xc = valid ? src[M - k] : make_float2(0.f, 0.f);
+    const int k = p1 + P * e;
+    float2 xk = v[e];
+    float2 xc = (k == 0) ? xnyq[t] : line[lds_pos<M>(M - k)];
     if (k == 0) { xk.y = 0.0f; xc.y = 0.0f; }
-    const float2 s = cadd(xk, cconj(xc));
+    const float2 sm = cadd(xk, cconj(xc));
     const float2 d = csub(xk, cconj(xc));
-    const float2 wd = cmul(cconj(twr[k]), d);
-    v[e] = make_float2(s.x - wd.y, s.y + wd.x);  // s + i*wd
+    const float2 wd = cmul(cconj(twr1[k]), d);
+    v[e] = make_float2(sm.x - wd.y, sm.y + wd.x);  // sm + i*wd
   }
+  __syncthreads();
   fft_line<M, true>(v, line, p, tw);
   int flags = 0;
   if (valid) {
     float2* dst = reinterpret_cast<float2*>(out + L * (uint64_t)(2 * M));
+    const int p2 = p + opaque_zero();
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const float2 y = cscale(v[e], scale);
-      dst[p + P * e] = y;
+      dst[p2 + P * e] = y;
       if (isnan(y.x) || isnan(y.y)) flags |= 1;
       if (y.x != 0.0f || y.y != 0.0f) flags |= 2;
     }
@@ -346,12 +415,13 @@ bool length_supported(int n) {
 
 template <int N, int MODE>
 hipError_t launch_strided_n(const StridedArgs& a, uint64_t n_outer, hipStream_t s) {
-  constexpr int T = strided_T(N);
+  constexpr int E = mode_E(N, MODE);
+  constexpr int T = strided_T(N, E);
   StridedArgs b = a;
   b.n_iblk = (a.n_inner + T - 1) / T;
   const uint64_t nblk = n_outer * b.n_iblk;
   if (nblk > 0x7fffffffull) return hipErrorInvalidConfiguration;
-  hipLaunchKernelGGL((k_strided<N, T, MODE>), dim3((uint32_t)nblk), dim3(T * threads_per_line(N)), 0, s, b);
+  hipLaunchKernelGGL((k_strided<N, E, T, MODE>), dim3((uint32_t)nblk), dim3(T * (N / E)), 0, s, b);
   return hipGetLastError();
 }
 
@@ -366,10 +436,13 @@ hipError_t launch_strided(int n, const StridedArgs& a, uint64_t n_outer, hipStre
   }
 }
 
-uint64_t strided_blocks(int n, uint64_t n_outer, uint32_t n_inner) {
+uint64_t strided_blocks(int n, int mode, uint64_t n_outer, uint32_t n_inner) {
   switch (n) {
 #define SMK_CASE(N) \
-  case N: return n_outer * ((n_inner + strided_T(N) - 1) / strided_T(N));
+  case N: {                                                         \
+    const int T = strided_T(N, mode_E(N, mode));                    \
+    return n_outer * ((n_inner + T - 1) / T);                       \
+  }
     SMK_FFT_LENGTHS(SMK_CASE)
 #undef SMK_CASE
     default: return 0;
@@ -377,13 +450,13 @@ uint64_t strided_blocks(int n, uint64_t n_outer, uint32_t n_inner) {
 }
 
 hipError_t launch_z_r2c(int m, const float* noise, uint64_t seed, float2* K, const float2* tw,
-                        const float2* twr, uint64_t np, hipStream_t s) {
+                        const float2* twr, uint64_t np, int trans, uint32_t ny, hipStream_t s) {
   switch (m) {
 #define SMK_CASE(N)                                                                                    \
   case N: {                                                                                            \
     constexpr int T = pencil_T(N);                                                                     \
     hipLaunchKernelGGL((k_z_r2c<N, T>), dim3((uint32_t)((np + T - 1) / T)), dim3(T * threads_per_line(N)), 0, \
-                       s, noise, seed, K, tw, twr, np);                                                 \
+                       s, noise, seed, K, tw, twr, np, trans, ny);                                      \
     return hipGetLastError();                                                                          \
   }
     SMK_FFT_LENGTHS(SMK_CASE)
@@ -468,7 +541,7 @@ extern "C" int smk_box_plan_create(smk_box_plan** plan, int nx, int ny, int nz, 
   smk_box_plan* p = (smk_box_plan*)calloc(1, sizeof(smk_box_plan));
   SMK_REQUIRE(p != nullptr, SMK_ERR_OOM, "host allocation failed");
   p->nx = nx; p->ny = ny; p->nz = nz; p->m = nz / 2; p->nzh = nz / 2 + 1; p->cell = cell;
-  p->n_partial = strided_blocks(nx, (uint64_t)ny, (uint32_t)p->nzh);
+  p->n_partial = strided_blocks(nx, MODE_SPECTRAL, (uint64_t)p->nzh, (uint32_t)ny);
 
   auto upload = [](void** dst, const void* src, size_t bytes) -> hipError_t {
     hipError_t e = hipMalloc(dst, bytes);
@@ -508,43 +581,87 @@ static uint64_t align256(uint64_t b) { return (b + 255) & ~(uint64_t)255; }
 extern "C" size_t smk_box_workspace_bytes(const smk_box_plan* p) {
   if (p == nullptr) return 0;
   const uint64_t kb = align256(kbox_elems(p) * sizeof(float2));
-  return (size_t)(2 * kb + align256(p->n_partial * sizeof(double)));
+  return (size_t)(3 * kb + align256(p->n_partial * sizeof(double)));
 }
 
 namespace {
 
-StridedArgs y_args(const smk_box_plan* p, float2* K) {
+// ---- natural chain (numpy layout [x][y][kz], in place) -------------------
+StridedArgs nat_y_args(const smk_box_plan* p, float2* K) {
   StridedArgs a{};
-  a.data = K;
+  a.in = K; a.out = K;
   a.tw = p->tw_y;
-  a.s_outer = (uint64_t)p->ny * p->nzh;  // outer = x
-  a.s_elem = (uint64_t)p->nzh;           // along y
-  a.n_inner = (uint32_t)p->nzh;          // runs along kz
-  return a;
-}
-
-StridedArgs x_args(const smk_box_plan* p, float2* K) {
-  StridedArgs a{};
-  a.data = K;
-  a.tw = p->tw_x;
-  a.s_outer = (uint64_t)p->nzh;              // outer = ky
-  a.s_elem = (uint64_t)p->ny * p->nzh;       // along x
+  a.in_so = a.out_so = (uint64_t)p->ny * p->nzh;  // outer = x
+  a.in_si = a.out_si = 1;                         // inner = kz
+  a.in_sn = a.out_sk = (uint64_t)p->nzh;          // along y
   a.n_inner = (uint32_t)p->nzh;
   return a;
 }
 
-int forward_to_k(smk_box_plan* p, uint64_t seed, const float* noise, float2* K, hipStream_t s) {
+StridedArgs nat_x_args(const smk_box_plan* p, float2* K) {
+  StridedArgs a{};
+  a.in = K; a.out = K;
+  a.tw = p->tw_x;
+  a.in_so = a.out_so = (uint64_t)p->nzh;          // outer = ky
+  a.in_si = a.out_si = 1;                         // inner = kz
+  a.in_sn = a.out_sk = (uint64_t)p->ny * p->nzh;  // along x
+  a.n_inner = (uint32_t)p->nzh;
+  return a;
+}
+
+int nat_forward(smk_box_plan* p, uint64_t seed, const float* noise, float2* K, hipStream_t s) {
   const uint64_t np = (uint64_t)p->nx * p->ny;
-  SMK_HIP_RET(launch_z_r2c(p->m, noise, seed, K, p->tw_z, p->tw_r, np, s));
-  SMK_HIP_RET(launch_strided<MODE_FWD>(p->ny, y_args(p, K), (uint64_t)p->nx, s));
+  SMK_HIP_RET(launch_z_r2c(p->m, noise, seed, K, p->tw_z, p->tw_r, np, 0, (uint32_t)p->ny, s));
+  SMK_HIP_RET(launch_strided<MODE_FWD>(p->ny, nat_y_args(p, K), (uint64_t)p->nx, s));
+  SMK_HIP_RET(launch_strided<MODE_FWD>(p->nx, nat_x_args(p, K), (uint64_t)p->ny, s));
   return SMK_OK;
 }
 
-int k_to_real(smk_box_plan* p, float2* K, float* out, float scale, int* status, hipStream_t s) {
-  const uint64_t np = (uint64_t)p->nx * p->ny;
-  SMK_HIP_RET(launch_strided<MODE_INV>(p->ny, y_args(p, K), (uint64_t)p->nx, s));
-  SMK_HIP_RET(launch_z_c2r(p->m, K, out, p->tw_z, p->tw_r, np, scale, status, s));
-  return SMK_OK;
+// ---- transposing chain (fused job) ---------------------------------------
+// A1 [x][kz][y] -> Y -> A2 [kz][ky][x]
+StridedArgs tr_y_fwd_args(const smk_box_plan* p, const float2* in, float2* out) {
+  StridedArgs a{};
+  a.in = in; a.out = out;
+  a.tw = p->tw_y;
+  a.in_so = (uint64_t)p->ny;                         // outer = kz
+  a.in_si = (uint64_t)p->nzh * p->ny;                // inner = x
+  a.in_sn = 1;                                       // along y
+  a.out_so = (uint64_t)p->ny * p->nx;
+  a.out_si = 1;
+  a.out_sk = (uint64_t)p->nx;
+  a.n_inner = (uint32_t)p->nx;
+  return a;
+}
+
+// A2 [kz][ky][x] -> X (spectral) -> [kz][x][ky]
+StridedArgs tr_x_args(const smk_box_plan* p, const float2* in, float2* out) {
+  StridedArgs a{};
+  a.in = in; a.out = out;
+  a.tw = p->tw_x;
+  a.in_so = (uint64_t)p->ny * p->nx;                 // outer = kz
+  a.in_si = (uint64_t)p->nx;                         // inner = ky
+  a.in_sn = 1;                                       // along x
+  a.out_so = (uint64_t)p->nx * p->ny;
+  a.out_si = 1;
+  a.out_sk = (uint64_t)p->ny;
+  a.n_inner = (uint32_t)p->ny;
+  a.kz_outer = 1;
+  return a;
+}
+
+// [kz][x][ky] -> Y^-1 -> [x][y][kz]
+StridedArgs tr_y_inv_args(const smk_box_plan* p, const float2* in, float2* out) {
+  StridedArgs a{};
+  a.in = in; a.out = out;
+  a.tw = p->tw_y;
+  a.in_so = (uint64_t)p->ny;                         // outer = x
+  a.in_si = (uint64_t)p->nx * p->ny;                 // inner = kz
+  a.in_sn = 1;                                       // along ky
+  a.out_so = (uint64_t)p->ny * p->nzh;
+  a.out_si = 1;
+  a.out_sk = (uint64_t)p->nzh;
+  a.n_inner = (uint32_t)p->nzh;
+  return a;
 }
 
 }  // namespace
@@ -561,19 +678,21 @@ extern "C" int smk_grf_box(smk_box_plan* p, uint64_t seed, const float* noise, c
   SMK_REQUIRE(pk_params[1] > 0.0, SMK_ERR_ARG, "pk k0 must be positive");
   hipStream_t s = (hipStream_t)stream;
   const uint64_t kb = align256(kbox_elems(p) * sizeof(float2));
-  float2* K = (float2*)workspace;
-  float2* A = (float2*)((char*)workspace + kb);
-  double* partial = (double*)((char*)workspace + 2 * kb);
+  float2* A1 = (float2*)workspace;
+  float2* A2 = (float2*)((char*)workspace + kb);
+  float2* A3 = (float2*)((char*)workspace + 2 * kb);
+  double* partial = (double*)((char*)workspace + 3 * kb);
+  const uint64_t np = (uint64_t)p->nx * p->ny;
 
   if (status_out) {
     hipLaunchKernelGGL(k_clear_ints, dim3(1), dim3(64), 0, s, status_out, 2);
     SMK_LAUNCH_CHECK();
   }
-  int rc = forward_to_k(p, seed, noise, K, s);
-  if (rc) return rc;
+  SMK_HIP_RET(launch_z_r2c(p->m, noise, seed, A1, p->tw_z, p->tw_r, np, 1, (uint32_t)p->ny, s));
+  SMK_HIP_RET(launch_strided<MODE_FWD>(p->ny, tr_y_fwd_args(p, A1, A2), (uint64_t)p->nzh, s));
 
-  StridedArgs a = x_args(p, K);
-  a.aux = A;
+  StridedArgs a = tr_x_args(p, A2, A1);
+  a.aux = A3;
   a.kxv = p->kx; a.kyv = p->ky; a.kzv = p->kz;
   a.pkA = (float)pk_params[0];
   a.pk_k0inv = (float)(1.0 / pk_params[1]);
@@ -586,28 +705,24 @@ extern "C" int smk_grf_box(smk_box_plan* p, uint64_t seed, const float* noise, c
   a.write_delta = 1;
   a.nz_nyq = (uint32_t)p->m;
   a.partial = partial;
-  SMK_HIP_RET(launch_strided<MODE_SPECTRAL>(p->nx, a, (uint64_t)p->ny, s));
+  SMK_HIP_RET(launch_strided<MODE_SPECTRAL>(p->nx, a, (uint64_t)p->nzh, s));
   if (var_out) {
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, s, partial, p->n_partial, var_out);
+    const uint64_t nb = strided_blocks(p->nx, MODE_SPECTRAL, (uint64_t)p->nzh, (uint32_t)p->ny);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, s, partial, nb, var_out);
     SMK_LAUNCH_CHECK();
   }
-  rc = k_to_real(p, K, delta_out, 1.0f, status_out, s);
-  if (rc) return rc;
+  SMK_HIP_RET(launch_strided<MODE_INV>(p->ny, tr_y_inv_args(p, A1, A2), (uint64_t)p->nx, s));
+  SMK_HIP_RET(launch_z_c2r(p->m, A2, delta_out, p->tw_z, p->tw_r, np, 1.0f, status_out, s));
   if (aux_kind != SMK_AUX_NONE) {
-    rc = k_to_real(p, A, aux_out, 1.0f, status_out ? status_out + 1 : nullptr, s);
-    if (rc) return rc;
+    SMK_HIP_RET(launch_strided<MODE_INV>(p->ny, tr_y_inv_args(p, A3, A1), (uint64_t)p->nx, s));
+    SMK_HIP_RET(launch_z_c2r(p->m, A1, aux_out, p->tw_z, p->tw_r, np, 1.0f, status_out ? status_out + 1 : nullptr, s));
   }
   return SMK_OK;
 }
 
 extern "C" int smk_draw_grf_boxk(smk_box_plan* p, uint64_t seed, const float* noise, float* boxk_out, void* stream) {
   SMK_REQUIRE(p != nullptr && boxk_out != nullptr, SMK_ERR_ARG, "plan/boxk_out is NULL");
-  hipStream_t s = (hipStream_t)stream;
-  float2* K = (float2*)boxk_out;
-  int rc = forward_to_k(p, seed, noise, K, s);
-  if (rc) return rc;
-  SMK_HIP_RET(launch_strided<MODE_FWD>(p->nx, x_args(p, K), (uint64_t)p->ny, s));
-  return SMK_OK;
+  return nat_forward(p, seed, noise, (float2*)boxk_out, (hipStream_t)stream);
 }
 
 extern "C" int smk_fft_and_store(smk_box_plan* p, float* boxk_inout, float* box_out, void* workspace,
@@ -616,23 +731,27 @@ extern "C" int smk_fft_and_store(smk_box_plan* p, float* boxk_inout, float* box_
   SMK_REQUIRE(sumsq_out == nullptr || workspace != nullptr, SMK_ERR_ARG, "sumsq_out needs a workspace");
   hipStream_t s = (hipStream_t)stream;
   float2* K = (float2*)boxk_inout;
+  const uint64_t np = (uint64_t)p->nx * p->ny;
   if (status_out) {
     hipLaunchKernelGGL(k_clear_ints, dim3(1), dim3(64), 0, s, status_out, 1);
     SMK_LAUNCH_CHECK();
   }
-  StridedArgs a = x_args(p, K);
+  StridedArgs a = nat_x_args(p, K);
   a.nz_nyq = (uint32_t)p->m;
   a.partial = sumsq_out ? (double*)workspace : nullptr;
   SMK_HIP_RET(launch_strided<MODE_INV>(p->nx, a, (uint64_t)p->ny, s));
   if (sumsq_out) {
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, s, a.partial, p->n_partial, sumsq_out);
+    const uint64_t nb = strided_blocks(p->nx, MODE_INV, (uint64_t)p->ny, (uint32_t)p->nzh);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, s, a.partial, nb, sumsq_out);
     SMK_LAUNCH_CHECK();
   }
   const float scale = (float)(1.0 / ((double)p->nx * p->ny * p->nz));
-  return k_to_real(p, K, box_out, scale, status_out, s);
+  SMK_HIP_RET(launch_strided<MODE_INV>(p->ny, nat_y_args(p, K), (uint64_t)p->nx, s));
+  SMK_HIP_RET(launch_z_c2r(p->m, K, box_out, p->tw_z, p->tw_r, np, scale, status_out, s));
+  return SMK_OK;
 }
 
 extern "C" size_t smk_fft_and_store_workspace_bytes(const smk_box_plan* p) {
   if (p == nullptr) return 0;
-  return (size_t)align256(p->n_partial * sizeof(double));
+  return (size_t)align256(strided_blocks(p->nx, MODE_INV, (uint64_t)p->ny, (uint32_t)p->nzh) * sizeof(double));
 }

@@ -192,10 +192,24 @@ __host__ __device__ constexpr int lds_line_len(int n) { return lds_padded(n) ? n
 // the points j + r*N/R (r = 0..R-1); outputs go to idxD + r*Ns with
 // idxD = (j/Ns)*Ns*R + j%Ns.  For the last stage that is j + r*N/R again, so
 // the result stays in the registers it was computed in.
+// A zero the compiler cannot see through.  Adding it to a table pointer at
+// the start of each stage keeps that stage's loads inside the stage: without
+// it the compiler merges the identical twiddle (and wavenumber) loads of the
+// several transforms of one kernel and keeps them live across all of them,
+// which spills the register tile.
+__device__ __forceinline__ int opaque_zero() {
+  int z = 0;
+  asm volatile("" : "+s"(z));
+  return z;
+}
+
 template <int N, int P, int R, int Ns, bool INV, bool LAST, int E>
-__device__ __forceinline__ void fft_stage(float2 (&v)[E], float2* __restrict__ line, int p,
-                                          const float2* __restrict__ tw) {
+__device__ __forceinline__ void fft_stage(float2 (&v)[E], float2* __restrict__ line, int p_in,
+                                          const float2* tw_in) {
   constexpr int B = E / R;
+  const int z = opaque_zero();
+  const float2* tw = tw_in + z;
+  const int p = p_in + z;
 #pragma unroll
   for (int b = 0; b < B; ++b) {
     const int j = p + P * b;
@@ -225,7 +239,8 @@ __device__ __forceinline__ void fft_stage(float2 (&v)[E], float2* __restrict__ l
 }
 
 template <int N, int P, int E>
-__device__ __forceinline__ void lds_gather(float2 (&v)[E], const float2* __restrict__ line, int p) {
+__device__ __forceinline__ void lds_gather(float2 (&v)[E], const float2* __restrict__ line, int p_in) {
+  const int p = p_in + opaque_zero();
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = line[lds_pos<N>(p + P * e)];
 }
@@ -238,7 +253,7 @@ __device__ __forceinline__ void lds_scatter(const float2 (&v)[E], float2* __rest
 
 template <int N, int E, bool INV, int S, int Ns>
 __device__ __forceinline__ void fft_stages(float2 (&v)[E], float2* __restrict__ line, int p,
-                                           const float2* __restrict__ tw) {
+                                           const float2* tw) {
   constexpr RadixPlan pl = make_plan(N, E);
   static_assert(pl.nstage > 0, "unsupported FFT length");
   constexpr int P = N / E;
@@ -261,8 +276,15 @@ __device__ __forceinline__ void fft_stages(float2 (&v)[E], float2* __restrict__ 
 // exp(+...) and is unnormalised.
 template <int N, bool INV>
 __device__ __forceinline__ void fft_line(float2 (&v)[elems_per_thread(N)], float2* __restrict__ line, int p,
-                                         const float2* __restrict__ tw) {
+                                         const float2* tw) {
   fft_stages<N, elems_per_thread(N), INV, 0, 1>(v, line, p, tw);
+}
+
+// Same with an explicit register tile of E points per thread (P = N / E).
+template <int N, int E, bool INV>
+__device__ __forceinline__ void fft_line_e(float2 (&v)[E], float2* __restrict__ line, int p,
+                                           const float2* tw) {
+  fft_stages<N, E, INV, 0, 1>(v, line, p, tw);
 }
 
 // ------------------------------------------------------ counter-based noise
