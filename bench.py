@@ -1,0 +1,148 @@
+#!/usr/bin/env python
+"""Benchmark of the SaclayMocks hot path on one MI355X.
+
+Headline (``value``): GRF box cells/s for the fused delta + eta_par job on the
+nominal 2560 x 2560 x 1536 chunk (cell 2.19 Mpc/h, analytic P(k), seeded
+N(0,1) noise drawn on the device), i.e. bin/make_boxes.py's density and
+eta_zz boxes produced in one k-space pass.  A step is one full job: noise ->
+R2C -> sqrt(P/Vcell) -> two C2R (delta, eta_zz), outputs resident in HBM.
+
+Also printed on the same JSON line:
+  roofline      the dominant kernel's algorithmic bytes / its mean duration
+                (HIP events recorded around each pass on the launch stream)
+  cpu_baseline  the FP64 oracle (scipy.fft, all host cores given to this
+                process) on a bounded 256^3 sample of the same job
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "GRF box cells/s (delta+eta_par), forest pixels/s, fit time vs plain GPU fit"
+HBM_PEAK_GBPS = 8000.0
+# algorithmic HBM bytes per real cell of each pass of the fused job
+# (complex half spectrum = 4 bytes per real cell, real field = 4 bytes)
+PASS_BYTES_PER_CELL = {
+    "z_r2c": 4,          # write k-space (noise is drawn in registers)
+    "y_fft": 8,          # read + write
+    "x_spectral": 12,    # read, write delta_k and eta_k
+    "y_ifft_delta": 8,
+    "z_c2r_delta": 8,    # read half spectrum, write float32 delta
+    "y_ifft_aux": 8,
+    "z_c2r_aux": 8,
+}
+MODEL_BYTES_PER_CELL = 64   # 3 passes per 3D transform, inverses sharing the k read
+
+
+def cpu_baseline(cell, seconds=12.0):
+    """FP64 oracle of the same job on a 256^3 sample, timed on this host."""
+    import numpy as np
+    from oracle import box_oracle
+    cores = max(1, min(len(os.sched_getaffinity(0)), 16))
+    shape = (256, 256, 256)
+    n = 0
+    t0 = time.perf_counter()
+    rng = np.random.default_rng(0)
+    while True:
+        w = rng.standard_normal(shape)
+        box_oracle.grf_box(w, cell, aux="eta_zz", workers=cores)
+        n += 1
+        if time.perf_counter() - t0 >= seconds:
+            break
+    dt = time.perf_counter() - t0
+    cells = n * shape[0] * shape[1] * shape[2]
+    return {"value": cells / dt, "unit": "cells/s", "cores": cores, "kind": "port",
+            "sample": f"{n} x 256^3 box: numpy N(0,1) noise, scipy.fft FP64 rfftn, "
+                      f"x sqrt(P/Vcell), irfftn delta + eta_zz ({dt:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--shape", type=int, nargs=3, default=[2560, 2560, 1536])
+    ap.add_argument("--cell", type=float, default=2.19)
+    ap.add_argument("--seed", type=int, default=42)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: the box and forest stages are single-GPU here "
+              "(multi-GPU sharding is a design note, not a measured path)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from saclaymocks_amd.box import BoxPlan, check_status
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    nx, ny, nz = args.shape
+    cells = nx * ny * nz
+    plan = BoxPlan(nx, ny, nz, cell=args.cell)
+    ws = plan.alloc_workspace(dev)
+    delta = torch.empty((nx, ny, nz), dtype=torch.float32, device=dev)
+    eta = torch.empty((nx, ny, nz), dtype=torch.float32, device=dev)
+    plan.enable_pass_timing(True)
+
+    for i in range(args.warmup):
+        plan.grf_box(seed=args.seed + 1000 + i, delta=delta, aux_out=eta, workspace=ws, check=False)
+    torch.cuda.synchronize()
+    plan.pass_times(reset=True)
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        _, _, var, status = plan.grf_box(seed=args.seed + i, delta=delta, aux_out=eta,
+                                         workspace=ws, check=False)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    pass_ms, ncalls = plan.pass_times(reset=True)
+    check_status(status, "box")          # null/NaN guard, outside the timed region
+    sigma = float(var.item()) ** 0.5
+
+    dt = t1 - t0
+    ms_per_step = dt / args.steps * 1e3
+    value = cells * args.steps / dt
+    dom = max(pass_ms, key=pass_ms.get)
+    dom_ms = pass_ms[dom]
+    achieved = PASS_BYTES_PER_CELL[dom] * cells / (dom_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: Philox N(0,1) noise drawn on the device from a seed",
+        "config": {"workload": f"GRF box delta+eta_par {nx}x{ny}x{nz}, cell {args.cell} Mpc/h, "
+                               "analytic P(k) = 2e4 (k/0.02)/(1+(k/0.02)^2.6) exp(-(1.5k)^2)",
+                   "box": [nx, ny, nz], "cell": args.cell},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "kernel": dom, "kernel_ms": dom_ms,
+                     "alg_bytes_per_cell": PASS_BYTES_PER_CELL[dom]},
+        "job": {"model_bytes_per_cell": MODEL_BYTES_PER_CELL,
+                "model_GBps": MODEL_BYTES_PER_CELL * cells / (ms_per_step * 1e-3) / 1e9,
+                "model_frac": MODEL_BYTES_PER_CELL * cells / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBPS,
+                "pass_ms": {k: round(v, 3) for k, v in pass_ms.items()},
+                "timed_calls": ncalls, "sigma_delta_fp64": sigma},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.cell)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -96,6 +96,15 @@ int smk_grf_box(smk_box_plan* plan, uint64_t seed, const float* noise, const dou
                 int aux_kind, double aux_coef, float* delta_out, float* aux_out, void* workspace,
                 double* var_out, int* status_out, void* stream);
 
+/* Per-pass timing of smk_grf_box with HIP events recorded on the call's
+ * stream (the reference prints wall times per step: make_boxes.py:44-56,90).
+ * Passes: 0 z R2C, 1 y FFT, 2 x spectral, 3 y IFFT delta, 4 z C2R delta,
+ * 5 y IFFT aux, 6 z C2R aux.  smk_box_pass_times waits for the last timed
+ * call, writes the per-pass sums (ms) over the calls since the last reset
+ * into ms_sum[0..nmax) and the number of calls into *ncalls. */
+int smk_box_enable_pass_timing(smk_box_plan* plan, int enable);
+int smk_box_pass_times(smk_box_plan* plan, double* ms_sum, int nmax, int* ncalls, int reset);
+
 /* DrawGRF_boxk (make_boxes.py:40-72): unnormalised forward R2C FFT of the
  * white noise into boxk, complex64 [nx][ny][nz/2+1].  noise == NULL draws
  * the Philox noise of `seed` on the device. */

@@ -57,28 +57,40 @@ def aux_factor(kind, nx, ny, nz, cell, coef=1.0):
     return np.broadcast_to(table[kind], (nx, ny, nz // 2 + 1))
 
 
-def draw_boxk(noise):
-    """DrawGRF_boxk, bin/make_boxes.py:40-54: unnormalised R2C of the noise, FP64."""
-    return np.fft.rfftn(np.asarray(noise, dtype=np.float64))
+def draw_boxk(noise, workers=None):
+    """DrawGRF_boxk, bin/make_boxes.py:40-54: unnormalised R2C of the noise, FP64.
+
+    workers=None uses numpy.fft (one thread); an integer uses scipy.fft with
+    that many threads (the CPU baseline; FFTW runs threads=ncpu there)."""
+    x = np.asarray(noise, dtype=np.float64)
+    if workers is None:
+        return np.fft.rfftn(x)
+    import scipy.fft
+    return scipy.fft.rfftn(x, workers=workers)
 
 
-def fft_and_store(boxk, nz):
+def fft_and_store(boxk, nz, workers=None):
     """FFTandStore's transform, bin/make_boxes.py:86-92: irfftn(boxk)/N and np.std."""
-    box = np.fft.irfftn(boxk, s=(boxk.shape[0], boxk.shape[1], nz), axes=(0, 1, 2))
+    s = (boxk.shape[0], boxk.shape[1], nz)
+    if workers is None:
+        box = np.fft.irfftn(boxk, s=s, axes=(0, 1, 2))
+    else:
+        import scipy.fft
+        box = scipy.fft.irfftn(boxk, s=s, axes=(0, 1, 2), workers=workers)
     return box, box.std()
 
 
-def grf_box(noise, cell, pk=PK_DEFAULT, aux="eta_zz", aux_coef=1.0):
+def grf_box(noise, cell, pk=PK_DEFAULT, aux="eta_zz", aux_coef=1.0, workers=None):
     """delta and one auxiliary field from the same noise, FP64.
 
     bin/make_boxes.py:289-292 (boxk *= P0; FFTandStore) and :341-352
     (boxk *= kz^2/kk; FFTandStore)."""
     nx, ny, nz = noise.shape
-    boxk = draw_boxk(noise) * sqrt_p_over_vcell(nx, ny, nz, cell, pk)
-    delta, _ = fft_and_store(boxk, nz)
+    boxk = draw_boxk(noise, workers) * sqrt_p_over_vcell(nx, ny, nz, cell, pk)
+    delta, _ = fft_and_store(boxk, nz, workers)
     out = {"delta": delta, "var": float(np.mean(delta * delta))}
     if aux is not None:
-        out[aux], _ = fft_and_store(boxk * aux_factor(aux, nx, ny, nz, cell, aux_coef), nz)
+        out[aux], _ = fft_and_store(boxk * aux_factor(aux, nx, ny, nz, cell, aux_coef), nz, workers)
     return out
 
 

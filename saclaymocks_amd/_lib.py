@@ -49,6 +49,8 @@ _SIGNATURES = {
     "smk_box_workspace_bytes": (_sz, [_vp]),
     "smk_grf_box": (_i, [_vp, _u64, _vp, _c.POINTER(_d), _i, _d, _vp, _vp, _vp, _vp, _vp, _vp]),
     "smk_draw_grf_boxk": (_i, [_vp, _u64, _vp, _vp, _vp]),
+    "smk_box_enable_pass_timing": (_i, [_vp, _i]),
+    "smk_box_pass_times": (_i, [_vp, _c.POINTER(_d), _i, _c.POINTER(_i), _i]),
     "smk_fft_and_store": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "smk_fft_and_store_workspace_bytes": (_sz, [_vp]),
 }

@@ -51,6 +51,21 @@ class BoxPlan:
         nx, ny, nz = self.shape
         return (nx, ny, nz // 2 + 1)
 
+    PASS_NAMES = ("z_r2c", "y_fft", "x_spectral", "y_ifft_delta", "z_c2r_delta", "y_ifft_aux", "z_c2r_aux")
+
+    def enable_pass_timing(self, enable=True):
+        _lib.check(_lib.lib().smk_box_enable_pass_timing(self._h, 1 if enable else 0),
+                   "smk_box_enable_pass_timing")
+
+    def pass_times(self, reset=True):
+        """Per-pass mean duration (ms) of the timed grf_box calls since the last reset."""
+        ms = (ctypes.c_double * 7)()
+        n = ctypes.c_int()
+        _lib.check(_lib.lib().smk_box_pass_times(self._h, ms, 7, ctypes.byref(n), 1 if reset else 0),
+                   "smk_box_pass_times")
+        k = max(n.value, 1)
+        return {name: ms[i] / k for i, name in enumerate(self.PASS_NAMES)}, n.value
+
     def workspace_bytes(self):
         return int(_lib.lib().smk_box_workspace_bytes(self._h))
 

@@ -514,9 +514,20 @@ std::vector<float> fftfreq_k(int n, double cell, bool half) {
 
 }  // namespace
 
+#define SMK_BOX_NPASS 7
+
 struct smk_box_plan {
   int nx, ny, nz, m, nzh;
   double cell;
+  // optional per-pass timing of smk_grf_box: events[i] is recorded before
+  // pass i, events[NPASS] after the last one; `pending` marks a call whose
+  // events have not been read yet
+  int timing;
+  int pending;
+  int npass_last;
+  hipEvent_t ev[SMK_BOX_NPASS + 1];
+  double acc_ms[SMK_BOX_NPASS];
+  int ncalls;
   float2* tw_x;
   float2* tw_y;
   float2* tw_z;
@@ -568,6 +579,9 @@ extern "C" int smk_box_plan_create(smk_box_plan** plan, int nx, int ny, int nz, 
 
 extern "C" int smk_box_plan_destroy(smk_box_plan* p) {
   if (p == nullptr) return SMK_OK;
+  if (p->timing) {
+    for (int i = 0; i <= SMK_BOX_NPASS; ++i) (void)hipEventDestroy(p->ev[i]);
+  }
   void* bufs[] = {p->tw_x, p->tw_y, p->tw_z, p->tw_r, p->kx, p->ky, p->kz};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -666,6 +680,55 @@ StridedArgs tr_y_inv_args(const smk_box_plan* p, const float2* in, float2* out) 
 
 }  // namespace
 
+namespace {
+
+// fold the events of the previous timed call into the accumulators
+int harvest_timing(smk_box_plan* p) {
+  if (!p->timing || !p->pending) return SMK_OK;
+  SMK_HIP_CHECK(hipEventSynchronize(p->ev[p->npass_last]));
+  for (int i = 0; i < p->npass_last; ++i) {
+    float ms = 0.f;
+    SMK_HIP_CHECK(hipEventElapsedTime(&ms, p->ev[i], p->ev[i + 1]));
+    p->acc_ms[i] += ms;
+  }
+  p->ncalls += 1;
+  p->pending = 0;
+  return SMK_OK;
+}
+
+}  // namespace
+
+extern "C" int smk_box_enable_pass_timing(smk_box_plan* p, int enable) {
+  SMK_REQUIRE(p != nullptr, SMK_ERR_ARG, "plan is NULL");
+  if (enable && !p->timing) {
+    for (int i = 0; i <= SMK_BOX_NPASS; ++i) SMK_HIP_CHECK(hipEventCreate(&p->ev[i]));
+    p->timing = 1;
+  } else if (!enable && p->timing) {
+    int rc = harvest_timing(p);
+    if (rc) return rc;
+    for (int i = 0; i <= SMK_BOX_NPASS; ++i) (void)hipEventDestroy(p->ev[i]);
+    p->timing = 0;
+  }
+  for (int i = 0; i < SMK_BOX_NPASS; ++i) p->acc_ms[i] = 0.0;
+  p->ncalls = 0;
+  p->pending = 0;
+  return SMK_OK;
+}
+
+extern "C" int smk_box_pass_times(smk_box_plan* p, double* ms_sum, int nmax, int* ncalls, int reset) {
+  SMK_REQUIRE(p != nullptr && ms_sum != nullptr && ncalls != nullptr, SMK_ERR_ARG, "NULL argument");
+  SMK_REQUIRE(p->timing, SMK_ERR_ARG, "pass timing is not enabled");
+  int rc = harvest_timing(p);
+  if (rc) return rc;
+  for (int i = 0; i < nmax && i < SMK_BOX_NPASS; ++i) ms_sum[i] = p->acc_ms[i];
+  *ncalls = p->ncalls;
+  if (reset) {
+    for (int i = 0; i < SMK_BOX_NPASS; ++i) p->acc_ms[i] = 0.0;
+    p->ncalls = 0;
+  }
+  return SMK_OK;
+}
+
 extern "C" int smk_grf_box(smk_box_plan* p, uint64_t seed, const float* noise, const double* pk_params, int aux_kind,
                            double aux_coef, float* delta_out, float* aux_out, void* workspace, double* var_out,
                            int* status_out, void* stream) {
@@ -684,12 +747,20 @@ extern "C" int smk_grf_box(smk_box_plan* p, uint64_t seed, const float* noise, c
   double* partial = (double*)((char*)workspace + 3 * kb);
   const uint64_t np = (uint64_t)p->nx * p->ny;
 
+  int rc = harvest_timing(p);
+  if (rc) return rc;
+  int ev = 0;
+  auto mark = [&]() -> hipError_t { return p->timing ? hipEventRecord(p->ev[ev++], s) : hipSuccess; };
+
   if (status_out) {
     hipLaunchKernelGGL(k_clear_ints, dim3(1), dim3(64), 0, s, status_out, 2);
     SMK_LAUNCH_CHECK();
   }
+  SMK_HIP_RET(mark());
   SMK_HIP_RET(launch_z_r2c(p->m, noise, seed, A1, p->tw_z, p->tw_r, np, 1, (uint32_t)p->ny, s));
+  SMK_HIP_RET(mark());
   SMK_HIP_RET(launch_strided<MODE_FWD>(p->ny, tr_y_fwd_args(p, A1, A2), (uint64_t)p->nzh, s));
+  SMK_HIP_RET(mark());
 
   StridedArgs a = tr_x_args(p, A2, A1);
   a.aux = A3;
@@ -706,16 +777,25 @@ extern "C" int smk_grf_box(smk_box_plan* p, uint64_t seed, const float* noise, c
   a.nz_nyq = (uint32_t)p->m;
   a.partial = partial;
   SMK_HIP_RET(launch_strided<MODE_SPECTRAL>(p->nx, a, (uint64_t)p->nzh, s));
+  SMK_HIP_RET(mark());
   if (var_out) {
     const uint64_t nb = strided_blocks(p->nx, MODE_SPECTRAL, (uint64_t)p->nzh, (uint32_t)p->ny);
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, s, partial, nb, var_out);
     SMK_LAUNCH_CHECK();
   }
   SMK_HIP_RET(launch_strided<MODE_INV>(p->ny, tr_y_inv_args(p, A1, A2), (uint64_t)p->nx, s));
+  SMK_HIP_RET(mark());
   SMK_HIP_RET(launch_z_c2r(p->m, A2, delta_out, p->tw_z, p->tw_r, np, 1.0f, status_out, s));
+  SMK_HIP_RET(mark());
   if (aux_kind != SMK_AUX_NONE) {
     SMK_HIP_RET(launch_strided<MODE_INV>(p->ny, tr_y_inv_args(p, A3, A1), (uint64_t)p->nx, s));
+    SMK_HIP_RET(mark());
     SMK_HIP_RET(launch_z_c2r(p->m, A1, aux_out, p->tw_z, p->tw_r, np, 1.0f, status_out ? status_out + 1 : nullptr, s));
+    SMK_HIP_RET(mark());
+  }
+  if (p->timing) {
+    p->npass_last = ev - 1;
+    p->pending = 1;
   }
   return SMK_OK;
 }
