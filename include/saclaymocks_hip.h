@@ -124,6 +124,83 @@ int smk_fft_and_store(smk_box_plan* plan, float* boxk_inout, float* box_out, voi
                       double* sumsq_out, int* status_out, void* stream);
 size_t smk_fft_and_store_workspace_bytes(const smk_box_plan* plan);
 
+/* ---------------------------------------------------------------- forest */
+#define SMK_GEOM_PLANE 0  /* line of sight along z at fixed (X, Y) */
+#define SMK_GEOM_RADIAL 1 /* line of sight from the observer, make_spectra.py:443-452 */
+#define SMK_INTERP_TRILINEAR 0
+#define SMK_INTERP_GAUSS343 1 /* 343-cell Gaussian kernel, make_spectra.py:39-68 */
+
+/* Resident density / tidal boxes, float32 [nx][ny][nz].  Box frame of
+ * make_spectra.py:196-206,300-305: X in [-LX/2, LX/2), Y likewise, Z in
+ * [R0 - LZ/2, R0 + LZ/2) with L = cell * n. */
+typedef struct smk_box_view {
+  const float* delta;
+  const float* eta[6];  /* eta_xx, eta_yy, eta_zz, eta_xy, eta_xz, eta_yz; PLANE reads eta_zz */
+  int32_t nx, ny, nz;
+  int32_t periodic;     /* 1: wrap cell indices; 0: a skewer leaving the box is flagged */
+  double cell;          /* DX = DY = DZ, Mpc/h */
+  double r0;            /* R0 = h chi(z0), Mpc/h */
+} smk_box_view;
+
+/* A batch of skewers on the global pixel grid R_g = rmin + g * dr (Mpc/h),
+ * make_spectra.py:311-321.  Pixels are packed: skewer s owns
+ * [off[s], off[s] + npix[s]) of every per-pixel array, with
+ * off[s+1] = off[s] + npix[s].  All pointers are device arrays of length n. */
+typedef struct smk_skewers {
+  int32_t n;
+  int32_t geometry;     /* SMK_GEOM_* */
+  int64_t npix_total;   /* off[n-1] + npix[n-1] */
+  double rmin, dr;
+  const double* qx;     /* RADIAL: X_QSO; PLANE: X of the line of sight */
+  const double* qy;     /* RADIAL: Y_QSO; PLANE: Y of the line of sight */
+  const double* qz;     /* RADIAL: Z_QSO; PLANE: unused */
+  const double* qr;     /* RADIAL: R_QSO; PLANE: unused */
+  const int32_t* g0;    /* global grid index of pixel 0 */
+  const int32_t* npix;
+  const int32_t* imin;  /* forest range [imin, imax) in local pixels; outside it */
+  const int32_t* imax;  /* delta_l = -1e6 and eta_par = 0 (make_spectra.py:99-105) */
+  const int64_t* off;
+  const int32_t* pset;  /* FGPA parameter row per skewer; NULL: row = global index g */
+  const uint64_t* noise_id; /* missing-power noise stream per skewer (e.g. THING_ID) */
+} smk_skewers;
+
+/* Gather along each line of sight (make_spectra.py:90-139) and, when `flux`
+ * is non-NULL, the FGPA (merge_spectra.py:335-339, util.py:421-433):
+ *   F = exp(-a exp(b G (delta_l + delta_s + c eta_par)))
+ *   abcg        float32 [abcg_rows][4] = (a, b, c, G), row = skewers.pset[s]
+ *               or, with pset NULL, the global pixel index g
+ *   delta_s     NULL or float32 per pixel (smk_forest_missing_power)
+ * Outputs (NULL to skip, float32 per pixel): delta_l, eta_par, flux;
+ *   cells       int32 [pixels][3], containing cell (ix, iy, iz), make_spectra.py:47-49
+ *   status      int32 per skewer (zeroed by the caller): bit 0 set if a kernel
+ *               cell left a non-periodic box (the reference skips such a QSO,
+ *               make_spectra.py:490-494); bit 1 if an FGPA row was out of
+ *               range (flux NaN). */
+int smk_forest_gather(const smk_skewers* sk, const smk_box_view* box, int interp, const float* abcg,
+                      int64_t abcg_rows, const float* delta_s, float* delta_l, float* eta_par, float* flux,
+                      int32_t* cells, int32_t* status, void* stream);
+
+/* Missing small-scale power, merge_spectra.py:303-322: for a skewer of n
+ * pixels, nz = 256 * 2^m >= n + 50; delta_s = irfft(rfft(w) * filt)[:n] with
+ * filt[j] = sqrt(P1D_miss(k_j) / pixel), k_j = j 2 pi / (nz pixel).
+ *   filt        float32 tables, one per length class c (nz = 256 << c, c < 6),
+ *               table c at filt + filt_off[c], nz/2 + 1 entries (device)
+ *   noise       NULL: the Fourier modes of w are drawn on the device (Philox,
+ *               key `seed`, stream skewers.noise_id[s]; the reference's own
+ *               comment at merge_spectra.py:310 plans this); else float32
+ *               N(0,1) white noise, nz values per skewer at noise + noise_off[s]
+ *   delta_s     float32 per pixel (output). */
+typedef struct smk_forest_plan smk_forest_plan;
+int smk_forest_plan_create(smk_forest_plan** plan); /* FFT tables for every class */
+int smk_forest_plan_destroy(smk_forest_plan* plan);
+size_t smk_forest_workspace_bytes(int n_skewers);   /* class bucketing scratch */
+/*   filt_off    HOST int64[6]
+ *   workspace   smk_forest_workspace_bytes(sk->n) bytes
+ * Returns SMK_ERR_RANGE if a skewer needs nz > 8192 (one host sync). */
+int smk_forest_missing_power(smk_forest_plan* plan, const smk_skewers* sk, const float* filt,
+                             const int64_t* filt_off, uint64_t seed, const float* noise,
+                             const int64_t* noise_off, void* workspace, float* delta_s, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,263 @@
+"""FP64 restatement of the forest stage (make_spectra.py + merge_spectra.py) and
+of the P1D / mean-flux estimators of the FGPA fit (fit_az.py).
+
+TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
+
+Pinned by (tests/test_oracle_forest.py):
+  - Cosmo: chi(z) against scipy.integrate.quad of c/H (1e-6 relative), and
+    r_2_z(r_comoving(z)) = z;
+  - compute_xyz2: the orthonormal-frame identities (|XYZ| = R, the centre
+    direction maps to (0, 0, R));
+  - gauss343: a constant field returns that constant exactly, a brute-force
+    loop over the 343 cells on a tiny box, and the cell index of a point at a
+    known cell centre;
+  - trilinear: exact on linear fields;
+  - missing_power: closed form with a single-frequency noise vector;
+  - fgpa: a = 0 gives F = 1, b = 0 gives exp(-a), delta = -1e6 gives F = 1;
+  - p1d_1spectrum: brute-force DFT and Parseval;
+  - profile_histo: hand-computed bins, scipy.binned_statistic semantics;
+  - fit gradients: central finite differences of chi2.
+"""
+import numpy as np
+
+# bin/../py/SaclayMocks/constant.py:6-36
+C_KMS = 299792458.0 / 1000.0
+LYA = 1215.67
+LYLIMIT = 0.0
+LAMBDA_MIN = 3476.0
+H = 0.6731
+OMEGA_M = 0.31457
+OMEGA_K = 0.0
+Z0 = 1.70975268202
+
+SIG_LYA_EDGE = LYA  # forest: lambda_rf < LYA
+
+
+class Cosmo:
+    """util.py:793-833: chi(z) by cumulative trapezoid on 10000 bins to z = 10 (Mpc)."""
+
+    def __init__(self, Om=OMEGA_M, Ok=0.0, Or=0.0, wl=-1.0, H0=100.0 * H):
+        c = C_KMS
+        Ol = 1.0 - Ok - Om - Or
+        nbins = 10000
+        zmax = 10.0
+        dz = zmax / nbins
+        z = np.arange(nbins) * dz
+        hubble = H0 * np.sqrt(Ol * (1.0 + z) ** (3.0 * (1.0 + wl)) + Ok * (1.0 + z) ** 2
+                              + Om * (1.0 + z) ** 3 + Or * (1.0 + z) ** 4)
+        chi = np.zeros(nbins)
+        for i in range(1, nbins):
+            chi[i] = chi[i - 1] + c * (1.0 / hubble[i - 1] + 1.0 / hubble[i]) / 2.0 * dz
+        self.z = z
+        self.chi = chi
+        self.hubble = hubble
+
+    def r_comoving(self, z):
+        return np.interp(z, self.z, self.chi)
+
+    def r_2_z(self, r):
+        return np.interp(r, self.chi, self.z)
+
+
+def fgrowth(z, Om0=OMEGA_M, unnormed=False):
+    """util.py:123-134, Carroll-Press-Turner growth D(z)/D(0)."""
+    Om = 1.0 / (1.0 + (1.0 - Om0) / (Om0 * (1.0 + z) ** 3))
+    Ol = 1.0 - Om
+    a = 1.0 / (1.0 + z)
+    norm = 1.0 if unnormed else 1.0 / fgrowth(0.0, Om0, unnormed=True)
+    return norm * (5.0 / 2.0) * a * Om / (Om ** (4.0 / 7.0) - Ol + (1.0 + Om / 2.0) * (1.0 + Ol / 70.0))
+
+
+def growth_factor_fgpa(z):
+    """merge_spectra.py:91,296: G(z) = D(2.4) (1 + 2.4) / (1 + z)."""
+    return fgrowth(2.4, OMEGA_M) * (1.0 + 2.4) / (1.0 + np.asarray(z, dtype=np.float64))
+
+
+def compute_xyz2(ra, dec, R, ra0, dec0):
+    """box.py:240-250 (angles in radians)."""
+    x = R * (np.cos(ra0) * np.cos(dec) * np.sin(ra) - np.sin(ra0) * np.cos(dec) * np.cos(ra))
+    y = R * (-np.sin(ra0) * np.sin(dec0) * np.cos(dec) * np.sin(ra) + np.cos(dec0) * np.sin(dec)
+             - np.cos(ra0) * np.sin(dec0) * np.cos(dec) * np.cos(ra))
+    z = R * (np.cos(dec0) * np.sin(ra0) * np.cos(dec) * np.sin(ra) + np.sin(dec0) * np.sin(dec)
+             + np.cos(ra0) * np.cos(dec0) * np.cos(dec) * np.cos(ra))
+    return x, y, z
+
+
+def pixel_grid(zmin, zmax, pixel, cosmo=None):
+    """make_spectra.py:300-321: R grid (Mpc/h) from zmin to zmax, lambda > 3476 cut.
+
+    Returns (Rmin, i_first, R_vec, lambda_vec, z_vec) where R_vec[k] =
+    Rmin + (i_first + k) * pixel, exactly as R_vec[cut] of the reference."""
+    cosmo = cosmo or Cosmo()
+    Rmin = H * cosmo.r_comoving(zmin)
+    Rmax = H * cosmo.r_comoving(zmax)
+    npixeltot = int((Rmax - Rmin) / pixel + 0.5)
+    R_vec = Rmin + np.arange(npixeltot) * pixel
+    lambda_vec = LYA * (1 + cosmo.r_2_z(R_vec / H))
+    cut = lambda_vec > LAMBDA_MIN
+    i_first = int(np.argmax(cut)) if cut.any() else npixeltot
+    R_vec = R_vec[cut]
+    lambda_vec = lambda_vec[cut]
+    z_vec = cosmo.r_2_z(R_vec / H)
+    return Rmin, i_first, R_vec, lambda_vec, z_vec
+
+
+# ------------------------------------------------------------------ gathers
+def cell_index(X, Y, Z, L, D, R0):
+    """make_spectra.py:47-49: int truncation of the float64 expressions."""
+    LX, LY, LZ = L
+    DX, DY, DZ = D
+    ix = ((X + LX / 2) / DX).astype(np.int64)
+    iy = ((Y + LY / 2) / DY).astype(np.int64)
+    iz = ((Z + LZ / 2 - R0) / DZ).astype(np.int64)
+    return ix, iy, iz
+
+
+def gauss343(fields, X, Y, Z, L, D, R0, dmax=3, periodic=False):
+    """ComputeWeight + computeRho, make_spectra.py:39-68,104-112, vectorised over pixels.
+
+    fields: list of float32 boxes [nx][ny][nz].  Returns (list of smoothed
+    values per field, (ix, iy, iz)).  Weights exp(-dr^2 / (2 DX^2)) over the
+    (2 dmax + 1)^3 cells around the containing cell, normalised by their sum."""
+    X, Y, Z = (np.asarray(a, dtype=np.float64) for a in (X, Y, Z))
+    LX, LY, LZ = L
+    DX, DY, DZ = D
+    ix, iy, iz = cell_index(X, Y, Z, L, D, R0)
+    sig2 = 2 * DX * DX
+    cx = (ix + 0.5) * DX - LX / 2
+    cy = (iy + 0.5) * DY - LY / 2
+    cz = (iz + 0.5) * DZ - LZ / 2 + R0
+    nx, ny, nz = fields[0].shape
+    num = [np.zeros_like(X) for _ in fields]
+    wsum = np.zeros_like(X)
+    r = np.arange(-dmax, dmax + 1)
+    for a in r:
+        for b in r:
+            for c in r:
+                xx = (a * DX + cx - X) ** 2 + (b * DY + cy - Y) ** 2 + (c * DZ + cz - Z) ** 2
+                w = np.exp(-xx / sig2)
+                jx, jy, jz = ix + a, iy + b, iz + c
+                if periodic:
+                    jx, jy, jz = jx % nx, jy % ny, jz % nz
+                elif (jx.min() < 0 or jy.min() < 0 or jz.min() < 0 or jx.max() >= nx
+                      or jy.max() >= ny or jz.max() >= nz):
+                    raise IndexError("gauss343 cell outside the box")
+                wsum += w
+                for k, f in enumerate(fields):
+                    num[k] += w * f[jx, jy, jz].astype(np.float64)
+    return [n / wsum for n in num], (ix, iy, iz)
+
+
+def trilinear(fields, X, Y, Z, L, D, R0):
+    """Trilinear interpolation of cell-centred values, periodic box."""
+    X, Y, Z = (np.asarray(a, dtype=np.float64) for a in (X, Y, Z))
+    LX, LY, LZ = L
+    DX, DY, DZ = D
+    nx, ny, nz = fields[0].shape
+    ux = (X + LX / 2) / DX - 0.5
+    uy = (Y + LY / 2) / DY - 0.5
+    uz = (Z + LZ / 2 - R0) / DZ - 0.5
+    i0x, i0y, i0z = np.floor(ux).astype(np.int64), np.floor(uy).astype(np.int64), np.floor(uz).astype(np.int64)
+    fx, fy, fz = ux - i0x, uy - i0y, uz - i0z
+    out = []
+    for f in fields:
+        acc = np.zeros_like(X)
+        for a in (0, 1):
+            wx = fx if a else 1 - fx
+            for b in (0, 1):
+                wy = fy if b else 1 - fy
+                for c in (0, 1):
+                    wz = fz if c else 1 - fz
+                    acc += wx * wy * wz * f[(i0x + a) % nx, (i0y + b) % ny, (i0z + c) % nz].astype(np.float64)
+        out.append(acc)
+    return out
+
+
+def eta_par_radial(eta6, X, Y, Z):
+    """make_spectra.py:114-129: x_hat . eta . x_hat from the six smoothed eta_ij."""
+    exx, eyy, ezz, exy, exz, eyz = eta6
+    RR = X ** 2 + Y ** 2 + Z ** 2
+    return (X * exx * X + Y * eyy * Y + Z * ezz * Z + 2 * X * exy * Y + 2 * X * exz * Z + 2 * Y * eyz * Z) / RR
+
+
+# ----------------------------------------------------------- missing power
+def fft_len_for(n):
+    """merge_spectra.py:308-309: nz = 256 * 2^m >= n + 50."""
+    nz = 256
+    while nz < n + 50:
+        nz *= 2
+    return nz
+
+
+def missing_power(noise, n, pixel, p1d):
+    """merge_spectra.py:310-322 with the noise fed in: irfft(rfft(w) sqrt(P1D(k)/pixel))[:n]."""
+    nz = len(noise)
+    k = np.fft.rfftfreq(nz) * 2 * (np.pi / pixel)
+    pm = np.maximum(p1d(k), 0.0)
+    dk = np.fft.rfft(np.asarray(noise, dtype=np.float64)) * np.sqrt(pm / pixel)
+    return np.fft.irfft(dk, n=nz)[:n]
+
+
+def p1d_missing_analytic(k, A=0.05, k1=1.8):
+    """P1D_miss(k) = A exp(-(k/k1)^2) (Mpc/h)."""
+    return A * np.exp(-(np.asarray(k) / k1) ** 2)
+
+
+def fgpa(delta, eta_par, growthf, a, b, c):
+    """util.py:421-433."""
+    tau_over_a = np.exp(b * growthf * (delta + c * eta_par))
+    return np.exp(-a * tau_over_a)
+
+
+# ------------------------------------------------------- P1D and mean flux
+def p1d_1spectrum(delta, pixel):
+    """powerspectrum.py:204-209: |rfft(delta)|^2 * pixel / n."""
+    z = np.fft.rfft(np.asarray(delta, dtype=np.float64))
+    return (z.real ** 2 + z.imag ** 2) * pixel / len(delta)
+
+
+def profile_histo(x, y, edges):
+    """util.py:46-75 with explicit bin edges (scipy binned_statistic semantics:
+    [e_i, e_i+1) and a closed last bin; empty bins dropped).
+
+    Returns (meanx, meany, erry, count, kept_bin_indices)."""
+    from scipy import stats
+    x = np.asarray(x, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    res = stats.binned_statistic(x, [y, y ** 2, x], bins=edges, statistic="mean")
+    meany, meany2, meanx = res.statistic
+    cnt = stats.binned_statistic(x, y, bins=edges, statistic="count").statistic
+    keep = ~np.isnan(meanx)
+    err = np.sqrt(np.maximum(meany2 - meany ** 2, 0)) / np.sqrt(np.where(keep, cnt, 1))
+    idx = np.nonzero(keep)[0]
+    return meanx[keep], meany[keep], err[keep], cnt[keep], idx
+
+
+def p1d_of_flux(flux_list, pixel, edges):
+    """fit_az.py:194-217: <F> over all forest pixels, delta_F = F/<F> - 1,
+    P1D per spectrum, binned with profile_histo."""
+    Fmean = float(np.mean(np.concatenate(flux_list)))
+    ks, ps = [], []
+    for f in flux_list:
+        d = f / Fmean - 1
+        ks.append(np.fft.rfftfreq(len(d)) * 2 * (np.pi / pixel))
+        ps.append(p1d_1spectrum(d, pixel))
+    return Fmean, profile_histo(np.concatenate(ks), np.concatenate(ps), edges)
+
+
+def binned_sums(flux_list, pixel, edges, Fmean):
+    """Per-bin count, sum k, sum P, sum P^2 (the device's FP64 reductions)."""
+    nb = len(edges) - 1
+    out = np.zeros((nb, 4))
+    for f in flux_list:
+        d = f / Fmean - 1
+        k = np.fft.rfftfreq(len(d)) * 2 * (np.pi / pixel)
+        p = p1d_1spectrum(d, pixel)
+        b = np.searchsorted(edges, k, side="right") - 1
+        b[k == edges[-1]] = nb - 1
+        ok = (b >= 0) & (b < nb)
+        np.add.at(out[:, 0], b[ok], 1.0)
+        np.add.at(out[:, 1], b[ok], k[ok])
+        np.add.at(out[:, 2], b[ok], p[ok])
+        np.add.at(out[:, 3], b[ok], p[ok] ** 2)
+    return out

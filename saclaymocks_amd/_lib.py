@@ -53,6 +53,11 @@ _SIGNATURES = {
     "smk_box_pass_times": (_i, [_vp, _c.POINTER(_d), _i, _c.POINTER(_i), _i]),
     "smk_fft_and_store": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "smk_fft_and_store_workspace_bytes": (_sz, [_vp]),
+    "smk_forest_plan_create": (_i, [_c.POINTER(_vp)]),
+    "smk_forest_plan_destroy": (_i, [_vp]),
+    "smk_forest_workspace_bytes": (_sz, [_i]),
+    "smk_forest_gather": (_i, [_vp, _vp, _i, _vp, _c.c_int64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "smk_forest_missing_power": (_i, [_vp, _vp, _vp, _c.POINTER(_c.c_int64), _u64, _vp, _vp, _vp, _vp, _vp]),
 }
 
 

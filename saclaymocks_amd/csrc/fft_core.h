@@ -173,6 +173,7 @@ __host__ __device__ constexpr int elems_per_thread(int n) {
     case 1536: return 24;
     case 2048: return 8;
     case 2560: return 40;
+    case 4096: return 16;
     default: return 0;
   }
 }
@@ -294,8 +295,8 @@ __device__ __forceinline__ void fft_line_e(float2 (&v)[E], float2* __restrict__ 
 // index), never on the launch geometry.
 struct U4 { uint32_t x, y, z, w; };
 
-__host__ __device__ __forceinline__ U4 philox4x32_10(uint64_t ctr, uint64_t seed) {
-  uint32_t c0 = (uint32_t)ctr, c1 = (uint32_t)(ctr >> 32), c2 = 0u, c3 = 0u;
+__host__ __device__ __forceinline__ U4 philox4x32_10_c4(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                                        uint64_t seed) {
   uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
   const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
 #pragma unroll
@@ -310,6 +311,10 @@ __host__ __device__ __forceinline__ U4 philox4x32_10(uint64_t ctr, uint64_t seed
     k0 += W0; k1 += W1;
   }
   return U4{c0, c1, c2, c3};
+}
+
+__host__ __device__ __forceinline__ U4 philox4x32_10(uint64_t ctr, uint64_t seed) {
+  return philox4x32_10_c4((uint32_t)ctr, (uint32_t)(ctr >> 32), 0u, 0u, seed);
 }
 
 // uniform in (0, 1): 24 high bits plus half an ulp, exactly representable

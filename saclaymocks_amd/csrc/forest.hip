@@ -1,0 +1,424 @@
+// Forest stage on gfx950: skewer gather, missing small-scale power and FGPA.
+//
+// Reference: bin/make_spectra.py:39-139,311-321,429-472 (gather along the
+// line of sight) and bin/merge_spectra.py:303-339 (missing power + FGPA).
+//
+//   k_gather          one thread per pixel: skewer by binary search over the
+//                     packed offsets, position and containing cell in the
+//                     reference's float64 evaluation order (bit-exact cell
+//                     indices), trilinear or 343-cell Gaussian smoothing of
+//                     delta and eta, then the FGPA
+//   k_missing_power   one FFT line per skewer, T skewers per block, one
+//                     launch per length class nz = 256 << c; the classes are
+//                     bucketed on the device
+#include <math.h>
+#include <stdlib.h>
+
+#include <vector>
+
+#include "fft_core.h"
+#include "smk_internal.h"
+
+using namespace smk;
+
+namespace {
+
+constexpr int N_CLASS = 6;  // nz = 256 .. 8192
+
+struct GatherArgs {
+  smk_skewers sk;
+  smk_box_view box;
+  int interp;
+  const float* abcg;
+  int64_t abcg_rows;
+  const float* delta_s;
+  float* delta_l;
+  float* eta_par;
+  float* flux;
+  int32_t* cells;
+  int32_t* status;
+  int64_t npix_total;
+  double hx, hy, hz;  // LX/2, LY/2, LZ/2
+};
+
+__device__ __forceinline__ int find_skewer(const int64_t* __restrict__ off, int n, int64_t pix) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= pix) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ int wrap(int64_t i, int n) {
+  int64_t r = i % n;
+  return (int)(r < 0 ? r + n : r);
+}
+
+__global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
+  const int64_t pix = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pix >= a.npix_total) return;
+  const smk_skewers& sk = a.sk;
+  const int s = find_skewer(sk.off, sk.n, pix);
+  const int i = (int)(pix - sk.off[s]);
+  const int64_t g = (int64_t)sk.g0[s] + i;
+  // R_g = rmin + g*dr (make_spectra.py:312), no contraction
+  const double R = __dadd_rn(sk.rmin, __dmul_rn((double)g, sk.dr));
+  double X, Y, Z;
+  if (sk.geometry == SMK_GEOM_RADIAL) {
+    // Xvec = Rvec * X_QSO / R_QSO (make_spectra.py:450-452)
+    const double qr = sk.qr[s];
+    X = __ddiv_rn(__dmul_rn(R, sk.qx[s]), qr);
+    Y = __ddiv_rn(__dmul_rn(R, sk.qy[s]), qr);
+    Z = __ddiv_rn(__dmul_rn(R, sk.qz[s]), qr);
+  } else {
+    X = sk.qx[s];
+    Y = sk.qy[s];
+    Z = R;
+  }
+  const smk_box_view& b = a.box;
+  const double D = b.cell;
+  // make_spectra.py:47-49: int((X + LX/2)/DX), int((Z + LZ/2 - R0)/DZ)
+  const int64_t ix = (int64_t)__ddiv_rn(__dadd_rn(X, a.hx), D);
+  const int64_t iy = (int64_t)__ddiv_rn(__dadd_rn(Y, a.hy), D);
+  const int64_t iz = (int64_t)__ddiv_rn(__dsub_rn(__dadd_rn(Z, a.hz), b.r0), D);
+  if (a.cells) {
+    a.cells[3 * pix + 0] = (int32_t)ix;
+    a.cells[3 * pix + 1] = (int32_t)iy;
+    a.cells[3 * pix + 2] = (int32_t)iz;
+  }
+  const bool in_forest = i >= sk.imin[s] && i < sk.imax[s];
+  const bool radial = sk.geometry == SMK_GEOM_RADIAL;
+  float dl = -1000000.0f, eta = 0.0f;
+  bool outside = false;
+  if (in_forest) {
+    const int nx = b.nx, ny = b.ny, nz = b.nz;
+    // fields: 0 = delta, 1..6 = eta_xx..eta_yz (radial) or 1 = eta_zz (plane)
+    const int nf = b.eta[2] == nullptr ? 1 : (radial ? 7 : 2);
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    const float* fld[7] = {b.delta, radial ? b.eta[0] : b.eta[2], b.eta[1], b.eta[2], b.eta[3], b.eta[4], b.eta[5]};
+    if (a.interp == SMK_INTERP_GAUSS343) {
+      // cell centres (i+0.5)D - L/2 (+R0 along Z), weights exp(-dr^2/(2 D^2))
+      const double cx = __dsub_rn(__dmul_rn((double)ix + 0.5, D), a.hx);
+      const double cy = __dsub_rn(__dmul_rn((double)iy + 0.5, D), a.hy);
+      const double cz = __dadd_rn(__dsub_rn(__dmul_rn((double)iz + 0.5, D), a.hz), b.r0);
+      const double sig2 = 2.0 * D * D;
+      double wsum = 0.0;
+      for (int da = -3; da <= 3; ++da) {
+        const double ex = __dsub_rn(__dadd_rn(da * D, cx), X);
+        int64_t jx = ix + da;
+        if (b.periodic) jx = wrap(jx, nx); else if (jx < 0 || jx >= nx) outside = true;
+        for (int db = -3; db <= 3; ++db) {
+          const double ey = __dsub_rn(__dadd_rn(db * D, cy), Y);
+          int64_t jy = iy + db;
+          if (b.periodic) jy = wrap(jy, ny); else if (jy < 0 || jy >= ny) outside = true;
+          for (int dc = -3; dc <= 3; ++dc) {
+            const double ez = __dsub_rn(__dadd_rn(dc * D, cz), Z);
+            int64_t jz = iz + dc;
+            if (b.periodic) jz = wrap(jz, nz); else if (jz < 0 || jz >= nz) outside = true;
+            if (outside) continue;
+            const double r2 = __dadd_rn(__dadd_rn(__dmul_rn(ex, ex), __dmul_rn(ey, ey)), __dmul_rn(ez, ez));
+            const double w = exp(-r2 / sig2);
+            wsum += w;
+            const int64_t flat = (jx * ny + jy) * nz + jz;
+            for (int f = 0; f < nf; ++f) acc[f] += w * (double)fld[f][flat];
+          }
+        }
+      }
+      for (int f = 0; f < nf; ++f) acc[f] /= wsum;
+    } else {
+      // trilinear on cell-centred values, periodic
+      const double ux = __ddiv_rn(__dadd_rn(X, a.hx), D) - 0.5;
+      const double uy = __ddiv_rn(__dadd_rn(Y, a.hy), D) - 0.5;
+      const double uz = __ddiv_rn(__dsub_rn(__dadd_rn(Z, a.hz), b.r0), D) - 0.5;
+      const double fx0 = floor(ux), fy0 = floor(uy), fz0 = floor(uz);
+      const double fx = ux - fx0, fy = uy - fy0, fz = uz - fz0;
+      const int64_t i0x = (int64_t)fx0, i0y = (int64_t)fy0, i0z = (int64_t)fz0;
+      for (int c = 0; c < 8; ++c) {
+        const int ox = c >> 2, oy = (c >> 1) & 1, oz = c & 1;
+        const double w = (ox ? fx : 1.0 - fx) * (oy ? fy : 1.0 - fy) * (oz ? fz : 1.0 - fz);
+        const int64_t flat = ((int64_t)wrap(i0x + ox, nx) * ny + wrap(i0y + oy, ny)) * nz + wrap(i0z + oz, nz);
+        for (int f = 0; f < nf; ++f) acc[f] += w * (double)fld[f][flat];
+      }
+    }
+    if (outside) {
+      dl = __int_as_float(0x7fc00000);
+      eta = dl;
+      if (a.status) atomicOr(&a.status[s], 1);
+    } else {
+      dl = (float)acc[0];
+      if (nf > 1) {
+        if (radial) {
+          // make_spectra.py:127-129 with Xtrue = X (no slab shift)
+          const double RR = X * X + Y * Y + Z * Z;
+          const double e = (X * acc[1] * X + Y * acc[2] * Y + Z * acc[3] * Z + 2 * X * acc[4] * Y +
+                            2 * X * acc[5] * Z + 2 * Y * acc[6] * Z) / RR;
+          eta = (float)e;
+        } else {
+          eta = (float)acc[1];
+        }
+      }
+    }
+  }
+  if (a.delta_l) a.delta_l[pix] = dl;
+  if (a.eta_par) a.eta_par[pix] = eta;
+  if (a.flux) {
+    const int64_t row = sk.pset ? (int64_t)sk.pset[s] : g;
+    if (row < 0 || row >= a.abcg_rows) {
+      a.flux[pix] = __int_as_float(0x7fc00000);
+      if (a.status) atomicOr(&a.status[s], 2);
+    } else {
+      const float4 p = reinterpret_cast<const float4*>(a.abcg)[row];
+      const float ds = a.delta_s ? a.delta_s[pix] : 0.0f;
+      // F = exp(-a exp(b G (delta + c eta)))
+      const float gfield = dl + ds + p.z * eta;
+      a.flux[pix] = expf(-p.x * expf(p.y * p.w * gfield));
+    }
+  }
+}
+
+// ------------------------------------------------------------ missing power
+__device__ __forceinline__ float2 mp_mode(uint64_t seed, uint64_t id, int k, int M, const float* __restrict__ filt) {
+  // Fourier mode k of real N(0,1) white noise of length 2M: interior modes
+  // have Re, Im ~ N(0, M); k = 0 and k = M are real with variance 2M
+  const U4 u = philox4x32_10_c4((uint32_t)k, (uint32_t)id, (uint32_t)(id >> 32), 0x6d697373u, seed);
+  float n0, n1;
+  box_muller(u.x, u.y, n0, n1);
+  const float f = filt[k];
+  if (k == 0 || k == M) return make_float2(n0 * sqrtf(2.0f * M) * f, 0.0f);
+  const float sm = sqrtf((float)M) * f;
+  return make_float2(n0 * sm, n1 * sm);
+}
+
+template <int M, int T>
+__global__ __launch_bounds__(T* (M / elems_per_thread(M))) void k_missing_power(
+    smk_skewers sk, const int32_t* __restrict__ list, const int32_t* __restrict__ count,
+    const float* __restrict__ filt, uint64_t seed, const float* __restrict__ noise,
+    const int64_t* __restrict__ noise_off, const float2* __restrict__ tw, const float2* __restrict__ twr,
+    float* __restrict__ delta_s) {
+  constexpr int E = elems_per_thread(M);
+  constexpr int P = M / E;
+  constexpr int LL = lds_line_len(M);
+  __shared__ float2 lds[T * LL];
+  const int ncls = *count;
+  const int base = blockIdx.x * T;
+  if (base >= ncls) return;  // uniform across the block
+  const int tid = threadIdx.x;
+  const int p = tid % P, t = tid / P;
+  const bool valid = base + t < ncls;
+  const int s = valid ? list[base + t] : 0;
+  float2* line = lds + t * LL;
+  float2 v[E];
+  if (noise == nullptr) {
+    const uint64_t id = sk.noise_id ? sk.noise_id[s] : (uint64_t)s;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = p + P * e;
+      const float2 yk = mp_mode(seed, id, k, M, filt);
+      const float2 yc = cconj(mp_mode(seed, id, M - k, M, filt));
+      const float2 sm = cadd(yk, yc);
+      const float2 d = csub(yk, yc);
+      const float2 wd = cmul(cconj(twr[k]), d);
+      v[e] = make_float2(sm.x - wd.y, sm.y + wd.x);
+    }
+  } else {
+    // forward R2C of the fed noise, times the filter, then the C2R pre-step
+    const float2* src = reinterpret_cast<const float2*>(noise + (valid ? noise_off[s] : 0));
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = valid ? src[p + P * e] : make_float2(0.f, 0.f);
+    fft_line<M, false>(v, line, p, tw);
+    lds_scatter<M, P, E>(v, line, p);
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = p + P * e;
+      const float2 zk = line[lds_pos<M>(k)];
+      const float2 zc = line[lds_pos<M>((M - k) % M)];
+      // X_k and X_{M-k} of the real transform
+      const float2 zcc = cconj(zc);
+      const float2 xk = cadd(cscale(cadd(zk, zcc), 0.5f), cmul(twr[k], mul_mi<false>(cscale(csub(zk, zcc), 0.5f))));
+      float2 xc;
+      if (k == 0) {
+        xc = make_float2(zk.x - zk.y, 0.0f);   // X_M
+      } else {
+        const float2 zkc = cconj(zk);
+        const float2 wm = cconj(twr[k]);       // W_{2M}^{M-k} = -conj(W_{2M}^k)
+        xc = csub(cscale(cadd(zc, zkc), 0.5f), cmul(wm, mul_mi<false>(cscale(csub(zc, zkc), 0.5f))));
+      }
+      float2 yk = cscale(xk, filt[k]);
+      float2 yc = cscale(xc, filt[M - k]);
+      if (k == 0) { yk.y = 0.0f; yc.y = 0.0f; }
+      const float2 ycc = cconj(yc);
+      const float2 sm = cadd(yk, ycc);
+      const float2 d = csub(yk, ycc);
+      const float2 wd = cmul(cconj(twr[k]), d);
+      v[e] = make_float2(sm.x - wd.y, sm.y + wd.x);
+    }
+    __syncthreads();
+  }
+  fft_line<M, true>(v, line, p, tw);
+  if (valid) {
+    const int n = sk.npix[s];
+    float* dst = delta_s + sk.off[s];
+    const float inv = 1.0f / (2.0f * M);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int j = 2 * (p + P * e);
+      if (j < n) dst[j] = v[e].x * inv;
+      if (j + 1 < n) dst[j + 1] = v[e].y * inv;
+    }
+  }
+}
+
+__global__ void k_classify(smk_skewers sk, int32_t* __restrict__ lists, int32_t* __restrict__ counts,
+                           int32_t* __restrict__ bad) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= sk.n) return;
+  const int n = sk.npix[s];
+  int c = 0, nz = 256;
+  while (nz < n + 50 && c < N_CLASS) { nz <<= 1; ++c; }
+  if (c >= N_CLASS) { atomicOr(bad, 1); return; }
+  const int pos = atomicAdd(&counts[c], 1);
+  lists[(int64_t)c * sk.n + pos] = s;
+}
+
+__global__ void k_zero_ints(int32_t* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+constexpr int mp_T(int m) {
+  const int P = m / elems_per_thread(m);
+  return P >= 256 ? 1 : 256 / P;
+}
+
+template <int M>
+hipError_t launch_mp(const smk_skewers& sk, const int32_t* list, const int32_t* count, const float* filt,
+                     uint64_t seed, const float* noise, const int64_t* noise_off, const float2* tw,
+                     const float2* twr, float* delta_s, hipStream_t s) {
+  constexpr int T = mp_T(M);
+  const int nblk = (sk.n + T - 1) / T;
+  if (nblk == 0) return hipSuccess;
+  hipLaunchKernelGGL((k_missing_power<M, T>), dim3(nblk), dim3(T * (M / elems_per_thread(M))), 0, s, sk, list,
+                     count, filt, seed, noise, noise_off, tw, twr, delta_s);
+  return hipGetLastError();
+}
+
+std::vector<float2> twiddles_f(int n, double scale) {
+  std::vector<float2> t(n);
+  for (int m = 0; m < n; ++m) {
+    const double ang = -2.0 * M_PI * (double)m / ((double)n * scale);
+    t[m] = make_float2((float)cos(ang), (float)sin(ang));
+  }
+  return t;
+}
+
+}  // namespace
+
+struct smk_forest_plan {
+  float2* tw[N_CLASS];   // W_M^m, M = 128 << c
+  float2* twr[N_CLASS];  // W_{2M}^k
+};
+
+extern "C" int smk_forest_plan_create(smk_forest_plan** plan) {
+  SMK_REQUIRE(plan != nullptr, SMK_ERR_ARG, "plan pointer is NULL");
+  smk_forest_plan* p = (smk_forest_plan*)calloc(1, sizeof(smk_forest_plan));
+  SMK_REQUIRE(p != nullptr, SMK_ERR_OOM, "host allocation failed");
+  for (int c = 0; c < N_CLASS; ++c) {
+    const int M = 128 << c;
+    auto a = twiddles_f(M, 1.0), b = twiddles_f(M, 2.0);
+    hipError_t e = hipMalloc((void**)&p->tw[c], M * sizeof(float2));
+    if (e == hipSuccess) e = hipMemcpy(p->tw[c], a.data(), M * sizeof(float2), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc((void**)&p->twr[c], M * sizeof(float2));
+    if (e == hipSuccess) e = hipMemcpy(p->twr[c], b.data(), M * sizeof(float2), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      smk_forest_plan_destroy(p);
+      return set_error(SMK_ERR_HIP, "forest plan tables: %s", hipGetErrorString(e));
+    }
+  }
+  *plan = p;
+  return SMK_OK;
+}
+
+extern "C" int smk_forest_plan_destroy(smk_forest_plan* p) {
+  if (p == nullptr) return SMK_OK;
+  for (int c = 0; c < N_CLASS; ++c) {
+    if (p->tw[c]) (void)hipFree(p->tw[c]);
+    if (p->twr[c]) (void)hipFree(p->twr[c]);
+  }
+  free(p);
+  return SMK_OK;
+}
+
+extern "C" size_t smk_forest_workspace_bytes(int n_skewers) {
+  return (size_t)(N_CLASS * (size_t)(n_skewers > 0 ? n_skewers : 0) + 16) * sizeof(int32_t);
+}
+
+extern "C" int smk_forest_gather(const smk_skewers* sk, const smk_box_view* box, int interp, const float* abcg,
+                                 int64_t abcg_rows, const float* delta_s, float* delta_l, float* eta_par, float* flux,
+                                 int32_t* cells, int32_t* status, void* stream) {
+  SMK_REQUIRE(sk != nullptr && box != nullptr, SMK_ERR_ARG, "NULL skewers/box");
+  SMK_REQUIRE(box->delta != nullptr, SMK_ERR_ARG, "box delta is NULL");
+  SMK_REQUIRE(box->nx > 0 && box->ny > 0 && box->nz > 0 && box->cell > 0, SMK_ERR_ARG, "bad box geometry");
+  SMK_REQUIRE(interp == SMK_INTERP_TRILINEAR || interp == SMK_INTERP_GAUSS343, SMK_ERR_ARG, "bad interp %d", interp);
+  SMK_REQUIRE(sk->geometry == SMK_GEOM_PLANE || sk->geometry == SMK_GEOM_RADIAL, SMK_ERR_ARG, "bad geometry");
+  SMK_REQUIRE(flux == nullptr || abcg != nullptr, SMK_ERR_ARG, "flux needs FGPA parameters");
+  SMK_REQUIRE(sk->geometry != SMK_GEOM_RADIAL ||
+                  (box->eta[0] == nullptr) == (box->eta[2] == nullptr),
+              SMK_ERR_ARG, "radial eta_par needs all six eta boxes");
+  if (sk->n <= 0) return SMK_OK;
+  GatherArgs a{};
+  a.sk = *sk;
+  a.box = *box;
+  a.interp = interp;
+  a.abcg = abcg;
+  a.abcg_rows = abcg_rows;
+  a.delta_s = delta_s;
+  a.delta_l = delta_l;
+  a.eta_par = eta_par;
+  a.flux = flux;
+  a.cells = cells;
+  a.status = status;
+  a.npix_total = sk->npix_total;
+  a.hx = box->cell * box->nx / 2;
+  a.hy = box->cell * box->ny / 2;
+  a.hz = box->cell * box->nz / 2;
+  if (a.npix_total <= 0) return SMK_OK;
+  const uint64_t nblk = (uint64_t)(a.npix_total + 255) / 256;
+  hipLaunchKernelGGL(k_gather, dim3((uint32_t)nblk), dim3(256), 0, (hipStream_t)stream, a);
+  SMK_LAUNCH_CHECK();
+  return SMK_OK;
+}
+
+extern "C" int smk_forest_missing_power(smk_forest_plan* plan, const smk_skewers* sk, const float* filt,
+                                        const int64_t* filt_off, uint64_t seed, const float* noise,
+                                        const int64_t* noise_off, void* workspace, float* delta_s, void* stream) {
+  SMK_REQUIRE(plan != nullptr && sk != nullptr && filt != nullptr && filt_off != nullptr && delta_s != nullptr,
+              SMK_ERR_ARG, "NULL argument");
+  SMK_REQUIRE(workspace != nullptr, SMK_ERR_ARG, "workspace is NULL");
+  SMK_REQUIRE(noise == nullptr || noise_off != nullptr, SMK_ERR_ARG, "fed noise needs noise_off");
+  if (sk->n <= 0) return SMK_OK;
+  hipStream_t s = (hipStream_t)stream;
+  int32_t* counts = (int32_t*)workspace;           // [N_CLASS], [N_CLASS] = bad flag
+  int32_t* lists = counts + 16;
+  hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, counts, 16);
+  SMK_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_classify, dim3((sk->n + 255) / 256), dim3(256), 0, s, *sk, lists, counts, counts + N_CLASS);
+  SMK_LAUNCH_CHECK();
+  const int64_t* foff = filt_off;
+  int32_t bad = 0;
+  SMK_HIP_CHECK(hipMemcpyAsync(&bad, counts + N_CLASS, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  SMK_HIP_CHECK(hipStreamSynchronize(s));
+  SMK_REQUIRE(bad == 0, SMK_ERR_RANGE, "a skewer needs an FFT longer than %d", 256 << (N_CLASS - 1));
+#define SMK_MP(C, M)                                                                                         \
+  SMK_HIP_CHECK(launch_mp<M>(*sk, lists + (int64_t)C * sk->n, counts + C, filt + foff[C], seed, noise,       \
+                             noise_off, plan->tw[C], plan->twr[C], delta_s, s))
+  SMK_MP(0, 128);
+  SMK_MP(1, 256);
+  SMK_MP(2, 512);
+  SMK_MP(3, 1024);
+  SMK_MP(4, 2048);
+  SMK_MP(5, 4096);
+#undef SMK_MP
+  return SMK_OK;
+}

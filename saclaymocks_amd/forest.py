@@ -1,0 +1,267 @@
+"""Forest stage: the MI355X replacement of make_spectra.py's gather and
+merge_spectra.py's missing power + FGPA.
+
+Reference flow (one OS process per x-slab, FITS files in between):
+    make_spectra.py:412-522   per QSO: line of sight, ReadSpec (343-cell gather)
+    merge_spectra.py:282-339  per QSO: concat slab pieces, + delta_s, FGPA
+Here the whole box is resident on the GPU, so a QSO is one skewer of the
+global pixel grid and the merged semantics (ascending lambda, full-length
+spectra, sentinels outside the forest) come out directly.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import util
+
+GEOM_PLANE = 0
+GEOM_RADIAL = 1
+INTERP = {"trilinear": 0, "gauss343": 1}
+N_CLASS = 6
+
+
+class SmkBoxView(ctypes.Structure):
+    _fields_ = [("delta", ctypes.c_void_p), ("eta", ctypes.c_void_p * 6),
+                ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("periodic", ctypes.c_int32), ("cell", ctypes.c_double), ("r0", ctypes.c_double)]
+
+
+class SmkSkewers(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32), ("geometry", ctypes.c_int32), ("npix_total", ctypes.c_int64),
+                ("rmin", ctypes.c_double), ("dr", ctypes.c_double),
+                ("qx", ctypes.c_void_p), ("qy", ctypes.c_void_p), ("qz", ctypes.c_void_p),
+                ("qr", ctypes.c_void_p), ("g0", ctypes.c_void_p), ("npix", ctypes.c_void_p),
+                ("imin", ctypes.c_void_p), ("imax", ctypes.c_void_p), ("off", ctypes.c_void_p),
+                ("pset", ctypes.c_void_p), ("noise_id", ctypes.c_void_p)]
+
+
+def _dev():
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _t(a, dtype, dev):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).to(dev)
+
+
+class BoxView:
+    """Resident boxes (torch float32 CUDA tensors [nx][ny][nz]) + box frame."""
+
+    def __init__(self, delta, eta=None, cell=2.19, r0=0.0, periodic=True):
+        self.delta = delta
+        self.eta = list(eta) if eta is not None else [None] * 6
+        if len(self.eta) != 6:
+            raise ValueError("eta must list 6 boxes (xx, yy, zz, xy, xz, yz); None where unused")
+        self.cell, self.r0, self.periodic = float(cell), float(r0), bool(periodic)
+        for t in [delta] + [e for e in self.eta if e is not None]:
+            if not (t.is_cuda and t.dtype == torch.float32 and t.is_contiguous() and t.shape == delta.shape):
+                raise ValueError("box fields must be contiguous float32 CUDA tensors of one shape")
+
+    @classmethod
+    def plane(cls, delta, eta_zz=None, cell=2.19, r0=0.0, periodic=True):
+        return cls(delta, [None, None, eta_zz, None, None, None], cell, r0, periodic)
+
+    def struct(self):
+        nx, ny, nz = self.delta.shape
+        v = SmkBoxView()
+        v.delta = self.delta.data_ptr()
+        for i, e in enumerate(self.eta):
+            v.eta[i] = e.data_ptr() if e is not None else None
+        v.nx, v.ny, v.nz = nx, ny, nz
+        v.periodic = 1 if self.periodic else 0
+        v.cell, v.r0 = self.cell, self.r0
+        return v
+
+
+class SkewerBatch:
+    """Device descriptor arrays of a batch of skewers on the grid R_g = rmin + g dr."""
+
+    def __init__(self, geometry, g0, npix, rmin, dr, qx, qy, qz=None, qr=None, imin=None, imax=None,
+                 pset=None, noise_id=None, device=None):
+        dev = device or _dev()
+        npix = np.asarray(npix, dtype=np.int32)
+        n = len(npix)
+        off = np.zeros(n, dtype=np.int64)
+        if n:
+            off[1:] = np.cumsum(npix[:-1], dtype=np.int64)
+        self.n = n
+        self.geometry = geometry
+        self.rmin, self.dr = float(rmin), float(dr)
+        self.npix_host = npix
+        self.off_host = off
+        self.npix_total = int(off[-1] + npix[-1]) if n else 0
+        zeros = np.zeros(n)
+        self.qx = _t(qx, torch.float64, dev)
+        self.qy = _t(qy, torch.float64, dev)
+        self.qz = _t(zeros if qz is None else qz, torch.float64, dev)
+        self.qr = _t(np.ones(n) if qr is None else qr, torch.float64, dev)
+        self.g0 = _t(g0, torch.int32, dev)
+        self.npix = _t(npix, torch.int32, dev)
+        self.imin = _t(np.zeros(n, np.int32) if imin is None else imin, torch.int32, dev)
+        self.imax = _t(npix if imax is None else imax, torch.int32, dev)
+        self.off = _t(off, torch.int64, dev)
+        self.pset = None if pset is None else _t(pset, torch.int32, dev)
+        nid = np.arange(n, dtype=np.uint64) if noise_id is None else np.asarray(noise_id, dtype=np.uint64)
+        self.noise_id = torch.as_tensor(nid.view(np.int64)).to(dev)
+        self.g0_host = np.asarray(g0, dtype=np.int64)
+
+    def struct(self):
+        s = SmkSkewers()
+        s.n, s.geometry, s.npix_total = self.n, self.geometry, self.npix_total
+        s.rmin, s.dr = self.rmin, self.dr
+        s.qx, s.qy, s.qz, s.qr = (t.data_ptr() for t in (self.qx, self.qy, self.qz, self.qr))
+        s.g0, s.npix, s.imin, s.imax, s.off = (t.data_ptr() for t in (self.g0, self.npix, self.imin, self.imax,
+                                                                       self.off))
+        s.pset = self.pset.data_ptr() if self.pset is not None else None
+        s.noise_id = self.noise_id.data_ptr()
+        return s
+
+    def split(self, x):
+        """Host list of per-skewer numpy arrays from a packed per-pixel tensor."""
+        a = x.detach().cpu().numpy() if isinstance(x, torch.Tensor) else np.asarray(x)
+        return [a[o:o + n] for o, n in zip(self.off_host, self.npix_host)]
+
+
+class ForestPlan:
+    """FFT tables of the missing-power kernel (one per length class)."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().smk_forest_plan_create(ctypes.byref(h)), "smk_forest_plan_create")
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            _lib.lib().smk_forest_plan_destroy(h)
+            self._h = None
+
+
+def missing_power_filters(pixel, p1d, device=None):
+    """sqrt(max(P1D(k_j), 0) / pixel) for every length class nz = 256 << c
+    (merge_spectra.py:312-320), concatenated float32 + host offsets."""
+    tabs, offs, pos = [], [], 0
+    for c in range(N_CLASS):
+        nz = 256 << c
+        k = np.fft.rfftfreq(nz) * 2 * (np.pi / pixel)
+        tabs.append(np.sqrt(np.maximum(p1d(k), 0.0) / pixel).astype(np.float32))
+        offs.append(pos)
+        pos += len(tabs[-1])
+    filt = torch.from_numpy(np.concatenate(tabs)).to(device or _dev())
+    return filt, (ctypes.c_int64 * N_CLASS)(*offs)
+
+
+def fft_len_for(n):
+    """merge_spectra.py:308-309."""
+    nz = 256
+    while nz < n + 50:
+        nz *= 2
+    return nz
+
+
+def missing_power(plan, batch, filt, filt_off, seed=0, noise=None, noise_off=None, out=None, stream=None):
+    """delta_s per pixel (float32 CUDA tensor).  noise None -> device Fourier draw."""
+    dev = _dev()
+    if out is None:
+        out = torch.empty(batch.npix_total, dtype=torch.float32, device=dev)
+    ws = torch.empty(int(_lib.lib().smk_forest_workspace_bytes(batch.n)), dtype=torch.uint8, device=dev)
+    noff = None
+    if noise is not None:
+        noff = noise_off if isinstance(noise_off, torch.Tensor) else _t(noise_off, torch.int64, dev)
+    sk = batch.struct()
+    rc = _lib.lib().smk_forest_missing_power(plan._h, ctypes.byref(sk), _lib.ptr(filt), filt_off,
+                                             ctypes.c_uint64(int(seed) & (2**64 - 1)), _lib.ptr(noise),
+                                             _lib.ptr(noff), _lib.ptr(ws), _lib.ptr(out),
+                                             _lib.stream_ptr(stream))
+    _lib.check(rc, "smk_forest_missing_power")
+    return out
+
+
+def gather(batch, box, interp="gauss343", abcg=None, delta_s=None, want=("delta_l", "eta_par", "flux"),
+           cells=False, stream=None):
+    """ReadSpec (+ FGPA when abcg is given).  Returns a dict of packed per-pixel
+    tensors and the per-skewer status (1 = left a non-periodic box)."""
+    dev = _dev()
+    n = batch.npix_total
+    out = {}
+    for k in ("delta_l", "eta_par", "flux"):
+        if k in want and (k != "flux" or abcg is not None):
+            out[k] = torch.empty(n, dtype=torch.float32, device=dev)
+    if cells:
+        out["cells"] = torch.empty((n, 3), dtype=torch.int32, device=dev)
+    status = torch.zeros(batch.n, dtype=torch.int32, device=dev)
+    sk = batch.struct()
+    bv = box.struct()
+    rows = int(abcg.shape[0]) if abcg is not None else 0
+    rc = _lib.lib().smk_forest_gather(ctypes.byref(sk), ctypes.byref(bv), INTERP[interp], _lib.ptr(abcg),
+                                      rows, _lib.ptr(delta_s), _lib.ptr(out.get("delta_l")),
+                                      _lib.ptr(out.get("eta_par")), _lib.ptr(out.get("flux")),
+                                      _lib.ptr(out.get("cells")), _lib.ptr(status), _lib.stream_ptr(stream))
+    _lib.check(rc, "smk_forest_gather")
+    out["status"] = status
+    return out
+
+
+def fgpa_rows(a, b, c, G):
+    """float32 [rows][4] parameter table (a, b, c, G) for the FGPA kernels."""
+    a, b, c, G = np.broadcast_arrays(*(np.asarray(x, dtype=np.float64) for x in (a, b, c, G)))
+    return torch.from_numpy(np.stack([a, b, c, G], axis=-1).astype(np.float32).reshape(-1, 4)).to(_dev())
+
+
+# ------------------------------------------------ reference-geometry builder
+def radial_skewers(ra, dec, z_qso, ra0, dec0, box_shape, cell, zmin=1.3, zmax=3.6, pixel=0.2, dmax=3,
+                   thing_id=None, cosmo=None, full_length_only=False):
+    """Line-of-sight set-up of make_spectra.py:300-472 for one slab covering the
+    whole box (NSlice = 1): pixel grid, per-QSO (X, Y, Z, R) of the QSO,
+    contiguous pixel range inside the box in X, forest limit lambda <
+    lambda_Lya (1 + z_QSO).  Angles in degrees.  Returns (SkewerBatch
+    arguments as a dict, grid dict, kept QSO indices)."""
+    cosmo = cosmo or util.Cosmo()
+    h = util.H
+    nx, ny, nz = box_shape
+    LX, LY, LZ = cell * nx, cell * ny, cell * nz
+    R0 = h * cosmo.r_comoving(util.Z0)
+    Rmin = h * cosmo.r_comoving(zmin)
+    Rmax = h * cosmo.r_comoving(zmax)
+    npixeltot = int((Rmax - Rmin) / pixel + 0.5)
+    R_all = Rmin + np.arange(npixeltot) * pixel
+    lam_all = util.LYA * (1 + cosmo.r_2_z(R_all / h))
+    keep = lam_all > util.LAMBDA_MIN
+    i_first = int(np.argmax(keep))
+    R_vec = R_all[keep]
+    lambda_vec = lam_all[keep]
+    xmin, xmax = -LX / 2, LX / 2
+    sinx_max = (LX / 2 - dmax * cell) / (R0 + LZ / 2 - dmax * cell)
+    tanx_max = sinx_max / np.sqrt(1 - sinx_max ** 2)
+    rows = {k: [] for k in ("qx", "qy", "qz", "qr", "g0", "npix", "imin", "imax", "noise_id")}
+    kept = []
+    for q in range(len(ra)):
+        zq = float(z_qso[q])
+        R_QSO = h * cosmo.r_comoving(zq)
+        X, Y, Z = util.compute_xyz2(np.radians(ra[q]), np.radians(dec[q]), R_QSO, np.radians(ra0),
+                                    np.radians(dec0))
+        if abs(X / Z) > tanx_max or zq < zmin or zq > zmax:
+            continue
+        xv = R_vec * X / R_QSO
+        sel = np.nonzero((xv > xmin) & (xv <= xmax))[0]
+        if len(sel) < 1:
+            continue
+        k0, k1 = int(sel[0]), int(sel[-1]) + 1
+        if full_length_only and (k1 - k0) != len(R_vec):
+            continue
+        lam = lambda_vec[k0:k1]
+        below = np.nonzero(lam < util.LYA * (1 + zq))[0]
+        rows["qx"].append(X)
+        rows["qy"].append(Y)
+        rows["qz"].append(Z)
+        rows["qr"].append(R_QSO)
+        rows["g0"].append(i_first + k0)
+        rows["npix"].append(k1 - k0)
+        rows["imin"].append(0)
+        rows["imax"].append(int(below[-1]) + 1 if len(below) else 0)
+        rows["noise_id"].append(int(thing_id[q]) if thing_id is not None else q)
+        kept.append(q)
+    grid = {"rmin": Rmin, "dr": pixel, "i_first": i_first, "R_vec": R_vec, "lambda_vec": lambda_vec,
+            "R0": R0, "npixeltot": len(R_vec)}
+    return rows, grid, np.asarray(kept, dtype=np.int64)

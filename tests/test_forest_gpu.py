@@ -1,0 +1,206 @@
+"""GPU parity of the forest stage (gather, missing power, FGPA) against the FP64 oracle."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import forest_oracle as fo
+
+pytestmark = pytest.mark.gpu
+
+
+def _field(shape, seed):
+    return np.random.default_rng(seed).standard_normal(shape).astype(np.float32)
+
+
+def _plane_case(rng, shape, cell, r0, n):
+    L = [cell * s for s in shape]
+    rmin = r0 - L[2] / 2 + 0.05
+    dr = 0.2
+    maxg = int((L[2] - 0.2) / dr)
+    npix = rng.integers(20, 300, n).astype(np.int32)
+    g0 = np.array([rng.integers(0, maxg - k) for k in npix], dtype=np.int32)
+    qx = rng.uniform(-L[0] / 2, L[0] / 2, n)
+    qy = rng.uniform(-L[1] / 2, L[1] / 2, n)
+    imin = np.array([rng.integers(0, k // 3) for k in npix], dtype=np.int32)
+    imax = np.array([rng.integers(2 * k // 3, k + 1) for k in npix], dtype=np.int32)
+    return dict(qx=qx, qy=qy, g0=g0, npix=npix, imin=imin, imax=imax, rmin=rmin, dr=dr, L=L)
+
+
+def _plane_positions(c):
+    X, Y, Z, sk = [], [], [], []
+    for s, (g0, n) in enumerate(zip(c["g0"], c["npix"])):
+        g = g0 + np.arange(n)
+        X.append(np.full(n, c["qx"][s]))
+        Y.append(np.full(n, c["qy"][s]))
+        Z.append(c["rmin"] + g * c["dr"])
+        sk.append(np.full(n, s))
+    return np.concatenate(X), np.concatenate(Y), np.concatenate(Z), np.concatenate(sk)
+
+
+def _forest_mask(c):
+    return np.concatenate([(np.arange(n) >= a) & (np.arange(n) < b)
+                           for n, a, b in zip(c["npix"], c["imin"], c["imax"])])
+
+
+@pytest.mark.parametrize("interp", ["gauss343", "trilinear"])
+def test_plane_gather_parity(cuda, interp):
+    from saclaymocks_amd import forest
+    print(f"[forest plane gather] interp={interp}", flush=True)
+    shape, cell, r0 = (48, 40, 64), 2.19, 3000.0
+    rng = np.random.default_rng(1)
+    c = _plane_case(rng, shape, cell, r0, 40)
+    d, e = _field(shape, 2), _field(shape, 3)
+    box = forest.BoxView.plane(torch.from_numpy(d).to(cuda), torch.from_numpy(e).to(cuda), cell=cell, r0=r0,
+                               periodic=True)
+    b = forest.SkewerBatch(forest.GEOM_PLANE, c["g0"], c["npix"], c["rmin"], c["dr"], c["qx"], c["qy"],
+                           imin=c["imin"], imax=c["imax"])
+    out = forest.gather(b, box, interp=interp, want=("delta_l", "eta_par"), cells=True)
+    X, Y, Z, _ = _plane_positions(c)
+    L, D = c["L"], (cell, cell, cell)
+    ix, iy, iz = fo.cell_index(X, Y, Z, L, D, r0)
+    cells = out["cells"].cpu().numpy()
+    assert np.array_equal(cells[:, 0], ix) and np.array_equal(cells[:, 1], iy) and np.array_equal(cells[:, 2], iz)
+    if interp == "gauss343":
+        (rd, re), _ = fo.gauss343([d, e], X, Y, Z, L, D, r0, periodic=True)
+    else:
+        rd, re = fo.trilinear([d, e], X, Y, Z, L, D, r0)
+    m = _forest_mask(c)
+    gd, ge = out["delta_l"].cpu().numpy(), out["eta_par"].cpu().numpy()
+    assert np.all(gd[~m] == -1e6) and np.all(ge[~m] == 0)
+    assert np.abs(gd[m] - rd[m]).max() < 2e-6
+    assert np.abs(ge[m] - re[m]).max() < 2e-6
+    assert out["status"].sum().item() == 0
+
+
+def test_radial_gather_parity_reference_geometry(cuda):
+    from saclaymocks_amd import forest, util
+    print("[forest radial gather gauss343]", flush=True)
+    shape, cell = (64, 64, 64), 20.0
+    cos = util.Cosmo()
+    R0 = util.H * cos.r_comoving(util.Z0)
+    L = [cell * s for s in shape]
+    zmax = float(cos.r_2_z((R0 + L[2] / 2 - 4 * cell) / util.H))
+    zmin = float(cos.r_2_z((R0 - L[2] / 2 + 4 * cell) / util.H))
+    rng = np.random.default_rng(7)
+    nq = 16
+    ang = np.degrees(0.25 * (L[0] / 2) / R0)
+    ra, dec = rng.uniform(-ang, ang, nq), rng.uniform(-ang, ang, nq)
+    zq = rng.uniform(0.5 * (zmin + zmax), zmax, nq)
+    rows, grid, kept = forest.radial_skewers(ra, dec, zq, 0.0, 0.0, shape, cell, zmin=zmin, zmax=zmax, pixel=0.2)
+    assert len(kept) == nq
+    fields = [_field(shape, 10 + i) for i in range(7)]
+    tens = [torch.from_numpy(f).to(cuda) for f in fields]
+    box = forest.BoxView(tens[0], tens[1:], cell=cell, r0=R0, periodic=False)
+    b = forest.SkewerBatch(forest.GEOM_RADIAL, rows["g0"], rows["npix"], grid["rmin"], grid["dr"], rows["qx"],
+                           rows["qy"], rows["qz"], rows["qr"], imin=rows["imin"], imax=rows["imax"])
+    out = forest.gather(b, box, interp="gauss343", want=("delta_l", "eta_par"), cells=True)
+    # oracle positions: R = rmin + g dr, X = R * X_Q / R_Q (make_spectra.py:312,450-452)
+    X, Y, Z, msk = [], [], [], []
+    for s in range(b.n):
+        g = rows["g0"][s] + np.arange(rows["npix"][s])
+        R = grid["rmin"] + g * grid["dr"]
+        X.append(R * rows["qx"][s] / rows["qr"][s])
+        Y.append(R * rows["qy"][s] / rows["qr"][s])
+        Z.append(R * rows["qz"][s] / rows["qr"][s])
+        i = np.arange(rows["npix"][s])
+        msk.append((i >= rows["imin"][s]) & (i < rows["imax"][s]))
+    X, Y, Z, msk = map(np.concatenate, (X, Y, Z, msk))
+    D = (cell, cell, cell)
+    ix, iy, iz = fo.cell_index(X, Y, Z, L, D, R0)
+    cells = out["cells"].cpu().numpy()
+    assert np.array_equal(cells[:, 0], ix) and np.array_equal(cells[:, 1], iy) and np.array_equal(cells[:, 2], iz)
+    vals, _ = fo.gauss343(fields, X[msk], Y[msk], Z[msk], L, D, R0)
+    eta = fo.eta_par_radial(vals[1:], X[msk], Y[msk], Z[msk])
+    gd, ge = out["delta_l"].cpu().numpy(), out["eta_par"].cpu().numpy()
+    assert np.abs(gd[msk] - vals[0]).max() < 2e-6
+    assert np.abs(ge[msk] - eta).max() < 2e-6
+    assert np.all(gd[~msk] == -1e6)
+    assert out["status"].sum().item() == 0
+
+
+def test_strict_box_flags_escaping_skewer(cuda):
+    from saclaymocks_amd import forest
+    shape, cell, r0 = (32, 32, 32), 2.0, 0.0
+    d = torch.from_numpy(_field(shape, 1)).to(cuda)
+    box = forest.BoxView.plane(d, None, cell=cell, r0=r0, periodic=False)
+    # skewer 0 at the centre, skewer 1 at the x edge (kernel leaves the box)
+    b = forest.SkewerBatch(forest.GEOM_PLANE, [40, 40], [50, 50], -32.0 + 0.05, 0.2, [0.0, -31.9], [0.0, 0.0])
+    out = forest.gather(b, box, interp="gauss343", want=("delta_l",))
+    st = out["status"].cpu().tolist()
+    assert st == [0, 1]
+    v = b.split(out["delta_l"])
+    assert np.all(np.isfinite(v[0])) and np.all(np.isnan(v[1]))
+
+
+def test_missing_power_fed_noise_parity(cuda):
+    from saclaymocks_amd import forest
+    print("[forest missing power fed noise]", flush=True)
+    pixel = 0.2
+    npix = np.array([100, 300, 462, 463, 900, 974, 975, 1500, 2000, 1998, 4000, 8000], dtype=np.int32)
+    nzs = [fo.fft_len_for(n) for n in npix]
+    rng = np.random.default_rng(3)
+    noises = [rng.standard_normal(nz).astype(np.float32) for nz in nzs]
+    noff = np.zeros(len(npix), dtype=np.int64)
+    noff[1:] = np.cumsum(nzs[:-1])
+    b = forest.SkewerBatch(forest.GEOM_PLANE, np.zeros(len(npix)), npix, 0.0, pixel, np.zeros(len(npix)),
+                           np.zeros(len(npix)))
+    plan = forest.ForestPlan()
+    filt, foff = forest.missing_power_filters(pixel, fo.p1d_missing_analytic)
+    noise_t = torch.from_numpy(np.concatenate(noises)).to(cuda)
+    ds = forest.missing_power(plan, b, filt, foff, noise=noise_t, noise_off=noff)
+    got = b.split(ds)
+    for s, n in enumerate(npix):
+        want = fo.missing_power(noises[s].astype(np.float64), n, pixel, fo.p1d_missing_analytic)
+        err = np.abs(got[s] - want).max() / np.abs(want).max()
+        assert err < 1e-5, (n, err)
+
+
+def test_missing_power_device_noise_statistics(cuda):
+    from saclaymocks_amd import forest
+    pixel, n, ns = 0.2, 974, 3000
+    b = forest.SkewerBatch(forest.GEOM_PLANE, np.zeros(ns), np.full(ns, n), 0.0, pixel, np.zeros(ns), np.zeros(ns),
+                           noise_id=np.arange(ns) * 7 + 11)
+    plan = forest.ForestPlan()
+    filt, foff = forest.missing_power_filters(pixel, fo.p1d_missing_analytic)
+    a = forest.missing_power(plan, b, filt, foff, seed=5).clone()
+    a2 = forest.missing_power(plan, b, filt, foff, seed=5)
+    assert torch.equal(a, a2)
+    x = a.double().cpu().numpy().reshape(ns, n)
+    nz = fo.fft_len_for(n)
+    k = np.fft.rfftfreq(nz) * 2 * np.pi / pixel
+    w = np.full(len(k), 2.0)
+    w[0] = w[-1] = 1.0
+    var_expect = np.sum(w * fo.p1d_missing_analytic(k) / pixel) / nz
+    assert abs(x.var() / var_expect - 1) < 0.02
+    assert abs(x.mean()) < 0.02 * np.sqrt(var_expect)
+    # the P1D of the pixels matches P1D_miss at low k (windowing aside)
+    p = np.mean([fo.p1d_1spectrum(r, pixel) for r in x], axis=0)
+    kk = np.fft.rfftfreq(n) * 2 * np.pi / pixel
+    sel = (kk > 0.2) & (kk < 2.0)
+    assert np.allclose(p[sel], fo.p1d_missing_analytic(kk[sel]), rtol=0.15)
+
+
+def test_fgpa_flux_parity(cuda):
+    from saclaymocks_amd import forest
+    shape, cell, r0 = (32, 32, 48), 2.19, 0.0
+    rng = np.random.default_rng(9)
+    c = _plane_case(rng, shape, cell, r0, 25)
+    d, e = _field(shape, 4), _field(shape, 5)
+    box = forest.BoxView.plane(torch.from_numpy(0.5 * d).to(cuda), torch.from_numpy(0.3 * e).to(cuda), cell=cell,
+                               r0=r0)
+    pset = rng.integers(0, 3, len(c["npix"])).astype(np.int32)
+    prm = np.array([[0.1, 1.5, 1.0, 1.0], [0.3, 1.2, 0.7, 0.9], [0.02, 1.8, 1.3, 1.1]])
+    b = forest.SkewerBatch(forest.GEOM_PLANE, c["g0"], c["npix"], c["rmin"], c["dr"], c["qx"], c["qy"],
+                           imin=c["imin"], imax=c["imax"], pset=pset)
+    ds = torch.from_numpy(rng.normal(0, 0.2, b.npix_total).astype(np.float32)).to(cuda)
+    abcg = forest.fgpa_rows(*prm.T)
+    out = forest.gather(b, box, interp="trilinear", abcg=abcg, delta_s=ds)
+    dl = out["delta_l"].double().cpu().numpy()
+    eta = out["eta_par"].double().cpu().numpy()
+    rows = np.concatenate([np.full(n, pset[s]) for s, n in enumerate(c["npix"])])
+    p = prm.astype(np.float32).astype(np.float64)[rows]
+    want = fo.fgpa(dl + ds.double().cpu().numpy(), eta, p[:, 3], p[:, 0], p[:, 1], p[:, 2])
+    got = out["flux"].double().cpu().numpy()
+    assert np.abs(got - want).max() < 2e-6
+    m = _forest_mask(c)
+    assert np.all(got[~m] == 1.0)

@@ -1,0 +1,142 @@
+"""Pins the FP64 forest oracle with closed forms and brute force (CPU)."""
+import numpy as np
+import pytest
+from scipy import integrate
+
+from oracle import forest_oracle as fo
+
+
+def test_cosmo_against_quadrature():
+    c = fo.Cosmo()
+    Om, H0 = fo.OMEGA_M, 100.0 * fo.H
+
+    def chi(z):
+        return integrate.quad(lambda x: fo.C_KMS / (H0 * np.sqrt(Om * (1 + x) ** 3 + 1 - Om)), 0, z)[0]
+
+    for z in (0.5, 1.7, 2.4, 3.6):
+        assert abs(c.r_comoving(z) / chi(z) - 1) < 1e-6
+        assert abs(c.r_2_z(c.r_comoving(z)) - z) < 1e-9
+
+
+def test_product_cosmo_matches_oracle_bitwise():
+    from saclaymocks_amd import util
+    a, b = fo.Cosmo(), util.Cosmo()
+    assert np.array_equal(a.chi, b.chi)
+    z = np.linspace(1.3, 3.6, 101)
+    assert np.array_equal(a.r_comoving(z), b.r_comoving(z))
+
+
+def test_growth():
+    assert fo.fgrowth(0.0) == pytest.approx(1.0)
+    # growth decreases with z, ~1/(1+z) in matter domination
+    g = fo.fgrowth(np.array([1.0, 2.0, 3.0]))
+    assert np.all(np.diff(g) < 0)
+    assert fo.fgrowth(10.0) * 11 == pytest.approx(fo.fgrowth(20.0) * 21, rel=2e-2)
+
+
+def test_compute_xyz2_frame():
+    rng = np.random.default_rng(0)
+    ra, dec = rng.uniform(0, 2 * np.pi, 50), rng.uniform(-1.2, 1.2, 50)
+    ra0, dec0 = 0.7, 0.3
+    x, y, z = fo.compute_xyz2(ra, dec, 1000.0, ra0, dec0)
+    assert np.allclose(np.sqrt(x * x + y * y + z * z), 1000.0)
+    x0, y0, z0 = fo.compute_xyz2(ra0, dec0, 5.0, ra0, dec0)
+    assert abs(x0) < 1e-12 and abs(y0) < 1e-12 and abs(z0 - 5.0) < 1e-12
+
+
+def _box(shape, seed=0):
+    return np.random.default_rng(seed).standard_normal(shape).astype(np.float32)
+
+
+def test_gauss343_constant_and_bruteforce():
+    shape = (12, 10, 14)
+    D, R0 = 2.0, 100.0
+    L = tuple(D * n for n in shape)
+    const = np.full(shape, 0.37, np.float32)
+    X = np.array([0.3, -1.1]); Y = np.array([0.2, 1.7]); Z = np.array([R0 + 0.4, R0 - 2.2])
+    (v,), _ = fo.gauss343([const], X, Y, Z, L, (D, D, D), R0)
+    assert np.allclose(v, np.float32(0.37), rtol=0, atol=1e-15)
+    f = _box(shape)
+    (v,), (ix, iy, iz) = fo.gauss343([f], X, Y, Z, L, (D, D, D), R0)
+    for p in range(2):
+        num = den = 0.0
+        for a in range(-3, 4):
+            for b in range(-3, 4):
+                for c in range(-3, 4):
+                    cc = np.array([(ix[p] + a + 0.5) * D - L[0] / 2, (iy[p] + b + 0.5) * D - L[1] / 2,
+                                   (iz[p] + c + 0.5) * D - L[2] / 2 + R0])
+                    w = np.exp(-np.sum((cc - [X[p], Y[p], Z[p]]) ** 2) / (2 * D * D))
+                    num += w * f[ix[p] + a, iy[p] + b, iz[p] + c]
+                    den += w
+        assert abs(v[p] - num / den) < 1e-12
+
+
+def test_cell_index_known_point():
+    D, R0 = 2.19, 3000.0
+    shape = (64, 64, 64)
+    L = tuple(D * n for n in shape)
+    # centre of cell (10, 20, 30)
+    X = (10 + 0.5) * D - L[0] / 2
+    Y = (20 + 0.5) * D - L[1] / 2
+    Z = (30 + 0.5) * D - L[2] / 2 + R0
+    ix, iy, iz = fo.cell_index(np.array([X]), np.array([Y]), np.array([Z]), L, (D, D, D), R0)
+    assert (ix[0], iy[0], iz[0]) == (10, 20, 30)
+
+
+def test_trilinear_exact_on_linear_field():
+    shape = (16, 16, 16)
+    D = 1.0
+    L = tuple(D * n for n in shape)
+    i, j, k = np.meshgrid(*[np.arange(n) for n in shape], indexing="ij")
+    f = (0.5 * i - 0.25 * j + 0.125 * k).astype(np.float32)
+    rng = np.random.default_rng(2)
+    # interior points (no wrap): cell-centre coordinate u in [1, 13]
+    u = rng.uniform(1, 13, (3, 40))
+    X, Y, Z = ((u[a] + 0.5) * D - L[a] / 2 for a in range(3))
+    (v,) = fo.trilinear([f], X, Y, Z, L, (D, D, D), 0.0)
+    assert np.allclose(v, 0.5 * u[0] - 0.25 * u[1] + 0.125 * u[2], atol=1e-12)
+
+
+def test_missing_power_single_mode():
+    nz, m, pixel = 512, 17, 0.2
+    n = np.arange(nz)
+    w = np.cos(2 * np.pi * m * n / nz)
+    out = fo.missing_power(w, 300, pixel, fo.p1d_missing_analytic)
+    k = m * 2 * np.pi / (nz * pixel)
+    want = w[:300] * np.sqrt(fo.p1d_missing_analytic(k) / pixel)
+    assert np.allclose(out, want, atol=1e-12)
+    assert fo.fft_len_for(462) == 512 and fo.fft_len_for(463) == 1024
+
+
+def test_fgpa_special_cases():
+    d = np.array([-0.5, 0.0, 1.3])
+    e = np.array([0.1, -0.2, 0.3])
+    assert np.allclose(fo.fgpa(d, e, 1.0, 0.0, 1.5, 1.0), 1.0)
+    assert np.allclose(fo.fgpa(d, e, 1.0, 0.3, 0.0, 1.0), np.exp(-0.3))
+    assert fo.fgpa(np.array([-1e6]), np.array([0.0]), 1.0, 0.3, 1.5, 1.0)[0] == 1.0
+
+
+def test_p1d_bruteforce_and_parseval():
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal(37)
+    p = fo.p1d_1spectrum(x, 0.2)
+    n = len(x)
+    dft = np.array([np.sum(x * np.exp(-2j * np.pi * j * np.arange(n) / n)) for j in range(n // 2 + 1)])
+    assert np.allclose(p, np.abs(dft) ** 2 * 0.2 / n)
+    w = np.full(len(p), 2.0); w[0] = 1.0
+    if n % 2 == 0:
+        w[-1] = 1.0
+    assert np.isclose(np.sum(w * p) * n / 0.2, n * np.sum(x * x))
+
+
+def test_profile_histo_by_hand():
+    x = np.array([0.05, 0.15, 0.16, 0.5, 1.0, 1.2])
+    y = np.array([1.0, 2.0, 4.0, 3.0, 5.0, 7.0])
+    edges = np.array([0.0, 0.1, 0.2, 0.6, 1.0])
+    mx, my, err, cnt, idx = fo.profile_histo(x, y, edges)
+    assert list(idx) == [0, 1, 2, 3]               # last bin closed: x = 1.0 is in it
+    assert np.allclose(my, [1.0, 3.0, 3.0, 5.0])
+    assert np.allclose(cnt, [1, 2, 1, 1])
+    assert np.allclose(err[1], np.sqrt(((4 + 16) / 2 - 9)) / np.sqrt(2))
+    s = fo.binned_sums([y / 3.0 * 0 + 1.0], 0.2, edges, 1.0)  # constant flux: all P = 0 except none
+    assert s.shape == (4, 4)
