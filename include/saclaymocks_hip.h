@@ -201,6 +201,46 @@ int smk_forest_missing_power(smk_forest_plan* plan, const smk_skewers* sk, const
                              const int64_t* filt_off, uint64_t seed, const float* noise,
                              const int64_t* noise_off, void* workspace, float* delta_s, void* stream);
 
+/* ------------------------------------------------------------- FGPA fit */
+/* The fit kernels need skewers.pset (one parameter row per set, e.g. a
+ * redshift bin) and use the forest range [imin, imax) of every skewer, the
+ * unmasked pixels of fit_az.py:88-91. */
+
+/* comp_spectra (fit_az.py:245-256): F = exp(-a exp(b G (delta + c eta)))
+ * for every pixel (delta = delta_l + delta_s, eta may be NULL); fsum[s] =
+ * FP64 sum of F over the forest of skewer s. */
+int smk_fgpa_flux(const smk_skewers* sk, const float* delta, const float* eta, const float* abcg,
+                  int64_t abcg_rows, float* flux, double* fsum, void* stream);
+
+/* out[set][c] = sum over the skewers of that set of per_skewer[s][c], FP64,
+ * deterministic (mean flux, P1D bins and adjoint terms per set). */
+int smk_reduce_by_set(const smk_skewers* sk, const double* per_skewer, int width, int n_sets, double* out,
+                      void* stream);
+
+/* compute_p1d (fit_az.py:194-217): per skewer, delta_F = regroup(F, m) /
+ * <F>_set - 1 over the forest (util.py:29-37), P_j = |rfft(delta_F)_j|^2 px / N
+ * with px = m * pixel (powerspectrum.py:204-209), k_j = rfftfreq(N) 2 pi / px,
+ * then per bin b of `edges` (scipy binned_statistic: [e_b, e_b+1), last bin
+ * closed): partial[s][b] = (count, sum k, sum P, sum P^2), FP64.
+ *   fmean       device double per set
+ *   max_forest  longest forest (imax - imin) in the batch, <= 8192 * m
+ *   status      NULL or int32 per skewer, 1 if its forest exceeds max_forest */
+int smk_p1d_binned(const smk_skewers* sk, const float* flux, const double* fmean, double pixel, int regroup,
+                   const double* edges, int nbins, int max_forest, double* partial, int32_t* status,
+                   void* stream);
+
+/* Adjoint of the binned P1D and of the FGPA, for a loss L(<F>_set, P1D bins).
+ *   gp    device double [sets][nbins]: dL/dP for one P_j sample of bin b
+ *         (dL/d mean_b divided by the bin count)
+ * out[s][7] = (sum_q A_q F'_q, sum_n A_q(n) dF_n/d(a, b, c) / m,
+ *              sum_n dF_n/d(a, b, c)) with A_q = dL/d delta_F,q at fixed <F>;
+ * the host adds the <F> chain rule per set.  No counterpart in the
+ * reference, whose fit is derivative-free (fit_az.py:266-279). */
+int smk_p1d_adjoint(const smk_skewers* sk, const float* delta, const float* eta, const float* abcg,
+                    int64_t abcg_rows, const float* flux, const double* fmean, double pixel, int regroup,
+                    const double* edges, int nbins, int max_forest, const double* gp, double* out,
+                    void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,211 @@
+"""FGPA parameter fit on the GPU: the fit_az.Fitter path (comp_spectra ->
+compute_p1d -> chi2, fit_az.py:194-279) with device kernels, plus a
+hand-written adjoint so a(z), b(z), c(z) can be fitted with gradients.
+
+The reference tunes `a` per redshift with a derivative-free iminuit migrad
+(fit_az.py:266-279).  Here both the plain (derivative-free, scipy Powell) and
+the gradient (L-BFGS-B on adjoint gradients) fits run on the same device
+forward model, so their wall times compare like for like.
+
+Loss over parameter sets s (redshift bins), with targets F_t and P_t:
+    chi2 = sum_s ((<F>_s - F_t,s) / sF_s)^2
+         + sum_s sum_b ((P1D_s,b - P_t,s,b) / sP_s,b)^2
+where <F>_s is the mean flux over the forest pixels of set s and P1D_s,b the
+binned mean of |rfft(F/<F>_s - 1)|^2 px / N (MakeProfileHisto, util.py:46-75).
+"""
+import ctypes
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+def _sync_host(t):
+    return t.detach().cpu().numpy()
+
+
+class P1DFitModel:
+    """Device state of one fit: skewer batch with a parameter set per skewer,
+    the fixed fields delta = delta_l + delta_s and eta_par, G per set, k bins."""
+
+    def __init__(self, batch, delta, eta, growth, edges, pixel=0.2, regroup=1):
+        if batch.pset is None:
+            raise ValueError("the batch needs a parameter set (pset) per skewer")
+        self.batch = batch
+        self.sk = batch.struct()
+        self.delta = delta.contiguous()
+        self.eta = None if eta is None else eta.contiguous()
+        self.growth = np.asarray(growth, dtype=np.float64)
+        self.n_sets = len(self.growth)
+        self.edges_host = np.asarray(edges, dtype=np.float64)
+        self.nb = len(self.edges_host) - 1
+        self.pixel = float(pixel)
+        self.regroup = int(regroup)
+        dev = delta.device
+        self.dev = dev
+        self.edges = torch.from_numpy(self.edges_host).to(dev)
+        imin = _sync_host(batch.imin).astype(np.int64)
+        imax = _sync_host(batch.imax).astype(np.int64)
+        pset = _sync_host(batch.pset).astype(np.int64)
+        nf = imax - imin
+        self.max_forest = int(nf.max()) if len(nf) else 0
+        self.count = np.bincount(pset, weights=nf, minlength=self.n_sets).astype(np.float64)
+        if np.any(self.count <= 0):
+            raise ValueError("every parameter set needs forest pixels")
+        n = batch.n
+        self.flux = torch.empty(batch.npix_total, dtype=torch.float32, device=dev)
+        self.fsum = torch.empty(n, dtype=torch.float64, device=dev)
+        self.fsum_set = torch.empty(self.n_sets, dtype=torch.float64, device=dev)
+        self.partial = torch.empty((n, self.nb, 4), dtype=torch.float64, device=dev)
+        self.bins_set = torch.empty((self.n_sets, self.nb, 4), dtype=torch.float64, device=dev)
+        self.adj = torch.empty((n, 7), dtype=torch.float64, device=dev)
+        self.adj_set = torch.empty((self.n_sets, 7), dtype=torch.float64, device=dev)
+        self.fmean_dev = torch.empty(self.n_sets, dtype=torch.float64, device=dev)
+        self.status = torch.zeros(n, dtype=torch.int32, device=dev)
+        self.n_eval = 0
+        self.n_grad = 0
+
+    # ------------------------------------------------------------ forward
+    def _abcg(self, theta):
+        th = np.asarray(theta, dtype=np.float64).reshape(self.n_sets, 3)
+        rows = np.concatenate([th, self.growth[:, None]], axis=1).astype(np.float32)
+        return torch.from_numpy(rows).to(self.dev)
+
+    def forward(self, theta):
+        """<F> per set and binned P1D sums per set (host numpy)."""
+        L = _lib.lib()
+        s = _lib.stream_ptr()
+        abcg = self._abcg(theta)
+        rows = abcg.shape[0]
+        _lib.check(L.smk_fgpa_flux(ctypes.byref(self.sk), _lib.ptr(self.delta), _lib.ptr(self.eta),
+                                   _lib.ptr(abcg), rows, _lib.ptr(self.flux), _lib.ptr(self.fsum), s),
+                   "smk_fgpa_flux")
+        _lib.check(L.smk_reduce_by_set(ctypes.byref(self.sk), _lib.ptr(self.fsum), 1, self.n_sets,
+                                       _lib.ptr(self.fsum_set), s), "smk_reduce_by_set")
+        fmean = _sync_host(self.fsum_set) / self.count
+        self.fmean_dev.copy_(torch.from_numpy(fmean))
+        _lib.check(L.smk_p1d_binned(ctypes.byref(self.sk), _lib.ptr(self.flux), _lib.ptr(self.fmean_dev),
+                                    self.pixel, self.regroup, _lib.ptr(self.edges), self.nb, self.max_forest,
+                                    _lib.ptr(self.partial), _lib.ptr(self.status), s), "smk_p1d_binned")
+        _lib.check(L.smk_reduce_by_set(ctypes.byref(self.sk), _lib.ptr(self.partial), self.nb * 4, self.n_sets,
+                                       _lib.ptr(self.bins_set), s), "smk_reduce_by_set")
+        bins = _sync_host(self.bins_set)
+        self.n_eval += 1
+        self._last = (np.array(theta, dtype=np.float64), abcg, fmean, bins)
+        return fmean, bins
+
+    @staticmethod
+    def profile(bins):
+        """MakeProfileHisto from the per-bin sums: (meanx, meany, erry, count)."""
+        c = bins[..., 0]
+        with np.errstate(invalid="ignore", divide="ignore"):
+            mx = bins[..., 1] / c
+            my = bins[..., 2] / c
+            my2 = bins[..., 3] / c
+            err = np.sqrt(np.maximum(my2 - my * my, 0)) / np.sqrt(c)
+        return mx, my, err, c
+
+    # --------------------------------------------------------------- loss
+    def set_targets(self, f_target, f_sigma, p_target, p_sigma):
+        self.f_target = np.asarray(f_target, dtype=np.float64)
+        self.f_sigma = np.asarray(f_sigma, dtype=np.float64)
+        self.p_target = np.asarray(p_target, dtype=np.float64)
+        self.p_sigma = np.asarray(p_sigma, dtype=np.float64)
+        self.p_mask = np.isfinite(self.p_target) & np.isfinite(self.p_sigma) & (self.p_sigma > 0)
+
+    def chi2(self, theta):
+        fmean, bins = self.forward(theta)
+        _, my, _, c = self.profile(bins)
+        m = self.p_mask & (c > 0)
+        rF = (fmean - self.f_target) / self.f_sigma
+        rP = np.where(m, (np.nan_to_num(my) - self.p_target) / np.where(m, self.p_sigma, 1.0), 0.0)
+        return float(np.sum(rF * rF) + np.sum(rP * rP))
+
+    def value_and_grad(self, theta):
+        """chi2 and its gradient w.r.t. theta = (a, b, c) per set, by the adjoint kernels."""
+        val = self.chi2(theta)
+        return val, self.grad_last()
+
+    def grad_last(self):
+        """Adjoint gradient at the parameters of the last forward()."""
+        theta, abcg, fmean, bins = self._last
+        _, my, _, c = self.profile(bins)
+        m = self.p_mask & (c > 0)
+        dmy = np.where(m, 2.0 * (np.nan_to_num(my) - self.p_target) / np.where(m, self.p_sigma, 1.0) ** 2, 0.0)
+        gp = np.where(c > 0, dmy / np.where(c > 0, c, 1.0), 0.0)
+        gF = 2.0 * (fmean - self.f_target) / self.f_sigma ** 2
+        L = _lib.lib()
+        s = _lib.stream_ptr()
+        gp_dev = torch.from_numpy(np.ascontiguousarray(gp)).to(self.dev)
+        _lib.check(L.smk_p1d_adjoint(ctypes.byref(self.sk), _lib.ptr(self.delta), _lib.ptr(self.eta),
+                                     _lib.ptr(abcg), abcg.shape[0], _lib.ptr(self.flux), _lib.ptr(self.fmean_dev),
+                                     self.pixel, self.regroup, _lib.ptr(self.edges), self.nb, self.max_forest,
+                                     _lib.ptr(gp_dev), _lib.ptr(self.adj), s), "smk_p1d_adjoint")
+        _lib.check(L.smk_reduce_by_set(ctypes.byref(self.sk), _lib.ptr(self.adj), 7, self.n_sets,
+                                       _lib.ptr(self.adj_set), s), "smk_reduce_by_set")
+        r = _sync_host(self.adj_set)
+        saf, t1, t2 = r[:, 0], r[:, 1:4], r[:, 4:7]
+        dLdF = gF - saf / fmean ** 2
+        grad = t1 / fmean[:, None] + (dLdF / self.count)[:, None] * t2
+        self.n_grad += 1
+        return grad.reshape(-1)
+
+
+class FGPAChi2(torch.autograd.Function):
+    """chi2(theta) with the adjoint kernels as backward (theta: float64 CPU tensor)."""
+
+    @staticmethod
+    def forward(ctx, theta, model):
+        ctx.model = model
+        val = model.chi2(theta.detach().cpu().numpy())
+        return torch.tensor(val, dtype=torch.float64)
+
+    @staticmethod
+    def backward(ctx, g):
+        grad = torch.from_numpy(ctx.model.grad_last())
+        return g * grad, None
+
+
+def fit_plain(model, theta0, bounds, maxfev=20000, xtol=1e-6, ftol=1e-9):
+    """Derivative-free fit (the reference's approach, fit_az.py:266-279) on the device forward."""
+    from scipy import optimize
+    t0 = time.perf_counter()
+    n0 = model.n_eval
+    res = optimize.minimize(model.chi2, np.asarray(theta0, dtype=np.float64), method="Powell", bounds=bounds,
+                            options={"maxfev": maxfev, "xtol": xtol, "ftol": ftol})
+    return {"x": res.x, "chi2": float(res.fun), "wall_s": time.perf_counter() - t0,
+            "n_eval": model.n_eval - n0, "message": str(res.message)}
+
+
+def fit_gradient(model, theta0, bounds, chi2_target=None, maxiter=500):
+    """L-BFGS-B on adjoint gradients; stops as soon as chi2 <= chi2_target when given."""
+    from scipy import optimize
+    t0 = time.perf_counter()
+    n0 = model.n_eval
+    best = {"chi2": np.inf, "x": None, "t": None}
+
+    class _Done(Exception):
+        pass
+
+    def fg(th):
+        th_t = torch.tensor(th, dtype=torch.float64, requires_grad=True)
+        val = FGPAChi2.apply(th_t, model)
+        val.backward()
+        v = float(val.item())
+        if v < best["chi2"]:
+            best.update(chi2=v, x=np.array(th), t=time.perf_counter() - t0)
+        if chi2_target is not None and v <= chi2_target:
+            raise _Done()
+        return v, th_t.grad.numpy().copy()
+
+    msg = "converged"
+    try:
+        res = optimize.minimize(fg, np.asarray(theta0, dtype=np.float64), jac=True, method="L-BFGS-B",
+                                bounds=bounds, options={"maxiter": maxiter, "ftol": 1e-12, "gtol": 1e-10})
+        msg = str(res.message)
+    except _Done:
+        msg = "reached target chi2"
+    return {"x": best["x"], "chi2": best["chi2"], "wall_s": time.perf_counter() - t0,
+            "time_to_best_s": best["t"], "n_eval": model.n_eval - n0, "message": msg}

@@ -1,0 +1,84 @@
+"""GPU parity of the FGPA-fit forward (mean flux, binned P1D) and adjoint gradient."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import fit_oracle as fto
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(cuda, seed=0, ns=3, per=40, m=1, lo=200, hi=900):
+    from saclaymocks_amd import fit, forest
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(lo, hi, ns * per)
+    # smooth-ish Gaussian fields along each skewer
+    deltas, etas = [], []
+    for n in lens:
+        w = np.convolve(rng.standard_normal(n + 8), np.ones(9) / 3.0, mode="valid")
+        deltas.append((0.8 * w).astype(np.float32))
+        etas.append((0.25 * rng.standard_normal(n)).astype(np.float32))
+    pset = np.repeat(np.arange(ns), per).astype(np.int32)
+    growth = np.array([1.0, 0.9, 0.8, 0.75, 0.7])[:ns]
+    edges = np.concatenate(([0.0, 1e-5, 0.03], np.arange(0.05, 0.5, 0.05), np.arange(0.5, 3, 0.1),
+                            np.arange(3, 15.5, 0.5)))
+    b = forest.SkewerBatch(forest.GEOM_PLANE, np.zeros(len(lens)), lens, 0.0, 0.2, np.zeros(len(lens)),
+                           np.zeros(len(lens)), pset=pset)
+    d = torch.from_numpy(np.concatenate(deltas)).to(cuda)
+    e = torch.from_numpy(np.concatenate(etas)).to(cuda)
+    model = fit.P1DFitModel(b, d, e, growth, edges, pixel=0.2, regroup=m)
+    theta_true = np.array([[0.2, 1.4, 0.9], [0.3, 1.2, 1.1], [0.15, 1.6, 0.7], [0.25, 1.5, 1.0],
+                           [0.35, 1.3, 0.8]])[:ns].reshape(-1)
+    d64 = [x.astype(np.float64) for x in deltas]
+    e64 = [x.astype(np.float64) for x in etas]
+    fm, bins, _ = fto.forward(d64, e64, pset, theta_true, growth, edges, 0.2, m)
+    c = bins[..., 0]
+    my = np.where(c > 0, bins[..., 2] / np.where(c > 0, c, 1), np.nan)
+    targets = (fm, np.full(ns, 0.005), my, 0.03 * np.abs(my) + 1e-4)
+    model.set_targets(*targets)
+    return model, (d64, e64, pset, growth, edges, 0.2, targets, m), theta_true
+
+
+@pytest.mark.parametrize("m", [1, 2])
+def test_forward_parity(cuda, m):
+    model, args, theta = _problem(cuda, m=m)
+    th = theta * np.array([1.1, 0.95, 1.05] * 3)
+    fmean, bins = model.forward(th)
+    d64, e64, pset, growth, edges, pixel, targets, mm = args
+    rf, rb, _ = fto.forward(d64, e64, pset, th, growth, edges, pixel, mm)
+    assert np.allclose(fmean, rf, rtol=2e-6)
+    assert np.array_equal(bins[..., 0], rb[..., 0])
+    assert np.allclose(bins[..., 1], rb[..., 1], rtol=1e-12)
+    assert np.allclose(bins[..., 2], rb[..., 2], rtol=1e-4, atol=1e-7 * rb[..., 2].max())
+    assert model.status.sum().item() == 0
+
+
+def test_adjoint_gradient_parity(cuda):
+    model, args, theta = _problem(cuda)
+    th = theta * np.array([1.2, 0.9, 1.1] * 3)
+    val, g = model.value_and_grad(th)
+    want = fto.adjoint_grad(th, *args)
+    fd = fto.grad_fd(th, args)
+    assert np.allclose(want, fd, rtol=1e-5, atol=1e-6 * np.abs(fd).max())
+    assert np.allclose(g, want, rtol=2e-3, atol=2e-3 * np.abs(want).max())
+    assert abs(val / fto.chi2(th, *args) - 1) < 1e-3
+
+
+def test_autograd_function_backward(cuda):
+    from saclaymocks_amd.fit import FGPAChi2
+    model, args, theta = _problem(cuda, ns=2, per=20)
+    th = torch.tensor(theta * 1.1, dtype=torch.float64, requires_grad=True)
+    v = FGPAChi2.apply(th, model)
+    v.backward()
+    want = fto.adjoint_grad(th.detach().numpy(), *args)
+    assert np.allclose(th.grad.numpy(), want, rtol=2e-3, atol=2e-3 * np.abs(want).max())
+
+
+def test_gradient_fit_recovers_parameters(cuda):
+    from saclaymocks_amd import fit
+    model, args, theta = _problem(cuda, ns=2, per=60)
+    th0 = theta * np.array([1.3, 0.85, 1.2] * 2)
+    bounds = [(1e-3, 3.0), (0.1, 4.0), (0.0, 4.0)] * 2
+    res = fit.fit_gradient(model, th0, bounds)
+    assert res["chi2"] < 1e-3
+    assert np.allclose(res["x"], theta, rtol=2e-2, atol=2e-2)
