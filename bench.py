@@ -21,6 +21,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
@@ -63,6 +65,92 @@ def cpu_baseline(cell, seconds=12.0):
                       f"x sqrt(P/Vcell), irfftn delta + eta_zz ({dt:.1f} s)"}
 
 
+def forest_bench(delta, eta, shape, cell, steps, warmup, n_skewers):
+    """200k uneven skewers on the resident box -> forest pixels/s."""
+    import torch
+    from saclaymocks_amd import forest, workloads
+    box = forest.BoxView.plane(delta, eta, cell=cell, r0=workloads.box_frame(shape, cell)["R0"], periodic=True)
+    batch, rows, _ = workloads.c3_batch(shape, cell, n=n_skewers)
+    plan = forest.ForestPlan()
+    filt, foff = forest.missing_power_filters(0.2, workloads.p1d_missing)
+    ds = torch.empty(batch.npix_total, dtype=torch.float32, device=delta.device)
+
+    def step(seed):
+        forest.missing_power(plan, batch, filt, foff, seed=seed, out=ds)
+        return forest.gather(batch, box, interp="trilinear", abcg=rows, delta_s=ds, want=("flux",))
+
+    for i in range(warmup):
+        step(100 + i)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for i in range(steps):
+        out = step(i)
+    e1.record()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1) / steps
+    f = out["flux"]
+    assert int(out["status"].sum().item()) == 0
+    npx = batch.npix_total
+    # algorithmic bytes: 4 z-columns x cells spanned x 2 fields x 4 B per skewer,
+    # 4 B of F per pixel, 32 B of descriptor per skewer
+    span = np.ceil(batch.npix_host * 0.2 / cell) + 1
+    alg = float(np.sum(4 * span * 2 * 4) + 4 * npx + 32 * batch.n)
+    return {"workload": f"{batch.n} plane-parallel skewers, 300-1500 px of 0.2 Mpc/h, trilinear gather of "
+                        "delta and eta_zz, missing power 0.05 exp(-(k/1.8)^2), FGPA at z_QSO",
+            "pixels": npx, "skewers": batch.n, "ms_per_step": ms, "pixels_per_s": npx * steps / dt,
+            "mean_flux": float(f.double().mean().item()),
+            "roofline": {"bound": "hbm", "alg_bytes": alg, "achieved": alg / (ms * 1e-3) / 1e9,
+                         "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS}}
+
+
+def fit_bench(delta, eta, shape, cell, per_bin, maxfev):
+    """15-parameter FGPA fit (5 z bins), plain derivative-free fit
+    vs adjoint-gradient fit on the same device forward model."""
+    from saclaymocks_amd import fit, forest, workloads
+    box = forest.BoxView.plane(delta, eta, cell=cell, r0=workloads.box_frame(shape, cell)["R0"], periodic=True)
+    edges = workloads.p1d_edges()
+    zb = np.array(workloads.FIT_Z_BINS)
+    # target realisation: b, c from etc/params.fits, a solved per bin for the
+    # mean-flux target; its P1D is the P1D target
+    tb, td, te, growth, table = workloads.c4_problem(box, shape, cell, per_bin=per_bin, seed=101)
+    tm = fit.P1DFitModel(tb, td, te, growth, edges)
+    f_target = workloads.mean_flux_target(zb)
+    a_true = workloads.solve_a_for_mean_flux(tm, table[:, 1:], f_target)
+    truth = np.concatenate([a_true[:, None], table[:, 1:]], axis=1)
+    _, tbins = tm.forward(truth.reshape(-1))
+    _, my, err, c = tm.profile(tbins)
+    del tm, td, te
+    # fit realisation
+    b, d, e, growth, _ = workloads.c4_problem(box, shape, cell, per_bin=per_bin, seed=202)
+    model = fit.P1DFitModel(b, d, e, growth, edges)
+    model.set_targets(f_target, np.full(len(zb), 0.002), np.where(c > 0, my, np.nan), np.where(c > 0, err, np.nan))
+    theta0 = np.tile([0.1, 1.5, 1.0], len(zb))
+    bounds = [(1e-7, 3.0), (0.1, 4.0), (0.0, 4.0)] * len(zb)
+    model.value_and_grad(theta0)  # warm-up of every kernel
+    plains = {m: fit.fit_plain(model, theta0, bounds, maxfev=maxfev, method=m) for m in ("powell", "fd-lbfgs")}
+    # the gradient fit has to reach the better of the plain fits' chi2
+    plain = min(plains.values(), key=lambda r: r["chi2"])
+    target = plain["chi2"] + 1e-6 * max(1.0, plain["chi2"])
+    grad = fit.fit_gradient(model, theta0, bounds, chi2_target=target)
+    return {"workload": f"{b.n} skewers in {len(zb)} z bins (zfix per bin), 15 params a,b,c(z); "
+                        "targets <F>=exp(-0.0025(1+z)^3.7) and the P1D of an independent realisation "
+                        "with b, c from etc/params.fits and a solved for the <F> target",
+            "truth": truth.tolist(),
+            "gradient_fit_x": None if grad["x"] is None else np.asarray(grad["x"]).reshape(-1, 3).tolist(),
+            "plain": {m: {"chi2": r["chi2"], "wall_s": r["wall_s"], "n_eval": r["n_eval"]}
+                      for m, r in plains.items()},
+            "plain_methods": {"powell": "scipy Powell (derivative-free)",
+                              "fd-lbfgs": "L-BFGS-B on 2-point finite-difference gradients (migrad-like)"},
+            "gradient": {"method": "L-BFGS-B on adjoint gradients", "chi2": grad["chi2"],
+                         "wall_s": grad["wall_s"], "n_eval": grad["n_eval"], "status": grad["message"]},
+            "target_chi2": target,
+            "reached_plain_chi2": bool(grad["chi2"] <= target),
+            "speedup": {m: r["wall_s"] / grad["wall_s"] for m, r in plains.items()}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +160,11 @@ def main():
     ap.add_argument("--cell", type=float, default=2.19)
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--skip-forest", action="store_true")
+    ap.add_argument("--skip-fit", action="store_true")
+    ap.add_argument("--forest-skewers", type=int, default=200000)
+    ap.add_argument("--fit-per-bin", type=int, default=4000)
+    ap.add_argument("--fit-maxfev", type=int, default=3000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: the box and forest stages are single-GPU here "
@@ -139,6 +232,13 @@ def main():
                 "pass_ms": {k: round(v, 3) for k, v in pass_ms.items()},
                 "timed_calls": ncalls, "sigma_delta_fp64": sigma},
     }
+    del ws
+    torch.cuda.empty_cache()
+    if not args.skip_forest:
+        out["forest"] = forest_bench(delta, eta, (nx, ny, nz), args.cell, args.steps, args.warmup,
+                                     args.forest_skewers)
+    if not args.skip_fit:
+        out["fit"] = fit_bench(delta, eta, (nx, ny, nz), args.cell, args.fit_per_bin, args.fit_maxfev)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cell)
     print(json.dumps(out), flush=True)

@@ -207,10 +207,13 @@ int smk_forest_missing_power(smk_forest_plan* plan, const smk_skewers* sk, const
  * unmasked pixels of fit_az.py:88-91. */
 
 /* comp_spectra (fit_az.py:245-256): F = exp(-a exp(b G (delta + c eta)))
- * for every pixel (delta = delta_l + delta_s, eta may be NULL); fsum[s] =
- * FP64 sum of F over the forest of skewer s. */
-int smk_fgpa_flux(const smk_skewers* sk, const float* delta, const float* eta, const float* abcg,
-                  int64_t abcg_rows, float* flux, double* fsum, void* stream);
+ * for every pixel (delta = delta_l + delta_s, eta may be NULL), FP64 as the
+ * reference (fit_az.py:101-110 promotes the fields to float64);
+ *   abcg   device double [rows][4] = (a, b, c, G), row = skewers.pset[s]
+ *   flux   device double per pixel
+ *   fsum   device double per skewer: sum of F over the forest. */
+int smk_fgpa_flux(const smk_skewers* sk, const float* delta, const float* eta, const double* abcg,
+                  int64_t abcg_rows, double* flux, double* fsum, void* stream);
 
 /* out[set][c] = sum over the skewers of that set of per_skewer[s][c], FP64,
  * deterministic (mean flux, P1D bins and adjoint terms per set). */
@@ -224,22 +227,27 @@ int smk_reduce_by_set(const smk_skewers* sk, const double* per_skewer, int width
  * closed): partial[s][b] = (count, sum k, sum P, sum P^2), FP64.
  *   fmean       device double per set
  *   max_forest  longest forest (imax - imin) in the batch, <= 8192 * m
- *   status      NULL or int32 per skewer, 1 if its forest exceeds max_forest */
-int smk_p1d_binned(const smk_skewers* sk, const float* flux, const double* fmean, double pixel, int regroup,
+ *   status      NULL or int32 per skewer, 1 if its forest exceeds max_forest
+ *   modes       NULL or device complex double: the rfft modes X_j of every
+ *               skewer, (N/2 + 1) of them at modes + 2 * modes_off[s], kept
+ *               for smk_p1d_adjoint */
+int smk_p1d_binned(const smk_skewers* sk, const double* flux, const double* fmean, double pixel, int regroup,
                    const double* edges, int nbins, int max_forest, double* partial, int32_t* status,
-                   void* stream);
+                   double* modes, const int64_t* modes_off, void* stream);
 
 /* Adjoint of the binned P1D and of the FGPA, for a loss L(<F>_set, P1D bins).
  *   gp    device double [sets][nbins]: dL/dP for one P_j sample of bin b
  *         (dL/d mean_b divided by the bin count)
+ *   modes NULL (recompute) or the modes saved by smk_p1d_binned at the same
+ *         parameters
  * out[s][7] = (sum_q A_q F'_q, sum_n A_q(n) dF_n/d(a, b, c) / m,
  *              sum_n dF_n/d(a, b, c)) with A_q = dL/d delta_F,q at fixed <F>;
  * the host adds the <F> chain rule per set.  No counterpart in the
  * reference, whose fit is derivative-free (fit_az.py:266-279). */
-int smk_p1d_adjoint(const smk_skewers* sk, const float* delta, const float* eta, const float* abcg,
-                    int64_t abcg_rows, const float* flux, const double* fmean, double pixel, int regroup,
-                    const double* edges, int nbins, int max_forest, const double* gp, double* out,
-                    void* stream);
+int smk_p1d_adjoint(const smk_skewers* sk, const float* delta, const float* eta, const double* abcg,
+                    int64_t abcg_rows, const double* flux, const double* fmean, double pixel, int regroup,
+                    const double* edges, int nbins, int max_forest, const double* modes,
+                    const int64_t* modes_off, const double* gp, double* out, void* stream);
 
 #ifdef __cplusplus
 }

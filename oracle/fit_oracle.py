@@ -107,8 +107,10 @@ def adjoint_grad(theta, deltas, etas, pset, growth, edges, pixel, targets, m=1):
         ok = (bi >= 0) & (bi < nb)
         cj = np.zeros(len(k), dtype=complex)
         cj[ok] = gp[s, bi[ok]] * 2 * px / nq * np.conj(X[ok])
-        n = np.arange(nq)
-        A = np.real(np.exp(-2j * np.pi * np.outer(n, np.arange(len(k))) / nq) @ cj)
+        # A_n = Re sum_j cj_j exp(-2 pi i j n / N): a length-N FFT of the zero-padded cj
+        pad = np.zeros(nq, dtype=complex)
+        pad[:len(cj)] = cj
+        A = np.real(np.fft.fft(pad))
         saf[s] += np.sum(A * fr)
         An = np.repeat(A, m)
         t1[s] += (An[:, None] * dF[:nq * m]).sum(0) / m

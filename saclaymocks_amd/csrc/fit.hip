@@ -34,27 +34,30 @@ __device__ __forceinline__ double block_reduce_sum(double v, double* red) {
   return s;  // valid on every thread
 }
 
-__device__ __forceinline__ float4 prow(const float* abcg, int64_t rows, int64_t r) {
-  if (r < 0 || r >= rows) return make_float4(__int_as_float(0x7fc00000), 0.f, 0.f, 0.f);
-  return reinterpret_cast<const float4*>(abcg)[r];
+__device__ __forceinline__ double4 prow(const double* abcg, int64_t rows, int64_t r) {
+  if (r < 0 || r >= rows) return make_double4(__longlong_as_double(0x7ff8000000000000ll), 0.0, 0.0, 0.0);
+  const double* q = abcg + 4 * r;
+  return make_double4(q[0], q[1], q[2], q[3]);
 }
 
-// F for every pixel; per-skewer FP64 sum of F over the forest [imin, imax)
+// F for every pixel, in FP64 so that chi2(theta) is smooth to FP64 rounding
+// (the optimisers' line searches need that); per-skewer sum of F over the
+// forest [imin, imax)
 __global__ __launch_bounds__(FIT_THREADS) void k_fgpa_flux(smk_skewers sk, const float* __restrict__ delta,
                                                            const float* __restrict__ eta,
-                                                           const float* __restrict__ abcg, int64_t rows,
-                                                           float* __restrict__ flux, double* __restrict__ fsum) {
+                                                           const double* __restrict__ abcg, int64_t rows,
+                                                           double* __restrict__ flux, double* __restrict__ fsum) {
   __shared__ double red[FIT_THREADS / 64];
   for (int s = blockIdx.x; s < sk.n; s += gridDim.x) {
     const int64_t o = sk.off[s];
     const int n = sk.npix[s], i0 = sk.imin[s], i1 = sk.imax[s];
-    const float4 p = prow(abcg, rows, sk.pset[s]);
+    const double4 p = prow(abcg, rows, sk.pset[s]);
     double acc = 0.0;
     for (int i = threadIdx.x; i < n; i += FIT_THREADS) {
-      const float g = delta[o + i] + p.z * (eta ? eta[o + i] : 0.0f);
-      const float f = expf(-p.x * expf(p.y * p.w * g));
+      const double g = (double)delta[o + i] + p.z * (eta ? (double)eta[o + i] : 0.0);
+      const double f = exp(-p.x * exp(p.y * p.w * g));
       flux[o + i] = f;
-      if (i >= i0 && i < i1) acc += (double)f;
+      if (i >= i0 && i < i1) acc += f;
     }
     const double tot = block_reduce_sum(acc, red);
     if (threadIdx.x == 0) fsum[s] = tot;
@@ -92,34 +95,81 @@ __device__ __forceinline__ int find_bin(const double* __restrict__ edges, int nb
 }
 
 // delta_F of the (regrouped) forest into LDS; returns the regrouped length
-__device__ __forceinline__ int load_delta_f(const float* __restrict__ flux, int64_t o, int i0, int i1, int m,
+__device__ __forceinline__ int load_delta_f(const double* __restrict__ flux, int64_t o, int i0, int i1, int m,
                                            double inv_fmean, double* __restrict__ x) {
   const int nf = i1 - i0;
   const int nq = nf / m;
   for (int q = threadIdx.x; q < nq; q += FIT_THREADS) {
     double acc = 0.0;
-    for (int r = 0; r < m; ++r) acc += (double)flux[o + i0 + q * m + r];
+    for (int r = 0; r < m; ++r) acc += flux[o + i0 + q * m + r];
     x[q] = (acc / m) * inv_fmean - 1.0;
   }
   return nq;
 }
 
-// X_j = sum_n x_n exp(-2 pi i j n / N) by Horner in FP64
-__device__ __forceinline__ double2 dft_mode(const double* __restrict__ x, int N, int j) {
-  double s, c;
-  sincospi(-2.0 * (double)j / (double)N, &s, &c);
-  double re = 0.0, im = 0.0;
+// S_r = sum_{n<N} y_n w_r^n, w_r = exp(i th_r), for R frequencies at once:
+// R independent FP64 Horner chains share each LDS read of y_n, so the
+// latency of one chain is covered by the others.
+template <int R, bool CPLX>
+__device__ __forceinline__ void horner_multi(const double* __restrict__ yr, const double2* __restrict__ yc, int N,
+                                             const double* c, const double* s, double* re, double* im) {
+#pragma unroll
+  for (int r = 0; r < R; ++r) { re[r] = 0.0; im[r] = 0.0; }
   for (int n = N - 1; n >= 0; --n) {
-    const double r2 = re * c - im * s + x[n];
-    im = re * s + im * c;
-    re = r2;
+    double xr, xi = 0.0;
+    if (CPLX) { xr = yc[n].x; xi = yc[n].y; } else { xr = yr[n]; }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const double r2 = fma(re[r], c[r], fma(-im[r], s[r], xr));
+      im[r] = fma(re[r], s[r], fma(im[r], c[r], xi));
+      re[r] = r2;
+    }
   }
-  return make_double2(re, im);
+}
+
+// For frequencies f = f0 + r*FIT_THREADS (r < R), f < nf:
+// out_r = sum_{n<N} y_n exp(-2 pi i f n / P); calls `emit(f, re, im)`.
+template <int R, bool CPLX, class Emit>
+__device__ __forceinline__ void dft_group(const double* yr, const double2* yc, int N, int P, int f0, int nf,
+                                          Emit emit) {
+  double c[R], s[R], re[R], im[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int f = f0 + r * FIT_THREADS;
+    if (f < nf) {
+      sincospi(-2.0 * (double)f / (double)P, &s[r], &c[r]);
+    } else {
+      s[r] = 0.0;
+      c[r] = 1.0;
+    }
+  }
+  horner_multi<R, CPLX>(yr, yc, N, c, s, re, im);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int f = f0 + r * FIT_THREADS;
+    if (f < nf) emit(f, re[r], im[r]);
+  }
+}
+
+// every frequency f < nf, FIT_THREADS * (up to 4) per sweep
+template <bool CPLX, class Emit>
+__device__ __forceinline__ void dft_all(const double* yr, const double2* yc, int N, int P, int nf, Emit emit) {
+  for (int base = 0; base < nf; base += 4 * FIT_THREADS) {
+    const int left = nf - base;
+    const int R = left > 3 * FIT_THREADS ? 4 : left > 2 * FIT_THREADS ? 3 : left > FIT_THREADS ? 2 : 1;
+    const int f0 = base + threadIdx.x;
+    switch (R) {
+      case 4: dft_group<4, CPLX>(yr, yc, N, P, f0, nf, emit); break;
+      case 3: dft_group<3, CPLX>(yr, yc, N, P, f0, nf, emit); break;
+      case 2: dft_group<2, CPLX>(yr, yc, N, P, f0, nf, emit); break;
+      default: dft_group<1, CPLX>(yr, yc, N, P, f0, nf, emit); break;
+    }
+  }
 }
 
 struct P1dArgs {
   smk_skewers sk;
-  const float* flux;
+  const double* flux;
   const double* fmean;   // per set
   double pixel;          // spectrum pixel (before regrouping)
   int regroup;           // m >= 1
@@ -128,6 +178,8 @@ struct P1dArgs {
   double* partial;       // [n][nb][4]: count, sum k, sum P, sum P^2
   int32_t* status;       // per skewer: 1 if the forest is longer than nmax
   int nmax;
+  double2* xj;           // NULL or the modes X_j of every skewer, at xj + xoff[s]
+  const int64_t* xoff;
 };
 
 __global__ __launch_bounds__(FIT_THREADS) void k_p1d(P1dArgs a) {
@@ -152,11 +204,12 @@ __global__ __launch_bounds__(FIT_THREADS) void k_p1d(P1dArgs a) {
     const double px = a.pixel * a.regroup;
     const double k_ny = M_PI / px;
     const int nj = nq > 0 ? nq / 2 + 1 : 0;
-    for (int j = threadIdx.x; j < nj; j += FIT_THREADS) {
-      const double2 X = dft_mode(x, nq, j);
-      pk[j] = (X.x * X.x + X.y * X.y) * px / nq;
+    double2* xj = a.xj ? a.xj + a.xoff[s] : nullptr;
+    dft_all<false>(x, nullptr, nq, nq, nj, [&](int j, double re, double im) {
+      pk[j] = (re * re + im * im) * px / nq;
       bins[j] = find_bin(a.edges, a.nb, kfreq(j, nq, k_ny));
-    }
+      if (xj) xj[j] = make_double2(re, im);
+    });
     __syncthreads();
     for (int b = threadIdx.x; b < a.nb; b += FIT_THREADS) {
       double c = 0.0, sk_ = 0.0, sp = 0.0, sp2 = 0.0;
@@ -180,9 +233,9 @@ struct AdjArgs {
   smk_skewers sk;
   const float* delta;
   const float* eta;
-  const float* abcg;
+  const double* abcg;
   int64_t rows;
-  const float* flux;
+  const double* flux;
   const double* fmean;
   double pixel;
   int regroup;
@@ -191,6 +244,8 @@ struct AdjArgs {
   const double* gp;      // [sets][nb]: dL/dP for one P_j sample of bin b
   double* out;           // [n][7]: sum A F', sum A dF/d(a,b,c) / m, sum dF/d(a,b,c)
   int nmax;
+  const double2* xj;     // NULL: recompute the modes; else those saved by k_p1d
+  const int64_t* xoff;
 };
 
 // Adjoint of the binned P1D through delta_F = F/<F> - 1 (at fixed <F>) and
@@ -219,23 +274,38 @@ __global__ __launch_bounds__(FIT_THREADS) void k_p1d_adjoint(AdjArgs a) {
     const double px = a.pixel * m;
     const double k_ny = M_PI / px;
     const int nj = nq > 0 ? nq / 2 + 1 : 0;
-    for (int j = threadIdx.x; j < nj; j += FIT_THREADS) {
+    const double* gps = a.gp + (int64_t)set * a.nb;
+    auto coef = [&](int j) -> double {
       const int b = find_bin(a.edges, a.nb, kfreq(j, nq, k_ny));
-      double2 c = make_double2(0.0, 0.0);
-      if (b >= 0) {
-        const double g = a.gp[(int64_t)set * a.nb + b] * 2.0 * px / nq;
-        if (g != 0.0) {
-          const double2 X = dft_mode(x, nq, j);
-          c = make_double2(g * X.x, -g * X.y);   // g conj(X_j)
-        }
+      return b >= 0 ? gps[b] * 2.0 * px / nq : 0.0;
+    };
+    if (a.xj) {
+      const double2* xs = a.xj + a.xoff[s];
+      for (int j = threadIdx.x; j < nj; j += FIT_THREADS) {
+        const double g = coef(j);
+        const double2 X = xs[j];
+        cj[j] = make_double2(g * X.x, -g * X.y);   // g conj(X_j)
       }
-      cj[j] = c;
+    } else {
+      dft_all<false>(x, nullptr, nq, nq, nj, [&](int j, double re, double im) {
+        const double g = coef(j);
+        cj[j] = make_double2(g * re, -g * im);
+      });
     }
     __syncthreads();
-    const float4 p = prow(a.abcg, a.rows, set);
+    // A_q = Re sum_j cj[j] W^{jq}: Horner over j with w = W^q, q < nq.  The
+    // thread that owns q accumulates A_q F'_q and then stores A_q over x[q]
+    // (x is not read by any other thread from here on).
+    double saf = 0.0;
+    dft_all<true>(nullptr, cj, nj, nq, nq, [&](int q, double re, double im) {
+      saf += re * (x[q] + 1.0) * fm;
+      x[q] = re;
+    });
+    __syncthreads();
+    const double* A = x;
+    const double4 p = prow(a.abcg, a.rows, set);
     const double pa = p.x, pb = p.y, pc = p.z, pg = p.w;
-    double saf = 0.0, t1a = 0.0, t1b = 0.0, t1c = 0.0, t2a = 0.0, t2b = 0.0, t2c = 0.0;
-    // pass over the forest pixels: A of the pixel's regroup cell, dF/dtheta
+    double t1a = 0.0, t1b = 0.0, t1c = 0.0, t2a = 0.0, t2b = 0.0, t2c = 0.0;
     for (int i = threadIdx.x; i < i1 - i0; i += FIT_THREADS) {
       const int q = i / m;
       const double d = a.delta[o + i0 + i];
@@ -248,20 +318,10 @@ __global__ __launch_bounds__(FIT_THREADS) void k_p1d_adjoint(AdjArgs a) {
       const double dfc = -pa * T * F * pb * pg * e;
       t2a += dfa; t2b += dfb; t2c += dfc;
       if (q < nq) {
-        // A_q = Re sum_j cj[j] W^{jq}, Horner in w = W^q
-        double sn, cs;
-        sincospi(-2.0 * (double)q / (double)nq, &sn, &cs);
-        double re = 0.0, im = 0.0;
-        for (int j = nj - 1; j >= 0; --j) {
-          const double r2 = re * cs - im * sn + cj[j].x;
-          im = re * sn + im * cs + cj[j].y;
-          re = r2;
-        }
-        const double A = re;
-        if (i % m == 0) saf += A * (x[q] + 1.0) * fm;   // A_q F'_q, once per regroup cell
-        t1a += A * dfa / m;
-        t1b += A * dfb / m;
-        t1c += A * dfc / m;
+        const double Aq = A[q] / m;
+        t1a += Aq * dfa;
+        t1b += Aq * dfb;
+        t1c += Aq * dfc;
       }
     }
     const double v[7] = {saf, t1a, t1b, t1c, t2a, t2b, t2c};
@@ -276,8 +336,8 @@ int grid_for(int n) { return n < 65536 ? (n > 0 ? n : 1) : 65536; }
 
 }  // namespace
 
-extern "C" int smk_fgpa_flux(const smk_skewers* sk, const float* delta, const float* eta, const float* abcg,
-                             int64_t abcg_rows, float* flux, double* fsum, void* stream) {
+extern "C" int smk_fgpa_flux(const smk_skewers* sk, const float* delta, const float* eta, const double* abcg,
+                             int64_t abcg_rows, double* flux, double* fsum, void* stream) {
   SMK_REQUIRE(sk != nullptr && delta != nullptr && abcg != nullptr && flux != nullptr && fsum != nullptr,
               SMK_ERR_ARG, "NULL argument");
   SMK_REQUIRE(sk->pset != nullptr, SMK_ERR_ARG, "the fit kernels need skewers.pset");
@@ -299,9 +359,9 @@ extern "C" int smk_reduce_by_set(const smk_skewers* sk, const double* per_skewer
   return SMK_OK;
 }
 
-extern "C" int smk_p1d_binned(const smk_skewers* sk, const float* flux, const double* fmean, double pixel,
+extern "C" int smk_p1d_binned(const smk_skewers* sk, const double* flux, const double* fmean, double pixel,
                               int regroup, const double* edges, int nbins, int max_forest, double* partial,
-                              int32_t* status, void* stream) {
+                              int32_t* status, double* modes, const int64_t* modes_off, void* stream) {
   SMK_REQUIRE(sk != nullptr && flux != nullptr && fmean != nullptr && edges != nullptr && partial != nullptr,
               SMK_ERR_ARG, "NULL argument");
   SMK_REQUIRE(sk->pset != nullptr, SMK_ERR_ARG, "the fit kernels need skewers.pset");
@@ -312,7 +372,8 @@ extern "C" int smk_p1d_binned(const smk_skewers* sk, const float* flux, const do
   int nmax = max_forest / regroup;
   nmax += nmax & 1;
   if (nmax < 2) nmax = 2;
-  P1dArgs a{*sk, flux, fmean, pixel, regroup, edges, nbins, partial, status, nmax};
+  SMK_REQUIRE(modes == nullptr || modes_off != nullptr, SMK_ERR_ARG, "modes need modes_off");
+  P1dArgs a{*sk, flux, fmean, pixel, regroup, edges, nbins, partial, status, nmax, (double2*)modes, modes_off};
   const size_t lds = p1d_lds_bytes(nmax);
   SMK_HIP_CHECK(hipFuncSetAttribute((const void*)k_p1d, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   hipLaunchKernelGGL(k_p1d, dim3(grid_for(sk->n)), dim3(FIT_THREADS), lds, (hipStream_t)stream, a);
@@ -320,10 +381,10 @@ extern "C" int smk_p1d_binned(const smk_skewers* sk, const float* flux, const do
   return SMK_OK;
 }
 
-extern "C" int smk_p1d_adjoint(const smk_skewers* sk, const float* delta, const float* eta, const float* abcg,
-                               int64_t abcg_rows, const float* flux, const double* fmean, double pixel,
-                               int regroup, const double* edges, int nbins, int max_forest, const double* gp,
-                               double* out, void* stream) {
+extern "C" int smk_p1d_adjoint(const smk_skewers* sk, const float* delta, const float* eta, const double* abcg,
+                               int64_t abcg_rows, const double* flux, const double* fmean, double pixel,
+                               int regroup, const double* edges, int nbins, int max_forest, const double* modes,
+                               const int64_t* modes_off, const double* gp, double* out, void* stream) {
   SMK_REQUIRE(sk != nullptr && delta != nullptr && abcg != nullptr && flux != nullptr && fmean != nullptr &&
                   edges != nullptr && gp != nullptr && out != nullptr,
               SMK_ERR_ARG, "NULL argument");
@@ -335,7 +396,9 @@ extern "C" int smk_p1d_adjoint(const smk_skewers* sk, const float* delta, const 
   int nmax = max_forest / regroup;
   nmax += nmax & 1;
   if (nmax < 2) nmax = 2;
-  AdjArgs a{*sk, delta, eta, abcg, abcg_rows, flux, fmean, pixel, regroup, edges, nbins, gp, out, nmax};
+  SMK_REQUIRE(modes == nullptr || modes_off != nullptr, SMK_ERR_ARG, "modes need modes_off");
+  AdjArgs a{*sk,  delta, eta,   abcg, abcg_rows, flux, fmean, pixel, regroup, edges,
+            nbins, gp,   out, nmax, (const double2*)modes, modes_off};
   const size_t lds = adj_lds_bytes(nmax);
   SMK_HIP_CHECK(
       hipFuncSetAttribute((const void*)k_p1d_adjoint, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

@@ -30,7 +30,7 @@ class P1DFitModel:
     """Device state of one fit: skewer batch with a parameter set per skewer,
     the fixed fields delta = delta_l + delta_s and eta_par, G per set, k bins."""
 
-    def __init__(self, batch, delta, eta, growth, edges, pixel=0.2, regroup=1):
+    def __init__(self, batch, delta, eta, growth, edges, pixel=0.2, regroup=1, save_modes=True):
         if batch.pset is None:
             raise ValueError("the batch needs a parameter set (pset) per skewer")
         self.batch = batch
@@ -55,7 +55,7 @@ class P1DFitModel:
         if np.any(self.count <= 0):
             raise ValueError("every parameter set needs forest pixels")
         n = batch.n
-        self.flux = torch.empty(batch.npix_total, dtype=torch.float32, device=dev)
+        self.flux = torch.empty(batch.npix_total, dtype=torch.float64, device=dev)
         self.fsum = torch.empty(n, dtype=torch.float64, device=dev)
         self.fsum_set = torch.empty(self.n_sets, dtype=torch.float64, device=dev)
         self.partial = torch.empty((n, self.nb, 4), dtype=torch.float64, device=dev)
@@ -64,14 +64,21 @@ class P1DFitModel:
         self.adj_set = torch.empty((self.n_sets, 7), dtype=torch.float64, device=dev)
         self.fmean_dev = torch.empty(self.n_sets, dtype=torch.float64, device=dev)
         self.status = torch.zeros(n, dtype=torch.int32, device=dev)
+        # rfft modes saved by the forward for the adjoint: N/2 + 1 per skewer
+        nj = np.where(nf // self.regroup > 0, nf // self.regroup // 2 + 1, 0).astype(np.int64)
+        moff = np.zeros(n, dtype=np.int64)
+        if n:
+            moff[1:] = np.cumsum(nj[:-1])
+        self.modes = torch.empty(2 * int(nj.sum()) + 2, dtype=torch.float64, device=dev) if save_modes else None
+        self.modes_off = torch.from_numpy(moff).to(dev) if save_modes else None
         self.n_eval = 0
         self.n_grad = 0
 
     # ------------------------------------------------------------ forward
     def _abcg(self, theta):
         th = np.asarray(theta, dtype=np.float64).reshape(self.n_sets, 3)
-        rows = np.concatenate([th, self.growth[:, None]], axis=1).astype(np.float32)
-        return torch.from_numpy(rows).to(self.dev)
+        rows = np.concatenate([th, self.growth[:, None]], axis=1)
+        return torch.from_numpy(np.ascontiguousarray(rows)).to(self.dev)
 
     def forward(self, theta):
         """<F> per set and binned P1D sums per set (host numpy)."""
@@ -88,7 +95,8 @@ class P1DFitModel:
         self.fmean_dev.copy_(torch.from_numpy(fmean))
         _lib.check(L.smk_p1d_binned(ctypes.byref(self.sk), _lib.ptr(self.flux), _lib.ptr(self.fmean_dev),
                                     self.pixel, self.regroup, _lib.ptr(self.edges), self.nb, self.max_forest,
-                                    _lib.ptr(self.partial), _lib.ptr(self.status), s), "smk_p1d_binned")
+                                    _lib.ptr(self.partial), _lib.ptr(self.status), _lib.ptr(self.modes),
+                                    _lib.ptr(self.modes_off), s), "smk_p1d_binned")
         _lib.check(L.smk_reduce_by_set(ctypes.byref(self.sk), _lib.ptr(self.partial), self.nb * 4, self.n_sets,
                                        _lib.ptr(self.bins_set), s), "smk_reduce_by_set")
         bins = _sync_host(self.bins_set)
@@ -142,7 +150,8 @@ class P1DFitModel:
         _lib.check(L.smk_p1d_adjoint(ctypes.byref(self.sk), _lib.ptr(self.delta), _lib.ptr(self.eta),
                                      _lib.ptr(abcg), abcg.shape[0], _lib.ptr(self.flux), _lib.ptr(self.fmean_dev),
                                      self.pixel, self.regroup, _lib.ptr(self.edges), self.nb, self.max_forest,
-                                     _lib.ptr(gp_dev), _lib.ptr(self.adj), s), "smk_p1d_adjoint")
+                                     _lib.ptr(self.modes), _lib.ptr(self.modes_off), _lib.ptr(gp_dev),
+                                     _lib.ptr(self.adj), s), "smk_p1d_adjoint")
         _lib.check(L.smk_reduce_by_set(ctypes.byref(self.sk), _lib.ptr(self.adj), 7, self.n_sets,
                                        _lib.ptr(self.adj_set), s), "smk_reduce_by_set")
         r = _sync_host(self.adj_set)
@@ -168,19 +177,57 @@ class FGPAChi2(torch.autograd.Function):
         return g * grad, None
 
 
-def fit_plain(model, theta0, bounds, maxfev=20000, xtol=1e-6, ftol=1e-9):
-    """Derivative-free fit (the reference's approach, fit_az.py:266-279) on the device forward."""
+def _to_u(theta, log_a):
+    u = np.array(theta, dtype=np.float64).reshape(-1, 3)
+    if log_a:
+        u[:, 0] = np.log(u[:, 0])
+    return u.reshape(-1)
+
+
+def _to_theta(u, log_a):
+    th = np.array(u, dtype=np.float64).reshape(-1, 3)
+    if log_a:
+        th[:, 0] = np.exp(th[:, 0])
+    return th.reshape(-1)
+
+
+def _u_bounds(bounds, log_a):
+    if bounds is None or not log_a:
+        return bounds
+    out = list(bounds)
+    for i in range(0, len(out), 3):
+        lo, hi = out[i]
+        out[i] = (np.log(lo) if lo and lo > 0 else None, np.log(hi) if hi else None)
+    return out
+
+
+def fit_plain(model, theta0, bounds, maxfev=20000, xtol=1e-6, ftol=1e-9, log_a=True, method="powell"):
+    """Fit without derivatives of the model, on the device forward (the
+    reference's approach, fit_az.py:266-279):
+      method "powell"     scipy Powell line searches
+      method "fd-lbfgs"   quasi-Newton on finite-difference gradients, like
+                          Minuit migrad (iminuit is not available here)
+    Optimises (ln a, b, c) per set when log_a."""
     from scipy import optimize
     t0 = time.perf_counter()
     n0 = model.n_eval
-    res = optimize.minimize(model.chi2, np.asarray(theta0, dtype=np.float64), method="Powell", bounds=bounds,
-                            options={"maxfev": maxfev, "xtol": xtol, "ftol": ftol})
-    return {"x": res.x, "chi2": float(res.fun), "wall_s": time.perf_counter() - t0,
+    fun = lambda u: model.chi2(_to_theta(u, log_a))  # noqa: E731
+    if method == "powell":
+        res = optimize.minimize(fun, _to_u(theta0, log_a), method="Powell", bounds=_u_bounds(bounds, log_a),
+                                options={"maxfev": maxfev, "xtol": xtol, "ftol": ftol})
+    elif method == "fd-lbfgs":
+        res = optimize.minimize(fun, _to_u(theta0, log_a), method="L-BFGS-B", jac="2-point",
+                                bounds=_u_bounds(bounds, log_a),
+                                options={"maxfun": maxfev, "ftol": 1e-14, "gtol": 1e-10, "eps": 1e-7})
+    else:
+        raise ValueError(f"unknown method {method}")
+    return {"x": _to_theta(res.x, log_a), "chi2": float(res.fun), "wall_s": time.perf_counter() - t0,
             "n_eval": model.n_eval - n0, "message": str(res.message)}
 
 
-def fit_gradient(model, theta0, bounds, chi2_target=None, maxiter=500):
-    """L-BFGS-B on adjoint gradients; stops as soon as chi2 <= chi2_target when given."""
+def fit_gradient(model, theta0, bounds, chi2_target=None, maxiter=500, log_a=True):
+    """L-BFGS-B on adjoint gradients (through FGPAChi2.backward); stops as soon
+    as chi2 <= chi2_target when given.  Optimises (ln a, b, c) per set when log_a."""
     from scipy import optimize
     t0 = time.perf_counter()
     n0 = model.n_eval
@@ -189,21 +236,26 @@ def fit_gradient(model, theta0, bounds, chi2_target=None, maxiter=500):
     class _Done(Exception):
         pass
 
-    def fg(th):
+    def fg(u):
+        th = _to_theta(u, log_a)
         th_t = torch.tensor(th, dtype=torch.float64, requires_grad=True)
         val = FGPAChi2.apply(th_t, model)
         val.backward()
         v = float(val.item())
+        g = th_t.grad.numpy().copy().reshape(-1, 3)
+        if log_a:
+            g[:, 0] *= th.reshape(-1, 3)[:, 0]     # dL/dln a = a dL/da
         if v < best["chi2"]:
             best.update(chi2=v, x=np.array(th), t=time.perf_counter() - t0)
         if chi2_target is not None and v <= chi2_target:
             raise _Done()
-        return v, th_t.grad.numpy().copy()
+        return v, g.reshape(-1)
 
     msg = "converged"
     try:
-        res = optimize.minimize(fg, np.asarray(theta0, dtype=np.float64), jac=True, method="L-BFGS-B",
-                                bounds=bounds, options={"maxiter": maxiter, "ftol": 1e-12, "gtol": 1e-10})
+        res = optimize.minimize(fg, _to_u(theta0, log_a), jac=True, method="L-BFGS-B",
+                                bounds=_u_bounds(bounds, log_a),
+                                options={"maxiter": maxiter, "ftol": 0.0, "gtol": 1e-10})
         msg = str(res.message)
     except _Done:
         msg = "reached target chi2"

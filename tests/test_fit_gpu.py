@@ -8,7 +8,7 @@ from oracle import fit_oracle as fto
 pytestmark = pytest.mark.gpu
 
 
-def _problem(cuda, seed=0, ns=3, per=40, m=1, lo=200, hi=900):
+def _problem(cuda, seed=0, ns=3, per=40, m=1, lo=200, hi=900, save_modes=True):
     from saclaymocks_amd import fit, forest
     rng = np.random.default_rng(seed)
     lens = rng.integers(lo, hi, ns * per)
@@ -26,7 +26,7 @@ def _problem(cuda, seed=0, ns=3, per=40, m=1, lo=200, hi=900):
                            np.zeros(len(lens)), pset=pset)
     d = torch.from_numpy(np.concatenate(deltas)).to(cuda)
     e = torch.from_numpy(np.concatenate(etas)).to(cuda)
-    model = fit.P1DFitModel(b, d, e, growth, edges, pixel=0.2, regroup=m)
+    model = fit.P1DFitModel(b, d, e, growth, edges, pixel=0.2, regroup=m, save_modes=save_modes)
     theta_true = np.array([[0.2, 1.4, 0.9], [0.3, 1.2, 1.1], [0.15, 1.6, 0.7], [0.25, 1.5, 1.0],
                            [0.35, 1.3, 0.8]])[:ns].reshape(-1)
     d64 = [x.astype(np.float64) for x in deltas]
@@ -46,22 +46,24 @@ def test_forward_parity(cuda, m):
     fmean, bins = model.forward(th)
     d64, e64, pset, growth, edges, pixel, targets, mm = args
     rf, rb, _ = fto.forward(d64, e64, pset, th, growth, edges, pixel, mm)
-    assert np.allclose(fmean, rf, rtol=2e-6)
+    assert np.allclose(fmean, rf, rtol=1e-12)
     assert np.array_equal(bins[..., 0], rb[..., 0])
     assert np.allclose(bins[..., 1], rb[..., 1], rtol=1e-12)
-    assert np.allclose(bins[..., 2], rb[..., 2], rtol=1e-4, atol=1e-7 * rb[..., 2].max())
+    assert np.allclose(bins[..., 2], rb[..., 2], rtol=1e-9, atol=1e-12 * rb[..., 2].max())
+    assert np.allclose(bins[..., 3], rb[..., 3], rtol=1e-9, atol=1e-12 * rb[..., 3].max())
     assert model.status.sum().item() == 0
 
 
-def test_adjoint_gradient_parity(cuda):
-    model, args, theta = _problem(cuda)
+@pytest.mark.parametrize("save_modes", [True, False])
+def test_adjoint_gradient_parity(cuda, save_modes):
+    model, args, theta = _problem(cuda, save_modes=save_modes, lo=100, hi=2500)
     th = theta * np.array([1.2, 0.9, 1.1] * 3)
     val, g = model.value_and_grad(th)
     want = fto.adjoint_grad(th, *args)
     fd = fto.grad_fd(th, args)
     assert np.allclose(want, fd, rtol=1e-5, atol=1e-6 * np.abs(fd).max())
-    assert np.allclose(g, want, rtol=2e-3, atol=2e-3 * np.abs(want).max())
-    assert abs(val / fto.chi2(th, *args) - 1) < 1e-3
+    assert np.allclose(g, want, rtol=1e-7, atol=1e-9 * np.abs(want).max())
+    assert abs(val / fto.chi2(th, *args) - 1) < 1e-10
 
 
 def test_autograd_function_backward(cuda):
@@ -71,7 +73,7 @@ def test_autograd_function_backward(cuda):
     v = FGPAChi2.apply(th, model)
     v.backward()
     want = fto.adjoint_grad(th.detach().numpy(), *args)
-    assert np.allclose(th.grad.numpy(), want, rtol=2e-3, atol=2e-3 * np.abs(want).max())
+    assert np.allclose(th.grad.numpy(), want, rtol=1e-7, atol=1e-9 * np.abs(want).max())
 
 
 def test_gradient_fit_recovers_parameters(cuda):
@@ -82,3 +84,5 @@ def test_gradient_fit_recovers_parameters(cuda):
     res = fit.fit_gradient(model, th0, bounds)
     assert res["chi2"] < 1e-3
     assert np.allclose(res["x"], theta, rtol=2e-2, atol=2e-2)
+    plain = fit.fit_plain(model, th0, bounds, maxfev=3000, method="fd-lbfgs")
+    assert plain["chi2"] < 1e-2
