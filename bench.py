@@ -164,7 +164,7 @@ def main():
     ap.add_argument("--skip-fit", action="store_true")
     ap.add_argument("--forest-skewers", type=int, default=200000)
     ap.add_argument("--fit-per-bin", type=int, default=4000)
-    ap.add_argument("--fit-maxfev", type=int, default=3000)
+    ap.add_argument("--fit-maxfev", type=int, default=10000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: the box and forest stages are single-GPU here "
