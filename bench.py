@@ -160,6 +160,10 @@ def main():
     ap.add_argument("--cell", type=float, default=2.19)
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--copy-probe-gib", type=float, default=0.0,
+                    help="run one plain device copy of this many GiB first (the calibration kernel of "
+                         "HBM byte counters)")
     ap.add_argument("--skip-forest", action="store_true")
     ap.add_argument("--skip-fit", action="store_true")
     ap.add_argument("--forest-skewers", type=int, default=200000)
@@ -176,12 +180,31 @@ def main():
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.copy_probe_gib > 0:
+        nb = int(args.copy_probe_gib * 2**30)
+        src = torch.ones(nb, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"copy probe: {nb} bytes read, {nb} bytes written", file=sys.stderr, flush=True)
+        del src, dst
+        torch.cuda.empty_cache()
     nx, ny, nz = args.shape
     cells = nx * ny * nz
     plan = BoxPlan(nx, ny, nz, cell=args.cell)
     ws = plan.alloc_workspace(dev)
     delta = torch.empty((nx, ny, nz), dtype=torch.float32, device=dev)
     eta = torch.empty((nx, ny, nz), dtype=torch.float32, device=dev)
+    tune = None
+    if not args.no_autotune:
+        # the FFTW_MEASURE analogue: pick each pass's kernel variant on these buffers
+        # (outside the timed region, as the reference's planning is)
+        t_tune = time.perf_counter()
+        chosen, table = plan.autotune(seed=args.seed + 999, delta=delta, aux_out=eta, workspace=ws, reps=2)
+        tune = {"chosen": chosen, "ms_per_variant": {k: [None if v is None else round(v, 3) for v in row]
+                                                      for k, row in table.items()},
+                "seconds": round(time.perf_counter() - t_tune, 2)}
     plan.enable_pass_timing(True)
 
     for i in range(args.warmup):
@@ -230,7 +253,8 @@ def main():
                 "model_GBps": MODEL_BYTES_PER_CELL * cells / (ms_per_step * 1e-3) / 1e9,
                 "model_frac": MODEL_BYTES_PER_CELL * cells / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBPS,
                 "pass_ms": {k: round(v, 3) for k, v in pass_ms.items()},
-                "timed_calls": ncalls, "sigma_delta_fp64": sigma},
+                "timed_calls": ncalls, "sigma_delta_fp64": sigma,
+                "variants": plan.get_variants(), "autotune": tune},
     }
     del ws
     torch.cuda.empty_cache()

@@ -105,6 +105,41 @@ int smk_grf_box(smk_box_plan* plan, uint64_t seed, const float* noise, const dou
 int smk_box_enable_pass_timing(smk_box_plan* plan, int enable);
 int smk_box_pass_times(smk_box_plan* plan, double* ms_sum, int nmax, int* ncalls, int reset);
 
+/* Kernel variants per pass kind.  The reference leaves this choice to FFTW's
+ * planner and its wisdom file (make_boxes.py:53-54,87, FFTW_MEASURE): here a
+ * strided pass picks its register tile, lines per block and LDS exchange
+ * form (variants 0..SMK_BOX_NVAR-1), a z-pencil pass its register tile
+ * (0..1).  Variant 0 is the default of every pass; results of all variants
+ * agree to FP32 rounding.  Kinds 0-4 are the passes of smk_grf_box; the
+ * drop-in steps use kinds 0, 1, 3, 4 (y and z axes) and 5, 6 (x axis). */
+#define SMK_PASS_Z_R2C 0
+#define SMK_PASS_Y_FWD 1
+#define SMK_PASS_X_SPECTRAL 2
+#define SMK_PASS_Y_INV 3
+#define SMK_PASS_Z_C2R 4
+#define SMK_PASS_X_FWD 5
+#define SMK_PASS_X_INV 6
+#define SMK_PASS_COUNT 7
+#define SMK_BOX_NVAR 7
+
+/* 1 if variant `vid` exists for pass `kind` of this plan's sizes, else 0. */
+int smk_box_variant_valid(const smk_box_plan* plan, int kind, int vid);
+/* Select a variant (SMK_ERR_UNSUPPORTED if it does not exist) / read it. */
+int smk_box_set_variant(smk_box_plan* plan, int kind, int vid);
+int smk_box_get_variant(const smk_box_plan* plan, int kind);
+
+/* Autotuner of the fused job (the counterpart of FFTW_MEASURE planning):
+ * runs smk_grf_box `reps` + 1 times per variant index (the first untimed)
+ * with the same arguments, times each pass with HIP events, and keeps for
+ * each of kinds 0-4 the fastest valid variant.  The outputs then hold a
+ * valid result of the call.  ms_out (host, may be NULL) receives
+ * [5][SMK_BOX_NVAR] mean milliseconds per call (-1 for a variant that does
+ * not exist); the passes of kinds 3 and 4 count their delta and aux runs.
+ * Pass timing is left disabled and its sums cleared. */
+int smk_box_autotune(smk_box_plan* plan, uint64_t seed, const float* noise, const double* pk_params, int aux_kind,
+                     double aux_coef, float* delta_out, float* aux_out, void* workspace, double* var_out,
+                     int* status_out, void* stream, int reps, double* ms_out);
+
 /* DrawGRF_boxk (make_boxes.py:40-72): unnormalised forward R2C FFT of the
  * white noise into boxk, complex64 [nx][ny][nz/2+1].  noise == NULL draws
  * the Philox noise of `seed` on the device. */

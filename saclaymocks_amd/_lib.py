@@ -51,6 +51,11 @@ _SIGNATURES = {
     "smk_draw_grf_boxk": (_i, [_vp, _u64, _vp, _vp, _vp]),
     "smk_box_enable_pass_timing": (_i, [_vp, _i]),
     "smk_box_pass_times": (_i, [_vp, _c.POINTER(_d), _i, _c.POINTER(_i), _i]),
+    "smk_box_variant_valid": (_i, [_vp, _i, _i]),
+    "smk_box_set_variant": (_i, [_vp, _i, _i]),
+    "smk_box_get_variant": (_i, [_vp, _i]),
+    "smk_box_autotune": (_i, [_vp, _u64, _vp, _c.POINTER(_d), _i, _d, _vp, _vp, _vp, _vp, _vp, _vp, _i,
+                              _c.POINTER(_d)]),
     "smk_fft_and_store": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "smk_fft_and_store_workspace_bytes": (_sz, [_vp]),
     "smk_forest_plan_create": (_i, [_c.POINTER(_vp)]),

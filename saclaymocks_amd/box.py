@@ -66,23 +66,49 @@ class BoxPlan:
         k = max(n.value, 1)
         return {name: ms[i] / k for i, name in enumerate(self.PASS_NAMES)}, n.value
 
+    # pass kinds of smk_box_set_variant (SMK_PASS_*)
+    PASS_KINDS = ("z_r2c", "y_fwd", "x_spectral", "y_inv", "z_c2r", "x_fwd", "x_inv")
+    NVAR = 7
+
+    def variants(self):
+        """{kind: [valid variant ids]} for this plan's sizes."""
+        L = _lib.lib()
+        return {k: [v for v in range(self.NVAR) if L.smk_box_variant_valid(self._h, i, v)]
+                for i, k in enumerate(self.PASS_KINDS)}
+
+    def set_variant(self, kind, vid):
+        _lib.check(_lib.lib().smk_box_set_variant(self._h, self.PASS_KINDS.index(kind), int(vid)),
+                   "smk_box_set_variant")
+
+    def get_variants(self):
+        L = _lib.lib()
+        return {k: int(L.smk_box_get_variant(self._h, i)) for i, k in enumerate(self.PASS_KINDS)}
+
+    def autotune(self, seed=0, noise=None, pk=PK_DEFAULT, aux="eta_zz", aux_coef=1.0, delta=None,
+                 aux_out=None, workspace=None, reps=2, stream=None):
+        """Time every variant of the fused job's passes on these buffers and keep
+        the fastest per pass (smk_box_autotune).  Returns ({kind: chosen id},
+        {kind: [ms per variant or None]}); the buffers then hold a valid box."""
+        kind = _lib.AUX_KINDS[aux]
+        delta, aux_out, workspace = self._buffers(noise, kind, delta, aux_out, workspace)
+        dev = delta.device
+        var = torch.zeros(1, dtype=torch.float64, device=dev)
+        status = torch.zeros(2, dtype=torch.int32, device=dev)
+        pkp = (ctypes.c_double * 4)(*[float(x) for x in pk])
+        ms = (ctypes.c_double * (5 * self.NVAR))()
+        rc = _lib.lib().smk_box_autotune(self._h, ctypes.c_uint64(int(seed) & (2**64 - 1)), _lib.ptr(noise), pkp,
+                                         kind, float(aux_coef), _lib.ptr(delta),
+                                         _lib.ptr(aux_out if kind else None), _lib.ptr(workspace), _lib.ptr(var),
+                                         _lib.ptr(status), _lib.stream_ptr(stream), int(reps), ms)
+        _lib.check(rc, "smk_box_autotune")
+        table = {k: [ms[i * self.NVAR + v] if ms[i * self.NVAR + v] >= 0 else None for v in range(self.NVAR)]
+                 for i, k in enumerate(self.PASS_KINDS[:5])}
+        return self.get_variants(), table
+
     def workspace_bytes(self):
         return int(_lib.lib().smk_box_workspace_bytes(self._h))
 
-    def alloc_workspace(self, device="cuda"):
-        return torch.empty(self.workspace_bytes(), dtype=torch.uint8, device=device)
-
-    # ------------------------------------------------------------ fused job
-    def grf_box(self, seed=0, noise=None, pk=PK_DEFAULT, aux="eta_zz", aux_coef=1.0,
-                delta=None, aux_out=None, workspace=None, check=True, stream=None):
-        """delta (and the aux field, eta_zz by default) from white noise.
-
-        noise None -> N(0,1) drawn on the device from Philox(seed); else a
-        float32 CUDA tensor of the box shape (parity mode).  Returns
-        (delta, aux_field_or_None, variance_fp64_device_tensor, status_tensor).
-        With check=True the null/NaN guard of make_boxes.py:100-105 runs
-        (one device sync) and raises ValueError."""
-        kind = _lib.AUX_KINDS[aux]
+    def _buffers(self, noise, kind, delta, aux_out, workspace):
         dev = torch.device("cuda", torch.cuda.current_device())
         if noise is not None:
             _require_cuda(noise, "noise")
@@ -100,6 +126,24 @@ class BoxPlan:
             workspace = self.alloc_workspace(dev)
         if workspace.numel() * workspace.element_size() < self.workspace_bytes():
             raise ValueError("workspace too small")
+        return delta, aux_out, workspace
+
+    def alloc_workspace(self, device="cuda"):
+        return torch.empty(self.workspace_bytes(), dtype=torch.uint8, device=device)
+
+    # ------------------------------------------------------------ fused job
+    def grf_box(self, seed=0, noise=None, pk=PK_DEFAULT, aux="eta_zz", aux_coef=1.0,
+                delta=None, aux_out=None, workspace=None, check=True, stream=None):
+        """delta (and the aux field, eta_zz by default) from white noise.
+
+        noise None -> N(0,1) drawn on the device from Philox(seed); else a
+        float32 CUDA tensor of the box shape (parity mode).  Returns
+        (delta, aux_field_or_None, variance_fp64_device_tensor, status_tensor).
+        With check=True the null/NaN guard of make_boxes.py:100-105 runs
+        (one device sync) and raises ValueError."""
+        kind = _lib.AUX_KINDS[aux]
+        delta, aux_out, workspace = self._buffers(noise, kind, delta, aux_out, workspace)
+        dev = delta.device
         var = torch.zeros(1, dtype=torch.float64, device=dev)
         status = torch.zeros(2, dtype=torch.int32, device=dev)
         pkp = (ctypes.c_double * 4)(*[float(x) for x in pk])

@@ -36,21 +36,11 @@ namespace {
 
 // --------------------------------------------------------------- launch shapes
 // Threads per line P = N / E.  Lines per block T: strided passes use runs of
-// T consecutive kz (T*8 bytes per HBM access run); pencil passes just batch.
-constexpr int threads_per_line(int n) { return n / elems_per_thread(n); }
-
-constexpr int mode_E(int n, int mode) { return elems_per_thread(n); }
-
-constexpr int strided_T(int n, int e) {
-  int T = 8;
-  while (T > 1 && (T * (n / e) > 512 || T * lds_line_len(n) * 8 > 81920)) T >>= 1;
-  return T;
-}
-constexpr int strided_T(int n) { return strided_T(n, elems_per_thread(n)); }
-
-constexpr int pencil_T(int m) {
+// T consecutive lines (T*8 bytes per HBM access run, see strided_variant);
+// pencil passes just batch T pencils of E points per thread.
+constexpr int pencil_T(int m, int e) {
   int T = 16;
-  while (T > 1 && (T * threads_per_line(m) > 512 || T * lds_line_len(m) * 8 > 65536)) T >>= 1;
+  while (T > 1 && (T * (m / e) > 512 || T * lds_line_len(m) * 8 > 65536)) T >>= 1;
   return T;
 }
 
@@ -116,28 +106,83 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   return s;
 }
 
-template <int N, int E, int T, int MODE>
+// One line FFT through the block's LDS in the exchange form XCH
+// (0: float2 line, 1: float line, split exchange), line swizzle bits
+// [SWS, SWS+SWB) (see LdsMap).
+template <int N, int E, bool INV, int XCH, int SWS, int SWB>
+__device__ __forceinline__ void line_fft(float2 (&v)[E], float2* lds, uint32_t t, uint32_t p, const float2* tw,
+                                         uint32_t sw) {
+  constexpr int LL = lds_line_len(N);
+  if constexpr (XCH == 1) {
+    fft_line_split<N, E, INV, SWS, SWB>(v, reinterpret_cast<float*>(lds) + t * LL, p, tw, sw);
+  } else {
+    fft_line_e<N, E, INV, SWS, SWB>(v, lds + t * LL, p, tw, sw);
+  }
+}
+
+constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x / 2); }
+
+// Aux factor of one mode: eta_ij = k_i k_j / kk, v_i = -i k_i / kk * coef.
+// Operand selectors sa, sb: 0 = kx, 1 = ky, 2 = kz, 3 = 1.
+struct AuxSel {
+  int sa, sb;
+  bool vel;
+  float coef;
+};
+__device__ __forceinline__ AuxSel aux_sel(int ak, float aux_coef) {
+  AuxSel s;
+  s.sa = (ak == SMK_AUX_ETA_XX || ak == SMK_AUX_ETA_XY || ak == SMK_AUX_ETA_XZ || ak == SMK_AUX_VX) ? 0
+         : (ak == SMK_AUX_ETA_YY || ak == SMK_AUX_ETA_YZ || ak == SMK_AUX_VY) ? 1 : 2;
+  s.sb = (ak == SMK_AUX_ETA_XX) ? 0 : (ak == SMK_AUX_ETA_YY || ak == SMK_AUX_ETA_XY) ? 1 : (ak >= SMK_AUX_VX) ? 3 : 2;
+  s.vel = ak >= SMK_AUX_VX;
+  s.coef = s.vel ? aux_coef : 1.0f;
+  return s;
+}
+__device__ __forceinline__ float2 aux_apply(float2 d, const AuxSel& s, float kx, float ky, float kz, float kk,
+                                            bool origin, float scale) {
+  const float kkp = origin ? 1.0f : kk;
+  const float ka = s.sa == 0 ? kx : (s.sa == 1 ? ky : kz);
+  const float kb = s.sb == 0 ? kx : (s.sb == 1 ? ky : (s.sb == 2 ? kz : 1.0f));
+  const float2 x = cscale(d, ka * kb * __builtin_amdgcn_rcpf(kkp) * (s.coef * scale));
+  return s.vel ? mul_mi<false>(x) : x;
+}
+
+// T lines of N points per block, P = N / E threads per line, lanes ordered
+// line-fastest.  XCH selects the LDS exchange (float2 line, or float line
+// with the split exchange: half the LDS per block).  TILES (spectral mode
+// only) is the number of register tiles: 1 recomputes the spectrum for the
+// aux field (4 line FFTs), 2 keeps delta_k and aux_k side by side (3 FFTs).
+// SWS places the line swizzle of unpadded lines (see LdsMap).
+// Global addresses are a block-uniform base, a per-lane offset computed once
+// and a uniform per-point offset; lanes past the last line read the block's
+// first line (always in range) and store nothing.
+template <int N, int E, int T, int MODE, int XCH, int TILES, int SWS>
 __global__ __launch_bounds__(T*(N / E)) void k_strided(StridedArgs a) {
   constexpr int P = N / E;
   constexpr int LL = lds_line_len(N);
   constexpr int NT = T * P;
-  __shared__ float2 lds[T * LL];
+  constexpr int SWB = lds_padded(N) ? 0 : ilog2(T);
+  __shared__ float2 lds[XCH == 1 ? (T * LL + 1) / 2 : T * LL];
 
-  const int tid = threadIdx.x;
-  const int t = tid % T, p = tid / T;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t t = tid % T, p = tid / T;
+  const uint32_t sw = SWB ? (t << SWS) : 0u;
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const uint32_t o = tile / a.n_iblk;
-  const uint32_t inner = (tile % a.n_iblk) * T + t;
+  const uint32_t inner0 = (tile % a.n_iblk) * T;
+  const uint32_t inner = inner0 + t;
   const bool valid = inner < a.n_inner;
-  const float2* src = a.in + (uint64_t)o * a.in_so + (uint64_t)inner * a.in_si;
-  float2* dst = a.out + (uint64_t)o * a.out_so + (uint64_t)inner * a.out_si;
+  const uint32_t tl = valid ? t : 0u;
+  const float2* src = a.in + (uint64_t)o * a.in_so + (uint64_t)inner0 * a.in_si + (uint64_t)tl * a.in_si +
+                      (uint64_t)p * a.in_sn;
+  const uint64_t in_step = (uint64_t)P * a.in_sn;
+  float2* dst = a.out + (uint64_t)o * a.out_so + (uint64_t)inner * a.out_si + (uint64_t)p * a.out_sk;
+  const uint64_t out_step = (uint64_t)P * a.out_sk;
 
   float2 v[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e)
-    v[e] = valid ? src[(uint64_t)(p + P * e) * a.in_sn] : make_float2(0.f, 0.f);
+  for (int e = 0; e < E; ++e) v[e] = src[(uint64_t)e * in_step];
 
-  float2* line = lds + t * LL;
   if constexpr (MODE == MODE_INV) {
     // optional FP64 Parseval sum of the input spectrum (natural x pass only:
     // the inner index is kz)
@@ -151,86 +196,80 @@ __global__ __launch_bounds__(T*(N / E)) void k_strided(StridedArgs a) {
       if (threadIdx.x == 0) a.partial[tile] = bs;
     }
   }
-  fft_line_e<N, E, MODE == MODE_INV>(v, line, p, a.tw);
+  line_fft<N, E, MODE == MODE_INV, XCH, SWS, SWB>(v, lds, t, p, a.tw, sw);
 
   if constexpr (MODE != MODE_SPECTRAL) {
     if (valid) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) dst[(uint64_t)(p + P * e) * a.out_sk] = v[e];
+      for (int e = 0; e < E; ++e) dst[(uint64_t)e * out_step] = v[e];
     }
   } else {
-    // v[e] holds X(kx = p + P e, ky, kz).  One register tile only: delta_k
-    // is inverse transformed and stored; for the aux field the same tile is
-    // forward transformed again (FFT(IFFT(d)) = N d, exact up to rounding),
-    // scaled by f/N and inverse transformed.
+    // v[e] holds X(kx = p + P e, ky, kz)
     const uint32_t iky = a.kz_outer ? inner : o;
     const uint32_t ikz = a.kz_outer ? o : inner;
     const float ky = valid ? a.kyv[iky] : 0.0f;
     const float kz = valid ? a.kzv[ikz] : 0.0f;
     const double wz = (ikz == 0 || ikz == a.nz_nyq) ? 1.0 : 2.0;
     const float inv_vcell = 1.0f / a.vcell;
+    const AuxSel sel = aux_sel(a.aux_kind, a.aux_coef);
+    const bool has_aux = a.aux_kind != SMK_AUX_NONE;
+    float2 w[TILES == 2 ? E : 1];
     double acc = 0.0;
-    const int z1 = opaque_zero();
-    const float* kxv = a.kxv + z1;
-    const int p1 = p + z1;
+    {
+      const int z1 = opaque_zero();
+      const float* kxv = a.kxv + z1;
+      const int p1 = p + z1;
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int i = p1 + P * e;
-      const float kx = kxv[i];
-      // kk = kx*kx + ky*ky + kz*kz in float32, numpy evaluation order
-      const float kk = __fadd_rn(__fadd_rn(__fmul_rn(kx, kx), __fmul_rn(ky, ky)), __fmul_rn(kz, kz));
-      const float k = __builtin_amdgcn_sqrtf(kk);
-      const float sp = __builtin_amdgcn_sqrtf(pk_analytic(k, a) * inv_vcell);
-      const float2 d = cscale(v[e], sp * a.norm);
-      acc += (double)d.x * (double)d.x + (double)d.y * (double)d.y;
-      v[e] = d;
+      for (int e = 0; e < E; ++e) {
+        const int i = p1 + P * e;
+        const float kx = kxv[i];
+        // kk = kx*kx + ky*ky + kz*kz in float32, numpy evaluation order
+        const float kk = __fadd_rn(__fadd_rn(__fmul_rn(kx, kx), __fmul_rn(ky, ky)), __fmul_rn(kz, kz));
+        const float k = __builtin_amdgcn_sqrtf(kk);
+        const float sp = __builtin_amdgcn_sqrtf(pk_analytic(k, a) * inv_vcell);
+        const float2 d = cscale(v[e], sp * a.norm);
+        acc += (double)d.x * (double)d.x + (double)d.y * (double)d.y;
+        v[e] = d;
+        if constexpr (TILES == 2) w[e] = aux_apply(d, sel, kx, ky, kz, kk, i == 0 && iky == 0 && ikz == 0, 1.0f);
+      }
     }
     acc = valid ? wz * acc : 0.0;
     const double bs = block_sum<NT>(acc, reinterpret_cast<double*>(lds));
     if (threadIdx.x == 0) a.partial[tile] = bs;
 
-    fft_line_e<N, E, true>(v, line, p, a.tw);
+    line_fft<N, E, true, XCH, SWS, SWB>(v, lds, t, p, a.tw, sw);
     if (valid && a.write_delta) {
-      const int z2 = opaque_zero();
-      const uint64_t sk = a.out_sk + z2;
-      const int p2 = p + z2;
+      const uint64_t st = out_step + (uint64_t)opaque_zero();
 #pragma unroll
-      for (int e = 0; e < E; ++e) dst[(uint64_t)(p2 + P * e) * sk] = v[e];
+      for (int e = 0; e < E; ++e) dst[(uint64_t)e * st] = v[e];
     }
-    if (a.aux_kind != SMK_AUX_NONE) {
-      // aux factor: eta_ij = k_i k_j / kk, v_i = -i k_i / kk * coef; operand
-      // selectors 0 = kx, 1 = ky, 2 = kz, 3 = 1
-      const int ak = a.aux_kind;
-      const int sa = (ak == SMK_AUX_ETA_XX || ak == SMK_AUX_ETA_XY || ak == SMK_AUX_ETA_XZ || ak == SMK_AUX_VX) ? 0
-                     : (ak == SMK_AUX_ETA_YY || ak == SMK_AUX_ETA_YZ || ak == SMK_AUX_VY) ? 1 : 2;
-      const int sb = (ak == SMK_AUX_ETA_XX) ? 0 : (ak == SMK_AUX_ETA_YY || ak == SMK_AUX_ETA_XY) ? 1
-                     : (ak >= SMK_AUX_VX) ? 3 : 2;
-      const bool vel = ak >= SMK_AUX_VX;
-      const float coef = (vel ? a.aux_coef : 1.0f) / (float)N;
-      __syncthreads();
-      fft_line_e<N, E, false>(v, line, p, a.tw);
-      const int z3 = opaque_zero();
-      const float* kxv2 = a.kxv + z3;
-      const int p3 = p + z3;
+    if (has_aux) {
+      if constexpr (TILES == 1) {
+        // FFT(IFFT(d)) = N d (exact up to rounding): recompute the spectrum,
+        // scale by f/N, inverse transform
+        __syncthreads();
+        line_fft<N, E, false, XCH, SWS, SWB>(v, lds, t, p, a.tw, sw);
+        const int z3 = opaque_zero();
+        const float* kxv2 = a.kxv + z3;
+        const int p3 = p + z3;
 #pragma unroll
-      for (int e = 0; e < E; ++e) {
-        const int i = p3 + P * e;
-        const float kx = kxv2[i];
-        const float kk = __fadd_rn(__fadd_rn(__fmul_rn(kx, kx), __fmul_rn(ky, ky)), __fmul_rn(kz, kz));
-        const float kkp = (i == 0 && iky == 0 && ikz == 0) ? 1.0f : kk;
-        const float ka = sa == 0 ? kx : (sa == 1 ? ky : kz);
-        const float kb = sb == 0 ? kx : (sb == 1 ? ky : (sb == 2 ? kz : 1.0f));
-        const float2 x = cscale(v[e], ka * kb * __builtin_amdgcn_rcpf(kkp) * coef);
-        v[e] = vel ? mul_mi<false>(x) : x;
+        for (int e = 0; e < E; ++e) {
+          const int i = p3 + P * e;
+          const float kx = kxv2[i];
+          const float kk = __fadd_rn(__fadd_rn(__fmul_rn(kx, kx), __fmul_rn(ky, ky)), __fmul_rn(kz, kz));
+          v[e] = aux_apply(v[e], sel, kx, ky, kz, kk, i == 0 && iky == 0 && ikz == 0, 1.0f / (float)N);
+        }
+        line_fft<N, E, true, XCH, SWS, SWB>(v, lds, t, p, a.tw, sw);
+      } else {
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = w[e];
+        line_fft<N, E, true, XCH, SWS, SWB>(v, lds, t, p, a.tw, sw);
       }
-      fft_line_e<N, E, true>(v, line, p, a.tw);
       if (valid) {
-        const int z4 = opaque_zero();
-        float2* adst = a.aux + (uint64_t)o * a.out_so + (uint64_t)inner * a.out_si;
-        const uint64_t sk = a.out_sk + z4;
-        const int p4 = p + z4;
+        float2* adst = a.aux + (uint64_t)o * a.out_so + (uint64_t)inner * a.out_si + (uint64_t)p * a.out_sk;
+        const uint64_t st = out_step + (uint64_t)opaque_zero();
 #pragma unroll
-        for (int e = 0; e < E; ++e) adst[(uint64_t)(p4 + P * e) * sk] = v[e];
+        for (int e = 0; e < E; ++e) adst[(uint64_t)e * st] = v[e];
       }
     }
   }
@@ -241,13 +280,11 @@ __global__ __launch_bounds__(T*(N / E)) void k_strided(StridedArgs a) {
 // Ze_k = (Z_k + conj Z_{M-k})/2, Zo_k = -i (Z_k - conj Z_{M-k})/2.
 // Output: trans == 0 -> numpy layout K[x][y][kz]; trans == 1 -> K[x][kz][y]
 // (the store phase then runs with lanes ordered pencil-fastest, i.e. along y).
-template <int M, int T>
-__global__ __launch_bounds__(T* threads_per_line(M)) void k_z_r2c(const float* __restrict__ noise,
-                                                                 uint64_t seed, float2* __restrict__ K,
-                                                                 const float2* __restrict__ tw,
-                                                                 const float2* __restrict__ twr,
-                                                                 uint64_t n_pencils, int trans, uint32_t ny) {
-  constexpr int E = elems_per_thread(M);
+template <int M, int E, int T>
+__global__ __launch_bounds__(T*(M / E)) void k_z_r2c(const float* __restrict__ noise, uint64_t seed,
+                                                     float2* __restrict__ K, const float2* __restrict__ tw,
+                                                     const float2* __restrict__ twr, uint64_t n_pencils, int trans,
+                                                     uint32_t ny) {
   constexpr int P = M / E;
   constexpr int LL = lds_line_len(M);
   constexpr int NT = T * P;
@@ -269,8 +306,8 @@ __global__ __launch_bounds__(T* threads_per_line(M)) void k_z_r2c(const float* _
       if (LL_ < n_pencils) {
         U4 u = philox4x32_10(LL_ * (uint64_t)Q + qq, seed);
         float n0, n1, n2, n3;
-        box_muller(u.x, u.y, n0, n1);
-        box_muller(u.z, u.w, n2, n3);
+        box_muller_hw(u.x, u.y, n0, n1);
+        box_muller_hw(u.z, u.w, n2, n3);
         float2* ln = lds + tt * LL;
         ln[lds_pos<M>(2 * qq)] = make_float2(n0, n1);
         ln[lds_pos<M>(2 * qq + 1)] = make_float2(n2, n3);
@@ -285,7 +322,7 @@ __global__ __launch_bounds__(T* threads_per_line(M)) void k_z_r2c(const float* _
     for (int e = 0; e < E; ++e) v[e] = valid ? src[p + P * e] : make_float2(0.f, 0.f);
   }
 
-  fft_line<M, false>(v, line, p, tw);
+  fft_line_e<M, E, false>(v, line, p, tw);
 
   lds_scatter<M, P, E>(v, line, p);
   __syncthreads();
@@ -325,14 +362,10 @@ __global__ __launch_bounds__(T* threads_per_line(M)) void k_z_r2c(const float* _
 // + i W_{2M}^{-k} (X_k - conj X_{M-k}); the inverse M-point FFT of Z' is
 // x_{2n} + i x_{2n+1}.  Imaginary parts of X_0 and X_M are ignored, as a C2R
 // transform does.
-template <int M, int T>
-__global__ __launch_bounds__(T* threads_per_line(M)) void k_z_c2r(const float2* __restrict__ K,
-                                                                 float* __restrict__ out,
-                                                                 const float2* __restrict__ tw,
-                                                                 const float2* __restrict__ twr,
-                                                                 uint64_t n_pencils, float scale,
-                                                                 int* __restrict__ status) {
-  constexpr int E = elems_per_thread(M);
+template <int M, int E, int T>
+__global__ __launch_bounds__(T*(M / E)) void k_z_c2r(const float2* __restrict__ K, float* __restrict__ out,
+                                                     const float2* __restrict__ tw, const float2* __restrict__ twr,
+                                                     uint64_t n_pencils, float scale, int* __restrict__ status) {
   constexpr int P = M / E;
   constexpr int LL = lds_line_len(M);
   __shared__ float2 lds[T * LL];
@@ -365,7 +398,7 @@ __global__ __launch_bounds__(T* threads_per_line(M)) void k_z_c2r(const float2* 
     v[e] = make_float2(sm.x - wd.y, sm.y + wd.x);  // sm + i*wd
   }
   __syncthreads();
-  fft_line<M, true>(v, line, p, tw);
+  fft_line_e<M, E, true>(v, line, p, tw);
   int flags = 0;
   if (valid) {
     float2* dst = reinterpret_cast<float2*>(out + L * (uint64_t)(2 * M));
@@ -413,68 +446,174 @@ bool length_supported(int n) {
   }
 }
 
+// --------------------------------------------------------- pass variants
+// A strided-pass variant: register tile E, lines per block T, LDS exchange
+// form XCH (0 float2 line, 1 float line with the split exchange), in
+// spectral mode the number of register tiles, and the swizzle shift SWS of
+// unpadded lines.  Variant 0 exists for every length; the others for
+// n >= 512, where the passes are long enough to tune.  Blocks of 1024
+// threads cap a kernel at 128 VGPRs, so the two-tile spectral variant keeps
+// to 512.
+struct SVar {
+  int e, t, xch, tiles, sws;
+};
+constexpr int SMK_NVAR = SMK_BOX_NVAR;
+
+constexpr bool plan_ok(int n, int e) { return e > 0 && n % e == 0 && make_plan(n, e).nstage > 0; }
+
+constexpr int lines_per_block(int n, int e, int xch, int max_threads) {
+  const int bytes = lds_line_len(n) * (xch ? 4 : 8);
+  int T = 8;
+  while (T > 1 && (T * (n / e) > max_threads || T * bytes > 81920)) T >>= 1;
+  return T;
+}
+
+// the smaller register tile of the alternative variants (0: none)
+constexpr int half_tile(int n) {
+  const int e0 = elems_per_thread(n);
+  return (e0 % 2 == 0 && plan_ok(n, e0 / 2) && n / (e0 / 2) <= 512) ? e0 / 2 : 0;
+}
+
+constexpr SVar strided_variant(int n, int mode, int vid) {
+  const int e0 = elems_per_thread(n);
+  const int e1 = half_tile(n);
+  const SVar none{0, 0, 0, 0, 0};
+  if (e0 == 0) return none;
+  const int t0 = lines_per_block(n, e0, 0, 512);
+  if (vid == 0) return SVar{e0, t0, 0, 1, 0};
+  if (n < 512) return none;
+  const bool spec = mode == MODE_SPECTRAL;
+  const bool swz = !lds_padded(n);
+  switch (vid) {
+    case 1: return swz ? SVar{e0, t0, 0, 1, 2} : none;
+    case 2: return swz ? SVar{e0, t0, 0, 1, 3} : none;
+    case 3: return SVar{e0, lines_per_block(n, e0, 1, 1024), 1, 1, 0};
+    case 4: return e1 ? SVar{e1, lines_per_block(n, e1, 0, 1024), 0, 1, 0} : none;
+    case 5: return (spec && e1) ? SVar{e1, lines_per_block(n, e1, 0, 512), 0, 2, 0} : none;
+    case 6: return e1 ? SVar{e1, lines_per_block(n, e1, 1, 1024), 1, 1, 0} : none;
+    default: return none;
+  }
+}
+
+template <int N, int MODE, int VID>
+hipError_t launch_strided_nv(const StridedArgs& a, uint64_t n_outer, hipStream_t s) {
+  constexpr SVar sv = strided_variant(N, MODE, VID);
+  if constexpr (sv.e == 0) {
+    return hipErrorInvalidValue;
+  } else {
+    StridedArgs b = a;
+    b.n_iblk = (a.n_inner + sv.t - 1) / sv.t;
+    const uint64_t nblk = n_outer * b.n_iblk;
+    if (nblk > 0x7fffffffull) return hipErrorInvalidConfiguration;
+    hipLaunchKernelGGL((k_strided<N, sv.e, sv.t, MODE, sv.xch, sv.tiles, sv.sws>), dim3((uint32_t)nblk),
+                       dim3(sv.t * (N / sv.e)), 0, s, b);
+    return hipGetLastError();
+  }
+}
+
 template <int N, int MODE>
-hipError_t launch_strided_n(const StridedArgs& a, uint64_t n_outer, hipStream_t s) {
-  constexpr int E = mode_E(N, MODE);
-  constexpr int T = strided_T(N, E);
-  StridedArgs b = a;
-  b.n_iblk = (a.n_inner + T - 1) / T;
-  const uint64_t nblk = n_outer * b.n_iblk;
-  if (nblk > 0x7fffffffull) return hipErrorInvalidConfiguration;
-  hipLaunchKernelGGL((k_strided<N, E, T, MODE>), dim3((uint32_t)nblk), dim3(T * (N / E)), 0, s, b);
-  return hipGetLastError();
+hipError_t launch_strided_n(int vid, const StridedArgs& a, uint64_t n_outer, hipStream_t s) {
+  switch (vid) {
+    case 0: return launch_strided_nv<N, MODE, 0>(a, n_outer, s);
+    case 1: return launch_strided_nv<N, MODE, 1>(a, n_outer, s);
+    case 2: return launch_strided_nv<N, MODE, 2>(a, n_outer, s);
+    case 3: return launch_strided_nv<N, MODE, 3>(a, n_outer, s);
+    case 4: return launch_strided_nv<N, MODE, 4>(a, n_outer, s);
+    case 5: return launch_strided_nv<N, MODE, 5>(a, n_outer, s);
+    case 6: return launch_strided_nv<N, MODE, 6>(a, n_outer, s);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 template <int MODE>
-hipError_t launch_strided(int n, const StridedArgs& a, uint64_t n_outer, hipStream_t s) {
+hipError_t launch_strided(int n, int vid, const StridedArgs& a, uint64_t n_outer, hipStream_t s) {
   switch (n) {
 #define SMK_CASE(N) \
-  case N: return launch_strided_n<N, MODE>(a, n_outer, s);
+  case N: return launch_strided_n<N, MODE>(vid, a, n_outer, s);
     SMK_FFT_LENGTHS(SMK_CASE)
 #undef SMK_CASE
     default: return hipErrorInvalidValue;
   }
 }
 
-uint64_t strided_blocks(int n, int mode, uint64_t n_outer, uint32_t n_inner) {
-  switch (n) {
-#define SMK_CASE(N) \
-  case N: {                                                         \
-    const int T = strided_T(N, mode_E(N, mode));                    \
-    return n_outer * ((n_inner + T - 1) / T);                       \
+bool strided_variant_ok(int n, int mode, int vid) {
+  return vid >= 0 && vid < SMK_NVAR && length_supported(n) && strided_variant(n, mode, vid).e != 0;
+}
+
+uint64_t strided_blocks(int n, int mode, int vid, uint64_t n_outer, uint32_t n_inner) {
+  const SVar sv = strided_variant(n, mode, vid);
+  if (sv.e == 0) return 0;
+  return n_outer * ((n_inner + sv.t - 1) / sv.t);
+}
+
+// most blocks any variant of this pass launches (sizes the partial sums)
+uint64_t strided_blocks_max(int n, int mode, uint64_t n_outer, uint32_t n_inner) {
+  uint64_t m = 0;
+  for (int v = 0; v < SMK_NVAR; ++v) {
+    const uint64_t b = strided_blocks(n, mode, v, n_outer, n_inner);
+    m = b > m ? b : m;
   }
-    SMK_FFT_LENGTHS(SMK_CASE)
-#undef SMK_CASE
-    default: return 0;
+  return m;
+}
+
+// Pencil-pass variants: 0 = the default register tile, 1 = half of it
+// (twice the threads per pencil; m >= 256 only).
+constexpr int SMK_NPVAR = 2;
+constexpr int pencil_E(int m, int vid) {
+  return vid == 0 ? elems_per_thread(m) : (vid == 1 && m >= 256) ? half_tile(m) : 0;
+}
+bool pencil_variant_ok(int m, int vid) { return vid >= 0 && vid < SMK_NPVAR && pencil_E(m, vid) != 0; }
+
+template <int M, int VID>
+hipError_t launch_z_r2c_v(const float* noise, uint64_t seed, float2* K, const float2* tw, const float2* twr,
+                          uint64_t np, int trans, uint32_t ny, hipStream_t s) {
+  constexpr int E = pencil_E(M, VID);
+  if constexpr (E == 0) {
+    return hipErrorInvalidValue;
+  } else {
+    constexpr int T = pencil_T(M, E);
+    if ((np + T - 1) / T > 0x7fffffffull) return hipErrorInvalidConfiguration;
+    hipLaunchKernelGGL((k_z_r2c<M, E, T>), dim3((uint32_t)((np + T - 1) / T)), dim3(T * (M / E)), 0, s, noise,
+                       seed, K, tw, twr, np, trans, ny);
+    return hipGetLastError();
   }
 }
 
-hipError_t launch_z_r2c(int m, const float* noise, uint64_t seed, float2* K, const float2* tw,
+template <int M, int VID>
+hipError_t launch_z_c2r_v(const float2* K, float* out, const float2* tw, const float2* twr, uint64_t np,
+                          float scale, int* status, hipStream_t s) {
+  constexpr int E = pencil_E(M, VID);
+  if constexpr (E == 0) {
+    return hipErrorInvalidValue;
+  } else {
+    constexpr int T = pencil_T(M, E);
+    if ((np + T - 1) / T > 0x7fffffffull) return hipErrorInvalidConfiguration;
+    hipLaunchKernelGGL((k_z_c2r<M, E, T>), dim3((uint32_t)((np + T - 1) / T)), dim3(T * (M / E)), 0, s, K, out,
+                       tw, twr, np, scale, status);
+    return hipGetLastError();
+  }
+}
+
+hipError_t launch_z_r2c(int m, int vid, const float* noise, uint64_t seed, float2* K, const float2* tw,
                         const float2* twr, uint64_t np, int trans, uint32_t ny, hipStream_t s) {
   switch (m) {
-#define SMK_CASE(N)                                                                                    \
-  case N: {                                                                                            \
-    constexpr int T = pencil_T(N);                                                                     \
-    hipLaunchKernelGGL((k_z_r2c<N, T>), dim3((uint32_t)((np + T - 1) / T)), dim3(T * threads_per_line(N)), 0, \
-                       s, noise, seed, K, tw, twr, np, trans, ny);                                      \
-    return hipGetLastError();                                                                          \
-  }
+#define SMK_CASE(N)                                                                         \
+  case N:                                                                                   \
+    return vid == 0 ? launch_z_r2c_v<N, 0>(noise, seed, K, tw, twr, np, trans, ny, s)       \
+                    : launch_z_r2c_v<N, 1>(noise, seed, K, tw, twr, np, trans, ny, s);
     SMK_FFT_LENGTHS(SMK_CASE)
 #undef SMK_CASE
     default: return hipErrorInvalidValue;
   }
 }
 
-hipError_t launch_z_c2r(int m, const float2* K, float* out, const float2* tw, const float2* twr, uint64_t np,
-                        float scale, int* status, hipStream_t s) {
+hipError_t launch_z_c2r(int m, int vid, const float2* K, float* out, const float2* tw, const float2* twr,
+                        uint64_t np, float scale, int* status, hipStream_t s) {
   switch (m) {
-#define SMK_CASE(N)                                                                                    \
-  case N: {                                                                                            \
-    constexpr int T = pencil_T(N);                                                                     \
-    hipLaunchKernelGGL((k_z_c2r<N, T>), dim3((uint32_t)((np + T - 1) / T)), dim3(T * threads_per_line(N)), 0, \
-                       s, K, out, tw, twr, np, scale, status);                                          \
-    return hipGetLastError();                                                                          \
-  }
+#define SMK_CASE(N)                                                                         \
+  case N:                                                                                   \
+    return vid == 0 ? launch_z_c2r_v<N, 0>(K, out, tw, twr, np, scale, status, s)           \
+                    : launch_z_c2r_v<N, 1>(K, out, tw, twr, np, scale, status, s);
     SMK_FFT_LENGTHS(SMK_CASE)
 #undef SMK_CASE
     default: return hipErrorInvalidValue;
@@ -536,6 +675,8 @@ struct smk_box_plan {
   float* ky;
   float* kz;
   uint64_t n_partial;
+  // pass variants, indexed by SMK_PASS_*
+  int vid[SMK_PASS_COUNT];
 };
 
 extern "C" int smk_fft_length_supported(int n) { return length_supported(n) ? 1 : 0; }
@@ -552,7 +693,7 @@ extern "C" int smk_box_plan_create(smk_box_plan** plan, int nx, int ny, int nz, 
   smk_box_plan* p = (smk_box_plan*)calloc(1, sizeof(smk_box_plan));
   SMK_REQUIRE(p != nullptr, SMK_ERR_OOM, "host allocation failed");
   p->nx = nx; p->ny = ny; p->nz = nz; p->m = nz / 2; p->nzh = nz / 2 + 1; p->cell = cell;
-  p->n_partial = strided_blocks(nx, MODE_SPECTRAL, (uint64_t)p->nzh, (uint32_t)ny);
+  p->n_partial = strided_blocks_max(nx, MODE_SPECTRAL, (uint64_t)p->nzh, (uint32_t)ny);
 
   auto upload = [](void** dst, const void* src, size_t bytes) -> hipError_t {
     hipError_t e = hipMalloc(dst, bytes);
@@ -625,9 +766,10 @@ StridedArgs nat_x_args(const smk_box_plan* p, float2* K) {
 
 int nat_forward(smk_box_plan* p, uint64_t seed, const float* noise, float2* K, hipStream_t s) {
   const uint64_t np = (uint64_t)p->nx * p->ny;
-  SMK_HIP_RET(launch_z_r2c(p->m, noise, seed, K, p->tw_z, p->tw_r, np, 0, (uint32_t)p->ny, s));
-  SMK_HIP_RET(launch_strided<MODE_FWD>(p->ny, nat_y_args(p, K), (uint64_t)p->nx, s));
-  SMK_HIP_RET(launch_strided<MODE_FWD>(p->nx, nat_x_args(p, K), (uint64_t)p->ny, s));
+  SMK_HIP_RET(launch_z_r2c(p->m, p->vid[SMK_PASS_Z_R2C], noise, seed, K, p->tw_z, p->tw_r, np, 0, (uint32_t)p->ny,
+                           s));
+  SMK_HIP_RET(launch_strided<MODE_FWD>(p->ny, p->vid[SMK_PASS_Y_FWD], nat_y_args(p, K), (uint64_t)p->nx, s));
+  SMK_HIP_RET(launch_strided<MODE_FWD>(p->nx, p->vid[SMK_PASS_X_FWD], nat_x_args(p, K), (uint64_t)p->ny, s));
   return SMK_OK;
 }
 
@@ -757,9 +899,11 @@ extern "C" int smk_grf_box(smk_box_plan* p, uint64_t seed, const float* noise, c
     SMK_LAUNCH_CHECK();
   }
   SMK_HIP_RET(mark());
-  SMK_HIP_RET(launch_z_r2c(p->m, noise, seed, A1, p->tw_z, p->tw_r, np, 1, (uint32_t)p->ny, s));
+  SMK_HIP_RET(launch_z_r2c(p->m, p->vid[SMK_PASS_Z_R2C], noise, seed, A1, p->tw_z, p->tw_r, np, 1, (uint32_t)p->ny,
+                           s));
   SMK_HIP_RET(mark());
-  SMK_HIP_RET(launch_strided<MODE_FWD>(p->ny, tr_y_fwd_args(p, A1, A2), (uint64_t)p->nzh, s));
+  SMK_HIP_RET(launch_strided<MODE_FWD>(p->ny, p->vid[SMK_PASS_Y_FWD], tr_y_fwd_args(p, A1, A2), (uint64_t)p->nzh,
+                                       s));
   SMK_HIP_RET(mark());
 
   StridedArgs a = tr_x_args(p, A2, A1);
@@ -776,21 +920,24 @@ extern "C" int smk_grf_box(smk_box_plan* p, uint64_t seed, const float* noise, c
   a.write_delta = 1;
   a.nz_nyq = (uint32_t)p->m;
   a.partial = partial;
-  SMK_HIP_RET(launch_strided<MODE_SPECTRAL>(p->nx, a, (uint64_t)p->nzh, s));
+  const int vx = p->vid[SMK_PASS_X_SPECTRAL];
+  SMK_HIP_RET(launch_strided<MODE_SPECTRAL>(p->nx, vx, a, (uint64_t)p->nzh, s));
   SMK_HIP_RET(mark());
   if (var_out) {
-    const uint64_t nb = strided_blocks(p->nx, MODE_SPECTRAL, (uint64_t)p->nzh, (uint32_t)p->ny);
+    const uint64_t nb = strided_blocks(p->nx, MODE_SPECTRAL, vx, (uint64_t)p->nzh, (uint32_t)p->ny);
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, s, partial, nb, var_out);
     SMK_LAUNCH_CHECK();
   }
-  SMK_HIP_RET(launch_strided<MODE_INV>(p->ny, tr_y_inv_args(p, A1, A2), (uint64_t)p->nx, s));
+  const int vyi = p->vid[SMK_PASS_Y_INV], vz = p->vid[SMK_PASS_Z_C2R];
+  SMK_HIP_RET(launch_strided<MODE_INV>(p->ny, vyi, tr_y_inv_args(p, A1, A2), (uint64_t)p->nx, s));
   SMK_HIP_RET(mark());
-  SMK_HIP_RET(launch_z_c2r(p->m, A2, delta_out, p->tw_z, p->tw_r, np, 1.0f, status_out, s));
+  SMK_HIP_RET(launch_z_c2r(p->m, vz, A2, delta_out, p->tw_z, p->tw_r, np, 1.0f, status_out, s));
   SMK_HIP_RET(mark());
   if (aux_kind != SMK_AUX_NONE) {
-    SMK_HIP_RET(launch_strided<MODE_INV>(p->ny, tr_y_inv_args(p, A3, A1), (uint64_t)p->nx, s));
+    SMK_HIP_RET(launch_strided<MODE_INV>(p->ny, vyi, tr_y_inv_args(p, A3, A1), (uint64_t)p->nx, s));
     SMK_HIP_RET(mark());
-    SMK_HIP_RET(launch_z_c2r(p->m, A1, aux_out, p->tw_z, p->tw_r, np, 1.0f, status_out ? status_out + 1 : nullptr, s));
+    SMK_HIP_RET(launch_z_c2r(p->m, vz, A1, aux_out, p->tw_z, p->tw_r, np, 1.0f, status_out ? status_out + 1 : nullptr,
+                             s));
     SMK_HIP_RET(mark());
   }
   if (p->timing) {
@@ -798,6 +945,115 @@ extern "C" int smk_grf_box(smk_box_plan* p, uint64_t seed, const float* noise, c
     p->pending = 1;
   }
   return SMK_OK;
+}
+
+namespace {
+
+// (type, length, mode) of a pass kind: type 0 strided, 1 z pencil
+struct PassKind {
+  int type, n, mode;
+};
+PassKind pass_kind(const smk_box_plan* p, int kind) {
+  switch (kind) {
+    case SMK_PASS_Z_R2C: return {1, p->m, MODE_FWD};
+    case SMK_PASS_Y_FWD: return {0, p->ny, MODE_FWD};
+    case SMK_PASS_X_SPECTRAL: return {0, p->nx, MODE_SPECTRAL};
+    case SMK_PASS_Y_INV: return {0, p->ny, MODE_INV};
+    case SMK_PASS_Z_C2R: return {1, p->m, MODE_INV};
+    case SMK_PASS_X_FWD: return {0, p->nx, MODE_FWD};
+    case SMK_PASS_X_INV: return {0, p->nx, MODE_INV};
+    default: return {-1, 0, 0};
+  }
+}
+
+bool variant_valid(const smk_box_plan* p, int kind, int vid) {
+  const PassKind k = pass_kind(p, kind);
+  if (k.type == 0) return strided_variant_ok(k.n, k.mode, vid);
+  if (k.type == 1) return pencil_variant_ok(k.n, vid);
+  return false;
+}
+
+}  // namespace
+
+extern "C" int smk_box_variant_valid(const smk_box_plan* p, int kind, int vid) {
+  return (p != nullptr && variant_valid(p, kind, vid)) ? 1 : 0;
+}
+
+extern "C" int smk_box_set_variant(smk_box_plan* p, int kind, int vid) {
+  SMK_REQUIRE(p != nullptr, SMK_ERR_ARG, "plan is NULL");
+  SMK_REQUIRE(kind >= 0 && kind < SMK_PASS_COUNT, SMK_ERR_ARG, "bad pass kind %d", kind);
+  SMK_REQUIRE(variant_valid(p, kind, vid), SMK_ERR_UNSUPPORTED, "variant %d does not exist for pass kind %d", vid,
+              kind);
+  p->vid[kind] = vid;
+  return SMK_OK;
+}
+
+extern "C" int smk_box_get_variant(const smk_box_plan* p, int kind) {
+  if (p == nullptr || kind < 0 || kind >= SMK_PASS_COUNT) return -1;
+  return p->vid[kind];
+}
+
+extern "C" int smk_box_autotune(smk_box_plan* p, uint64_t seed, const float* noise, const double* pk_params,
+                                int aux_kind, double aux_coef, float* delta_out, float* aux_out, void* workspace,
+                                double* var_out, int* status_out, void* stream, int reps, double* ms_out) {
+  SMK_REQUIRE(p != nullptr, SMK_ERR_ARG, "plan is NULL");
+  SMK_REQUIRE(reps >= 1 && reps <= 100, SMK_ERR_ARG, "reps must be in [1, 100] (got %d)", reps);
+  constexpr int NK = 5;  // kinds of the fused job
+  const int nv = SMK_NVAR > SMK_NPVAR ? SMK_NVAR : SMK_NPVAR;
+  int saved[SMK_PASS_COUNT];
+  memcpy(saved, p->vid, sizeof(saved));
+  int rc = smk_box_enable_pass_timing(p, 1);
+  if (rc) return rc;
+  double best_ms[NK];
+  int best[NK];
+  for (int k = 0; k < NK; ++k) {
+    best_ms[k] = -1.0;
+    best[k] = saved[k];
+  }
+  if (ms_out)
+    for (int i = 0; i < NK * SMK_NVAR; ++i) ms_out[i] = -1.0;
+  for (int v = 0; v < nv && rc == SMK_OK; ++v) {
+    bool any = false;
+    for (int k = 0; k < NK; ++k) {
+      const bool ok = variant_valid(p, k, v);
+      p->vid[k] = ok ? v : 0;
+      any = any || ok;
+    }
+    if (!any) continue;
+    for (int r = 0; r <= reps && rc == SMK_OK; ++r) {
+      rc = smk_grf_box(p, seed, noise, pk_params, aux_kind, aux_coef, delta_out, aux_out, workspace, var_out,
+                       status_out, stream);
+      if (rc == SMK_OK && r == 0) {
+        // the first call is untimed (module load, cold caches)
+        double tmp[SMK_BOX_NPASS];
+        int nc = 0;
+        rc = smk_box_pass_times(p, tmp, SMK_BOX_NPASS, &nc, 1);
+      }
+    }
+    if (rc) break;
+    double ms[SMK_BOX_NPASS] = {0};
+    int nc = 0;
+    rc = smk_box_pass_times(p, ms, SMK_BOX_NPASS, &nc, 1);
+    if (rc) break;
+    // pass slots: 0 z R2C, 1 y FFT, 2 x spectral, 3 y IFFT, 4 z C2R, 5 y IFFT aux, 6 z C2R aux
+    const double per[NK] = {ms[0], ms[1], ms[2], ms[3] + ms[5], ms[4] + ms[6]};
+    for (int k = 0; k < NK; ++k) {
+      if (!variant_valid(p, k, v)) continue;
+      const double t = per[k] / (nc > 0 ? nc : 1);
+      if (ms_out && v < SMK_NVAR) ms_out[k * SMK_NVAR + v] = t;
+      if (best_ms[k] < 0.0 || t < best_ms[k]) {
+        best_ms[k] = t;
+        best[k] = v;
+      }
+    }
+  }
+  const int rc2 = smk_box_enable_pass_timing(p, 0);
+  if (rc) {
+    memcpy(p->vid, saved, sizeof(saved));
+    return rc;
+  }
+  for (int k = 0; k < NK; ++k) p->vid[k] = best[k];
+  return rc2;
 }
 
 extern "C" int smk_draw_grf_boxk(smk_box_plan* p, uint64_t seed, const float* noise, float* boxk_out, void* stream) {
@@ -819,19 +1075,20 @@ extern "C" int smk_fft_and_store(smk_box_plan* p, float* boxk_inout, float* box_
   StridedArgs a = nat_x_args(p, K);
   a.nz_nyq = (uint32_t)p->m;
   a.partial = sumsq_out ? (double*)workspace : nullptr;
-  SMK_HIP_RET(launch_strided<MODE_INV>(p->nx, a, (uint64_t)p->ny, s));
+  const int vx = p->vid[SMK_PASS_X_INV];
+  SMK_HIP_RET(launch_strided<MODE_INV>(p->nx, vx, a, (uint64_t)p->ny, s));
   if (sumsq_out) {
-    const uint64_t nb = strided_blocks(p->nx, MODE_INV, (uint64_t)p->ny, (uint32_t)p->nzh);
+    const uint64_t nb = strided_blocks(p->nx, MODE_INV, vx, (uint64_t)p->ny, (uint32_t)p->nzh);
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, s, a.partial, nb, sumsq_out);
     SMK_LAUNCH_CHECK();
   }
   const float scale = (float)(1.0 / ((double)p->nx * p->ny * p->nz));
-  SMK_HIP_RET(launch_strided<MODE_INV>(p->ny, nat_y_args(p, K), (uint64_t)p->nx, s));
-  SMK_HIP_RET(launch_z_c2r(p->m, K, box_out, p->tw_z, p->tw_r, np, scale, status_out, s));
+  SMK_HIP_RET(launch_strided<MODE_INV>(p->ny, p->vid[SMK_PASS_Y_INV], nat_y_args(p, K), (uint64_t)p->nx, s));
+  SMK_HIP_RET(launch_z_c2r(p->m, p->vid[SMK_PASS_Z_C2R], K, box_out, p->tw_z, p->tw_r, np, scale, status_out, s));
   return SMK_OK;
 }
 
 extern "C" size_t smk_fft_and_store_workspace_bytes(const smk_box_plan* p) {
   if (p == nullptr) return 0;
-  return (size_t)align256(strided_blocks(p->nx, MODE_INV, (uint64_t)p->ny, (uint32_t)p->nzh) * sizeof(double));
+  return (size_t)align256(strided_blocks_max(p->nx, MODE_INV, (uint64_t)p->ny, (uint32_t)p->nzh) * sizeof(double));
 }

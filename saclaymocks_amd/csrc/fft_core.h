@@ -121,8 +121,11 @@ template <bool INV> struct DFT<8, INV> {
 };
 
 // --------------------------------------------------------- compile-time plan
-// Radices are picked greedily from {8,5,4,3,2}; every radix must divide E so
-// that each thread owns a whole number of butterflies in every stage.
+// Every radix must divide E so that each thread owns a whole number of
+// butterflies in every stage.  Power-of-two radices come first (largest
+// first), the odd radix (3 or 5) last: every stage that writes LDS then has a
+// power-of-two Ns, which makes its butterfly indices a per-thread base plus
+// compile-time offsets (see fft_stage).
 struct RadixPlan {
   int n, e, p, nstage;
   int r[8];
@@ -131,7 +134,7 @@ struct RadixPlan {
 __host__ __device__ constexpr RadixPlan make_plan(int n, int e) {
   RadixPlan pl{n, e, n / e, 0, {0, 0, 0, 0, 0, 0, 0, 0}};
   int rem = n;
-  const int cand[5] = {8, 5, 4, 3, 2};
+  const int cand[5] = {8, 4, 2, 5, 3};
   while (rem > 1 && pl.nstage < 8) {
     int pick = 0;
     for (int i = 0; i < 5; ++i) {
@@ -188,11 +191,6 @@ template <int N> __host__ __device__ constexpr int lds_pos(int pos) {
 }
 __host__ __device__ constexpr int lds_line_len(int n) { return lds_padded(n) ? n + (n >> 5) + 1 : n; }
 
-// ------------------------------------------------------------- stage driver
-// One radix-R stage of a Stockham autosort FFT.  Inputs of butterfly j are
-// the points j + r*N/R (r = 0..R-1); outputs go to idxD + r*Ns with
-// idxD = (j/Ns)*Ns*R + j%Ns.  For the last stage that is j + r*N/R again, so
-// the result stays in the registers it was computed in.
 // A zero the compiler cannot see through.  Adding it to a table pointer at
 // the start of each stage keeps that stage's loads inside the stage: without
 // it the compiler merges the identical twiddle (and wavenumber) loads of the
@@ -204,25 +202,115 @@ __device__ __forceinline__ int opaque_zero() {
   return z;
 }
 
-template <int N, int P, int R, int Ns, bool INV, bool LAST, int E>
-__device__ __forceinline__ void fft_stage(float2 (&v)[E], float2* __restrict__ line, int p_in,
-                                          const float2* tw_in) {
+// Slot of point `pos` in an LDS line.  Padded lines (every length but 2560)
+// insert one slot every 32 points.  The unpadded 2560-point line instead
+// XORs bits [SWS, SWS + SWB) of the position with the lane's swizzle `sw`
+// (its line index within the block, shifted by SWS): the lines of a block
+// start a multiple of 64 banks apart and the lanes of a wave interleave them.
+// fold(slot(x), C) = slot(x + C) for a compile-time C that is a multiple of G,
+// which turns the LDS addresses of a stage into one base plus immediates.
+template <int N, int SWS, int SWB> struct LdsMap {
+  static constexpr bool padded = lds_padded(N);
+  static constexpr int G = padded ? 32 : (1 << (SWS + SWB));
+  static __device__ __forceinline__ uint32_t slot(uint32_t pos, uint32_t sw) {
+    if constexpr (padded) return pos + (pos >> 5);
+    else if constexpr (SWB == 0) return pos;
+    else return pos ^ sw;
+  }
+  // c must be a multiple of G (checked through folds() at compile time by
+  // the callers); c is a constant after unrolling
+  static __device__ __forceinline__ uint32_t fold(uint32_t s, int c) {
+    return s + (uint32_t)(padded ? c + c / 32 : c);
+  }
+  static constexpr bool folds(int c) { return c % G == 0; }
+};
+
+// ------------------------------------------------------------- stage driver
+// One radix-R stage of a Stockham autosort FFT.  Butterfly j = p + P*b
+// (b < B = E/R) takes the points j + r*N/R, i.e. register slots b + r*B;
+// its twiddles are W_N^(r * (j % Ns) * N/(Ns R)) and its outputs go to
+// idxD + r*Ns with idxD = (j/Ns)*Ns*R + j%Ns.  For the last stage that is
+// j + r*N/R again, so the result stays in registers (slot b + r*B holds
+// output r of butterfly b).
+// Index classes, fixed at compile time:
+//   Ns == 1       no twiddles, idxD = R*p + b*P*R
+//   P % Ns == 0   j % Ns = p % Ns for every b: R-1 twiddle loads per stage,
+//                 idxD = base(p) + b*P*R
+//   Ns % P == 0   j % Ns = p + (P b) % Ns, j / Ns = (P b) / Ns
+//   otherwise     per-butterfly division
+template <int N, int P, int R, int Ns> struct StageIdx {
+  static constexpr int KIND = Ns == 1 ? 0 : (P % Ns == 0 ? 1 : (Ns % P == 0 ? 2 : 3));
+  static constexpr int step = N / (Ns * R);
+  // compile-time parts for butterfly b (classes 0-2)
+  static constexpr int offD(int b) {
+    return KIND == 2 ? ((P * b) / Ns) * Ns * R + (P * b) % Ns : b * P * R;
+  }
+  static constexpr int offJ(int b) { return KIND == 2 ? (P * b) % Ns : 0; }
+};
+
+template <int N, int P, int R, int Ns, bool INV, bool LAST, int E, int SWS, int SWB>
+__device__ __forceinline__ void fft_stage(float2 (&v)[E], float2* __restrict__ line, uint32_t p_in,
+                                          const float2* tw_in, uint32_t sw) {
+  using I = StageIdx<N, P, R, Ns>;
+  using M = LdsMap<N, SWS, SWB>;
   constexpr int B = E / R;
   const int z = opaque_zero();
   const float2* tw = tw_in + z;
-  const int p = p_in + z;
+  const uint32_t p = p_in + (uint32_t)z;
+
+  // per-thread bases
+  uint32_t jj0 = 0, base0 = 0;
+  if constexpr (I::KIND == 0) {
+    base0 = p * R;
+  } else if constexpr (I::KIND == 1) {
+    jj0 = p % Ns;
+    base0 = (p / Ns) * (Ns * R) + jj0;
+  } else if constexpr (I::KIND == 2) {
+    jj0 = p;
+    base0 = p;
+  }
+  // stage-uniform twiddles (class 1)
+  float2 wu[R];
+  if constexpr (I::KIND == 1) {
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      wu[r] = tw[r * jj0 * I::step];
+      if (INV) wu[r].y = -wu[r].y;
+    }
+  }
+  // slots of the outputs relative to base0, when every butterfly offset folds
+  constexpr bool fold_b = []() {
+    for (int b = 0; b < B; ++b)
+      if (!M::folds(I::offD(b))) return false;
+    return true;
+  }();
+  uint32_t sr[R];
+  if constexpr (!LAST && I::KIND != 3 && fold_b) {
+    if constexpr (M::folds(Ns)) {
+      const uint32_t s0 = M::slot(base0, sw);
+#pragma unroll
+      for (int r = 0; r < R; ++r) sr[r] = s0 + (uint32_t)(M::padded ? r * Ns + (r * Ns) / 32 : r * Ns);
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; ++r) sr[r] = M::slot(base0 + (uint32_t)(r * Ns), sw);
+    }
+  }
+
 #pragma unroll
   for (int b = 0; b < B; ++b) {
-    const int j = p + P * b;
     float2 a[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) a[r] = v[b + r * B];
-    const int jj = (Ns > 1) ? (j % Ns) : 0;
-    if constexpr (Ns > 1) {
-      constexpr int step = N / (Ns * R);
+    uint32_t jj = 0;
+    if constexpr (I::KIND == 3) jj = (p + (uint32_t)(P * b)) % Ns;
+    if constexpr (I::KIND == 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) a[r] = cmul(a[r], wu[r]);
+    } else if constexpr (I::KIND == 2 || I::KIND == 3) {
+      const uint32_t jb = I::KIND == 2 ? jj0 + (uint32_t)I::offJ(b) : jj;
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        float2 w = tw[r * jj * step];
+        float2 w = tw[r * jb * I::step];
         if (INV) w.y = -w.y;
         a[r] = cmul(a[r], w);
       }
@@ -231,42 +319,110 @@ __device__ __forceinline__ void fft_stage(float2 (&v)[E], float2* __restrict__ l
     if constexpr (LAST) {
 #pragma unroll
       for (int r = 0; r < R; ++r) v[b + r * B] = a[r];
-    } else {
-      const int idxD = (j / Ns) * Ns * R + jj;
+    } else if constexpr (I::KIND == 3) {
+      const uint32_t j = p + (uint32_t)(P * b);
+      const uint32_t idxD = (j / Ns) * (Ns * R) + jj;
 #pragma unroll
-      for (int r = 0; r < R; ++r) line[lds_pos<N>(idxD + r * Ns)] = a[r];
+      for (int r = 0; r < R; ++r) line[M::slot(idxD + (uint32_t)(r * Ns), sw)] = a[r];
+    } else if constexpr (fold_b) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) line[M::fold(sr[r], I::offD(b))] = a[r];
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; ++r) line[M::slot(base0 + (uint32_t)(I::offD(b) + r * Ns), sw)] = a[r];
     }
   }
 }
 
-template <int N, int P, int E>
-__device__ __forceinline__ void lds_gather(float2 (&v)[E], const float2* __restrict__ line, int p_in) {
-  const int p = p_in + opaque_zero();
+// thread p loads points p + P*e of the line
+template <int N, int P, int E, int SWS = 0, int SWB = 0>
+__device__ __forceinline__ void lds_gather(float2 (&v)[E], const float2* __restrict__ line, uint32_t p_in,
+                                           uint32_t sw = 0) {
+  using M = LdsMap<N, SWS, SWB>;
+  const uint32_t p = p_in + (uint32_t)opaque_zero();
+  if constexpr (M::folds(P)) {
+    const uint32_t s0 = M::slot(p, sw);
 #pragma unroll
-  for (int e = 0; e < E; ++e) v[e] = line[lds_pos<N>(p + P * e)];
+    for (int e = 0; e < E; ++e) v[e] = line[s0 + (uint32_t)(M::padded ? P * e + (P * e) / 32 : P * e)];
+  } else {
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = line[M::slot(p + (uint32_t)(P * e), sw)];
+  }
 }
 
-template <int N, int P, int E>
-__device__ __forceinline__ void lds_scatter(const float2 (&v)[E], float2* __restrict__ line, int p) {
+template <int N, int P, int E, int SWS = 0, int SWB = 0>
+__device__ __forceinline__ void lds_scatter(const float2 (&v)[E], float2* __restrict__ line, uint32_t p,
+                                            uint32_t sw = 0) {
+  using M = LdsMap<N, SWS, SWB>;
+  if constexpr (M::folds(P)) {
+    const uint32_t s0 = M::slot(p, sw);
 #pragma unroll
-  for (int e = 0; e < E; ++e) line[lds_pos<N>(p + P * e)] = v[e];
+    for (int e = 0; e < E; ++e) line[s0 + (uint32_t)(M::padded ? P * e + (P * e) / 32 : P * e)] = v[e];
+  } else {
+#pragma unroll
+    for (int e = 0; e < E; ++e) line[M::slot(p + (uint32_t)(P * e), sw)] = v[e];
+  }
 }
 
-template <int N, int E, bool INV, int S, int Ns>
-__device__ __forceinline__ void fft_stages(float2 (&v)[E], float2* __restrict__ line, int p,
-                                           const float2* tw) {
+// Split exchange after a stage computed in the LAST form: output r of
+// butterfly b (slot b + r*B) moves to point idxD + r*Ns.  The real parts go
+// through a float LDS line, then the imaginary parts: half the LDS of the
+// float2 exchange for two more barriers.
+template <int N, int P, int R, int Ns, int E, int SWS, int SWB>
+__device__ __forceinline__ void exchange_split(float2 (&v)[E], float* __restrict__ lf, uint32_t p_in, uint32_t sw) {
+  using M = LdsMap<N, SWS, SWB>;
+  constexpr int B = E / R;
+  const uint32_t p = p_in + (uint32_t)opaque_zero();
+#pragma unroll
+  for (int half = 0; half < 2; ++half) {
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const uint32_t j = p + (uint32_t)(P * b);
+      const uint32_t idxD = (j / Ns) * (Ns * R) + j % Ns;
+#pragma unroll
+      for (int r = 0; r < R; ++r) lf[M::slot(idxD + (uint32_t)(r * Ns), sw)] = half ? v[b + r * B].y : v[b + r * B].x;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const float x = lf[M::slot(p + (uint32_t)(P * e), sw)];
+      if (half) v[e].y = x; else v[e].x = x;
+    }
+    __syncthreads();
+  }
+}
+
+template <int N, int E, bool INV, int S, int Ns, int SWS, int SWB>
+__device__ __forceinline__ void fft_stages(float2 (&v)[E], float2* __restrict__ line, uint32_t p, const float2* tw,
+                                           uint32_t sw) {
   constexpr RadixPlan pl = make_plan(N, E);
   static_assert(pl.nstage > 0, "unsupported FFT length");
   constexpr int P = N / E;
   if constexpr (S < pl.nstage) {
     constexpr int R = pl.r[S];
     constexpr bool LAST = (S + 1 == pl.nstage);
-    fft_stage<N, P, R, Ns, INV, LAST, E>(v, line, p, tw);
+    fft_stage<N, P, R, Ns, INV, LAST, E, SWS, SWB>(v, line, p, tw, sw);
     if constexpr (!LAST) {
       __syncthreads();
-      lds_gather<N, P, E>(v, line, p);
+      lds_gather<N, P, E, SWS, SWB>(v, line, p, sw);
       __syncthreads();
-      fft_stages<N, E, INV, S + 1, Ns * R>(v, line, p, tw);
+      fft_stages<N, E, INV, S + 1, Ns * R, SWS, SWB>(v, line, p, tw, sw);
+    }
+  }
+}
+
+template <int N, int E, bool INV, int S, int Ns, int SWS, int SWB>
+__device__ __forceinline__ void fft_stages_split(float2 (&v)[E], float* __restrict__ lf, uint32_t p,
+                                                 const float2* tw, uint32_t sw) {
+  constexpr RadixPlan pl = make_plan(N, E);
+  static_assert(pl.nstage > 0, "unsupported FFT length");
+  constexpr int P = N / E;
+  if constexpr (S < pl.nstage) {
+    constexpr int R = pl.r[S];
+    fft_stage<N, P, R, Ns, INV, true, E, SWS, SWB>(v, nullptr, p, tw, sw);
+    if constexpr (S + 1 < pl.nstage) {
+      exchange_split<N, P, R, Ns, E, SWS, SWB>(v, lf, p, sw);
+      fft_stages_split<N, E, INV, S + 1, Ns * R, SWS, SWB>(v, lf, p, tw, sw);
     }
   }
 }
@@ -276,16 +432,25 @@ __device__ __forceinline__ void fft_stages(float2 (&v)[E], float2* __restrict__ 
 // p + P*e.  Forward: X_k = sum_n x_n exp(-2 pi i n k / N); inverse uses
 // exp(+...) and is unnormalised.
 template <int N, bool INV>
-__device__ __forceinline__ void fft_line(float2 (&v)[elems_per_thread(N)], float2* __restrict__ line, int p,
+__device__ __forceinline__ void fft_line(float2 (&v)[elems_per_thread(N)], float2* __restrict__ line, uint32_t p,
                                          const float2* tw) {
-  fft_stages<N, elems_per_thread(N), INV, 0, 1>(v, line, p, tw);
+  fft_stages<N, elems_per_thread(N), INV, 0, 1, 0, 0>(v, line, p, tw, 0);
 }
 
-// Same with an explicit register tile of E points per thread (P = N / E).
-template <int N, int E, bool INV>
-__device__ __forceinline__ void fft_line_e(float2 (&v)[E], float2* __restrict__ line, int p,
-                                           const float2* tw) {
-  fft_stages<N, E, INV, 0, 1>(v, line, p, tw);
+// Same with an explicit register tile of E points per thread (P = N / E) and
+// an optional line swizzle (see LdsMap; sw must be < 2^(SWS+SWB) with its
+// bits in [SWS, SWS+SWB)).
+template <int N, int E, bool INV, int SWS = 0, int SWB = 0>
+__device__ __forceinline__ void fft_line_e(float2 (&v)[E], float2* __restrict__ line, uint32_t p,
+                                           const float2* tw, uint32_t sw = 0) {
+  fft_stages<N, E, INV, 0, 1, SWS, SWB>(v, line, p, tw, sw);
+}
+
+// Same through a float LDS line of lds_line_len(N) floats (split exchange).
+template <int N, int E, bool INV, int SWS = 0, int SWB = 0>
+__device__ __forceinline__ void fft_line_split(float2 (&v)[E], float* __restrict__ lf, uint32_t p,
+                                               const float2* tw, uint32_t sw = 0) {
+  fft_stages_split<N, E, INV, 0, 1, SWS, SWB>(v, lf, p, tw, sw);
 }
 
 // ------------------------------------------------------ counter-based noise
@@ -328,6 +493,17 @@ __device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, float& n0, fl
   sincospif(2.0f * u01(b), &s, &c);
   n0 = r * c;
   n1 = r * s;
+}
+
+// The same transform on the hardware transcendentals: v_log_f32 is log2
+// (ln u = ln2 * log2 u), and v_sin_f32 / v_cos_f32 take their argument in
+// revolutions, so sin(2 pi u) is one instruction for u in (0, 1).  The
+// error against the FP64 transform is a few 1e-7 absolute.
+__device__ __forceinline__ void box_muller_hw(uint32_t a, uint32_t b, float& n0, float& n1) {
+  const float r = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u01(a)));
+  const float ub = u01(b);
+  n0 = r * __builtin_amdgcn_cosf(ub);
+  n1 = r * __builtin_amdgcn_sinf(ub);
 }
 
 // XCD-aware tile order: consecutive hardware block ids land on different
