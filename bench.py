@@ -161,6 +161,8 @@ def main():
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--slabs", type=int, default=0,
+                    help="slab schedule of the timed steps (0: sequential only)")
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
                     help="run one plain device copy of this many GiB first (the calibration kernel of "
                          "HBM byte counters)")
@@ -205,25 +207,44 @@ def main():
         tune = {"chosen": chosen, "ms_per_variant": {k: [None if v is None else round(v, 3) for v in row]
                                                       for k, row in table.items()},
                 "seconds": round(time.perf_counter() - t_tune, 2)}
+
+    def timed_steps(ws):
+        """W warmup + K timed job steps; returns (seconds, pass ms, calls, var, status)."""
+        for i in range(args.warmup):
+            plan.grf_box(seed=args.seed + 1000 + i, delta=delta, aux_out=eta, workspace=ws, check=False)
+        torch.cuda.synchronize()
+        plan.pass_times(reset=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            _, _, var, status = plan.grf_box(seed=args.seed + i, delta=delta, aux_out=eta,
+                                             workspace=ws, check=False)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        pass_ms, ncalls = plan.pass_times(reset=True)
+        return t1 - t0, pass_ms, ncalls, var, status
+
+    # sequential schedule: one pass at a time, which gives each kernel's own
+    # duration (HIP events on its launch stream) for the roofline
     plan.enable_pass_timing(True)
-
-    for i in range(args.warmup):
-        plan.grf_box(seed=args.seed + 1000 + i, delta=delta, aux_out=eta, workspace=ws, check=False)
-    torch.cuda.synchronize()
-    plan.pass_times(reset=True)
-
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        _, _, var, status = plan.grf_box(seed=args.seed + i, delta=delta, aux_out=eta,
-                                         workspace=ws, check=False)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    pass_ms, ncalls = plan.pass_times(reset=True)
+    dt_seq, pass_ms, ncalls, var, status = timed_steps(ws)
     check_status(status, "box")          # null/NaN guard, outside the timed region
+    sched = {"sequential": {"ms_per_step": dt_seq / args.steps * 1e3,
+                            "pass_ms": {k: round(v, 3) for k, v in pass_ms.items()}}}
+    dt = dt_seq
+    if args.slabs >= 2:
+        # the production schedule: compute-bound and memory-bound passes overlap
+        del ws
+        torch.cuda.empty_cache()
+        plan.set_schedule(args.slabs)
+        ws = plan.alloc_workspace(dev)
+        dt_slab, pass_slab, _, var, status = timed_steps(ws)
+        check_status(status, "box")
+        sched["slab"] = {"nslab": args.slabs, "ms_per_step": dt_slab / args.steps * 1e3,
+                         "pass_ms_overlapped": {k: round(v, 3) for k, v in pass_slab.items()}}
+        dt = dt_slab
     sigma = float(var.item()) ** 0.5
 
-    dt = t1 - t0
     ms_per_step = dt / args.steps * 1e3
     value = cells * args.steps / dt
     dom = max(pass_ms, key=pass_ms.get)
@@ -252,8 +273,7 @@ def main():
         "job": {"model_bytes_per_cell": MODEL_BYTES_PER_CELL,
                 "model_GBps": MODEL_BYTES_PER_CELL * cells / (ms_per_step * 1e-3) / 1e9,
                 "model_frac": MODEL_BYTES_PER_CELL * cells / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBPS,
-                "pass_ms": {k: round(v, 3) for k, v in pass_ms.items()},
-                "timed_calls": ncalls, "sigma_delta_fp64": sigma,
+                "schedule": sched, "timed_calls": ncalls, "sigma_delta_fp64": sigma,
                 "variants": plan.get_variants(), "autotune": tune},
     }
     del ws

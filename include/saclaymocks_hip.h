@@ -69,8 +69,9 @@ typedef struct smk_box_plan smk_box_plan;
 int smk_box_plan_create(smk_box_plan** plan, int nx, int ny, int nz, double cell);
 int smk_box_plan_destroy(smk_box_plan* plan);
 
-/* Bytes of device workspace smk_grf_box needs (three k-space boxes and
- * per-block FP64 partial sums: 3 * nx*ny*(nz/2+1)*8 bytes + small). */
+/* Bytes of device workspace smk_grf_box needs under the plan's current
+ * schedule: three k-space boxes (four in the slab schedule) and per-block
+ * FP64 partial sums, i.e. 3 (or 4) * nx*ny*(nz/2+1)*8 bytes + small. */
 size_t smk_box_workspace_bytes(const smk_box_plan* plan);
 
 /* Fused GRF box: noise -> R2C -> x sqrt(P(k)/Vcell) -> C2R giving delta, and
@@ -96,22 +97,38 @@ int smk_grf_box(smk_box_plan* plan, uint64_t seed, const float* noise, const dou
                 int aux_kind, double aux_coef, float* delta_out, float* aux_out, void* workspace,
                 double* var_out, int* status_out, void* stream);
 
-/* Per-pass timing of smk_grf_box with HIP events recorded on the call's
- * stream (the reference prints wall times per step: make_boxes.py:44-56,90).
- * Passes: 0 z R2C, 1 y FFT, 2 x spectral, 3 y IFFT delta, 4 z C2R delta,
- * 5 y IFFT aux, 6 z C2R aux.  smk_box_pass_times waits for the last timed
- * call, writes the per-pass sums (ms) over the calls since the last reset
- * into ms_sum[0..nmax) and the number of calls into *ncalls. */
+/* Per-pass timing of smk_grf_box with HIP events recorded around each
+ * launch on the stream it runs on (the reference prints wall times per
+ * step: make_boxes.py:44-56,90).  Passes: 0 z R2C, 1 y FFT, 2 x spectral,
+ * 3 y IFFT delta, 4 z C2R delta, 5 y IFFT aux, 6 z C2R aux; in the slab
+ * schedule a pass sums its slab launches, which overlap other passes.
+ * smk_box_pass_times waits for the last timed call, writes the per-pass sums
+ * (ms) over the calls since the last reset into ms_sum[0..nmax) and the
+ * number of calls into *ncalls. */
 int smk_box_enable_pass_timing(smk_box_plan* plan, int enable);
 int smk_box_pass_times(smk_box_plan* plan, double* ms_sum, int nmax, int* ncalls, int reset);
+
+/* Schedule of smk_grf_box.  nslab = 0: the passes run one after the other on
+ * the caller's stream (3 work boxes).  nslab in [2, 16]: slab schedule on
+ * two plan-owned streams beside the caller's, joined back into it before
+ * the call's work ends (stream order is kept for the caller): z R2C and
+ * y FFT are pipelined over nslab x slabs, the spectral pass and the delta
+ * y IFFT over nslab kz slabs, so that compute-bound and memory-bound passes
+ * overlap.  Needs 4 work boxes: query smk_box_workspace_bytes after setting
+ * the schedule.  Results are identical to the sequential schedule's.  The
+ * reference's counterpart is its multiprocessing pool over x planes
+ * (make_boxes.py:100-117, ncpu workers). */
+int smk_box_set_schedule(smk_box_plan* plan, int nslab);
+int smk_box_get_schedule(const smk_box_plan* plan);
 
 /* Kernel variants per pass kind.  The reference leaves this choice to FFTW's
  * planner and its wisdom file (make_boxes.py:53-54,87, FFTW_MEASURE): here a
  * strided pass picks its register tile, lines per block and LDS exchange
- * form (variants 0..SMK_BOX_NVAR-1), a z-pencil pass its register tile
- * (0..1).  Variant 0 is the default of every pass; results of all variants
- * agree to FP32 rounding.  Kinds 0-4 are the passes of smk_grf_box; the
- * drop-in steps use kinds 0, 1, 3, 4 (y and z axes) and 5, 6 (x axis). */
+ * form and the swizzle of its LDS lines (variants 0..SMK_BOX_NVAR-1), a
+ * z-pencil pass its register tile (0..1).  Variant 0 is the default of every
+ * pass; results of all variants agree to FP32 rounding.  Kinds 0-4 are the
+ * passes of smk_grf_box; the drop-in steps use kinds 0, 1, 3, 4 (y and z
+ * axes) and 5, 6 (x axis). */
 #define SMK_PASS_Z_R2C 0
 #define SMK_PASS_Y_FWD 1
 #define SMK_PASS_X_SPECTRAL 2
@@ -120,8 +137,10 @@ int smk_box_pass_times(smk_box_plan* plan, double* ms_sum, int nmax, int* ncalls
 #define SMK_PASS_X_FWD 5
 #define SMK_PASS_X_INV 6
 #define SMK_PASS_COUNT 7
-#define SMK_BOX_NVAR 7
+#define SMK_BOX_NVAR 10
 
+/* SMK_BOX_NVAR of the library (for bindings that cannot read the header). */
+int smk_box_variant_count(void);
 /* 1 if variant `vid` exists for pass `kind` of this plan's sizes, else 0. */
 int smk_box_variant_valid(const smk_box_plan* plan, int kind, int vid);
 /* Select a variant (SMK_ERR_UNSUPPORTED if it does not exist) / read it. */
@@ -135,7 +154,9 @@ int smk_box_get_variant(const smk_box_plan* plan, int kind);
  * valid result of the call.  ms_out (host, may be NULL) receives
  * [5][SMK_BOX_NVAR] mean milliseconds per call (-1 for a variant that does
  * not exist); the passes of kinds 3 and 4 count their delta and aux runs.
- * Pass timing is left disabled and its sums cleared. */
+ * Variants are timed in the sequential schedule whatever the plan's
+ * schedule (which is kept).  Pass timing is left disabled and its sums
+ * cleared. */
 int smk_box_autotune(smk_box_plan* plan, uint64_t seed, const float* noise, const double* pk_params, int aux_kind,
                      double aux_coef, float* delta_out, float* aux_out, void* workspace, double* var_out,
                      int* status_out, void* stream, int reps, double* ms_out);

@@ -51,6 +51,9 @@ _SIGNATURES = {
     "smk_draw_grf_boxk": (_i, [_vp, _u64, _vp, _vp, _vp]),
     "smk_box_enable_pass_timing": (_i, [_vp, _i]),
     "smk_box_pass_times": (_i, [_vp, _c.POINTER(_d), _i, _c.POINTER(_i), _i]),
+    "smk_box_set_schedule": (_i, [_vp, _i]),
+    "smk_box_get_schedule": (_i, [_vp]),
+    "smk_box_variant_count": (_i, []),
     "smk_box_variant_valid": (_i, [_vp, _i, _i]),
     "smk_box_set_variant": (_i, [_vp, _i, _i]),
     "smk_box_get_variant": (_i, [_vp, _i]),

@@ -66,9 +66,22 @@ class BoxPlan:
         k = max(n.value, 1)
         return {name: ms[i] / k for i, name in enumerate(self.PASS_NAMES)}, n.value
 
+    def set_schedule(self, nslab):
+        """0: passes in sequence on the caller's stream; 2..16: slab schedule
+        overlapping compute- and memory-bound passes on plan-owned streams
+        (4 work boxes: allocate the workspace after this call)."""
+        _lib.check(_lib.lib().smk_box_set_schedule(self._h, int(nslab)), "smk_box_set_schedule")
+
+    @property
+    def schedule(self):
+        return int(_lib.lib().smk_box_get_schedule(self._h))
+
     # pass kinds of smk_box_set_variant (SMK_PASS_*)
     PASS_KINDS = ("z_r2c", "y_fwd", "x_spectral", "y_inv", "z_c2r", "x_fwd", "x_inv")
-    NVAR = 7
+
+    @property
+    def NVAR(self):
+        return int(_lib.lib().smk_box_variant_count())
 
     def variants(self):
         """{kind: [valid variant ids]} for this plan's sizes."""
@@ -95,13 +108,14 @@ class BoxPlan:
         var = torch.zeros(1, dtype=torch.float64, device=dev)
         status = torch.zeros(2, dtype=torch.int32, device=dev)
         pkp = (ctypes.c_double * 4)(*[float(x) for x in pk])
-        ms = (ctypes.c_double * (5 * self.NVAR))()
+        nvar = self.NVAR
+        ms = (ctypes.c_double * (5 * nvar))()
         rc = _lib.lib().smk_box_autotune(self._h, ctypes.c_uint64(int(seed) & (2**64 - 1)), _lib.ptr(noise), pkp,
                                          kind, float(aux_coef), _lib.ptr(delta),
                                          _lib.ptr(aux_out if kind else None), _lib.ptr(workspace), _lib.ptr(var),
                                          _lib.ptr(status), _lib.stream_ptr(stream), int(reps), ms)
         _lib.check(rc, "smk_box_autotune")
-        table = {k: [ms[i * self.NVAR + v] if ms[i * self.NVAR + v] >= 0 else None for v in range(self.NVAR)]
+        table = {k: [ms[i * nvar + v] if ms[i * nvar + v] >= 0 else None for v in range(nvar)]
                  for i, k in enumerate(self.PASS_KINDS[:5])}
         return self.get_variants(), table
 

@@ -71,6 +71,7 @@ struct StridedArgs {
   int aux_kind;
   int write_delta;
   uint32_t nz_nyq;       // kz index of the Nyquist plane (nz/2)
+  uint32_t kz0;          // kz index of this launch's first kz (slab launches)
   double* partial;       // per-block FP64 sums (spectral: w(kz)|delta_k|^2)
 };
 
@@ -206,7 +207,7 @@ __global__ __launch_bounds__(T*(N / E)) void k_strided(StridedArgs a) {
   } else {
     // v[e] holds X(kx = p + P e, ky, kz)
     const uint32_t iky = a.kz_outer ? inner : o;
-    const uint32_t ikz = a.kz_outer ? o : inner;
+    const uint32_t ikz = (a.kz_outer ? o : inner) + a.kz0;
     const float ky = valid ? a.kyv[iky] : 0.0f;
     const float kz = valid ? a.kzv[ikz] : 0.0f;
     const double wz = (ikz == 0 || ikz == a.nz_nyq) ? 1.0 : 2.0;
@@ -280,18 +281,19 @@ __global__ __launch_bounds__(T*(N / E)) void k_strided(StridedArgs a) {
 // Ze_k = (Z_k + conj Z_{M-k})/2, Zo_k = -i (Z_k - conj Z_{M-k})/2.
 // Output: trans == 0 -> numpy layout K[x][y][kz]; trans == 1 -> K[x][kz][y]
 // (the store phase then runs with lanes ordered pencil-fastest, i.e. along y).
+// Pencils pencil0 .. n_pencils-1 (a launch may cover a slab of the box).
 template <int M, int E, int T>
 __global__ __launch_bounds__(T*(M / E)) void k_z_r2c(const float* __restrict__ noise, uint64_t seed,
                                                      float2* __restrict__ K, const float2* __restrict__ tw,
-                                                     const float2* __restrict__ twr, uint64_t n_pencils, int trans,
-                                                     uint32_t ny) {
+                                                     const float2* __restrict__ twr, uint64_t pencil0,
+                                                     uint64_t n_pencils, int trans, uint32_t ny) {
   constexpr int P = M / E;
   constexpr int LL = lds_line_len(M);
   constexpr int NT = T * P;
   __shared__ float2 lds[T * LL];
   const int tid = threadIdx.x;
   const int p = tid % P, t = tid / P;
-  const uint64_t L0 = (uint64_t)blockIdx.x * T;
+  const uint64_t L0 = pencil0 + (uint64_t)blockIdx.x * T;
   const uint64_t L = L0 + t;
   const bool valid = L < n_pencils;
   float2* line = lds + t * LL;
@@ -474,6 +476,14 @@ constexpr int half_tile(int n) {
   return (e0 % 2 == 0 && plan_ok(n, e0 / 2) && n / (e0 / 2) <= 512) ? e0 / 2 : 0;
 }
 
+// Variants (n >= 512; "swz" ones only for the unpadded 2560-point line,
+// whose LDS exchange needs a swizzle; the two-tile ones only in spectral mode):
+//   0  E0, float2 exchange            5  E0, split exchange, SWS 3 (swz)
+//   1  E0, float2, SWS 2 (swz)        6  E1, float2, SWS 2 (swz) / SWS 0
+//   2  E0, float2, SWS 3 (swz)        7  E1, float2, SWS 3 (swz)
+//   3  E0, split exchange             8  E0, two register tiles, SWS 2 / 0
+//   4  E0, split, SWS 2 (swz)         9  E1, two register tiles, SWS 2 / 0
+// E0 = elems_per_thread(n), E1 = half_tile(n).
 constexpr SVar strided_variant(int n, int mode, int vid) {
   const int e0 = elems_per_thread(n);
   const int e1 = half_tile(n);
@@ -484,13 +494,19 @@ constexpr SVar strided_variant(int n, int mode, int vid) {
   if (n < 512) return none;
   const bool spec = mode == MODE_SPECTRAL;
   const bool swz = !lds_padded(n);
+  const int ts = lines_per_block(n, e0, 1, 1024);
+  const int t1 = e1 ? lines_per_block(n, e1, 0, 1024) : 0;
+  const int t1s = e1 ? lines_per_block(n, e1, 0, 512) : 0;
   switch (vid) {
     case 1: return swz ? SVar{e0, t0, 0, 1, 2} : none;
     case 2: return swz ? SVar{e0, t0, 0, 1, 3} : none;
-    case 3: return SVar{e0, lines_per_block(n, e0, 1, 1024), 1, 1, 0};
-    case 4: return e1 ? SVar{e1, lines_per_block(n, e1, 0, 1024), 0, 1, 0} : none;
-    case 5: return (spec && e1) ? SVar{e1, lines_per_block(n, e1, 0, 512), 0, 2, 0} : none;
-    case 6: return e1 ? SVar{e1, lines_per_block(n, e1, 1, 1024), 1, 1, 0} : none;
+    case 3: return SVar{e0, ts, 1, 1, 0};
+    case 4: return swz ? SVar{e0, ts, 1, 1, 2} : none;
+    case 5: return swz ? SVar{e0, ts, 1, 1, 3} : none;
+    case 6: return e1 ? SVar{e1, t1, 0, 1, swz ? 2 : 0} : none;
+    case 7: return (e1 && swz) ? SVar{e1, t1, 0, 1, 3} : none;
+    case 8: return spec ? SVar{e0, t0, 0, 2, swz ? 2 : 0} : none;
+    case 9: return (spec && e1) ? SVar{e1, t1s, 0, 2, swz ? 2 : 0} : none;
     default: return none;
   }
 }
@@ -521,6 +537,9 @@ hipError_t launch_strided_n(int vid, const StridedArgs& a, uint64_t n_outer, hip
     case 4: return launch_strided_nv<N, MODE, 4>(a, n_outer, s);
     case 5: return launch_strided_nv<N, MODE, 5>(a, n_outer, s);
     case 6: return launch_strided_nv<N, MODE, 6>(a, n_outer, s);
+    case 7: return launch_strided_nv<N, MODE, 7>(a, n_outer, s);
+    case 8: return launch_strided_nv<N, MODE, 8>(a, n_outer, s);
+    case 9: return launch_strided_nv<N, MODE, 9>(a, n_outer, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -566,15 +585,17 @@ bool pencil_variant_ok(int m, int vid) { return vid >= 0 && vid < SMK_NPVAR && p
 
 template <int M, int VID>
 hipError_t launch_z_r2c_v(const float* noise, uint64_t seed, float2* K, const float2* tw, const float2* twr,
-                          uint64_t np, int trans, uint32_t ny, hipStream_t s) {
+                          uint64_t p0, uint64_t np, int trans, uint32_t ny, hipStream_t s) {
   constexpr int E = pencil_E(M, VID);
   if constexpr (E == 0) {
     return hipErrorInvalidValue;
   } else {
     constexpr int T = pencil_T(M, E);
-    if ((np + T - 1) / T > 0x7fffffffull) return hipErrorInvalidConfiguration;
-    hipLaunchKernelGGL((k_z_r2c<M, E, T>), dim3((uint32_t)((np + T - 1) / T)), dim3(T * (M / E)), 0, s, noise,
-                       seed, K, tw, twr, np, trans, ny);
+    if (np <= p0) return hipSuccess;
+    const uint64_t nblk = (np - p0 + T - 1) / T;
+    if (nblk > 0x7fffffffull) return hipErrorInvalidConfiguration;
+    hipLaunchKernelGGL((k_z_r2c<M, E, T>), dim3((uint32_t)nblk), dim3(T * (M / E)), 0, s, noise, seed, K, tw, twr,
+                       p0, np, trans, ny);
     return hipGetLastError();
   }
 }
@@ -594,13 +615,14 @@ hipError_t launch_z_c2r_v(const float2* K, float* out, const float2* tw, const f
   }
 }
 
+// pencils [p0, np)
 hipError_t launch_z_r2c(int m, int vid, const float* noise, uint64_t seed, float2* K, const float2* tw,
-                        const float2* twr, uint64_t np, int trans, uint32_t ny, hipStream_t s) {
+                        const float2* twr, uint64_t p0, uint64_t np, int trans, uint32_t ny, hipStream_t s) {
   switch (m) {
 #define SMK_CASE(N)                                                                         \
   case N:                                                                                   \
-    return vid == 0 ? launch_z_r2c_v<N, 0>(noise, seed, K, tw, twr, np, trans, ny, s)       \
-                    : launch_z_r2c_v<N, 1>(noise, seed, K, tw, twr, np, trans, ny, s);
+    return vid == 0 ? launch_z_r2c_v<N, 0>(noise, seed, K, tw, twr, p0, np, trans, ny, s)   \
+                    : launch_z_r2c_v<N, 1>(noise, seed, K, tw, twr, p0, np, trans, ny, s);
     SMK_FFT_LENGTHS(SMK_CASE)
 #undef SMK_CASE
     default: return hipErrorInvalidValue;
@@ -654,19 +676,31 @@ std::vector<float> fftfreq_k(int n, double cell, bool half) {
 }  // namespace
 
 #define SMK_BOX_NPASS 7
+#define SMK_BOX_MAXSLAB 16
+// timed launches per call: 7 in the sequential schedule, at most
+// 5 * slabs + 2 in the slab schedule
+#define SMK_BOX_MAXMARK (5 * SMK_BOX_MAXSLAB + 2)
+#define SMK_BOX_NSEV (3 * SMK_BOX_MAXSLAB + 8)
 
 struct smk_box_plan {
   int nx, ny, nz, m, nzh;
   double cell;
-  // optional per-pass timing of smk_grf_box: events[i] is recorded before
-  // pass i, events[NPASS] after the last one; `pending` marks a call whose
-  // events have not been read yet
+  // optional per-pass timing of smk_grf_box: one (start, end) event pair
+  // per launch, recorded on the launch's stream and tagged with its pass
+  // slot; `pending` marks a call whose events have not been read yet
   int timing;
   int pending;
-  int npass_last;
-  hipEvent_t ev[SMK_BOX_NPASS + 1];
+  int nmark;
+  hipEvent_t tev[2 * SMK_BOX_MAXMARK];
+  int tslot[SMK_BOX_MAXMARK];
   double acc_ms[SMK_BOX_NPASS];
   int ncalls;
+  // slab schedule (nslab >= 2): plan-owned worker streams (default priority)
+  // and dependency events; 0 = sequential schedule on the caller's stream
+  int nslab;
+  hipStream_t wstream[2];
+  hipEvent_t sev[SMK_BOX_NSEV];
+  int have_sched;
   float2* tw_x;
   float2* tw_y;
   float2* tw_z;
@@ -721,7 +755,11 @@ extern "C" int smk_box_plan_create(smk_box_plan** plan, int nx, int ny, int nz, 
 extern "C" int smk_box_plan_destroy(smk_box_plan* p) {
   if (p == nullptr) return SMK_OK;
   if (p->timing) {
-    for (int i = 0; i <= SMK_BOX_NPASS; ++i) (void)hipEventDestroy(p->ev[i]);
+    for (int i = 0; i < 2 * SMK_BOX_MAXMARK; ++i) (void)hipEventDestroy(p->tev[i]);
+  }
+  if (p->have_sched) {
+    for (int i = 0; i < 2; ++i) (void)hipStreamDestroy(p->wstream[i]);
+    for (int i = 0; i < SMK_BOX_NSEV; ++i) (void)hipEventDestroy(p->sev[i]);
   }
   void* bufs[] = {p->tw_x, p->tw_y, p->tw_z, p->tw_r, p->kx, p->ky, p->kz};
   for (void* b : bufs)
@@ -733,10 +771,15 @@ extern "C" int smk_box_plan_destroy(smk_box_plan* p) {
 static uint64_t kbox_elems(const smk_box_plan* p) { return (uint64_t)p->nx * p->ny * p->nzh; }
 static uint64_t align256(uint64_t b) { return (b + 255) & ~(uint64_t)255; }
 
+// work boxes of smk_grf_box: 3 in the sequential schedule, 4 in the slab
+// schedule (the aux branch's y inverse needs its own output box while the
+// delta branch still reads A1)
+static int work_boxes(const smk_box_plan* p) { return p->nslab >= 2 ? 4 : 3; }
+
 extern "C" size_t smk_box_workspace_bytes(const smk_box_plan* p) {
   if (p == nullptr) return 0;
   const uint64_t kb = align256(kbox_elems(p) * sizeof(float2));
-  return (size_t)(3 * kb + align256(p->n_partial * sizeof(double)));
+  return (size_t)(work_boxes(p) * kb + align256(p->n_partial * sizeof(double)));
 }
 
 namespace {
@@ -766,7 +809,7 @@ StridedArgs nat_x_args(const smk_box_plan* p, float2* K) {
 
 int nat_forward(smk_box_plan* p, uint64_t seed, const float* noise, float2* K, hipStream_t s) {
   const uint64_t np = (uint64_t)p->nx * p->ny;
-  SMK_HIP_RET(launch_z_r2c(p->m, p->vid[SMK_PASS_Z_R2C], noise, seed, K, p->tw_z, p->tw_r, np, 0, (uint32_t)p->ny,
+  SMK_HIP_RET(launch_z_r2c(p->m, p->vid[SMK_PASS_Z_R2C], noise, seed, K, p->tw_z, p->tw_r, 0, np, 0, (uint32_t)p->ny,
                            s));
   SMK_HIP_RET(launch_strided<MODE_FWD>(p->ny, p->vid[SMK_PASS_Y_FWD], nat_y_args(p, K), (uint64_t)p->nx, s));
   SMK_HIP_RET(launch_strided<MODE_FWD>(p->nx, p->vid[SMK_PASS_X_FWD], nat_x_args(p, K), (uint64_t)p->ny, s));
@@ -827,15 +870,30 @@ namespace {
 // fold the events of the previous timed call into the accumulators
 int harvest_timing(smk_box_plan* p) {
   if (!p->timing || !p->pending) return SMK_OK;
-  SMK_HIP_CHECK(hipEventSynchronize(p->ev[p->npass_last]));
-  for (int i = 0; i < p->npass_last; ++i) {
+  for (int k = 0; k < p->nmark; ++k) {
+    SMK_HIP_CHECK(hipEventSynchronize(p->tev[2 * k + 1]));
     float ms = 0.f;
-    SMK_HIP_CHECK(hipEventElapsedTime(&ms, p->ev[i], p->ev[i + 1]));
-    p->acc_ms[i] += ms;
+    SMK_HIP_CHECK(hipEventElapsedTime(&ms, p->tev[2 * k], p->tev[2 * k + 1]));
+    p->acc_ms[p->tslot[k]] += ms;
   }
   p->ncalls += 1;
   p->pending = 0;
+  p->nmark = 0;
   return SMK_OK;
+}
+
+// launch `f` on stream s, bracketed by timing events tagged `slot` when
+// timing is enabled
+template <class F>
+hipError_t timed(smk_box_plan* p, int slot, hipStream_t s, F&& f) {
+  if (!p->timing || p->nmark >= SMK_BOX_MAXMARK) return f();
+  const int k = p->nmark++;
+  p->tslot[k] = slot;
+  hipError_t e = hipEventRecord(p->tev[2 * k], s);
+  if (e != hipSuccess) return e;
+  e = f();
+  if (e != hipSuccess) return e;
+  return hipEventRecord(p->tev[2 * k + 1], s);
 }
 
 }  // namespace
@@ -843,17 +901,18 @@ int harvest_timing(smk_box_plan* p) {
 extern "C" int smk_box_enable_pass_timing(smk_box_plan* p, int enable) {
   SMK_REQUIRE(p != nullptr, SMK_ERR_ARG, "plan is NULL");
   if (enable && !p->timing) {
-    for (int i = 0; i <= SMK_BOX_NPASS; ++i) SMK_HIP_CHECK(hipEventCreate(&p->ev[i]));
+    for (int i = 0; i < 2 * SMK_BOX_MAXMARK; ++i) SMK_HIP_CHECK(hipEventCreate(&p->tev[i]));
     p->timing = 1;
   } else if (!enable && p->timing) {
     int rc = harvest_timing(p);
     if (rc) return rc;
-    for (int i = 0; i <= SMK_BOX_NPASS; ++i) (void)hipEventDestroy(p->ev[i]);
+    for (int i = 0; i < 2 * SMK_BOX_MAXMARK; ++i) (void)hipEventDestroy(p->tev[i]);
     p->timing = 0;
   }
   for (int i = 0; i < SMK_BOX_NPASS; ++i) p->acc_ms[i] = 0.0;
   p->ncalls = 0;
   p->pending = 0;
+  p->nmark = 0;
   return SMK_OK;
 }
 
@@ -871,6 +930,74 @@ extern "C" int smk_box_pass_times(smk_box_plan* p, double* ms_sum, int nmax, int
   return SMK_OK;
 }
 
+extern "C" int smk_box_set_schedule(smk_box_plan* p, int nslab) {
+  SMK_REQUIRE(p != nullptr, SMK_ERR_ARG, "plan is NULL");
+  SMK_REQUIRE(nslab >= 0 && nslab <= SMK_BOX_MAXSLAB, SMK_ERR_ARG, "nslab must be in [0, %d] (got %d)",
+              SMK_BOX_MAXSLAB, nslab);
+  if (nslab >= 2 && !p->have_sched) {
+    for (int i = 0; i < 2; ++i) SMK_HIP_CHECK(hipStreamCreateWithFlags(&p->wstream[i], hipStreamNonBlocking));
+    for (int i = 0; i < SMK_BOX_NSEV; ++i) SMK_HIP_CHECK(hipEventCreateWithFlags(&p->sev[i], hipEventDisableTiming));
+    p->have_sched = 1;
+  }
+  p->nslab = nslab >= 2 ? nslab : 0;
+  return SMK_OK;
+}
+
+extern "C" int smk_box_get_schedule(const smk_box_plan* p) { return p ? p->nslab : -1; }
+
+namespace {
+
+// Arguments of the spectral x pass over kz planes [k0, k1).
+StridedArgs spectral_args(const smk_box_plan* p, const float2* in, float2* out, float2* aux, const double* pk,
+                          int aux_kind, double aux_coef, double* partial, uint32_t k0) {
+  StridedArgs a = tr_x_args(p, in, out);
+  a.in = in + (uint64_t)k0 * a.in_so;
+  a.out = out + (uint64_t)k0 * a.out_so;
+  a.aux = aux ? aux + (uint64_t)k0 * a.out_so : nullptr;
+  a.kxv = p->kx; a.kyv = p->ky; a.kzv = p->kz;
+  a.pkA = (float)pk[0];
+  a.pk_k0inv = (float)(1.0 / pk[1]);
+  a.pk_n = (float)pk[2];
+  a.pk_R = (float)pk[3];
+  a.vcell = (float)(p->cell * p->cell * p->cell);
+  a.norm = (float)(1.0 / ((double)p->nx * p->ny * p->nz));
+  a.aux_coef = (float)aux_coef;
+  a.aux_kind = aux_kind;
+  a.write_delta = 1;
+  a.nz_nyq = (uint32_t)p->m;
+  a.kz0 = k0;
+  a.partial = partial + strided_blocks(p->nx, MODE_SPECTRAL, p->vid[SMK_PASS_X_SPECTRAL], k0, (uint32_t)p->ny);
+  return a;
+}
+
+// y FFT of the lines x in [x0, x1) (all kz): A1 [x][kz][y] -> A2 [kz][ky][x]
+StridedArgs y_fwd_slab(const smk_box_plan* p, const float2* A1, float2* A2, uint32_t x0, uint32_t x1) {
+  StridedArgs a = tr_y_fwd_args(p, A1, A2);
+  a.in += (uint64_t)x0 * a.in_si;
+  a.out += (uint64_t)x0 * a.out_si;
+  a.n_inner = x1 - x0;
+  return a;
+}
+
+// y IFFT of the kz planes [k0, k1) (all x): [kz][x][ky] -> [x][y][kz]
+StridedArgs y_inv_slab(const smk_box_plan* p, const float2* in, float2* out, uint32_t k0, uint32_t k1) {
+  StridedArgs a = tr_y_inv_args(p, in, out);
+  a.in += (uint64_t)k0 * a.in_si;
+  a.out += (uint64_t)k0 * a.out_si;
+  a.n_inner = k1 - k0;
+  return a;
+}
+
+// slab boundaries: [b(i), b(i+1)) for i < n, multiples of `align` except the last
+uint32_t slab_bound(uint32_t total, int n, int i, uint32_t align) {
+  if (i >= n) return total;
+  uint64_t b = (uint64_t)total * (uint64_t)i / (uint64_t)n;
+  b = (b + align / 2) / align * align;
+  return (uint32_t)(b < total ? b : total);
+}
+
+}  // namespace
+
 extern "C" int smk_grf_box(smk_box_plan* p, uint64_t seed, const float* noise, const double* pk_params, int aux_kind,
                            double aux_coef, float* delta_out, float* aux_out, void* workspace, double* var_out,
                            int* status_out, void* stream) {
@@ -883,67 +1010,124 @@ extern "C" int smk_grf_box(smk_box_plan* p, uint64_t seed, const float* noise, c
   SMK_REQUIRE(pk_params[1] > 0.0, SMK_ERR_ARG, "pk k0 must be positive");
   hipStream_t s = (hipStream_t)stream;
   const uint64_t kb = align256(kbox_elems(p) * sizeof(float2));
+  const int nbox = work_boxes(p);
   float2* A1 = (float2*)workspace;
   float2* A2 = (float2*)((char*)workspace + kb);
   float2* A3 = (float2*)((char*)workspace + 2 * kb);
-  double* partial = (double*)((char*)workspace + 3 * kb);
+  float2* A4 = nbox == 4 ? (float2*)((char*)workspace + 3 * kb) : nullptr;
+  double* partial = (double*)((char*)workspace + nbox * kb);
   const uint64_t np = (uint64_t)p->nx * p->ny;
+  const bool has_aux = aux_kind != SMK_AUX_NONE;
+  const int vzr = p->vid[SMK_PASS_Z_R2C], vyf = p->vid[SMK_PASS_Y_FWD], vx = p->vid[SMK_PASS_X_SPECTRAL];
+  const int vyi = p->vid[SMK_PASS_Y_INV], vz = p->vid[SMK_PASS_Z_C2R];
 
   int rc = harvest_timing(p);
   if (rc) return rc;
-  int ev = 0;
-  auto mark = [&]() -> hipError_t { return p->timing ? hipEventRecord(p->ev[ev++], s) : hipSuccess; };
 
   if (status_out) {
     hipLaunchKernelGGL(k_clear_ints, dim3(1), dim3(64), 0, s, status_out, 2);
     SMK_LAUNCH_CHECK();
   }
-  SMK_HIP_RET(mark());
-  SMK_HIP_RET(launch_z_r2c(p->m, p->vid[SMK_PASS_Z_R2C], noise, seed, A1, p->tw_z, p->tw_r, np, 1, (uint32_t)p->ny,
-                           s));
-  SMK_HIP_RET(mark());
-  SMK_HIP_RET(launch_strided<MODE_FWD>(p->ny, p->vid[SMK_PASS_Y_FWD], tr_y_fwd_args(p, A1, A2), (uint64_t)p->nzh,
-                                       s));
-  SMK_HIP_RET(mark());
+  const uint64_t nb_spec = strided_blocks(p->nx, MODE_SPECTRAL, vx, (uint64_t)p->nzh, (uint32_t)p->ny);
 
-  StridedArgs a = tr_x_args(p, A2, A1);
-  a.aux = A3;
-  a.kxv = p->kx; a.kyv = p->ky; a.kzv = p->kz;
-  a.pkA = (float)pk_params[0];
-  a.pk_k0inv = (float)(1.0 / pk_params[1]);
-  a.pk_n = (float)pk_params[2];
-  a.pk_R = (float)pk_params[3];
-  a.vcell = (float)(p->cell * p->cell * p->cell);
-  a.norm = (float)(1.0 / ((double)p->nx * p->ny * p->nz));
-  a.aux_coef = (float)aux_coef;
-  a.aux_kind = aux_kind;
-  a.write_delta = 1;
-  a.nz_nyq = (uint32_t)p->m;
-  a.partial = partial;
-  const int vx = p->vid[SMK_PASS_X_SPECTRAL];
-  SMK_HIP_RET(launch_strided<MODE_SPECTRAL>(p->nx, vx, a, (uint64_t)p->nzh, s));
-  SMK_HIP_RET(mark());
-  if (var_out) {
-    const uint64_t nb = strided_blocks(p->nx, MODE_SPECTRAL, vx, (uint64_t)p->nzh, (uint32_t)p->ny);
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, s, partial, nb, var_out);
-    SMK_LAUNCH_CHECK();
+  if (p->nslab < 2) {
+    // ---- sequential schedule on the caller's stream
+    SMK_HIP_RET(timed(p, 0, s, [&] {
+      return launch_z_r2c(p->m, vzr, noise, seed, A1, p->tw_z, p->tw_r, 0, np, 1, (uint32_t)p->ny, s);
+    }));
+    SMK_HIP_RET(timed(p, 1, s, [&] {
+      return launch_strided<MODE_FWD>(p->ny, vyf, tr_y_fwd_args(p, A1, A2), (uint64_t)p->nzh, s);
+    }));
+    const StridedArgs a = spectral_args(p, A2, A1, has_aux ? A3 : nullptr, pk_params, aux_kind, aux_coef, partial, 0);
+    SMK_HIP_RET(timed(p, 2, s, [&] { return launch_strided<MODE_SPECTRAL>(p->nx, vx, a, (uint64_t)p->nzh, s); }));
+    if (var_out) {
+      hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, s, partial, nb_spec, var_out);
+      SMK_LAUNCH_CHECK();
+    }
+    SMK_HIP_RET(timed(p, 3, s, [&] {
+      return launch_strided<MODE_INV>(p->ny, vyi, tr_y_inv_args(p, A1, A2), (uint64_t)p->nx, s);
+    }));
+    SMK_HIP_RET(timed(p, 4, s, [&] {
+      return launch_z_c2r(p->m, vz, A2, delta_out, p->tw_z, p->tw_r, np, 1.0f, status_out, s);
+    }));
+    if (has_aux) {
+      SMK_HIP_RET(timed(p, 5, s, [&] {
+        return launch_strided<MODE_INV>(p->ny, vyi, tr_y_inv_args(p, A3, A1), (uint64_t)p->nx, s);
+      }));
+      SMK_HIP_RET(timed(p, 6, s, [&] {
+        return launch_z_c2r(p->m, vz, A1, aux_out, p->tw_z, p->tw_r, np, 1.0f, status_out ? status_out + 1 : nullptr,
+                            s);
+      }));
+    }
+  } else {
+    // ---- slab schedule.  Stage A (x slabs): z R2C on stream 1 feeding the
+    // y FFT on stream 2.  Stage B (kz slabs): spectral x pass on stream 1
+    // feeding the delta y IFFT (A1 -> A4) on stream 2.  Stage C: once the
+    // last spectral slab has read A2, the aux y IFFT (A3 -> A2) and its z C2R
+    // run on the caller's stream beside the delta z C2R on stream 2.  The
+    // compute-bound passes (R2C with its noise, spectral) overlap the
+    // memory-bound ones.
+    const int S = p->nslab;
+    hipStream_t s1 = p->wstream[0], s2 = p->wstream[1];
+    hipEvent_t* ev = p->sev;
+    int ne = 0;
+    auto rec = [&](hipStream_t st) -> hipEvent_t {
+      hipEvent_t e = ev[ne++];
+      (void)hipEventRecord(e, st);
+      return e;
+    };
+    hipEvent_t start = rec(s);
+    SMK_HIP_RET(hipStreamWaitEvent(s1, start, 0));
+    SMK_HIP_RET(hipStreamWaitEvent(s2, start, 0));
+    for (int i = 0; i < S; ++i) {
+      const uint32_t x0 = slab_bound((uint32_t)p->nx, S, i, 8), x1 = slab_bound((uint32_t)p->nx, S, i + 1, 8);
+      if (x1 <= x0) continue;
+      SMK_HIP_RET(timed(p, 0, s1, [&] {
+        return launch_z_r2c(p->m, vzr, noise, seed, A1, p->tw_z, p->tw_r, (uint64_t)x0 * p->ny, (uint64_t)x1 * p->ny,
+                            1, (uint32_t)p->ny, s1);
+      }));
+      SMK_HIP_RET(hipStreamWaitEvent(s2, rec(s1), 0));
+      SMK_HIP_RET(timed(p, 1, s2, [&] {
+        return launch_strided<MODE_FWD>(p->ny, vyf, y_fwd_slab(p, A1, A2, x0, x1), (uint64_t)p->nzh, s2);
+      }));
+    }
+    SMK_HIP_RET(hipStreamWaitEvent(s1, rec(s2), 0));
+    for (int i = 0; i < S; ++i) {
+      const uint32_t k0 = slab_bound((uint32_t)p->nzh, S, i, 8), k1 = slab_bound((uint32_t)p->nzh, S, i + 1, 8);
+      if (k1 <= k0) continue;
+      const StridedArgs a = spectral_args(p, A2, A1, has_aux ? A3 : nullptr, pk_params, aux_kind, aux_coef, partial,
+                                          k0);
+      SMK_HIP_RET(timed(p, 2, s1, [&] {
+        return launch_strided<MODE_SPECTRAL>(p->nx, vx, a, (uint64_t)(k1 - k0), s1);
+      }));
+      SMK_HIP_RET(hipStreamWaitEvent(s2, rec(s1), 0));
+      SMK_HIP_RET(timed(p, 3, s2, [&] {
+        return launch_strided<MODE_INV>(p->ny, vyi, y_inv_slab(p, A1, A4, k0, k1), (uint64_t)p->nx, s2);
+      }));
+    }
+    if (var_out) {
+      hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, s1, partial, nb_spec, var_out);
+      SMK_LAUNCH_CHECK();
+    }
+    hipEvent_t b_done = rec(s1);
+    SMK_HIP_RET(timed(p, 4, s2, [&] {
+      return launch_z_c2r(p->m, vz, A4, delta_out, p->tw_z, p->tw_r, np, 1.0f, status_out, s2);
+    }));
+    SMK_HIP_RET(hipStreamWaitEvent(s, b_done, 0));
+    if (has_aux) {
+      SMK_HIP_RET(timed(p, 5, s, [&] {
+        return launch_strided<MODE_INV>(p->ny, vyi, tr_y_inv_args(p, A3, A2), (uint64_t)p->nx, s);
+      }));
+      SMK_HIP_RET(timed(p, 6, s, [&] {
+        return launch_z_c2r(p->m, vz, A2, aux_out, p->tw_z, p->tw_r, np, 1.0f, status_out ? status_out + 1 : nullptr,
+                            s);
+      }));
+    }
+    // join: the caller's stream waits for stream 2 (stream 1 is already joined
+    // through b_done)
+    SMK_HIP_RET(hipStreamWaitEvent(s, rec(s2), 0));
   }
-  const int vyi = p->vid[SMK_PASS_Y_INV], vz = p->vid[SMK_PASS_Z_C2R];
-  SMK_HIP_RET(launch_strided<MODE_INV>(p->ny, vyi, tr_y_inv_args(p, A1, A2), (uint64_t)p->nx, s));
-  SMK_HIP_RET(mark());
-  SMK_HIP_RET(launch_z_c2r(p->m, vz, A2, delta_out, p->tw_z, p->tw_r, np, 1.0f, status_out, s));
-  SMK_HIP_RET(mark());
-  if (aux_kind != SMK_AUX_NONE) {
-    SMK_HIP_RET(launch_strided<MODE_INV>(p->ny, vyi, tr_y_inv_args(p, A3, A1), (uint64_t)p->nx, s));
-    SMK_HIP_RET(mark());
-    SMK_HIP_RET(launch_z_c2r(p->m, vz, A1, aux_out, p->tw_z, p->tw_r, np, 1.0f, status_out ? status_out + 1 : nullptr,
-                             s));
-    SMK_HIP_RET(mark());
-  }
-  if (p->timing) {
-    p->npass_last = ev - 1;
-    p->pending = 1;
-  }
+  if (p->timing) p->pending = 1;
   return SMK_OK;
 }
 
@@ -975,6 +1159,8 @@ bool variant_valid(const smk_box_plan* p, int kind, int vid) {
 
 }  // namespace
 
+extern "C" int smk_box_variant_count(void) { return SMK_BOX_NVAR; }
+
 extern "C" int smk_box_variant_valid(const smk_box_plan* p, int kind, int vid) {
   return (p != nullptr && variant_valid(p, kind, vid)) ? 1 : 0;
 }
@@ -1002,8 +1188,15 @@ extern "C" int smk_box_autotune(smk_box_plan* p, uint64_t seed, const float* noi
   const int nv = SMK_NVAR > SMK_NPVAR ? SMK_NVAR : SMK_NPVAR;
   int saved[SMK_PASS_COUNT];
   memcpy(saved, p->vid, sizeof(saved));
+  // variants are timed in the sequential schedule (one pass at a time); the
+  // workspace of either schedule is large enough for it
+  const int saved_nslab = p->nslab;
+  p->nslab = 0;
   int rc = smk_box_enable_pass_timing(p, 1);
-  if (rc) return rc;
+  if (rc) {
+    p->nslab = saved_nslab;
+    return rc;
+  }
   double best_ms[NK];
   int best[NK];
   for (int k = 0; k < NK; ++k) {
@@ -1048,6 +1241,7 @@ extern "C" int smk_box_autotune(smk_box_plan* p, uint64_t seed, const float* noi
     }
   }
   const int rc2 = smk_box_enable_pass_timing(p, 0);
+  p->nslab = saved_nslab;
   if (rc) {
     memcpy(p->vid, saved, sizeof(saved));
     return rc;

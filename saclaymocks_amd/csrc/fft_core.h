@@ -248,28 +248,60 @@ template <int N, int P, int R, int Ns> struct StageIdx {
   static constexpr int offJ(int b) { return KIND == 2 ? (P * b) % Ns : 0; }
 };
 
+// LDS slots of a stage's outputs: output r of butterfly b goes to point
+// idxD(b) + r*Ns.  Classes 0-2 keep one per-thread base (and R per-r slots
+// when an r*Ns offset does not fold); the butterfly offsets are immediates.
+template <int N, int P, int R, int Ns, int E, int SWS, int SWB> struct OutSlots {
+  using I = StageIdx<N, P, R, Ns>;
+  using M = LdsMap<N, SWS, SWB>;
+  static constexpr int B = E / R;
+  static constexpr bool fold_b = []() {
+    for (int b = 0; b < B; ++b)
+      if (!M::folds(I::offD(b))) return false;
+    return true;
+  }();
+  uint32_t p, sw, base0;
+  uint32_t sr[R];
+  __device__ __forceinline__ OutSlots(uint32_t p_, uint32_t sw_) : p(p_), sw(sw_), base0(0) {
+    if constexpr (I::KIND == 0) base0 = p * R;
+    else if constexpr (I::KIND == 1) base0 = (p / Ns) * (Ns * R) + p % Ns;
+    else if constexpr (I::KIND == 2) base0 = p;
+    if constexpr (I::KIND != 3 && fold_b) {
+      if constexpr (M::folds(Ns)) {
+        const uint32_t s0 = M::slot(base0, sw);
+#pragma unroll
+        for (int r = 0; r < R; ++r) sr[r] = s0 + (uint32_t)(M::padded ? r * Ns + (r * Ns) / 32 : r * Ns);
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) sr[r] = M::slot(base0 + (uint32_t)(r * Ns), sw);
+      }
+    }
+  }
+  __device__ __forceinline__ uint32_t at(int b, int r) const {
+    if constexpr (I::KIND == 3) {
+      const uint32_t j = p + (uint32_t)(P * b);
+      return M::slot((j / Ns) * (Ns * R) + j % Ns + (uint32_t)(r * Ns), sw);
+    } else if constexpr (fold_b) {
+      return M::fold(sr[r], I::offD(b));
+    } else {
+      return M::slot(base0 + (uint32_t)(I::offD(b) + r * Ns), sw);
+    }
+  }
+};
+
 template <int N, int P, int R, int Ns, bool INV, bool LAST, int E, int SWS, int SWB>
 __device__ __forceinline__ void fft_stage(float2 (&v)[E], float2* __restrict__ line, uint32_t p_in,
                                           const float2* tw_in, uint32_t sw) {
   using I = StageIdx<N, P, R, Ns>;
-  using M = LdsMap<N, SWS, SWB>;
   constexpr int B = E / R;
   const int z = opaque_zero();
   const float2* tw = tw_in + z;
   const uint32_t p = p_in + (uint32_t)z;
 
-  // per-thread bases
-  uint32_t jj0 = 0, base0 = 0;
-  if constexpr (I::KIND == 0) {
-    base0 = p * R;
-  } else if constexpr (I::KIND == 1) {
-    jj0 = p % Ns;
-    base0 = (p / Ns) * (Ns * R) + jj0;
-  } else if constexpr (I::KIND == 2) {
-    jj0 = p;
-    base0 = p;
-  }
-  // stage-uniform twiddles (class 1)
+  // twiddle exponent bases: uniform over b (class 1) or p + offJ(b) (class 2)
+  uint32_t jj0 = 0;
+  if constexpr (I::KIND == 1) jj0 = p % Ns;
+  else if constexpr (I::KIND == 2) jj0 = p;
   float2 wu[R];
   if constexpr (I::KIND == 1) {
 #pragma unroll
@@ -278,36 +310,18 @@ __device__ __forceinline__ void fft_stage(float2 (&v)[E], float2* __restrict__ l
       if (INV) wu[r].y = -wu[r].y;
     }
   }
-  // slots of the outputs relative to base0, when every butterfly offset folds
-  constexpr bool fold_b = []() {
-    for (int b = 0; b < B; ++b)
-      if (!M::folds(I::offD(b))) return false;
-    return true;
-  }();
-  uint32_t sr[R];
-  if constexpr (!LAST && I::KIND != 3 && fold_b) {
-    if constexpr (M::folds(Ns)) {
-      const uint32_t s0 = M::slot(base0, sw);
-#pragma unroll
-      for (int r = 0; r < R; ++r) sr[r] = s0 + (uint32_t)(M::padded ? r * Ns + (r * Ns) / 32 : r * Ns);
-    } else {
-#pragma unroll
-      for (int r = 0; r < R; ++r) sr[r] = M::slot(base0 + (uint32_t)(r * Ns), sw);
-    }
-  }
+  [[maybe_unused]] OutSlots<N, P, R, Ns, E, SWS, SWB> os(p, sw);
 
 #pragma unroll
   for (int b = 0; b < B; ++b) {
     float2 a[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) a[r] = v[b + r * B];
-    uint32_t jj = 0;
-    if constexpr (I::KIND == 3) jj = (p + (uint32_t)(P * b)) % Ns;
     if constexpr (I::KIND == 1) {
 #pragma unroll
       for (int r = 1; r < R; ++r) a[r] = cmul(a[r], wu[r]);
     } else if constexpr (I::KIND == 2 || I::KIND == 3) {
-      const uint32_t jb = I::KIND == 2 ? jj0 + (uint32_t)I::offJ(b) : jj;
+      const uint32_t jb = I::KIND == 2 ? jj0 + (uint32_t)I::offJ(b) : (p + (uint32_t)(P * b)) % Ns;
 #pragma unroll
       for (int r = 1; r < R; ++r) {
         float2 w = tw[r * jb * I::step];
@@ -319,17 +333,9 @@ __device__ __forceinline__ void fft_stage(float2 (&v)[E], float2* __restrict__ l
     if constexpr (LAST) {
 #pragma unroll
       for (int r = 0; r < R; ++r) v[b + r * B] = a[r];
-    } else if constexpr (I::KIND == 3) {
-      const uint32_t j = p + (uint32_t)(P * b);
-      const uint32_t idxD = (j / Ns) * (Ns * R) + jj;
-#pragma unroll
-      for (int r = 0; r < R; ++r) line[M::slot(idxD + (uint32_t)(r * Ns), sw)] = a[r];
-    } else if constexpr (fold_b) {
-#pragma unroll
-      for (int r = 0; r < R; ++r) line[M::fold(sr[r], I::offD(b))] = a[r];
     } else {
 #pragma unroll
-      for (int r = 0; r < R; ++r) line[M::slot(base0 + (uint32_t)(I::offD(b) + r * Ns), sw)] = a[r];
+      for (int r = 0; r < R; ++r) line[os.at(b, r)] = a[r];
     }
   }
 }
@@ -373,19 +379,20 @@ __device__ __forceinline__ void exchange_split(float2 (&v)[E], float* __restrict
   using M = LdsMap<N, SWS, SWB>;
   constexpr int B = E / R;
   const uint32_t p = p_in + (uint32_t)opaque_zero();
+  const OutSlots<N, P, R, Ns, E, SWS, SWB> os(p, sw);
+  const uint32_t g0 = M::slot(p, sw);
 #pragma unroll
   for (int half = 0; half < 2; ++half) {
 #pragma unroll
     for (int b = 0; b < B; ++b) {
-      const uint32_t j = p + (uint32_t)(P * b);
-      const uint32_t idxD = (j / Ns) * (Ns * R) + j % Ns;
 #pragma unroll
-      for (int r = 0; r < R; ++r) lf[M::slot(idxD + (uint32_t)(r * Ns), sw)] = half ? v[b + r * B].y : v[b + r * B].x;
+      for (int r = 0; r < R; ++r) lf[os.at(b, r)] = half ? v[b + r * B].y : v[b + r * B].x;
     }
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      const float x = lf[M::slot(p + (uint32_t)(P * e), sw)];
+      const uint32_t sl = M::folds(P) ? M::fold(g0, P * e) : M::slot(p + (uint32_t)(P * e), sw);
+      const float x = lf[sl];
       if (half) v[e].y = x; else v[e].x = x;
     }
     __syncthreads();

@@ -101,3 +101,58 @@ def test_set_variant_rejects_missing_variants(cuda):
     with pytest.raises(ValueError, match="does not exist"):
         plan.set_variant("x_spectral", 3)
     assert plan.get_variants()["x_spectral"] == 0
+
+
+@pytest.mark.parametrize("shape,nslab", [((32, 40, 48), 2), ((64, 48, 40), 3), ((40, 80, 24), 8),
+                                         ((2560, 8, 16), 4), ((16, 12, 1536), 16)])
+def test_slab_schedule_matches_sequential_bitwise(cuda, shape, nslab):
+    """The slab schedule runs the same kernels on the same lines: outputs,
+    variance and status must equal the sequential schedule's exactly."""
+    from saclaymocks_amd.box import BoxPlan
+    print(f"[slab schedule] shape={shape} nslab={nslab}", flush=True)
+    w = _noise(shape, 10)
+    noise = torch.from_numpy(w).to(cuda)
+    plan = BoxPlan(*shape, cell=2.19)
+    d0, e0, v0, s0 = plan.grf_box(noise=noise, aux="eta_zz")
+    d0, e0 = d0.clone(), e0.clone()
+    plan.set_schedule(nslab)
+    assert plan.schedule == nslab
+    ws = plan.alloc_workspace(cuda)
+    for _ in range(2):   # a second call reuses the plan's events and streams
+        d1, e1, v1, s1 = plan.grf_box(noise=noise, aux="eta_zz", workspace=ws)
+        torch.cuda.synchronize()
+        assert torch.equal(d1, d0) and torch.equal(e1, e0)
+        assert v1.item() == v0.item()
+        assert s1.cpu().tolist() == s0.cpu().tolist() == [2, 2]
+    ref = bo.grf_box(w.astype(np.float64), 2.19, aux="eta_zz")
+    assert relinf(d1.cpu().numpy(), ref["delta"]) < TOL
+    assert relinf(e1.cpu().numpy(), ref["eta_zz"]) < TOL
+    # device noise and no aux field
+    d2, _, _, _ = plan.grf_box(seed=3, aux=None, workspace=ws)
+    plan.set_schedule(0)
+    d3, _, _, _ = plan.grf_box(seed=3, aux=None)
+    assert torch.equal(d2, d3)
+
+
+def test_slab_schedule_on_a_side_stream_and_timing(cuda):
+    """Stream order for the caller: work queued on a side stream after the
+    call sees the finished box; per-pass timing sums the slab launches."""
+    from saclaymocks_amd.box import BoxPlan
+    shape = (64, 48, 40)
+    w = _noise(shape, 11)
+    noise = torch.from_numpy(w).to(cuda)
+    plan = BoxPlan(*shape, cell=2.19)
+    d0, _, _, _ = plan.grf_box(noise=noise, aux="eta_zz")
+    d0 = d0.clone()
+    plan.set_schedule(4)
+    plan.enable_pass_timing(True)
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        d1, e1, _, _ = plan.grf_box(noise=noise, aux="eta_zz", stream=side)
+        total = d1.double().sum()        # queued on `side` after the call
+    torch.cuda.synchronize()
+    assert torch.equal(d1, d0)
+    assert abs(total.item() - d0.double().sum().item()) == 0.0
+    times, ncalls = plan.pass_times()
+    assert ncalls == 1
+    assert all(t > 0 for t in times.values())
