@@ -43,6 +43,32 @@ PASS_BYTES_PER_CELL = {
 MODEL_BYTES_PER_CELL = 64   # 3 passes per 3D transform, inverses sharing the k read
 
 
+PASS_KIND = {"z_r2c": "z_r2c", "y_fft": "y_fwd", "x_spectral": "x_spectral", "y_ifft_delta": "y_inv",
+             "z_c2r_delta": "z_c2r", "y_ifft_aux": "y_inv", "z_c2r_aux": "z_c2r"}
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "box_c2_pmc_traffic.md")
+
+
+def measured_traffic(pass_name, vid, shape):
+    """HBM bytes per launch of a box pass from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE runs, each divided by what the counter reads for a
+    plain 4 GiB device copy), or None when the summary does not cover this kernel."""
+    try:
+        text = open(TRAFFIC_PROFILE).read()
+        doc = json.loads(text.split("```json", 1)[1].split("```", 1)[0])
+    except (OSError, IndexError, ValueError):
+        return None, None
+    if list(doc.get("shape", [])) != list(shape):
+        return None, None
+    row = doc["passes"].get(PASS_KIND[pass_name], {}).get(str(vid))
+    if row is None:
+        return None, None
+    copy_gbps = doc["copy_probe_bytes"] / (doc["copy_probe_ms"] * 1e-3) / 1e9
+    return row["fetch_bytes"] + row["write_bytes"], {"kernel": row["kernel"], "fetch_bytes": row["fetch_bytes"],
+                                                     "write_bytes": row["write_bytes"],
+                                                     "copy_probe_GBps": round(copy_gbps, 1),
+                                                     "source": os.path.relpath(TRAFFIC_PROFILE, ROOT)}
+
+
 def cpu_baseline(cell, seconds=12.0):
     """FP64 oracle of the same job on a 256^3 sample, timed on this host."""
     import numpy as np
@@ -250,6 +276,7 @@ def main():
     dom = max(pass_ms, key=pass_ms.get)
     dom_ms = pass_ms[dom]
     achieved = PASS_BYTES_PER_CELL[dom] * cells / (dom_ms * 1e-3) / 1e9
+    traffic, traffic_info = measured_traffic(dom, plan.get_variants()[PASS_KIND[dom]], (nx, ny, nz))
     out = {
         "metric": METRIC,
         "value": value,
@@ -267,9 +294,9 @@ def main():
                                "analytic P(k) = 2e4 (k/0.02)/(1+(k/0.02)^2.6) exp(-(1.5k)^2)",
                    "box": [nx, ny, nz], "cell": args.cell},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
                      "kernel": dom, "kernel_ms": dom_ms,
-                     "alg_bytes_per_cell": PASS_BYTES_PER_CELL[dom]},
+                     "alg_bytes_per_cell": PASS_BYTES_PER_CELL[dom], "traffic_pmc": traffic_info},
         "job": {"model_bytes_per_cell": MODEL_BYTES_PER_CELL,
                 "model_GBps": MODEL_BYTES_PER_CELL * cells / (ms_per_step * 1e-3) / 1e9,
                 "model_frac": MODEL_BYTES_PER_CELL * cells / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBPS,

@@ -151,8 +151,11 @@ __device__ __forceinline__ float2 aux_apply(float2 d, const AuxSel& s, float kx,
 // T lines of N points per block, P = N / E threads per line, lanes ordered
 // line-fastest.  XCH selects the LDS exchange (float2 line, or float line
 // with the split exchange: half the LDS per block).  TILES (spectral mode
-// only) is the number of register tiles: 1 recomputes the spectrum for the
-// aux field (4 line FFTs), 2 keeps delta_k and aux_k side by side (3 FFTs).
+// only) says where the aux spectrum waits while delta is inverse transformed:
+// 1 recomputes it (FFT(IFFT(d)) = N d: 4 line FFTs), 3 stashes it in the aux
+// output rows the same thread overwrites later (3 FFTs; the rows are still
+// in L2 / MALL when they are read back).  (A second register tile, 2, does
+// not fit: 2 x 40 points spill at E = 40.)
 // SWS places the line swizzle of unpadded lines (see LdsMap).
 // Global addresses are a block-uniform base, a per-lane offset computed once
 // and a uniform per-point offset; lanes past the last line read the block's
@@ -214,7 +217,8 @@ __global__ __launch_bounds__(T*(N / E)) void k_strided(StridedArgs a) {
     const float inv_vcell = 1.0f / a.vcell;
     const AuxSel sel = aux_sel(a.aux_kind, a.aux_coef);
     const bool has_aux = a.aux_kind != SMK_AUX_NONE;
-    float2 w[TILES == 2 ? E : 1];
+    float2* adst = has_aux ? a.aux + (uint64_t)o * a.out_so + (uint64_t)inner * a.out_si + (uint64_t)p * a.out_sk
+                           : nullptr;
     double acc = 0.0;
     {
       const int z1 = opaque_zero();
@@ -231,7 +235,10 @@ __global__ __launch_bounds__(T*(N / E)) void k_strided(StridedArgs a) {
         const float2 d = cscale(v[e], sp * a.norm);
         acc += (double)d.x * (double)d.x + (double)d.y * (double)d.y;
         v[e] = d;
-        if constexpr (TILES == 2) w[e] = aux_apply(d, sel, kx, ky, kz, kk, i == 0 && iky == 0 && ikz == 0, 1.0f);
+        if constexpr (TILES == 3) {
+          if (valid && has_aux) adst[(uint64_t)e * out_step] = aux_apply(d, sel, kx, ky, kz, kk,
+                                                                         i == 0 && iky == 0 && ikz == 0, 1.0f);
+        }
       }
     }
     acc = valid ? wz * acc : 0.0;
@@ -262,12 +269,13 @@ __global__ __launch_bounds__(T*(N / E)) void k_strided(StridedArgs a) {
         }
         line_fft<N, E, true, XCH, SWS, SWB>(v, lds, t, p, a.tw, sw);
       } else {
+        // read back the stash this thread wrote (same addresses)
+        const uint64_t st = out_step + (uint64_t)opaque_zero();
 #pragma unroll
-        for (int e = 0; e < E; ++e) v[e] = w[e];
+        for (int e = 0; e < E; ++e) v[e] = valid ? adst[(uint64_t)e * st] : make_float2(0.f, 0.f);
         line_fft<N, E, true, XCH, SWS, SWB>(v, lds, t, p, a.tw, sw);
       }
       if (valid) {
-        float2* adst = a.aux + (uint64_t)o * a.out_so + (uint64_t)inner * a.out_si + (uint64_t)p * a.out_sk;
         const uint64_t st = out_step + (uint64_t)opaque_zero();
 #pragma unroll
         for (int e = 0; e < E; ++e) adst[(uint64_t)e * st] = v[e];
@@ -451,11 +459,9 @@ bool length_supported(int n) {
 // --------------------------------------------------------- pass variants
 // A strided-pass variant: register tile E, lines per block T, LDS exchange
 // form XCH (0 float2 line, 1 float line with the split exchange), in
-// spectral mode the number of register tiles, and the swizzle shift SWS of
-// unpadded lines.  Variant 0 exists for every length; the others for
-// n >= 512, where the passes are long enough to tune.  Blocks of 1024
-// threads cap a kernel at 128 VGPRs, so the two-tile spectral variant keeps
-// to 512.
+// spectral mode where the aux spectrum waits (TILES, see k_strided), and the
+// swizzle shift SWS of unpadded lines.  Variant 0 exists for every length;
+// the others for n >= 512, where the passes are long enough to tune.
 struct SVar {
   int e, t, xch, tiles, sws;
 };
@@ -477,12 +483,12 @@ constexpr int half_tile(int n) {
 }
 
 // Variants (n >= 512; "swz" ones only for the unpadded 2560-point line,
-// whose LDS exchange needs a swizzle; the two-tile ones only in spectral mode):
+// whose LDS exchange needs a swizzle; the stash ones only in spectral mode):
 //   0  E0, float2 exchange            5  E0, split exchange, SWS 3 (swz)
 //   1  E0, float2, SWS 2 (swz)        6  E1, float2, SWS 2 (swz) / SWS 0
 //   2  E0, float2, SWS 3 (swz)        7  E1, float2, SWS 3 (swz)
-//   3  E0, split exchange             8  E0, two register tiles, SWS 2 / 0
-//   4  E0, split, SWS 2 (swz)         9  E1, two register tiles, SWS 2 / 0
+//   3  E0, split exchange             8  E0, aux stash in the output, SWS 2 / 0
+//   4  E0, split, SWS 2 (swz)         9  E1, aux stash, SWS 2 / 0
 // E0 = elems_per_thread(n), E1 = half_tile(n).
 constexpr SVar strided_variant(int n, int mode, int vid) {
   const int e0 = elems_per_thread(n);
@@ -496,7 +502,6 @@ constexpr SVar strided_variant(int n, int mode, int vid) {
   const bool swz = !lds_padded(n);
   const int ts = lines_per_block(n, e0, 1, 1024);
   const int t1 = e1 ? lines_per_block(n, e1, 0, 1024) : 0;
-  const int t1s = e1 ? lines_per_block(n, e1, 0, 512) : 0;
   switch (vid) {
     case 1: return swz ? SVar{e0, t0, 0, 1, 2} : none;
     case 2: return swz ? SVar{e0, t0, 0, 1, 3} : none;
@@ -505,8 +510,8 @@ constexpr SVar strided_variant(int n, int mode, int vid) {
     case 5: return swz ? SVar{e0, ts, 1, 1, 3} : none;
     case 6: return e1 ? SVar{e1, t1, 0, 1, swz ? 2 : 0} : none;
     case 7: return (e1 && swz) ? SVar{e1, t1, 0, 1, 3} : none;
-    case 8: return spec ? SVar{e0, t0, 0, 2, swz ? 2 : 0} : none;
-    case 9: return (spec && e1) ? SVar{e1, t1s, 0, 2, swz ? 2 : 0} : none;
+    case 8: return spec ? SVar{e0, t0, 0, 3, swz ? 2 : 0} : none;
+    case 9: return (spec && e1) ? SVar{e1, t1, 0, 3, swz ? 2 : 0} : none;
     default: return none;
   }
 }

@@ -289,27 +289,36 @@ template <int N, int P, int R, int Ns, int E, int SWS, int SWB> struct OutSlots 
   }
 };
 
+// Twiddles of butterfly b = 0 of a stage: W^(r * (p % Ns) * step), r < R
+// (entry 0 unused).  For class 1 they serve every butterfly of the stage.
+// fft_stages loads them for stage S+1 before the barriers of stage S, so the
+// loads are in flight while the block synchronises.
+template <int N, int R, int Ns, bool INV>
+__device__ __forceinline__ void stage_twiddles0(float2 (&w0)[8], const float2* tw_in, uint32_t p_in) {
+  if constexpr (Ns > 1) {
+    constexpr int step = N / (Ns * R);
+    const int z = opaque_zero();
+    const float2* tw = tw_in + z;
+    const uint32_t jj = (p_in + (uint32_t)z) % Ns;
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      w0[r] = tw[r * jj * step];
+      if (INV) w0[r].y = -w0[r].y;
+    }
+  }
+}
+
 template <int N, int P, int R, int Ns, bool INV, bool LAST, int E, int SWS, int SWB>
 __device__ __forceinline__ void fft_stage(float2 (&v)[E], float2* __restrict__ line, uint32_t p_in,
-                                          const float2* tw_in, uint32_t sw) {
+                                          const float2* tw_in, uint32_t sw, const float2 (&w0)[8]) {
   using I = StageIdx<N, P, R, Ns>;
   constexpr int B = E / R;
   const int z = opaque_zero();
   const float2* tw = tw_in + z;
   const uint32_t p = p_in + (uint32_t)z;
 
-  // twiddle exponent bases: uniform over b (class 1) or p + offJ(b) (class 2)
-  uint32_t jj0 = 0;
-  if constexpr (I::KIND == 1) jj0 = p % Ns;
-  else if constexpr (I::KIND == 2) jj0 = p;
-  float2 wu[R];
-  if constexpr (I::KIND == 1) {
-#pragma unroll
-    for (int r = 1; r < R; ++r) {
-      wu[r] = tw[r * jj0 * I::step];
-      if (INV) wu[r].y = -wu[r].y;
-    }
-  }
+  // twiddle exponent base of butterfly b: p % Ns (class 1), p + offJ(b)
+  // (class 2); butterfly 0 takes the preloaded w0
   [[maybe_unused]] OutSlots<N, P, R, Ns, E, SWS, SWB> os(p, sw);
 
 #pragma unroll
@@ -319,14 +328,19 @@ __device__ __forceinline__ void fft_stage(float2 (&v)[E], float2* __restrict__ l
     for (int r = 0; r < R; ++r) a[r] = v[b + r * B];
     if constexpr (I::KIND == 1) {
 #pragma unroll
-      for (int r = 1; r < R; ++r) a[r] = cmul(a[r], wu[r]);
+      for (int r = 1; r < R; ++r) a[r] = cmul(a[r], w0[r]);
     } else if constexpr (I::KIND == 2 || I::KIND == 3) {
-      const uint32_t jb = I::KIND == 2 ? jj0 + (uint32_t)I::offJ(b) : (p + (uint32_t)(P * b)) % Ns;
+      if (b == 0) {
 #pragma unroll
-      for (int r = 1; r < R; ++r) {
-        float2 w = tw[r * jb * I::step];
-        if (INV) w.y = -w.y;
-        a[r] = cmul(a[r], w);
+        for (int r = 1; r < R; ++r) a[r] = cmul(a[r], w0[r]);
+      } else {
+        const uint32_t jb = I::KIND == 2 ? p + (uint32_t)I::offJ(b) : (p + (uint32_t)(P * b)) % Ns;
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          float2 w = tw[r * jb * I::step];
+          if (INV) w.y = -w.y;
+          a[r] = cmul(a[r], w);
+        }
       }
     }
     DFT<R, INV>::run(a);
@@ -401,35 +415,39 @@ __device__ __forceinline__ void exchange_split(float2 (&v)[E], float* __restrict
 
 template <int N, int E, bool INV, int S, int Ns, int SWS, int SWB>
 __device__ __forceinline__ void fft_stages(float2 (&v)[E], float2* __restrict__ line, uint32_t p, const float2* tw,
-                                           uint32_t sw) {
+                                           uint32_t sw, const float2 (&w0)[8]) {
   constexpr RadixPlan pl = make_plan(N, E);
   static_assert(pl.nstage > 0, "unsupported FFT length");
   constexpr int P = N / E;
   if constexpr (S < pl.nstage) {
     constexpr int R = pl.r[S];
     constexpr bool LAST = (S + 1 == pl.nstage);
-    fft_stage<N, P, R, Ns, INV, LAST, E, SWS, SWB>(v, line, p, tw, sw);
+    fft_stage<N, P, R, Ns, INV, LAST, E, SWS, SWB>(v, line, p, tw, sw, w0);
     if constexpr (!LAST) {
+      float2 w1[8];
+      stage_twiddles0<N, pl.r[S + 1], Ns * R, INV>(w1, tw, p);
       __syncthreads();
       lds_gather<N, P, E, SWS, SWB>(v, line, p, sw);
       __syncthreads();
-      fft_stages<N, E, INV, S + 1, Ns * R, SWS, SWB>(v, line, p, tw, sw);
+      fft_stages<N, E, INV, S + 1, Ns * R, SWS, SWB>(v, line, p, tw, sw, w1);
     }
   }
 }
 
 template <int N, int E, bool INV, int S, int Ns, int SWS, int SWB>
 __device__ __forceinline__ void fft_stages_split(float2 (&v)[E], float* __restrict__ lf, uint32_t p,
-                                                 const float2* tw, uint32_t sw) {
+                                                 const float2* tw, uint32_t sw, const float2 (&w0)[8]) {
   constexpr RadixPlan pl = make_plan(N, E);
   static_assert(pl.nstage > 0, "unsupported FFT length");
   constexpr int P = N / E;
   if constexpr (S < pl.nstage) {
     constexpr int R = pl.r[S];
-    fft_stage<N, P, R, Ns, INV, true, E, SWS, SWB>(v, nullptr, p, tw, sw);
+    fft_stage<N, P, R, Ns, INV, true, E, SWS, SWB>(v, nullptr, p, tw, sw, w0);
     if constexpr (S + 1 < pl.nstage) {
+      float2 w1[8];
+      stage_twiddles0<N, pl.r[S + 1], Ns * R, INV>(w1, tw, p);
       exchange_split<N, P, R, Ns, E, SWS, SWB>(v, lf, p, sw);
-      fft_stages_split<N, E, INV, S + 1, Ns * R, SWS, SWB>(v, lf, p, tw, sw);
+      fft_stages_split<N, E, INV, S + 1, Ns * R, SWS, SWB>(v, lf, p, tw, sw, w1);
     }
   }
 }
@@ -441,7 +459,8 @@ __device__ __forceinline__ void fft_stages_split(float2 (&v)[E], float* __restri
 template <int N, bool INV>
 __device__ __forceinline__ void fft_line(float2 (&v)[elems_per_thread(N)], float2* __restrict__ line, uint32_t p,
                                          const float2* tw) {
-  fft_stages<N, elems_per_thread(N), INV, 0, 1, 0, 0>(v, line, p, tw, 0);
+  const float2 w0[8] = {};
+  fft_stages<N, elems_per_thread(N), INV, 0, 1, 0, 0>(v, line, p, tw, 0, w0);
 }
 
 // Same with an explicit register tile of E points per thread (P = N / E) and
@@ -450,14 +469,16 @@ __device__ __forceinline__ void fft_line(float2 (&v)[elems_per_thread(N)], float
 template <int N, int E, bool INV, int SWS = 0, int SWB = 0>
 __device__ __forceinline__ void fft_line_e(float2 (&v)[E], float2* __restrict__ line, uint32_t p,
                                            const float2* tw, uint32_t sw = 0) {
-  fft_stages<N, E, INV, 0, 1, SWS, SWB>(v, line, p, tw, sw);
+  const float2 w0[8] = {};
+  fft_stages<N, E, INV, 0, 1, SWS, SWB>(v, line, p, tw, sw, w0);
 }
 
 // Same through a float LDS line of lds_line_len(N) floats (split exchange).
 template <int N, int E, bool INV, int SWS = 0, int SWB = 0>
 __device__ __forceinline__ void fft_line_split(float2 (&v)[E], float* __restrict__ lf, uint32_t p,
                                                const float2* tw, uint32_t sw = 0) {
-  fft_stages_split<N, E, INV, 0, 1, SWS, SWB>(v, lf, p, tw, sw);
+  const float2 w0[8] = {};
+  fft_stages_split<N, E, INV, 0, 1, SWS, SWB>(v, lf, p, tw, sw, w0);
 }
 
 // ------------------------------------------------------ counter-based noise
