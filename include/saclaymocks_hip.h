@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define SMK_ABI_VERSION 1
+#define SMK_ABI_VERSION 2
 
 #define SMK_OK 0
 #define SMK_ERR_ARG (-1)         /* bad argument (null pointer, size, enum) */
@@ -196,6 +196,11 @@ typedef struct smk_box_view {
   int32_t periodic;     /* 1: wrap cell indices; 0: a skewer leaving the box is flagged */
   double cell;          /* DX = DY = DZ, Mpc/h */
   double r0;            /* R0 = h chi(z0), Mpc/h */
+  /* x slab resident on the device (make_spectra.py:220-226 reads planes
+   * [iXmin, iXmax) of the box files): nxs > 0 means the field pointers point
+   * at plane x0 and hold planes [x0, x0 + nxs) of the nx-plane box; a cell
+   * outside them is never read (status bit 2).  nxs = 0: the whole box. */
+  int32_t x0, nxs;
 } smk_box_view;
 
 /* A batch of skewers on the global pixel grid R_g = rmin + g * dr (Mpc/h),
@@ -231,7 +236,8 @@ typedef struct smk_skewers {
  *   status      int32 per skewer (zeroed by the caller): bit 0 set if a kernel
  *               cell left a non-periodic box (the reference skips such a QSO,
  *               make_spectra.py:490-494); bit 1 if an FGPA row was out of
- *               range (flux NaN). */
+ *               range (flux NaN); bit 2 if a kernel cell lay outside the
+ *               resident x slab (smk_box_view.x0, nxs). */
 int smk_forest_gather(const smk_skewers* sk, const smk_box_view* box, int interp, const float* abcg,
                       int64_t abcg_rows, const float* delta_s, float* delta_l, float* eta_par, float* flux,
                       int32_t* cells, int32_t* status, void* stream);
@@ -257,6 +263,22 @@ int smk_forest_missing_power(smk_forest_plan* plan, const smk_skewers* sk, const
                              const int64_t* filt_off, uint64_t seed, const float* noise,
                              const int64_t* noise_off, void* workspace, float* delta_s, void* stream);
 
+/* Missing power with a P1D_miss table chosen per skewer, merge_spectra.py:311-322
+ * without --fit-p1d: P1D_miss(z, k) is the table at the redshift nearest the
+ * QSO's mean forest redshift z_eff (util.py:448-474, InterpP1Dmissing), and
+ * delta_s is then scaled per pixel by sigma_s(z) / sigma_s(z_eff).
+ *   filt        n_sets filter sets, set i at filt + i * set_stride, each laid
+ *               out as the filt of smk_forest_missing_power (same filt_off)
+ *   filt_set    device int32 per skewer in [0, n_sets) (SMK_ERR_RANGE else),
+ *               or NULL: set 0 for every skewer
+ *   scale       NULL or device float32 per pixel multiplying delta_s
+ * Other arguments as smk_forest_missing_power. */
+int smk_forest_missing_power_sets(smk_forest_plan* plan, const smk_skewers* sk, const float* filt,
+                                  const int64_t* filt_off, int64_t set_stride, int32_t n_sets,
+                                  const int32_t* filt_set, const float* scale, uint64_t seed,
+                                  const float* noise, const int64_t* noise_off, void* workspace,
+                                  float* delta_s, void* stream);
+
 /* ------------------------------------------------------------- FGPA fit */
 /* The fit kernels need skewers.pset (one parameter row per set, e.g. a
  * redshift bin) and use the forest range [imin, imax) of every skewer, the
@@ -264,8 +286,11 @@ int smk_forest_missing_power(smk_forest_plan* plan, const smk_skewers* sk, const
 
 /* comp_spectra (fit_az.py:245-256): F = exp(-a exp(b G (delta + c eta)))
  * for every pixel (delta = delta_l + delta_s, eta may be NULL), FP64 as the
- * reference (fit_az.py:101-110 promotes the fields to float64);
- *   abcg   device double [rows][4] = (a, b, c, G), row = skewers.pset[s]
+ * reference (fit_az.py:101-110 promotes the fields to float64); also the
+ * FGPA of merged spectra (merge_spectra.py:335-339);
+ *   abcg   device double [rows][4] = (a, b, c, G), row = skewers.pset[s], or
+ *          with pset NULL the pixel's global grid index g0[s] + i (a, b, c
+ *          and G at each pixel's redshift, merge_spectra.py:282-296)
  *   flux   device double per pixel
  *   fsum   device double per skewer: sum of F over the forest. */
 int smk_fgpa_flux(const smk_skewers* sk, const float* delta, const float* eta, const double* abcg,

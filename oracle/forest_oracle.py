@@ -198,6 +198,68 @@ def missing_power(noise, n, pixel, p1d):
     return np.fft.irfft(dk, n=nz)[:n]
 
 
+MP_STREAM_TAG = 0x6D697373   # Philox counter word 3 of the missing-power modes
+
+
+def device_noise_modes(seed, noise_id, nz):
+    """The Fourier modes of N(0,1) white noise of length nz = 2M that the
+    device draws instead of numpy's normal draw of merge_spectra.py:310-311:
+    mode k in [0, M] takes Philox4x32-10(counter = (k, id_lo, id_hi, tag),
+    key = seed) and Box-Muller of its words 0 and 1 -> (n0, n1); interior
+    modes are (n0 + i n1) sqrt(M) (Re, Im ~ N(0, M), as rfft of white
+    noise), k = 0 and k = M are real, n0 sqrt(2M)."""
+    from oracle import philox
+    M = nz // 2
+    k = np.arange(M + 1, dtype=np.uint64)
+    nid = np.uint64(noise_id)
+    seed = np.uint64(seed)
+    lo = np.uint32(nid & np.uint64(0xFFFFFFFF))
+    hi = np.uint32(nid >> np.uint64(32))
+    w = philox.philox4x32_10(k.astype(np.uint32), np.full(M + 1, lo), np.full(M + 1, hi),
+                             np.full(M + 1, MP_STREAM_TAG, np.uint32), np.uint32(seed & np.uint64(0xFFFFFFFF)),
+                             np.uint32(seed >> np.uint64(32)))
+    r = np.sqrt(-2.0 * np.log(philox.u01(w[0])))
+    th = 2.0 * np.pi * philox.u01(w[1])
+    n0, n1 = r * np.cos(th), r * np.sin(th)
+    y = (n0 + 1j * n1) * np.sqrt(M)
+    y[0] = n0[0] * np.sqrt(2.0 * M)
+    y[M] = n0[M] * np.sqrt(2.0 * M)
+    return y
+
+
+def missing_power_device(seed, noise_id, n, pixel, p1d):
+    """merge_spectra.py:308-322 on the device-drawn modes: irfft(Y sqrt(P1D(k)
+    / pixel))[:n], nz = 256 * 2^m >= n + 50."""
+    nz = fft_len_for(n)
+    k = np.fft.rfftfreq(nz) * 2 * (np.pi / pixel)
+    y = device_noise_modes(seed, noise_id, nz) * np.sqrt(np.maximum(p1d(k), 0.0) / pixel)
+    return np.fft.irfft(y, n=nz)[:n]
+
+
+def p1d_table_index(z_tab, zeff):
+    """util.py:469-474 (InterpP1Dmissing.__call__): the table whose redshift is
+    nearest zeff (first one on a tie)."""
+    return int(np.argmin(np.abs(np.asarray(z_tab, dtype=np.float64) - zeff)))
+
+
+def p1d_table(k_tab, pk_row):
+    """util.py:462-467: P1D_miss(k) of one table row, linear in k
+    (scipy.interpolate.interp1d, which raises outside [k_min, k_max])."""
+    from scipy.interpolate import interp1d
+    return interp1d(np.asarray(k_tab, dtype=np.float64), np.asarray(pk_row, dtype=np.float64))
+
+
+def missing_power_zdep(noise, n, pixel, z_tab, k_tab, pk_tab, sig_z, sig, z_pix, zeff):
+    """merge_spectra.py:310-322 without --fit-p1d: the P1D_miss table nearest
+    zeff, then delta_s scaled by sigma_s(z) / sigma_s(zeff) (sigma_s linear in
+    z, merge_spectra.py:116)."""
+    from scipy.interpolate import interp1d
+    i = p1d_table_index(z_tab, zeff)
+    ds = missing_power(noise, n, pixel, p1d_table(k_tab, pk_tab[i]))
+    s = interp1d(np.asarray(sig_z, dtype=np.float64), np.asarray(sig, dtype=np.float64))
+    return ds * s(np.asarray(z_pix, dtype=np.float64)) / s(zeff)
+
+
 def p1d_missing_analytic(k, A=0.05, k1=1.8):
     """P1D_miss(k) = A exp(-(k/k1)^2) (Mpc/h)."""
     return A * np.exp(-(np.asarray(k) / k1) ** 2)

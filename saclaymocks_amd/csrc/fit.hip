@@ -42,7 +42,8 @@ __device__ __forceinline__ double4 prow(const double* abcg, int64_t rows, int64_
 
 // F for every pixel, in FP64 so that chi2(theta) is smooth to FP64 rounding
 // (the optimisers' line searches need that); per-skewer sum of F over the
-// forest [imin, imax)
+// forest [imin, imax).  Parameter row: pset[s], or with pset NULL the pixel's
+// global grid index g0[s] + i (parameters at the pixel's redshift).
 __global__ __launch_bounds__(FIT_THREADS) void k_fgpa_flux(smk_skewers sk, const float* __restrict__ delta,
                                                            const float* __restrict__ eta,
                                                            const double* __restrict__ abcg, int64_t rows,
@@ -51,9 +52,11 @@ __global__ __launch_bounds__(FIT_THREADS) void k_fgpa_flux(smk_skewers sk, const
   for (int s = blockIdx.x; s < sk.n; s += gridDim.x) {
     const int64_t o = sk.off[s];
     const int n = sk.npix[s], i0 = sk.imin[s], i1 = sk.imax[s];
-    const double4 p = prow(abcg, rows, sk.pset[s]);
+    const bool per_pixel = sk.pset == nullptr;
+    const double4 ps = per_pixel ? make_double4(0.0, 0.0, 0.0, 0.0) : prow(abcg, rows, sk.pset[s]);
     double acc = 0.0;
     for (int i = threadIdx.x; i < n; i += FIT_THREADS) {
+      const double4 p = per_pixel ? prow(abcg, rows, (int64_t)sk.g0[s] + i) : ps;
       const double g = (double)delta[o + i] + p.z * (eta ? (double)eta[o + i] : 0.0);
       const double f = exp(-p.x * exp(p.y * p.w * g));
       flux[o + i] = f;
@@ -340,7 +343,6 @@ extern "C" int smk_fgpa_flux(const smk_skewers* sk, const float* delta, const fl
                              int64_t abcg_rows, double* flux, double* fsum, void* stream) {
   SMK_REQUIRE(sk != nullptr && delta != nullptr && abcg != nullptr && flux != nullptr && fsum != nullptr,
               SMK_ERR_ARG, "NULL argument");
-  SMK_REQUIRE(sk->pset != nullptr, SMK_ERR_ARG, "the fit kernels need skewers.pset");
   if (sk->n <= 0) return SMK_OK;
   hipLaunchKernelGGL(k_fgpa_flux, dim3(grid_for(sk->n)), dim3(FIT_THREADS), 0, (hipStream_t)stream, *sk, delta,
                      eta, abcg, abcg_rows, flux, fsum);

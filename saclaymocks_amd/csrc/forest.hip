@@ -90,9 +90,11 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
   const bool in_forest = i >= sk.imin[s] && i < sk.imax[s];
   const bool radial = sk.geometry == SMK_GEOM_RADIAL;
   float dl = -1000000.0f, eta = 0.0f;
-  bool outside = false;
+  bool outside = false, off_slab = false;
   if (in_forest) {
     const int nx = b.nx, ny = b.ny, nz = b.nz;
+    // resident x planes [x0, x0 + nxs); the field pointers point at plane x0
+    const int64_t x0 = b.nxs > 0 ? b.x0 : 0, nxs = b.nxs > 0 ? b.nxs : nx;
     // fields: 0 = delta, 1..6 = eta_xx..eta_yz (radial) or 1 = eta_zz (plane)
     const int nf = b.eta[2] == nullptr ? 1 : (radial ? 7 : 2);
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -108,6 +110,7 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
         const double ex = __dsub_rn(__dadd_rn(da * D, cx), X);
         int64_t jx = ix + da;
         if (b.periodic) jx = wrap(jx, nx); else if (jx < 0 || jx >= nx) outside = true;
+        if (jx < x0 || jx >= x0 + nxs) off_slab = true;
         for (int db = -3; db <= 3; ++db) {
           const double ey = __dsub_rn(__dadd_rn(db * D, cy), Y);
           int64_t jy = iy + db;
@@ -116,18 +119,18 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
             const double ez = __dsub_rn(__dadd_rn(dc * D, cz), Z);
             int64_t jz = iz + dc;
             if (b.periodic) jz = wrap(jz, nz); else if (jz < 0 || jz >= nz) outside = true;
-            if (outside) continue;
+            if (outside || off_slab) continue;
             const double r2 = __dadd_rn(__dadd_rn(__dmul_rn(ex, ex), __dmul_rn(ey, ey)), __dmul_rn(ez, ez));
             const double w = exp(-r2 / sig2);
             wsum += w;
-            const int64_t flat = (jx * ny + jy) * nz + jz;
+            const int64_t flat = ((jx - x0) * ny + jy) * nz + jz;
             for (int f = 0; f < nf; ++f) acc[f] += w * (double)fld[f][flat];
           }
         }
       }
       for (int f = 0; f < nf; ++f) acc[f] /= wsum;
     } else {
-      // trilinear on cell-centred values, periodic
+      // trilinear on cell-centred values; indices wrap in a periodic box
       const double ux = __ddiv_rn(__dadd_rn(X, a.hx), D) - 0.5;
       const double uy = __ddiv_rn(__dadd_rn(Y, a.hy), D) - 0.5;
       const double uz = __ddiv_rn(__dsub_rn(__dadd_rn(Z, a.hz), b.r0), D) - 0.5;
@@ -137,14 +140,24 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
       for (int c = 0; c < 8; ++c) {
         const int ox = c >> 2, oy = (c >> 1) & 1, oz = c & 1;
         const double w = (ox ? fx : 1.0 - fx) * (oy ? fy : 1.0 - fy) * (oz ? fz : 1.0 - fz);
-        const int64_t flat = ((int64_t)wrap(i0x + ox, nx) * ny + wrap(i0y + oy, ny)) * nz + wrap(i0z + oz, nz);
+        int64_t jx = i0x + ox, jy = i0y + oy, jz = i0z + oz;
+        if (b.periodic) {
+          jx = wrap(jx, nx);
+          jy = wrap(jy, ny);
+          jz = wrap(jz, nz);
+        } else if (jx < 0 || jx >= nx || jy < 0 || jy >= ny || jz < 0 || jz >= nz) {
+          outside = true;
+        }
+        if (jx < x0 || jx >= x0 + nxs) off_slab = true;
+        if (outside || off_slab) continue;
+        const int64_t flat = ((jx - x0) * ny + jy) * nz + jz;
         for (int f = 0; f < nf; ++f) acc[f] += w * (double)fld[f][flat];
       }
     }
-    if (outside) {
+    if (outside || off_slab) {
       dl = __int_as_float(0x7fc00000);
       eta = dl;
-      if (a.status) atomicOr(&a.status[s], 1);
+      if (a.status) atomicOr(&a.status[s], (outside ? 1 : 0) | (off_slab ? 4 : 0));
     } else {
       dl = (float)acc[0];
       if (nf > 1) {
@@ -193,7 +206,8 @@ __device__ __forceinline__ float2 mp_mode(uint64_t seed, uint64_t id, int k, int
 template <int M, int T>
 __global__ __launch_bounds__(T* (M / elems_per_thread(M))) void k_missing_power(
     smk_skewers sk, const int32_t* __restrict__ list, const int32_t* __restrict__ count,
-    const float* __restrict__ filt, uint64_t seed, const float* __restrict__ noise,
+    const float* __restrict__ filt_all, const int32_t* __restrict__ fset, int64_t set_stride,
+    const float* __restrict__ scale, uint64_t seed, const float* __restrict__ noise,
     const int64_t* __restrict__ noise_off, const float2* __restrict__ tw, const float2* __restrict__ twr,
     float* __restrict__ delta_s) {
   constexpr int E = elems_per_thread(M);
@@ -207,6 +221,8 @@ __global__ __launch_bounds__(T* (M / elems_per_thread(M))) void k_missing_power(
   const int p = tid % P, t = tid / P;
   const bool valid = base + t < ncls;
   const int s = valid ? list[base + t] : 0;
+  // the skewer's filter set (one P1D_miss table per set)
+  const float* __restrict__ filt = filt_all + (fset ? (int64_t)fset[s] * set_stride : 0);
   float2* line = lds + t * LL;
   float2 v[E];
   if (noise == nullptr) {
@@ -261,19 +277,30 @@ __global__ __launch_bounds__(T* (M / elems_per_thread(M))) void k_missing_power(
     const int n = sk.npix[s];
     float* dst = delta_s + sk.off[s];
     const float inv = 1.0f / (2.0f * M);
+    if (scale == nullptr) {
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int j = 2 * (p + P * e);
-      if (j < n) dst[j] = v[e].x * inv;
-      if (j + 1 < n) dst[j + 1] = v[e].y * inv;
+      for (int e = 0; e < E; ++e) {
+        const int j = 2 * (p + P * e);
+        if (j < n) dst[j] = v[e].x * inv;
+        if (j + 1 < n) dst[j + 1] = v[e].y * inv;
+      }
+    } else {
+      const float* sc = scale + sk.off[s];
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int j = 2 * (p + P * e);
+        if (j < n) dst[j] = v[e].x * inv * sc[j];
+        if (j + 1 < n) dst[j + 1] = v[e].y * inv * sc[j + 1];
+      }
     }
   }
 }
 
-__global__ void k_classify(smk_skewers sk, int32_t* __restrict__ lists, int32_t* __restrict__ counts,
-                           int32_t* __restrict__ bad) {
+__global__ void k_classify(smk_skewers sk, const int32_t* __restrict__ fset, int32_t n_sets,
+                           int32_t* __restrict__ lists, int32_t* __restrict__ counts, int32_t* __restrict__ bad) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= sk.n) return;
+  if (fset && (fset[s] < 0 || fset[s] >= n_sets)) { atomicOr(bad, 2); return; }
   const int n = sk.npix[s];
   int c = 0, nz = 256;
   while (nz < n + 50 && c < N_CLASS) { nz <<= 1; ++c; }
@@ -294,13 +321,14 @@ constexpr int mp_T(int m) {
 
 template <int M>
 hipError_t launch_mp(const smk_skewers& sk, const int32_t* list, const int32_t* count, const float* filt,
-                     uint64_t seed, const float* noise, const int64_t* noise_off, const float2* tw,
-                     const float2* twr, float* delta_s, hipStream_t s) {
+                     const int32_t* fset, int64_t set_stride, const float* scale, uint64_t seed,
+                     const float* noise, const int64_t* noise_off, const float2* tw, const float2* twr,
+                     float* delta_s, hipStream_t s) {
   constexpr int T = mp_T(M);
   const int nblk = (sk.n + T - 1) / T;
   if (nblk == 0) return hipSuccess;
   hipLaunchKernelGGL((k_missing_power<M, T>), dim3(nblk), dim3(T * (M / elems_per_thread(M))), 0, s, sk, list,
-                     count, filt, seed, noise, noise_off, tw, twr, delta_s);
+                     count, filt, fset, set_stride, scale, seed, noise, noise_off, tw, twr, delta_s);
   return hipGetLastError();
 }
 
@@ -361,6 +389,8 @@ extern "C" int smk_forest_gather(const smk_skewers* sk, const smk_box_view* box,
   SMK_REQUIRE(box->delta != nullptr, SMK_ERR_ARG, "box delta is NULL");
   SMK_REQUIRE(box->nx > 0 && box->ny > 0 && box->nz > 0 && box->cell > 0, SMK_ERR_ARG, "bad box geometry");
   SMK_REQUIRE(interp == SMK_INTERP_TRILINEAR || interp == SMK_INTERP_GAUSS343, SMK_ERR_ARG, "bad interp %d", interp);
+  SMK_REQUIRE(box->nxs >= 0 && box->x0 >= 0 && (box->nxs == 0 || box->x0 + box->nxs <= box->nx), SMK_ERR_ARG,
+              "bad resident x slab [%d, %d + %d) of a %d-plane box", box->x0, box->x0, box->nxs, box->nx);
   SMK_REQUIRE(sk->geometry == SMK_GEOM_PLANE || sk->geometry == SMK_GEOM_RADIAL, SMK_ERR_ARG, "bad geometry");
   SMK_REQUIRE(flux == nullptr || abcg != nullptr, SMK_ERR_ARG, "flux needs FGPA parameters");
   SMK_REQUIRE(sk->geometry != SMK_GEOM_RADIAL ||
@@ -390,29 +420,36 @@ extern "C" int smk_forest_gather(const smk_skewers* sk, const smk_box_view* box,
   return SMK_OK;
 }
 
-extern "C" int smk_forest_missing_power(smk_forest_plan* plan, const smk_skewers* sk, const float* filt,
-                                        const int64_t* filt_off, uint64_t seed, const float* noise,
-                                        const int64_t* noise_off, void* workspace, float* delta_s, void* stream) {
+extern "C" int smk_forest_missing_power_sets(smk_forest_plan* plan, const smk_skewers* sk, const float* filt,
+                                             const int64_t* filt_off, int64_t set_stride, int32_t n_sets,
+                                             const int32_t* filt_set, const float* scale, uint64_t seed,
+                                             const float* noise, const int64_t* noise_off, void* workspace,
+                                             float* delta_s, void* stream) {
   SMK_REQUIRE(plan != nullptr && sk != nullptr && filt != nullptr && filt_off != nullptr && delta_s != nullptr,
               SMK_ERR_ARG, "NULL argument");
   SMK_REQUIRE(workspace != nullptr, SMK_ERR_ARG, "workspace is NULL");
   SMK_REQUIRE(noise == nullptr || noise_off != nullptr, SMK_ERR_ARG, "fed noise needs noise_off");
+  SMK_REQUIRE(filt_set == nullptr || (n_sets >= 1 && set_stride >= 0), SMK_ERR_ARG,
+              "filter sets need n_sets >= 1 and set_stride >= 0");
   if (sk->n <= 0) return SMK_OK;
   hipStream_t s = (hipStream_t)stream;
-  int32_t* counts = (int32_t*)workspace;           // [N_CLASS], [N_CLASS] = bad flag
+  int32_t* counts = (int32_t*)workspace;           // [N_CLASS], [N_CLASS] = bad flags
   int32_t* lists = counts + 16;
   hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, counts, 16);
   SMK_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_classify, dim3((sk->n + 255) / 256), dim3(256), 0, s, *sk, lists, counts, counts + N_CLASS);
+  hipLaunchKernelGGL(k_classify, dim3((sk->n + 255) / 256), dim3(256), 0, s, *sk, filt_set, n_sets, lists, counts,
+                     counts + N_CLASS);
   SMK_LAUNCH_CHECK();
   const int64_t* foff = filt_off;
   int32_t bad = 0;
   SMK_HIP_CHECK(hipMemcpyAsync(&bad, counts + N_CLASS, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   SMK_HIP_CHECK(hipStreamSynchronize(s));
+  SMK_REQUIRE((bad & 2) == 0, SMK_ERR_RANGE, "a filter set index is outside [0, %d)", (int)n_sets);
   SMK_REQUIRE(bad == 0, SMK_ERR_RANGE, "a skewer needs an FFT longer than %d", 256 << (N_CLASS - 1));
-#define SMK_MP(C, M)                                                                                         \
-  SMK_HIP_CHECK(launch_mp<M>(*sk, lists + (int64_t)C * sk->n, counts + C, filt + foff[C], seed, noise,       \
-                             noise_off, plan->tw[C], plan->twr[C], delta_s, s))
+  const int64_t stride = filt_set ? set_stride : 0;
+#define SMK_MP(C, M)                                                                                            \
+  SMK_HIP_CHECK(launch_mp<M>(*sk, lists + (int64_t)C * sk->n, counts + C, filt + foff[C], filt_set, stride,     \
+                             scale, seed, noise, noise_off, plan->tw[C], plan->twr[C], delta_s, s))
   SMK_MP(0, 128);
   SMK_MP(1, 256);
   SMK_MP(2, 512);
@@ -421,4 +458,11 @@ extern "C" int smk_forest_missing_power(smk_forest_plan* plan, const smk_skewers
   SMK_MP(5, 4096);
 #undef SMK_MP
   return SMK_OK;
+}
+
+extern "C" int smk_forest_missing_power(smk_forest_plan* plan, const smk_skewers* sk, const float* filt,
+                                        const int64_t* filt_off, uint64_t seed, const float* noise,
+                                        const int64_t* noise_off, void* workspace, float* delta_s, void* stream) {
+  return smk_forest_missing_power_sets(plan, sk, filt, filt_off, 0, 1, nullptr, nullptr, seed, noise, noise_off,
+                                       workspace, delta_s, stream);
 }

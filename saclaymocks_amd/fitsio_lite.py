@@ -53,26 +53,38 @@ def _parse_value(v):
         return v
 
 
-def _read_header(buf, pos):
-    hdr = {}
-    order = []
+class Header(dict):
+    """Header cards by keyword; lookups ignore case, as FITS keywords do."""
+
+    def __getitem__(self, k):
+        return dict.__getitem__(self, k.upper())
+
+    def get(self, k, default=None):
+        return dict.get(self, k.upper(), default)
+
+    def __contains__(self, k):
+        return dict.__contains__(self, k.upper())
+
+
+def _read_header(f):
+    """Parse one header from the file object positioned at its first block;
+    None at the end of the file."""
+    hdr = Header()
+    first = True
     while True:
-        blk = buf[pos:pos + BLOCK]
+        blk = f.read(BLOCK)
+        if first and (len(blk) < 8 or not blk[:8].strip(b" \0")):
+            return None
+        first = False
         if len(blk) < BLOCK:
             raise ValueError("truncated FITS header")
-        pos += BLOCK
-        done = False
         for i in range(0, BLOCK, 80):
             card = blk[i:i + 80].decode("ascii", "replace")
             key = card[:8].strip()
             if key == "END":
-                done = True
-                break
+                return hdr
             if card[8:10] == "= ":
                 hdr[key] = _parse_value(card[10:])
-                order.append(key)
-        if done:
-            return hdr, pos
 
 
 def _data_bytes(h):
@@ -107,6 +119,20 @@ def _table_dtype(h):
     return np.dtype(fields)
 
 
+def _decode(h, raw):
+    if h.get("XTENSION", "").strip() == "BINTABLE":
+        return np.frombuffer(raw, dtype=_table_dtype(h), count=h["NAXIS2"]).copy()
+    if h.get("NAXIS", 0) > 0:
+        shape = tuple(h["NAXIS%d" % i] for i in range(h["NAXIS"], 0, -1))
+        data = np.frombuffer(raw, dtype=_BITPIX[h["BITPIX"]]).reshape(shape)
+        bscale, bzero = h.get("BSCALE", 1), h.get("BZERO", 0)
+        data = data.astype(data.dtype.newbyteorder("="))
+        if bscale != 1 or bzero != 0:
+            data = data * bscale + bzero
+        return data
+    return None
+
+
 class HDU:
     def __init__(self, header, data):
         self.header = header
@@ -117,56 +143,69 @@ class HDU:
         return str(self.header.get("EXTNAME", "")).strip()
 
 
+def _scan(path, want):
+    """Walk the HDUs of a file; decode the data of those for which
+    want(index, header) is true and skip the others' data blocks."""
+    out = []
+    with _open(path, "rb") as f:
+        i = 0
+        while True:
+            h = _read_header(f)
+            if h is None:
+                break
+            nbytes = _data_bytes(h)
+            padded = (nbytes + BLOCK - 1) // BLOCK * BLOCK
+            if want(i, h):
+                raw = f.read(nbytes)
+                if len(raw) < nbytes:
+                    raise ValueError("truncated FITS data")
+                f.seek(padded - nbytes, 1)
+                out.append(HDU(h, _decode(h, raw)))
+            else:
+                f.seek(padded, 1)
+                out.append(HDU(h, None))
+            i += 1
+    return out
+
+
 def read_all(path):
     """Every HDU of a FITS file: list of HDU(header dict, numpy data or None)."""
-    with _open(path, "rb") as f:
-        buf = f.read()
-    pos = 0
-    hdus = []
-    while pos < len(buf):
-        if not buf[pos:pos + 8].strip(b" \0"):
-            break
-        h, pos = _read_header(buf, pos)
-        nbytes = _data_bytes(h)
-        raw = buf[pos:pos + nbytes]
-        pos += (nbytes + BLOCK - 1) // BLOCK * BLOCK
-        data = None
-        if h.get("XTENSION", "").strip() == "BINTABLE":
-            dt = _table_dtype(h)
-            data = np.frombuffer(raw, dtype=dt, count=h["NAXIS2"]).copy()
-        elif h.get("NAXIS", 0) > 0:
-            shape = tuple(h["NAXIS%d" % i] for i in range(h["NAXIS"], 0, -1))
-            data = np.frombuffer(raw, dtype=_BITPIX[h["BITPIX"]]).reshape(shape)
-            bscale, bzero = h.get("BSCALE", 1), h.get("BZERO", 0)
-            data = data.astype(data.dtype.newbyteorder("="))
-            if bscale != 1 or bzero != 0:
-                data = data * bscale + bzero
-        hdus.append(HDU(h, data))
-    return hdus
+    return _scan(path, lambda i, h: True)
 
 
-def _select(hdus, ext):
+def headers(path):
+    """Headers of every HDU, without reading their data."""
+    return [h.header for h in _scan(path, lambda i, h: False)]
+
+
+def _match(ext, i, h):
     if isinstance(ext, str):
-        for h in hdus:
-            if h.name.upper() == ext.upper():
-                return h
-        raise KeyError(f"no HDU named {ext}")
-    return hdus[ext]
+        return str(h.get("EXTNAME", "")).strip().upper() == ext.upper()
+    return i == ext
 
 
 def read(path, ext=None):
-    """fitsio.read: the data of `ext` (index or EXTNAME); default the first HDU with data."""
-    hdus = read_all(path)
+    """fitsio.read: the data of `ext` (index or EXTNAME); default the first HDU
+    with data.  Only that HDU's data is read from the file."""
     if ext is None:
-        for h in hdus:
-            if h.data is not None:
-                return h.data
-        return None
-    return _select(hdus, ext).data
+        for i, h in enumerate(headers(path)):
+            if _data_bytes(h) > 0:
+                ext = i
+                break
+        else:
+            return None
+    hdus = _scan(path, lambda i, h: _match(ext, i, h))
+    for i, h in enumerate(hdus):
+        if _match(ext, i, h.header):
+            return h.data
+    raise KeyError(f"no HDU {ext!r} in {path}")
 
 
 def read_header(path, ext=0):
-    return _select(read_all(path), ext).header
+    for i, h in enumerate(headers(path)):
+        if _match(ext, i, h):
+            return h
+    raise KeyError(f"no HDU {ext!r} in {path}")
 
 
 # ------------------------------------------------------------------ writing

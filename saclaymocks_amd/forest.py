@@ -25,7 +25,8 @@ N_CLASS = 6
 class SmkBoxView(ctypes.Structure):
     _fields_ = [("delta", ctypes.c_void_p), ("eta", ctypes.c_void_p * 6),
                 ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
-                ("periodic", ctypes.c_int32), ("cell", ctypes.c_double), ("r0", ctypes.c_double)]
+                ("periodic", ctypes.c_int32), ("cell", ctypes.c_double), ("r0", ctypes.c_double),
+                ("x0", ctypes.c_int32), ("nxs", ctypes.c_int32)]
 
 
 class SmkSkewers(ctypes.Structure):
@@ -48,8 +49,14 @@ def _t(a, dtype, dev):
 class BoxView:
     """Resident boxes (torch float32 CUDA tensors [nx][ny][nz]) + box frame."""
 
-    def __init__(self, delta, eta=None, cell=2.19, r0=0.0, periodic=True):
+    def __init__(self, delta, eta=None, cell=2.19, r0=0.0, periodic=True, x0=0, nx=None):
+        """delta/eta: [nxs][ny][nz] tensors holding x planes [x0, x0 + nxs) of an
+        nx-plane box (default: the whole box, nx = nxs, x0 = 0)."""
         self.delta = delta
+        self.x0 = int(x0)
+        self.nx_box = int(nx) if nx is not None else int(delta.shape[0])
+        if self.x0 < 0 or self.x0 + delta.shape[0] > self.nx_box:
+            raise ValueError(f"x planes [{self.x0}, {self.x0 + delta.shape[0]}) outside a {self.nx_box}-plane box")
         self.eta = list(eta) if eta is not None else [None] * 6
         if len(self.eta) != 6:
             raise ValueError("eta must list 6 boxes (xx, yy, zz, xy, xz, yz); None where unused")
@@ -59,16 +66,18 @@ class BoxView:
                 raise ValueError("box fields must be contiguous float32 CUDA tensors of one shape")
 
     @classmethod
-    def plane(cls, delta, eta_zz=None, cell=2.19, r0=0.0, periodic=True):
-        return cls(delta, [None, None, eta_zz, None, None, None], cell, r0, periodic)
+    def plane(cls, delta, eta_zz=None, cell=2.19, r0=0.0, periodic=True, x0=0, nx=None):
+        return cls(delta, [None, None, eta_zz, None, None, None], cell, r0, periodic, x0, nx)
 
     def struct(self):
-        nx, ny, nz = self.delta.shape
+        nxs, ny, nz = self.delta.shape
         v = SmkBoxView()
         v.delta = self.delta.data_ptr()
         for i, e in enumerate(self.eta):
             v.eta[i] = e.data_ptr() if e is not None else None
-        v.nx, v.ny, v.nz = nx, ny, nz
+        v.nx, v.ny, v.nz = self.nx_box, ny, nz
+        whole = self.x0 == 0 and nxs == self.nx_box
+        v.x0, v.nxs = (0, 0) if whole else (self.x0, nxs)
         v.periodic = 1 if self.periodic else 0
         v.cell, v.r0 = self.cell, self.r0
         return v
@@ -152,6 +161,14 @@ def missing_power_filters(pixel, p1d, device=None):
     return filt, (ctypes.c_int64 * N_CLASS)(*offs)
 
 
+def missing_power_filter_sets(pixel, p1ds, device=None):
+    """One filter set (as missing_power_filters) per P1D_miss function of
+    `p1ds`, stacked; returns (filt, filt_off, set_stride)."""
+    sets = [missing_power_filters(pixel, p, device="cpu") for p in p1ds]
+    filt = torch.cat([f for f, _ in sets]).to(device or _dev())
+    return filt, sets[0][1], int(sets[0][0].numel())
+
+
 def fft_len_for(n):
     """merge_spectra.py:308-309."""
     nz = 256
@@ -160,8 +177,11 @@ def fft_len_for(n):
     return nz
 
 
-def missing_power(plan, batch, filt, filt_off, seed=0, noise=None, noise_off=None, out=None, stream=None):
-    """delta_s per pixel (float32 CUDA tensor).  noise None -> device Fourier draw."""
+def missing_power(plan, batch, filt, filt_off, seed=0, noise=None, noise_off=None, out=None, stream=None,
+                  filt_set=None, set_stride=0, n_sets=1, scale=None):
+    """delta_s per pixel (float32 CUDA tensor).  noise None -> device Fourier draw.
+    filt_set (int32 per skewer) picks one of n_sets stacked filter sets
+    (missing_power_filter_sets); scale (float32 per pixel) multiplies delta_s."""
     dev = _dev()
     if out is None:
         out = torch.empty(batch.npix_total, dtype=torch.float32, device=dev)
@@ -170,11 +190,25 @@ def missing_power(plan, batch, filt, filt_off, seed=0, noise=None, noise_off=Non
     if noise is not None:
         noff = noise_off if isinstance(noise_off, torch.Tensor) else _t(noise_off, torch.int64, dev)
     sk = batch.struct()
-    rc = _lib.lib().smk_forest_missing_power(plan._h, ctypes.byref(sk), _lib.ptr(filt), filt_off,
-                                             ctypes.c_uint64(int(seed) & (2**64 - 1)), _lib.ptr(noise),
-                                             _lib.ptr(noff), _lib.ptr(ws), _lib.ptr(out),
-                                             _lib.stream_ptr(stream))
-    _lib.check(rc, "smk_forest_missing_power")
+    seed = ctypes.c_uint64(int(seed) & (2**64 - 1))
+    if filt_set is None and scale is None:
+        rc = _lib.lib().smk_forest_missing_power(plan._h, ctypes.byref(sk), _lib.ptr(filt), filt_off, seed,
+                                                 _lib.ptr(noise), _lib.ptr(noff), _lib.ptr(ws), _lib.ptr(out),
+                                                 _lib.stream_ptr(stream))
+        _lib.check(rc, "smk_forest_missing_power")
+        return out
+    fs = None if filt_set is None else (filt_set if isinstance(filt_set, torch.Tensor)
+                                        else _t(filt_set, torch.int32, dev))
+    sc = None if scale is None else (scale if isinstance(scale, torch.Tensor) else _t(scale, torch.float32, dev))
+    if sc is not None and (sc.dtype != torch.float32 or sc.numel() != batch.npix_total):
+        raise ValueError("scale must be float32 with one value per pixel")
+    if fs is not None and (fs.dtype != torch.int32 or fs.numel() != batch.n):
+        raise ValueError("filt_set must be int32 with one value per skewer")
+    rc = _lib.lib().smk_forest_missing_power_sets(plan._h, ctypes.byref(sk), _lib.ptr(filt), filt_off,
+                                                  int(set_stride), int(n_sets), _lib.ptr(fs), _lib.ptr(sc), seed,
+                                                  _lib.ptr(noise), _lib.ptr(noff), _lib.ptr(ws), _lib.ptr(out),
+                                                  _lib.stream_ptr(stream))
+    _lib.check(rc, "smk_forest_missing_power_sets")
     return out
 
 
@@ -211,12 +245,14 @@ def fgpa_rows(a, b, c, G):
 
 # ------------------------------------------------ reference-geometry builder
 def radial_skewers(ra, dec, z_qso, ra0, dec0, box_shape, cell, zmin=1.3, zmax=3.6, pixel=0.2, dmax=3,
-                   thing_id=None, cosmo=None, full_length_only=False):
-    """Line-of-sight set-up of make_spectra.py:300-472 for one slab covering the
-    whole box (NSlice = 1): pixel grid, per-QSO (X, Y, Z, R) of the QSO,
-    contiguous pixel range inside the box in X, forest limit lambda <
-    lambda_Lya (1 + z_QSO).  Angles in degrees.  Returns (SkewerBatch
-    arguments as a dict, grid dict, kept QSO indices)."""
+                   thing_id=None, cosmo=None, full_length_only=False, i_slice=0, n_slice=1):
+    """Line-of-sight set-up of make_spectra.py:300-472 for x slice i_slice of
+    n_slice (default: one slice covering the whole box): pixel grid, per-QSO
+    (X, Y, Z, R) of the QSO, the QSO cut |X/Z| <= tanx_slice_max of
+    make_spectra.py:327-346, the contiguous pixel range with X in the slice
+    (:423-428), forest limit lambda < lambda_Lya (1 + z_QSO) (:467-472).
+    Angles in degrees.  Returns (SkewerBatch arguments as a dict, with k0 =
+    first pixel's index in the lambda grid; grid dict; kept QSO indices)."""
     cosmo = cosmo or util.Cosmo()
     h = util.H
     nx, ny, nz = box_shape
@@ -231,10 +267,13 @@ def radial_skewers(ra, dec, z_qso, ra0, dec0, box_shape, cell, zmin=1.3, zmax=3.
     i_first = int(np.argmax(keep))
     R_vec = R_all[keep]
     lambda_vec = lam_all[keep]
-    xmin, xmax = -LX / 2, LX / 2
-    sinx_max = (LX / 2 - dmax * cell) / (R0 + LZ / 2 - dmax * cell)
-    tanx_max = sinx_max / np.sqrt(1 - sinx_max ** 2)
-    rows = {k: [] for k in ("qx", "qy", "qz", "qr", "g0", "npix", "imin", "imax", "noise_id")}
+    xmin = LX * i_slice / n_slice - LX / 2
+    xmax = LX * (i_slice + 1) / n_slice - LX / 2
+    if i_slice >= n_slice // 2:
+        tanx_max = (cell * (nx / n_slice) * (i_slice + 1) - LX / 2) / (R0 - LZ / 2)
+    else:
+        tanx_max = abs((cell * (nx / n_slice) * i_slice - LX / 2) / (R0 - LZ / 2))
+    rows = {k: [] for k in ("qx", "qy", "qz", "qr", "g0", "npix", "imin", "imax", "noise_id", "k0")}
     kept = []
     for q in range(len(ra)):
         zq = float(z_qso[q])
@@ -257,6 +296,7 @@ def radial_skewers(ra, dec, z_qso, ra0, dec0, box_shape, cell, zmin=1.3, zmax=3.
         rows["qz"].append(Z)
         rows["qr"].append(R_QSO)
         rows["g0"].append(i_first + k0)
+        rows["k0"].append(k0)
         rows["npix"].append(k1 - k0)
         rows["imin"].append(0)
         rows["imax"].append(int(below[-1]) + 1 if len(below) else 0)

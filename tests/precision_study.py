@@ -152,7 +152,9 @@ def study_seed(seed, shape, workers, n_skewers=4096, device="cuda"):
     r0 = util.H * util.Cosmo().r_comoving(util.Z0)
     c = skewer_case(shape, n_skewers, seed, r0)
     a, b, cc, G = fgpa_params()
-    batch = forest.SkewerBatch(forest.GEOM_PLANE, c["g0"], c["npix"], c["rmin"], c["dr"], c["qx"], c["qy"])
+    # one FGPA parameter row (z fixed) shared by every skewer
+    batch = forest.SkewerBatch(forest.GEOM_PLANE, c["g0"], c["npix"], c["rmin"], c["dr"], c["qx"], c["qy"],
+                               pset=np.zeros(len(c["npix"]), np.int32))
     box = forest.BoxView.plane(d, e, cell=CELL, r0=r0, periodic=True)
     out = forest.gather(batch, box, interp="trilinear", abcg=forest.fgpa_rows(a, b, cc, G),
                         want=("delta_l", "eta_par", "flux"))

@@ -1,0 +1,151 @@
+"""The make_boxes -> make_spectra -> merge_spectra drop-ins end to end on a
+small box, against the FP64 oracle: Gaussian-smoothed skewers of the written
+box files, the device-drawn missing power, the FGPA flux per pixel."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from oracle import forest_oracle as fo
+
+pytestmark = pytest.mark.gpu
+
+NX, NY, NZ, CELL = 32, 32, 256, 2.19
+ZMIN, ZMAX, PIXEL = 1.3, 1.94, 0.2
+NSLICE, SEED = 4, 21
+QSO_NAMES = ["RA", "DEC", "Z_QSO_RSD", "Z_QSO_NO_RSD", "THING_ID", "HDU", "PLATE", "MJD", "FIBERID", "PMF"]
+
+
+def _qsos(R0):
+    """QSOs in box coordinates (ra0 = dec0 = 0: X = R cos(dec) sin(ra),
+    Y = R sin(dec)); two of them cross the x boundary between two slices in
+    the middle of the pixel grid, one has no forest pixel (z < 1.8594)."""
+    r_mid = R0 + 206.0
+    xb = CELL * NX / NSLICE          # slice boundaries at x = -xb, 0, +xb
+    spec = [  # (X at r_mid, Y at r_mid, z_qso, hdu)
+        (xb, 3.0, 1.93, 0), (-xb, -5.0, 1.91, 0), (12.0, 20.0, 1.90, 0), (-22.0, 8.0, 1.935, 0),
+        (5.0, -18.0, 1.88, 1), (-9.0, -12.0, 1.92, 1), (20.0, 0.5, 1.85, 0), (-3.0, 1.0, 1.89, 1),
+    ]
+    rows = []
+    for q, (x, y, z, hdu) in enumerate(spec):
+        dec = np.degrees(np.arcsin(y / r_mid))
+        ra = np.degrees(np.arcsin(x / (r_mid * np.cos(np.radians(dec)))))
+        rows.append((ra, dec, z, z - 0.001, 1000 + 17 * q, hdu, 4000 + q, 55000 + q, 10 + q, 0))
+    return rows
+
+
+def _write_qsos(qdir, rows):
+    from saclaymocks_amd import fitsio_lite
+    os.makedirs(qdir, exist_ok=True)
+    east = [r for r in rows if r[0] > 0]
+    west = [r for r in rows if r[0] <= 0]
+    files = {0: west[: len(west) // 2], 1: west[len(west) // 2:], 2: east[: len(east) // 2], 3: east[len(east) // 2:]}
+    dtypes = [np.float64, np.float64, np.float64, np.float64, np.int64, np.int32, np.int32, np.int32, np.int32,
+              np.int32]
+    for i, rs in files.items():
+        cols = [np.array([r[c] for r in rs], dtype=dt) for c, dt in enumerate(dtypes)]
+        with fitsio_lite.FITS(os.path.join(qdir, f"QSO-{i}-{NSLICE}.fits"), "rw", clobber=True) as f:
+            f.write(cols, names=QSO_NAMES, header=[{"name": "RA0", "value": 0.0}, {"name": "DEC0", "value": 0.0}])
+
+
+def _read_box(d, name):
+    from saclaymocks_amd import fitsio_lite
+    planes = [fitsio_lite.read(os.path.join(d, f"{name}-{i}.fits"), 0) for i in range(NX)]
+    return np.concatenate([np.asarray(p, dtype=np.float32).reshape(-1, NY, NZ) for p in planes])
+
+
+def _p1d_file(path):
+    """z-dependent P1D_miss file: extensions z, k, pk [nz][nk], sigma."""
+    from saclaymocks_amd import fitsio_lite
+    z = np.array([1.8, 1.88, 1.95, 2.2])
+    k = np.linspace(0.0, np.pi / PIXEL + 0.5, 300)
+    pk = np.array([(0.03 + 0.015 * i) * np.exp(-(k / (1.3 + 0.4 * i)) ** 2) for i in range(len(z))])
+    sig = 0.3 + 0.2 * z
+    with fitsio_lite.FITS(path, "rw", clobber=True) as f:
+        f.write(np.zeros(1, np.float32))
+        f.write(z, extname="z")
+        f.write(k, extname="k")
+        f.write(pk, extname="pk")
+        f.write(sig, extname="sigma")
+    return z, k, pk, sig
+
+
+def test_spectra_pipeline_matches_oracle(cuda, tmp_path):
+    from saclaymocks_amd import fitsio_lite, healpix, workloads
+    from saclaymocks_amd.bin import make_boxes, make_spectra, merge_spectra
+    boxdir, qdir = str(tmp_path / "boxes"), str(tmp_path / "qso")
+    sdir, mdir = str(tmp_path / "spectra"), str(tmp_path / "merged")
+    print("[pipeline make_boxes]", flush=True)
+    assert make_boxes.main(["-NX", str(NX), "-NY", str(NY), "-NZ", str(NZ), "-seed", "9", "-outDir", boxdir]) == 0
+    cos = fo.Cosmo()
+    R0 = fo.H * cos.r_comoving(fo.Z0)
+    rows = _qsos(R0)
+    _write_qsos(qdir, rows)
+    print("[pipeline make_spectra]", flush=True)
+    for i in range(NSLICE):
+        assert make_spectra.main(["-QSOfile", os.path.join(qdir, "QSO-"), "-boxdir", boxdir, "-outDir", sdir,
+                                  "-i", str(i), "-N", str(NSLICE), "-zmin", str(ZMIN), "-zmax", str(ZMAX)]) == 0
+    assert len(glob.glob(os.path.join(sdir, "spectra-*-*.fits.gz"))) >= NSLICE
+    p1d_path = str(tmp_path / "p1dmiss.fits")
+    z_tab, k_tab, pk_tab, sig_tab = _p1d_file(p1d_path)
+    print("[pipeline merge_spectra]", flush=True)
+    assert merge_spectra.main(["-inDir", sdir, "-outDir", mdir, "-i", "0", "-seed", str(SEED), "-p1dfile", p1d_path,
+                               "--store-g", "True"]) == 0
+    assert merge_spectra.main(["-inDir", sdir, "-outDir", mdir, "-i", "1", "-seed", str(SEED)]) == 0
+
+    # ---- oracle
+    fields = [_read_box(boxdir, k) for k in ("box",) + make_boxes.ETA_KINDS]
+    L, D = (CELL * NX, CELL * NY, CELL * NZ), (CELL,) * 3
+    Rmin, Rmax = fo.H * cos.r_comoving(ZMIN), fo.H * cos.r_comoving(ZMAX)
+    R_all = Rmin + np.arange(int((Rmax - Rmin) / PIXEL + 0.5)) * PIXEL
+    lam_all = fo.LYA * (1 + cos.r_2_z(R_all / fo.H))
+    keep = lam_all > 3476.0
+    R_vec, lam32 = R_all[keep], np.float32(lam_all[keep])
+    zpix = np.float32(cos.r_2_z(R_vec / fo.H))
+    tabs = workloads.fgpa_tables()
+    a, b, c = (tabs[k].interp(zpix.astype(np.float64)) for k in ("a", "b", "c"))
+    G = fo.fgrowth(2.4) * 3.4 / (1 + zpix.astype(np.float64))
+    seen = 0
+    for hdu in (0, 1):
+        files = sorted(glob.glob(os.path.join(mdir, f"spectra_merged-*-{hdu}.fits.gz")))
+        assert files
+        for path in files:
+            hd = {h.name: h for h in fitsio_lite.read_all(path)}
+            meta = hd["METADATA"].data
+            pix = int(os.path.basename(path).split("-")[1])
+            assert np.all(healpix.radec2pix(16, meta["RA"], meta["DEC"], nest=True) == pix)
+            assert np.all(meta["HDU"] == hdu) and np.all(np.diff(meta["THING_ID"]) > 0)
+            assert np.array_equal(hd["LAMBDA"].data, lam32)
+            for q, tid in enumerate(meta["THING_ID"]):
+                r = next(r for r in rows if r[4] == tid)
+                assert r[5] == hdu
+                seen += 1
+                R_Q = fo.H * cos.r_comoving(r[2])
+                X, Y, Z = fo.compute_xyz2(np.radians(r[0]), np.radians(r[1]), R_Q, 0.0, 0.0)
+                vals, _ = fo.gauss343(fields, R_vec * X / R_Q, R_vec * Y / R_Q, R_vec * Z / R_Q, L, D, R0)
+                dl = np.float32(vals[0])
+                eta = np.float32(fo.eta_par_radial(vals[1:], R_vec * X / R_Q, R_vec * Y / R_Q, R_vec * Z / R_Q))
+                rf = lam32 / (1 + meta["Z"][q])
+                mmm = np.nonzero((rf < fo.LYA) & (rf > 0.0))[0]
+                if len(mmm) == 0:
+                    ds = np.zeros(len(R_vec))
+                elif hdu == 0:
+                    zeff = np.mean(zpix[mmm])
+                    i = fo.p1d_table_index(z_tab, zeff)
+                    ds = fo.missing_power_device(SEED + hdu, tid, len(R_vec), PIXEL, fo.p1d_table(k_tab, pk_tab[i]))
+                    from scipy.interpolate import interp1d
+                    s = interp1d(z_tab, sig_tab)
+                    ds = ds * s(zpix) / s(zeff)
+                else:
+                    ds = fo.missing_power_device(SEED + hdu, tid, len(R_vec), PIXEL, fo.p1d_missing_analytic)
+                flux = fo.fgpa(dl.astype(np.float64) + ds, eta.astype(np.float64), G, a, b, c)
+                got = hd["FLUX"].data[q]
+                assert np.abs(got - flux).max() < 2e-5, (tid, np.abs(got - flux).max())
+                if hdu == 0:
+                    assert np.abs(hd["DELTA_L"].data[q] - dl).max() < 2e-6
+                    assert np.abs(hd["ETA_PAR"].data[q] - eta).max() < 2e-6
+                    assert np.abs(hd["DELTA_S"].data[q] - ds).max() < 1e-5 * max(1.0, np.abs(ds).max())
+                    if len(mmm) == 0:
+                        assert np.all(hd["DELTA_S"].data[q] == 0)
+    assert seen == len(rows)

@@ -155,6 +155,70 @@ def test_missing_power_fed_noise_parity(cuda):
         assert err < 1e-5, (n, err)
 
 
+def test_missing_power_table_per_skewer_parity(cuda):
+    """smk_forest_missing_power_sets: the nearest-redshift P1D_miss table per
+    skewer and the sigma(z)/sigma(z_eff) pixel scale, against the oracle."""
+    from saclaymocks_amd import _lib, forest
+    print("[forest missing power table per skewer]", flush=True)
+    pixel = 0.2
+    z_tab = np.array([1.8, 2.2, 2.6, 3.0, 3.6])
+    k_tab = np.linspace(0.0, np.pi / pixel + 1.0, 400)
+    pk = np.array([(0.03 + 0.02 * i) * np.exp(-(k_tab / (1.2 + 0.3 * i)) ** 2) for i in range(len(z_tab))])
+    sig_z, sig = np.linspace(1.5, 4.0, 11), 0.2 + 0.1 * np.linspace(1.5, 4.0, 11) ** 1.5
+    npix = np.array([100, 463, 974, 975, 2000, 4000, 300, 1200], dtype=np.int32)
+    rng = np.random.default_rng(8)
+    zeff = rng.uniform(1.9, 3.5, len(npix))
+    nzs = [fo.fft_len_for(n) for n in npix]
+    noises = [rng.standard_normal(nz).astype(np.float32) for nz in nzs]
+    noff = np.zeros(len(npix), dtype=np.int64)
+    noff[1:] = np.cumsum(nzs[:-1])
+    zpix = [np.linspace(ze - 0.3, ze + 0.2, n) for ze, n in zip(zeff, npix)]
+    from scipy.interpolate import interp1d
+    sfun = interp1d(sig_z, sig)
+    scale = np.concatenate([sfun(z) / sfun(ze) for z, ze in zip(zpix, zeff)]).astype(np.float32)
+    fset = np.array([fo.p1d_table_index(z_tab, ze) for ze in zeff], dtype=np.int32)
+    b = forest.SkewerBatch(forest.GEOM_PLANE, np.zeros(len(npix)), npix, 0.0, pixel, np.zeros(len(npix)),
+                           np.zeros(len(npix)))
+    plan = forest.ForestPlan()
+    filt, foff, stride = forest.missing_power_filter_sets(pixel, [fo.p1d_table(k_tab, row) for row in pk])
+    noise_t = torch.from_numpy(np.concatenate(noises)).to(cuda)
+    ds = forest.missing_power(plan, b, filt, foff, noise=noise_t, noise_off=noff, filt_set=fset,
+                              set_stride=stride, n_sets=len(z_tab), scale=scale)
+    got = b.split(ds)
+    for s, n in enumerate(npix):
+        want = fo.missing_power_zdep(noises[s].astype(np.float64), n, pixel, z_tab, k_tab, pk, sig_z, sig, zpix[s],
+                                     zeff[s])
+        err = np.abs(got[s] - want).max() / np.abs(want).max()
+        assert err < 1e-5, (s, n, err)
+    # device-drawn modes: set 0 with unit scale is the plain entry point
+    a = forest.missing_power(plan, b, filt, foff, seed=3).clone()
+    a2 = forest.missing_power(plan, b, filt, foff, seed=3, filt_set=np.zeros(len(npix), np.int32),
+                              set_stride=stride, n_sets=len(z_tab), scale=np.ones(b.npix_total, np.float32))
+    assert torch.equal(a, a2)
+    with pytest.raises(_lib.SmkError):
+        forest.missing_power(plan, b, filt, foff, seed=3, filt_set=np.full(len(npix), 5, np.int32),
+                             set_stride=stride, n_sets=len(z_tab))
+
+
+def test_missing_power_device_modes_parity(cuda):
+    """Device-drawn Fourier modes (Philox keyed by seed, stream noise_id)
+    against the oracle's restatement of the same draw."""
+    from saclaymocks_amd import forest
+    print("[forest missing power device modes]", flush=True)
+    pixel = 0.2
+    npix = np.array([100, 462, 463, 975, 2000, 4000, 8000], dtype=np.int32)
+    ids = np.array([0, 1, 2**32 + 5, 39_000_123_456, 2**63 + 17, 7, 123456789], dtype=np.uint64)
+    b = forest.SkewerBatch(forest.GEOM_PLANE, np.zeros(len(npix)), npix, 0.0, pixel, np.zeros(len(npix)),
+                           np.zeros(len(npix)), noise_id=ids)
+    filt, foff = forest.missing_power_filters(pixel, fo.p1d_missing_analytic)
+    seed = 2**40 + 99
+    got = b.split(forest.missing_power(forest.ForestPlan(), b, filt, foff, seed=seed))
+    for s, n in enumerate(npix):
+        want = fo.missing_power_device(seed, ids[s], n, pixel, fo.p1d_missing_analytic)
+        err = np.abs(got[s] - want).max() / np.abs(want).max()
+        assert err < 1e-5, (s, n, err)
+
+
 def test_missing_power_device_noise_statistics(cuda):
     from saclaymocks_amd import forest
     pixel, n, ns = 0.2, 974, 3000
@@ -204,3 +268,66 @@ def test_fgpa_flux_parity(cuda):
     assert np.abs(got - want).max() < 2e-6
     m = _forest_mask(c)
     assert np.all(got[~m] == 1.0)
+
+
+def test_radial_slices_on_resident_slabs_match_the_whole_box(cuda):
+    """make_spectra.py's slicing: slice i of N reads planes [iXmin, iXmax) (dmax
+    halo, :220-221) and keeps the pixels with X in the slice.  Gathering each
+    slice from its slab gives the whole-box values pixel for pixel."""
+    from saclaymocks_amd import forest, util
+    shape, cell, nslice, dmax = (64, 48, 64), 20.0, 4, 3
+    cos = util.Cosmo()
+    R0 = util.H * cos.r_comoving(util.Z0)
+    L = [cell * s for s in shape]
+    zmax = float(cos.r_2_z((R0 + L[2] / 2 - 4 * cell) / util.H))
+    zmin = float(cos.r_2_z((R0 - L[2] / 2 + 4 * cell) / util.H))
+    rng = np.random.default_rng(8)
+    nq = 40
+    ang = np.degrees(0.6 * (L[0] / 2) / R0)
+    ra, dec = rng.uniform(-ang, ang, nq), rng.uniform(-0.2 * ang, 0.2 * ang, nq)
+    zq = rng.uniform(0.5 * (zmin + zmax), zmax, nq)
+    fields = [_field(shape, 20 + i) for i in range(7)]
+    tens = [torch.from_numpy(f).to(cuda) for f in fields]
+    whole = forest.BoxView(tens[0], tens[1:], cell=cell, r0=R0, periodic=False)
+    nx = shape[0]
+    seen = 0
+    for i in range(nslice):
+        print(f"[radial slab] slice {i}", flush=True)
+        rows, grid, kept = forest.radial_skewers(ra, dec, zq, 0.0, 0.0, shape, cell, zmin=zmin, zmax=zmax,
+                                                 pixel=0.2, i_slice=i, n_slice=nslice)
+        if len(kept) == 0:
+            continue
+        b = forest.SkewerBatch(forest.GEOM_RADIAL, rows["g0"], rows["npix"], grid["rmin"], grid["dr"], rows["qx"],
+                               rows["qy"], rows["qz"], rows["qr"], imin=rows["imin"], imax=rows["imax"])
+        x0 = max(i * nx // nslice - dmax, 0)
+        x1 = min((i + 1) * nx // nslice + dmax, nx)
+        slab = forest.BoxView(tens[0][x0:x1].contiguous(), [t[x0:x1].contiguous() for t in tens[1:]], cell=cell,
+                              r0=R0, periodic=False, x0=x0, nx=nx)
+        a = forest.gather(b, whole, interp="gauss343", want=("delta_l", "eta_par"))
+        s = forest.gather(b, slab, interp="gauss343", want=("delta_l", "eta_par"))
+        ok = (s["status"] == 0) & (a["status"] == 0)
+        assert ok.sum().item() > 0
+        pix_ok = torch.repeat_interleave(ok, b.npix)
+        assert torch.equal(a["delta_l"][pix_ok], s["delta_l"][pix_ok])
+        assert torch.equal(a["eta_par"][pix_ok], s["eta_par"][pix_ok])
+        # a slab skewer that fails only because of the slab reports bit 2
+        assert set(s["status"].cpu().tolist()) <= {0, 1, 4, 5}
+        seen += int(ok.sum().item())
+    assert seen >= 20
+
+
+def test_trilinear_respects_a_strict_box_and_a_slab(cuda):
+    from saclaymocks_amd import forest
+    shape, cell = (32, 32, 32), 2.0
+    d = torch.from_numpy(_field(shape, 1)).to(cuda)
+    b = forest.SkewerBatch(forest.GEOM_PLANE, [40, 40], [50, 50], -32.0 + 0.05, 0.2, [0.0, -31.9], [0.0, 0.0])
+    strict = forest.BoxView.plane(d, None, cell=cell, r0=0.0, periodic=False)
+    assert forest.gather(b, strict, interp="trilinear", want=("delta_l",))["status"].cpu().tolist() == [0, 1]
+    periodic = forest.BoxView.plane(d, None, cell=cell, r0=0.0, periodic=True)
+    assert forest.gather(b, periodic, interp="trilinear", want=("delta_l",))["status"].cpu().tolist() == [0, 0]
+    # planes [10, 20) resident: the centre skewer (x = 0 -> planes 15, 16) is inside, the edge one is not
+    slab = forest.BoxView.plane(d[10:20].contiguous(), None, cell=cell, r0=0.0, periodic=True, x0=10, nx=32)
+    out = forest.gather(b, slab, interp="trilinear", want=("delta_l",))
+    assert out["status"].cpu().tolist() == [0, 4]
+    full = forest.gather(b, periodic, interp="trilinear", want=("delta_l",))
+    assert torch.equal(out["delta_l"][:50], full["delta_l"][:50])

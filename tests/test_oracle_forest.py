@@ -108,6 +108,44 @@ def test_missing_power_single_mode():
     assert fo.fft_len_for(462) == 512 and fo.fft_len_for(463) == 1024
 
 
+def test_device_noise_modes_are_white_noise_modes():
+    """With a unit filter the drawn modes invert to N(0,1) white noise: unit
+    variance, zero mean, flat periodogram; k = 0 and k = M are real."""
+    nz = 8192
+    y = fo.device_noise_modes(12345, 77, nz)
+    assert y[0].imag == 0 and y[-1].imag == 0
+    w = np.fft.irfft(y, n=nz)
+    assert abs(w.var() - 1) < 0.05 and abs(w.mean()) < 0.05
+    assert abs(np.mean(np.abs(y[1:-1]) ** 2) / nz - 1) < 0.05
+    # different streams are uncorrelated
+    w2 = np.fft.irfft(fo.device_noise_modes(12345, 78, nz), n=nz)
+    assert abs(np.mean(w * w2)) < 0.05
+    # a unit P1D (sqrt(P / pixel) = 1) returns the noise itself
+    ds = fo.missing_power_device(12345, 77, 1000, 0.2, lambda k: np.full_like(k, 0.2))
+    nz2 = fo.fft_len_for(1000)
+    assert np.allclose(ds, np.fft.irfft(fo.device_noise_modes(12345, 77, nz2), n=nz2)[:1000])
+
+
+def test_missing_power_zdep_table_choice_and_scaling():
+    """Nearest-table choice by hand, the table reproduced at its own k nodes,
+    and a single Fourier mode through the chosen table times the linear
+    sigma ratio."""
+    z_tab = np.array([1.8, 2.2, 2.6, 3.0, 3.6])
+    assert [fo.p1d_table_index(z_tab, z) for z in (1.0, 1.99, 2.01, 2.39, 2.41, 3.31, 4.0)] == [0, 0, 1, 1, 2, 4, 4]
+    k_tab = np.linspace(0.0, 16.0, 9)
+    pk = np.array([(i + 1) * 0.01 * np.exp(-k_tab / (2 + i)) for i in range(len(z_tab))])
+    f = fo.p1d_table(k_tab, pk[3])
+    assert np.allclose(f(k_tab), pk[3]) and np.isclose(f(1.0), 0.5 * (pk[3][0] + pk[3][1]))
+    nz, m, pixel, n = 512, 17, 0.2, 300
+    w = np.cos(2 * np.pi * m * np.arange(nz) / nz)
+    sig_z, sig = np.array([1.5, 4.0]), np.array([0.5, 1.75])    # sigma = 0.5 (z - 0.5)
+    z_pix = np.linspace(2.0, 2.5, n)
+    out = fo.missing_power_zdep(w, n, pixel, z_tab, k_tab, pk, sig_z, sig, z_pix, zeff=2.9)
+    k = m * 2 * np.pi / (nz * pixel)
+    amp = np.sqrt(np.interp(k, k_tab, pk[3]) / pixel)
+    assert np.allclose(out, w[:n] * amp * (z_pix - 0.5) / (2.9 - 0.5), atol=1e-12)
+
+
 def test_fgpa_special_cases():
     d = np.array([-0.5, 0.0, 1.3])
     e = np.array([0.1, -0.2, 0.3])
