@@ -45,7 +45,7 @@ MODEL_BYTES_PER_CELL = 64   # 3 passes per 3D transform, inverses sharing the k 
 
 PASS_KIND = {"z_r2c": "z_r2c", "y_fft": "y_fwd", "x_spectral": "x_spectral", "y_ifft_delta": "y_inv",
              "z_c2r_delta": "z_c2r", "y_ifft_aux": "y_inv", "z_c2r_aux": "z_c2r"}
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "box_c2_pmc_traffic.md")
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "grf_box_2560x2560x1536_pmc_traffic.md")
 
 
 def measured_traffic(pass_name, vid, shape):

@@ -152,10 +152,11 @@ __device__ __forceinline__ float2 aux_apply(float2 d, const AuxSel& s, float kx,
 // line-fastest.  XCH selects the LDS exchange (float2 line, or float line
 // with the split exchange: half the LDS per block).  TILES (spectral mode
 // only) says where the aux spectrum waits while delta is inverse transformed:
-// 1 recomputes it (FFT(IFFT(d)) = N d: 4 line FFTs), 3 stashes it in the aux
+// 1 recomputes it (FFT(IFFT(d)) = N d: 4 line FFTs), 2 keeps it in a second
+// register tile (3 FFTs; used with the smaller tile E1 only: 2 x 40 points
+// per thread do not fit the register file at E = 40), 3 stashes it in the aux
 // output rows the same thread overwrites later (3 FFTs; the rows are still
-// in L2 / MALL when they are read back).  (A second register tile, 2, does
-// not fit: 2 x 40 points spill at E = 40.)
+// in L2 / MALL when they are read back).
 // SWS places the line swizzle of unpadded lines (see LdsMap).
 // Global addresses are a block-uniform base, a per-lane offset computed once
 // and a uniform per-point offset; lanes past the last line read the block's
@@ -220,6 +221,7 @@ __global__ __launch_bounds__(T*(N / E)) void k_strided(StridedArgs a) {
     float2* adst = has_aux ? a.aux + (uint64_t)o * a.out_so + (uint64_t)inner * a.out_si + (uint64_t)p * a.out_sk
                            : nullptr;
     double acc = 0.0;
+    float2 w[TILES == 2 ? E : 1];
     {
       const int z1 = opaque_zero();
       const float* kxv = a.kxv + z1;
@@ -239,6 +241,7 @@ __global__ __launch_bounds__(T*(N / E)) void k_strided(StridedArgs a) {
           if (valid && has_aux) adst[(uint64_t)e * out_step] = aux_apply(d, sel, kx, ky, kz, kk,
                                                                          i == 0 && iky == 0 && ikz == 0, 1.0f);
         }
+        if constexpr (TILES == 2) w[e] = aux_apply(d, sel, kx, ky, kz, kk, i == 0 && iky == 0 && ikz == 0, 1.0f);
       }
     }
     acc = valid ? wz * acc : 0.0;
@@ -267,6 +270,10 @@ __global__ __launch_bounds__(T*(N / E)) void k_strided(StridedArgs a) {
           const float kk = __fadd_rn(__fadd_rn(__fmul_rn(kx, kx), __fmul_rn(ky, ky)), __fmul_rn(kz, kz));
           v[e] = aux_apply(v[e], sel, kx, ky, kz, kk, i == 0 && iky == 0 && ikz == 0, 1.0f / (float)N);
         }
+        line_fft<N, E, true, XCH, SWS, SWB>(v, lds, t, p, a.tw, sw);
+      } else if constexpr (TILES == 2) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = w[e];
         line_fft<N, E, true, XCH, SWS, SWB>(v, lds, t, p, a.tw, sw);
       } else {
         // read back the stash this thread wrote (same addresses)
@@ -489,6 +496,7 @@ constexpr int half_tile(int n) {
 //   2  E0, float2, SWS 3 (swz)        7  E1, float2, SWS 3 (swz)
 //   3  E0, split exchange             8  E0, aux stash in the output, SWS 2 / 0
 //   4  E0, split, SWS 2 (swz)         9  E1, aux stash, SWS 2 / 0
+//  10  E1, aux in registers, SWS 2 / 0  11  E1, aux in registers, SWS 3 (swz)
 // E0 = elems_per_thread(n), E1 = half_tile(n).
 constexpr SVar strided_variant(int n, int mode, int vid) {
   const int e0 = elems_per_thread(n);
@@ -512,6 +520,8 @@ constexpr SVar strided_variant(int n, int mode, int vid) {
     case 7: return (e1 && swz) ? SVar{e1, t1, 0, 1, 3} : none;
     case 8: return spec ? SVar{e0, t0, 0, 3, swz ? 2 : 0} : none;
     case 9: return (spec && e1) ? SVar{e1, t1, 0, 3, swz ? 2 : 0} : none;
+    case 10: return (spec && e1) ? SVar{e1, t1, 0, 2, swz ? 2 : 0} : none;
+    case 11: return (spec && e1 && swz) ? SVar{e1, t1, 0, 2, 3} : none;
     default: return none;
   }
 }
@@ -545,6 +555,8 @@ hipError_t launch_strided_n(int vid, const StridedArgs& a, uint64_t n_outer, hip
     case 7: return launch_strided_nv<N, MODE, 7>(a, n_outer, s);
     case 8: return launch_strided_nv<N, MODE, 8>(a, n_outer, s);
     case 9: return launch_strided_nv<N, MODE, 9>(a, n_outer, s);
+    case 10: return launch_strided_nv<N, MODE, 10>(a, n_outer, s);
+    case 11: return launch_strided_nv<N, MODE, 11>(a, n_outer, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -110,7 +110,8 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
         const double ex = __dsub_rn(__dadd_rn(da * D, cx), X);
         int64_t jx = ix + da;
         if (b.periodic) jx = wrap(jx, nx); else if (jx < 0 || jx >= nx) outside = true;
-        if (jx < x0 || jx >= x0 + nxs) off_slab = true;
+        // a cell inside the box but not among the resident planes
+        if (jx >= 0 && jx < nx && (jx < x0 || jx >= x0 + nxs)) off_slab = true;
         for (int db = -3; db <= 3; ++db) {
           const double ey = __dsub_rn(__dadd_rn(db * D, cy), Y);
           int64_t jy = iy + db;
@@ -148,7 +149,7 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
         } else if (jx < 0 || jx >= nx || jy < 0 || jy >= ny || jz < 0 || jz >= nz) {
           outside = true;
         }
-        if (jx < x0 || jx >= x0 + nxs) off_slab = true;
+        if (jx >= 0 && jx < nx && (jx < x0 || jx >= x0 + nxs)) off_slab = true;
         if (outside || off_slab) continue;
         const int64_t flat = ((jx - x0) * ny + jy) * nz + jz;
         for (int f = 0; f < nf; ++f) acc[f] += w * (double)fld[f][flat];
