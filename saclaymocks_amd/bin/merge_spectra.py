@@ -200,7 +200,7 @@ def run(args):
     # forest pixels lambda / (1 + z_qso) in (lylimit, lya) (:301-302) and their
     # mean redshift z_eff (:313), in the dtypes of the files as the reference
     imax = np.zeros(n, np.int32)
-    zeff = np.zeros(n)
+    zeff = np.full(n, zz[0])      # kept for spectra without forest pixels (their delta_s is 0)
     for q, m in enumerate(merged):
         rf = m["lam"] / (1 + m["meta"]["Z"])
         mmm = np.nonzero((rf < util.LYA) & (rf > util.LYLIMIT))[0]

@@ -157,12 +157,14 @@ __device__ __forceinline__ float2 aux_apply(float2 d, const AuxSel& s, float kx,
 // per thread do not fit the register file at E = 40), 3 stashes it in the aux
 // output rows the same thread overwrites later (3 FFTs; the rows are still
 // in L2 / MALL when they are read back).
-// SWS places the line swizzle of unpadded lines (see LdsMap).
+// SWS places the line swizzle of unpadded lines (see LdsMap).  OCC > 0 asks
+// the compiler for at least OCC waves per SIMD (it then caps the registers).
 // Global addresses are a block-uniform base, a per-lane offset computed once
 // and a uniform per-point offset; lanes past the last line read the block's
 // first line (always in range) and store nothing.
-template <int N, int E, int T, int MODE, int XCH, int TILES, int SWS>
-__global__ __launch_bounds__(T*(N / E)) void k_strided(StridedArgs a) {
+template <int N, int E, int T, int MODE, int XCH, int TILES, int SWS, int OCC>
+__global__ __launch_bounds__(T*(N / E)) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_strided(
+    StridedArgs a) {
   constexpr int P = N / E;
   constexpr int LL = lds_line_len(N);
   constexpr int NT = T * P;
@@ -470,7 +472,7 @@ bool length_supported(int n) {
 // swizzle shift SWS of unpadded lines.  Variant 0 exists for every length;
 // the others for n >= 512, where the passes are long enough to tune.
 struct SVar {
-  int e, t, xch, tiles, sws;
+  int e, t, xch, tiles, sws, occ;
 };
 constexpr int SMK_NVAR = SMK_BOX_NVAR;
 
@@ -497,31 +499,35 @@ constexpr int half_tile(int n) {
 //   3  E0, split exchange             8  E0, aux stash in the output, SWS 2 / 0
 //   4  E0, split, SWS 2 (swz)         9  E1, aux stash, SWS 2 / 0
 //  10  E1, aux in registers, SWS 2 / 0  11  E1, aux in registers, SWS 3 (swz)
+//  12  as 10 with half the lines per block and >= 3 waves per SIMD
+//  13  as 12 with >= 4 waves per SIMD
 // E0 = elems_per_thread(n), E1 = half_tile(n).
 constexpr SVar strided_variant(int n, int mode, int vid) {
   const int e0 = elems_per_thread(n);
   const int e1 = half_tile(n);
-  const SVar none{0, 0, 0, 0, 0};
+  const SVar none{0, 0, 0, 0, 0, 0};
   if (e0 == 0) return none;
   const int t0 = lines_per_block(n, e0, 0, 512);
-  if (vid == 0) return SVar{e0, t0, 0, 1, 0};
+  if (vid == 0) return SVar{e0, t0, 0, 1, 0, 0};
   if (n < 512) return none;
   const bool spec = mode == MODE_SPECTRAL;
   const bool swz = !lds_padded(n);
   const int ts = lines_per_block(n, e0, 1, 1024);
   const int t1 = e1 ? lines_per_block(n, e1, 0, 1024) : 0;
   switch (vid) {
-    case 1: return swz ? SVar{e0, t0, 0, 1, 2} : none;
-    case 2: return swz ? SVar{e0, t0, 0, 1, 3} : none;
-    case 3: return SVar{e0, ts, 1, 1, 0};
-    case 4: return swz ? SVar{e0, ts, 1, 1, 2} : none;
-    case 5: return swz ? SVar{e0, ts, 1, 1, 3} : none;
-    case 6: return e1 ? SVar{e1, t1, 0, 1, swz ? 2 : 0} : none;
-    case 7: return (e1 && swz) ? SVar{e1, t1, 0, 1, 3} : none;
-    case 8: return spec ? SVar{e0, t0, 0, 3, swz ? 2 : 0} : none;
-    case 9: return (spec && e1) ? SVar{e1, t1, 0, 3, swz ? 2 : 0} : none;
-    case 10: return (spec && e1) ? SVar{e1, t1, 0, 2, swz ? 2 : 0} : none;
-    case 11: return (spec && e1 && swz) ? SVar{e1, t1, 0, 2, 3} : none;
+    case 1: return swz ? SVar{e0, t0, 0, 1, 2, 0} : none;
+    case 2: return swz ? SVar{e0, t0, 0, 1, 3, 0} : none;
+    case 3: return SVar{e0, ts, 1, 1, 0, 0};
+    case 4: return swz ? SVar{e0, ts, 1, 1, 2, 0} : none;
+    case 5: return swz ? SVar{e0, ts, 1, 1, 3, 0} : none;
+    case 6: return e1 ? SVar{e1, t1, 0, 1, swz ? 2 : 0, 0} : none;
+    case 7: return (e1 && swz) ? SVar{e1, t1, 0, 1, 3, 0} : none;
+    case 8: return spec ? SVar{e0, t0, 0, 3, swz ? 2 : 0, 0} : none;
+    case 9: return (spec && e1) ? SVar{e1, t1, 0, 3, swz ? 2 : 0, 0} : none;
+    case 10: return (spec && e1) ? SVar{e1, t1, 0, 2, swz ? 2 : 0, 0} : none;
+    case 11: return (spec && e1 && swz) ? SVar{e1, t1, 0, 2, 3, 0} : none;
+    case 12: return (spec && e1 && t1 >= 2) ? SVar{e1, t1 / 2, 0, 2, swz ? 2 : 0, 3} : none;
+    case 13: return (spec && e1 && t1 >= 2) ? SVar{e1, t1 / 2, 0, 2, swz ? 2 : 0, 4} : none;
     default: return none;
   }
 }
@@ -536,7 +542,7 @@ hipError_t launch_strided_nv(const StridedArgs& a, uint64_t n_outer, hipStream_t
     b.n_iblk = (a.n_inner + sv.t - 1) / sv.t;
     const uint64_t nblk = n_outer * b.n_iblk;
     if (nblk > 0x7fffffffull) return hipErrorInvalidConfiguration;
-    hipLaunchKernelGGL((k_strided<N, sv.e, sv.t, MODE, sv.xch, sv.tiles, sv.sws>), dim3((uint32_t)nblk),
+    hipLaunchKernelGGL((k_strided<N, sv.e, sv.t, MODE, sv.xch, sv.tiles, sv.sws, sv.occ>), dim3((uint32_t)nblk),
                        dim3(sv.t * (N / sv.e)), 0, s, b);
     return hipGetLastError();
   }
@@ -557,6 +563,8 @@ hipError_t launch_strided_n(int vid, const StridedArgs& a, uint64_t n_outer, hip
     case 9: return launch_strided_nv<N, MODE, 9>(a, n_outer, s);
     case 10: return launch_strided_nv<N, MODE, 10>(a, n_outer, s);
     case 11: return launch_strided_nv<N, MODE, 11>(a, n_outer, s);
+    case 12: return launch_strided_nv<N, MODE, 12>(a, n_outer, s);
+    case 13: return launch_strided_nv<N, MODE, 13>(a, n_outer, s);
     default: return hipErrorInvalidValue;
   }
 }

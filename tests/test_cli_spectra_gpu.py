@@ -104,7 +104,9 @@ def test_spectra_pipeline_matches_oracle(cuda, tmp_path):
     R_vec, lam32 = R_all[keep], np.float32(lam_all[keep])
     zpix = np.float32(cos.r_2_z(R_vec / fo.H))
     tabs = workloads.fgpa_tables()
-    a, b, c = (tabs[k].interp(zpix.astype(np.float64)) for k in ("a", "b", "c"))
+    # -aa and -cc default to the table at each pixel's redshift, -bb to 1.58 (merge_spectra.py:25-27, 60-75)
+    a, c = (tabs[k].interp(zpix.astype(np.float64)) for k in ("a", "c"))
+    b = 1.58
     G = fo.fgrowth(2.4) * 3.4 / (1 + zpix.astype(np.float64))
     seen = 0
     for hdu in (0, 1):
@@ -126,6 +128,11 @@ def test_spectra_pipeline_matches_oracle(cuda, tmp_path):
                 vals, _ = fo.gauss343(fields, R_vec * X / R_Q, R_vec * Y / R_Q, R_vec * Z / R_Q, L, D, R0)
                 dl = np.float32(vals[0])
                 eta = np.float32(fo.eta_par_radial(vals[1:], R_vec * X / R_Q, R_vec * Y / R_Q, R_vec * Z / R_Q))
+                # outside the forest (lambda >= lambda_Lya (1 + z_QSO)): delta = -1e6, eta = 0
+                # (make_spectra.py:99-100, :467-472), so F = 1 there
+                forest = lam_all[keep] < fo.LYA * (1 + r[2])
+                dl = np.where(forest, dl, np.float32(-1e6))
+                eta = np.where(forest, eta, np.float32(0.0))
                 rf = lam32 / (1 + meta["Z"][q])
                 mmm = np.nonzero((rf < fo.LYA) & (rf > 0.0))[0]
                 if len(mmm) == 0:
@@ -139,13 +146,14 @@ def test_spectra_pipeline_matches_oracle(cuda, tmp_path):
                     ds = ds * s(zpix) / s(zeff)
                 else:
                     ds = fo.missing_power_device(SEED + hdu, tid, len(R_vec), PIXEL, fo.p1d_missing_analytic)
+                if hdu == 0:
+                    assert np.abs(hd["DELTA_L"].data[q] - dl).max() < 2e-6, tid
+                    assert np.abs(hd["ETA_PAR"].data[q] - eta).max() < 2e-6, tid
+                    assert np.abs(hd["DELTA_S"].data[q] - ds).max() < 1e-5 * max(1.0, np.abs(ds).max()), tid
+                    if len(mmm) == 0:
+                        assert np.all(hd["DELTA_S"].data[q] == 0)
                 flux = fo.fgpa(dl.astype(np.float64) + ds, eta.astype(np.float64), G, a, b, c)
                 got = hd["FLUX"].data[q]
                 assert np.abs(got - flux).max() < 2e-5, (tid, np.abs(got - flux).max())
-                if hdu == 0:
-                    assert np.abs(hd["DELTA_L"].data[q] - dl).max() < 2e-6
-                    assert np.abs(hd["ETA_PAR"].data[q] - eta).max() < 2e-6
-                    assert np.abs(hd["DELTA_S"].data[q] - ds).max() < 1e-5 * max(1.0, np.abs(ds).max())
-                    if len(mmm) == 0:
-                        assert np.all(hd["DELTA_S"].data[q] == 0)
+                assert np.all(got[~forest] == 1.0)
     assert seen == len(rows)
