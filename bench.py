@@ -92,11 +92,11 @@ def cpu_baseline(cell, seconds=12.0):
 
 
 def forest_bench(delta, eta, shape, cell, steps, warmup, n_skewers):
-    """200k uneven skewers on the resident box -> forest pixels/s."""
+    """200k uneven skewers (300-1500 pixels) on the resident box -> forest pixels/s."""
     import torch
     from saclaymocks_amd import forest, workloads
     box = forest.BoxView.plane(delta, eta, cell=cell, r0=workloads.box_frame(shape, cell)["R0"], periodic=True)
-    batch, rows, _ = workloads.c3_batch(shape, cell, n=n_skewers)
+    batch, rows, _ = workloads.uneven_quasar_batch(shape, cell, n=n_skewers)
     plan = forest.ForestPlan()
     filt, foff = forest.missing_power_filters(0.2, workloads.p1d_missing)
     ds = torch.empty(batch.npix_total, dtype=torch.float32, device=delta.device)
@@ -133,7 +133,7 @@ def forest_bench(delta, eta, shape, cell, steps, warmup, n_skewers):
 
 
 def fit_bench(delta, eta, shape, cell, per_bin, maxfev):
-    """15-parameter FGPA fit (5 z bins), plain derivative-free fit
+    """15-parameter FGPA fit (5 z bins of 4000 skewers), plain derivative-free fit
     vs adjoint-gradient fit on the same device forward model."""
     from saclaymocks_amd import fit, forest, workloads
     box = forest.BoxView.plane(delta, eta, cell=cell, r0=workloads.box_frame(shape, cell)["R0"], periodic=True)
@@ -141,7 +141,7 @@ def fit_bench(delta, eta, shape, cell, per_bin, maxfev):
     zb = np.array(workloads.FIT_Z_BINS)
     # target realisation: b, c from etc/params.fits, a solved per bin for the
     # mean-flux target; its P1D is the P1D target
-    tb, td, te, growth, table = workloads.c4_problem(box, shape, cell, per_bin=per_bin, seed=101)
+    tb, td, te, growth, table = workloads.fit_bins_problem(box, shape, cell, per_bin=per_bin, seed=101)
     tm = fit.P1DFitModel(tb, td, te, growth, edges)
     f_target = workloads.mean_flux_target(zb)
     a_true = workloads.solve_a_for_mean_flux(tm, table[:, 1:], f_target)
@@ -150,7 +150,7 @@ def fit_bench(delta, eta, shape, cell, per_bin, maxfev):
     _, my, err, c = tm.profile(tbins)
     del tm, td, te
     # fit realisation
-    b, d, e, growth, _ = workloads.c4_problem(box, shape, cell, per_bin=per_bin, seed=202)
+    b, d, e, growth, _ = workloads.fit_bins_problem(box, shape, cell, per_bin=per_bin, seed=202)
     model = fit.P1DFitModel(b, d, e, growth, edges)
     model.set_targets(f_target, np.full(len(zb), 0.002), np.where(c > 0, my, np.nan), np.where(c > 0, err, np.nan))
     theta0 = np.tile([0.1, 1.5, 1.0], len(zb))

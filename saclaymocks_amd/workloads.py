@@ -54,8 +54,8 @@ def quasar_skewers(shape, cell, n, z_qso, lengths, seed, dr=0.2, fr=None):
     return {"x": x, "y": y, "z_qso": zq, "g0": g0.astype(np.int32), "npix": ln.astype(np.int32), "fr": fr}
 
 
-def c3_batch(shape, cell, n=200000, seed=7, dr=0.2):
-    """z_QSO ~ N(2.4, 0.4) clipped to [1.8, 3.6], lengths uniform in
+def uneven_quasar_batch(shape, cell, n=200000, seed=7, dr=0.2):
+    """n quasars with z_QSO ~ N(2.4, 0.4) clipped to [1.8, 3.6], lengths uniform in
     [300, 1500] pixels, FGPA parameters a, b, c and G at z_QSO (zfix per skewer)."""
     rng = np.random.default_rng(seed)
     zq = np.clip(rng.normal(2.4, 0.4, n), 1.8, 3.6)
@@ -68,8 +68,8 @@ def c3_batch(shape, cell, n=200000, seed=7, dr=0.2):
     return batch, rows, h
 
 
-def c4_problem(box, shape, cell, per_bin=4000, seed=11, dr=0.2, lengths=(300, 1500)):
-    """skewers in 5 redshift bins (zfix per bin), fields delta_l +
+def fit_bins_problem(box, shape, cell, per_bin=4000, seed=11, dr=0.2, lengths=(300, 1500)):
+    """Skewers in 5 redshift bins (zfix per bin), fields delta_l +
     delta_s and eta_par gathered from the resident box.  Returns the batch,
     delta, eta, growth per bin and the bins' true (a, b, c) from the table."""
     zb = np.array(FIT_Z_BINS)
@@ -93,7 +93,7 @@ def c4_problem(box, shape, cell, per_bin=4000, seed=11, dr=0.2, lengths=(300, 15
 
 
 def mean_flux_target(z):
-    """<F>(z) = exp(-0.0025 (1 + z)^3.7) of configs[3]."""
+    """Target mean flux of the fit workload: <F>(z) = exp(-0.0025 (1 + z)^3.7)."""
     return np.exp(-0.0025 * (1 + np.asarray(z, dtype=np.float64)) ** 3.7)
 
 
