@@ -330,6 +330,72 @@ int smk_p1d_adjoint(const smk_skewers* sk, const float* delta, const float* eta,
                     const double* edges, int nbins, int max_forest, const double* modes,
                     const int64_t* modes_off, const double* gp, double* out, void* stream);
 
+/* ---------------------------------------------------------- QSO catalogue */
+/* Quasar draw of one x slice of the lognormal boxes, draw_qso.py:186-462:
+ * per cell, ptot = c(z)(p12 - p23) + p23 of the three fields exp(p_k)^a_k(z)
+ * (float32 power as the reference, z at the cell centre), then
+ *   cond1  rnd1 < norm * ptot                 (randoms: rnd1 > 1 - 0.006)
+ *   cond2  density_max * rnd2 < dn_cell(z) / <ptot>(z)   (randoms: dn_cell(z))
+ *   cond3  |ra - ra0| < dra, |dec - dec0| < ddec, zmin < z (RSD) < zmax
+ * at a position drawn uniformly in the cell; z from R by the comoving-distance
+ * table, the RSD shift R += v_par (1 + z) D'(z) / (D'(0) H0) (:443-452).
+ * Uniforms: Philox4x32-10 keyed by `seed`, cell c = (ix*ny + iy)*nz + mz:
+ * counter (c_lo, c_hi, 0x71736f63, 0) gives rnd1, rnd2 (53 bits each from
+ * word pairs); (c_lo, c_hi, 0x71736f7a, 0) the z offset; (ix, mz, 0x71736f78, 0)
+ * and (iy, mz, 0x71736f79, 0) the x and y offsets, shared along a plane's
+ * rows and columns as in the reference (:433-436).  The reference draws the
+ * same quantities from numpy's legacy RNG plane after plane. */
+typedef struct smk_qso_params {
+  int32_t nx, ny, nz;            /* slab planes, y and z cells */
+  int32_t randoms;               /* 1: random catalogue (no fields) */
+  int32_t rsd;                   /* 1: velocity boxes given, cut on the RSD redshift */
+  int32_t zfix_on;               /* 1: every redshift is zfix */
+  double zfix;
+  double dx, dy, dz;             /* cell sides (Mpc/h) */
+  const double* x_axis;          /* device [nx], [ny], [nz]: cell-centre coordinates */
+  const double* y_axis;
+  const double* z_axis;
+  double h;                      /* z(R) is looked up at R / h */
+  const double* r2z_chi;         /* device [r2z_n]: comoving distance nodes (Mpc), ascending */
+  const double* r2z_z;           /* device [r2z_n]: their redshifts */
+  int32_t r2z_n;
+  const int32_t* r2z_guess;      /* device [r2z_ng]: last node <= r2z_lo + b * r2z_step */
+  int32_t r2z_ng;
+  double r2z_lo, r2z_step;
+  const double* coef_z;          /* device [coef_n]: c(z) nodes (padded to z = 0 and 10) */
+  const double* coef_c;
+  int32_t coef_n;
+  const double* dn_cell;         /* device [dn_n]: QSOs per cell on z = dn_z0 + i * dn_dz */
+  int32_t dn_n;
+  double dn_z0, dn_dz;
+  double sigma[3];               /* std of the three fields over the slab */
+  double norm, density_max;
+  double zmin, zmax;
+  double ra0, dec0;              /* radians */
+  double dra, ddec;              /* degrees */
+  const double* dg_z;            /* device [dg_n]: dD/dz table, ascending z */
+  const double* dg_v;
+  int32_t dg_n;
+  double dgrowth0;               /* dD/dz at z = 0 */
+} smk_qso_params;
+
+/* One drawn QSO: key = (mz * nx + ix) * ny + iy (the reference's output
+ * order), position (Mpc/h), R, redshift without and with RSD, ra, dec (deg). */
+typedef struct smk_qso_record {
+  int64_t key;
+  double x, y, z, r, z_qso, z_qso_rsd, ra, dec;
+} smk_qso_record;
+
+/*   p1, p2, p3   float32 [nx][ny][nz] lognormal Gaussian fields (NULL for randoms)
+ *   vx, vy, vz   float32 [nx][ny][nz] (used when rsd)
+ *   records      device [capacity]; filled in no particular order
+ *   count        device int64[1]: number of QSOs drawn (may exceed capacity:
+ *                then only `capacity` are stored; call again with a larger
+ *                buffer, the draw is deterministic) */
+int smk_qso_draw(const smk_qso_params* p, const float* p1, const float* p2, const float* p3, const float* vx,
+                 const float* vy, const float* vz, uint64_t seed, smk_qso_record* records, int64_t capacity,
+                 int64_t* count, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -39,6 +39,8 @@ def _parse_value(v):
         return v[1:end].replace("''", "'").rstrip()
     if "/" in v:
         v = v[:v.index("/")].strip()
+        if not v:
+            return None                    # undefined value
     if v == "T":
         return True
     if v == "F":
@@ -210,6 +212,11 @@ def read_header(path, ext=0):
 
 # ------------------------------------------------------------------ writing
 def _card(key, value, comment=None):
+    if value is None:                      # keyword with an undefined value
+        card = "%-8s= %20s" % (key[:8], "")
+        if comment:
+            card += " / " + comment
+        return card[:80].ljust(80)
     if isinstance(value, (bool, np.bool_)):
         v = "%20s" % ("T" if value else "F")
     elif isinstance(value, (int, np.integer)):
