@@ -132,6 +132,44 @@ def forest_bench(delta, eta, shape, cell, steps, warmup, n_skewers):
                          "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS}}
 
 
+def qso_bench(delta, eta, shape, cell, steps, warmup, nslice=16):
+    """Quasar draw over one x slice (nx / nslice planes) of the resident box:
+    the delta and eta_zz boxes stand in for the three lognormal fields and the
+    velocities (synthetic), footprint |ra|, |dec| < 20 deg inside the box."""
+    import torch
+    from saclaymocks_amd import qso
+    nx, ny, nz = shape
+    npl = nx // nslice
+    isl = nslice // 2                      # the slice at x in [0, LX / nslice): inside the footprint
+    p1, p2 = delta[isl * npl:(isl + 1) * npl], eta[isl * npl:(isl + 1) * npl]
+    sig = (float(p1.double().std()), float(p2.double().std()), float(p1.double().std()))
+    setup = qso.DrawSetup(npl, ny, nz, nx, cell, cell, cell, isl, nslice, sigma=sig, dra=20.0, ddec=20.0)
+    vel = (p2, p2, p2)
+    drawer = qso.Drawer(setup)
+    for i in range(warmup):
+        rec = drawer(p1, p2, p1, vel=vel, seed=500 + i)
+    torch.cuda.synchronize()
+    ms = []
+    for i in range(steps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        drawer.launch(p1, p2, p1, vel=vel, seed=i)
+        e1.record()
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    rec = drawer(p1, p2, p1, vel=vel, seed=0)
+    ms = float(np.mean(ms))
+    cells = npl * ny * nz
+    alg = 12.0 * cells       # three float32 fields per cell (velocities only at drawn cells)
+    return {"workload": f"QSO draw, one slice of {nslice}: {npl}x{ny}x{nz} cells, three lognormal fields "
+                        "(synthetic: the bench's delta and eta_zz), RSD, |ra|, |dec| < 20 deg",
+            "cells": cells, "qsos": int(len(rec)), "expected": setup.n_expected, "ms_per_slice": ms,
+            "cells_per_s": cells / (ms * 1e-3),
+            "roofline": {"bound": "hbm", "alg_bytes": alg, "achieved": alg / (ms * 1e-3) / 1e9,
+                         "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS,
+                         "limiter": "VALU: Philox, FP64 table lookups, a power and three exponentials per cell"}}
+
+
 def fit_bench(delta, eta, shape, cell, per_bin, maxfev):
     """15-parameter FGPA fit (5 z bins of 4000 skewers), plain derivative-free fit
     vs adjoint-gradient fit on the same device forward model."""
@@ -194,6 +232,7 @@ def main():
                          "HBM byte counters)")
     ap.add_argument("--skip-forest", action="store_true")
     ap.add_argument("--skip-fit", action="store_true")
+    ap.add_argument("--skip-qso", action="store_true")
     ap.add_argument("--forest-skewers", type=int, default=200000)
     ap.add_argument("--fit-per-bin", type=int, default=4000)
     ap.add_argument("--fit-maxfev", type=int, default=10000)
@@ -308,6 +347,8 @@ def main():
     if not args.skip_forest:
         out["forest"] = forest_bench(delta, eta, (nx, ny, nz), args.cell, args.steps, args.warmup,
                                      args.forest_skewers)
+    if not args.skip_qso:
+        out["qso"] = qso_bench(delta, eta, (nx, ny, nz), args.cell, args.steps, args.warmup)
     if not args.skip_fit:
         out["fit"] = fit_bench(delta, eta, (nx, ny, nz), args.cell, args.fit_per_bin, args.fit_maxfev)
     if not args.no_cpu_baseline:

@@ -25,6 +25,10 @@ constexpr uint32_t TAG_Z = 0x71736f7au;
 constexpr double ZB1 = 1.9, ZB2 = 2.75, ZB3 = 3.6;   // constant.py:30-32
 constexpr double H0_KMS = 100.0;                     // constant.py:36
 constexpr double RAND_QSO_NB = 0.006;                // constant.py:10
+// (1 + z_k)^-0.7 of the three bias redshifts
+constexpr float INV1 = 0.47459308f;   // 2.9^-0.7
+constexpr float INV2 = 0.39644059f;   // 3.75^-0.7
+constexpr float INV3 = 0.34361295f;   // 4.6^-0.7
 
 // uniform in [0, 1) with 53 bits from two Philox words
 __device__ __forceinline__ double u53(uint32_t a, uint32_t b) {
@@ -43,9 +47,10 @@ __device__ __forceinline__ double interp_from(double x, const double* __restrict
   return slope * (x - xp[j]) + fp[j];
 }
 
-// start node for a uniform grid of nodes x0 + i * dx
+// start node for a uniform grid of nodes x0 + i * dx (a guess: interp_from
+// walks to the exact node, so a reciprocal is good enough)
 __device__ __forceinline__ int uniform_guess(double x, double x0, double dx) {
-  const double g = floor((x - x0) / dx);
+  const double g = floor((x - x0) * __builtin_amdgcn_rcp(dx));
   return g < 0.0 ? 0 : (g > 2.0e9 ? 2000000000 : (int)g);
 }
 
@@ -73,9 +78,12 @@ __device__ __forceinline__ double coef_of_z(const smk_qso_params& p, double z) {
 // (util.py:508-517), i.e. ((1+z)/(1+z_k))^0.7
 __device__ __forceinline__ double a_of_z(double z, double zk) { return pow((1.0 + z) / (1.0 + zk), 0.7); }
 
+// the two linear interpolations between the bias redshifts and their mix
+// (draw_qso.py:247-250), with the constant divisions as products
 __device__ __forceinline__ double mix(double z, double c, double e1, double e2, double e3) {
-  const double p12 = e1 * (ZB2 - z) / (ZB2 - ZB1) + e2 * (z - ZB1) / (ZB2 - ZB1);
-  const double p23 = e2 * (ZB3 - z) / (ZB3 - ZB2) + e3 * (z - ZB2) / (ZB3 - ZB2);
+  constexpr double R12 = 1.0 / (ZB2 - ZB1), R23 = 1.0 / (ZB3 - ZB2);
+  const double p12 = (e1 * (ZB2 - z) + e2 * (z - ZB1)) * R12;
+  const double p23 = (e2 * (ZB3 - z) + e3 * (z - ZB2)) * R23;
   return c * (p12 - p23) + p23;
 }
 
@@ -114,10 +122,13 @@ __global__ __launch_bounds__(256) void k_qso_draw(smk_qso_params p, const float*
   if (p.randoms) {
     if (!(rnd1 > 1.0 - RAND_QSO_NB)) return;
   } else {
-    // exp(p)**a(z) in float32 as the reference's in-place float32 arrays (:219-223, :236-238)
-    const float e1 = powf(expf(p1[c]), (float)a_of_z(zc, ZB1));
-    const float e2 = powf(expf(p2[c]), (float)a_of_z(zc, ZB2));
-    const float e3 = powf(expf(p3[c]), (float)a_of_z(zc, ZB3));
+    // exp(p)**a_k(z), float32 as the reference's in-place float32 arrays
+    // (:219-223, :236-238), evaluated as exp(a_k p) with a_k = ((1+z)/(1+z_k))^0.7
+    // from one power per cell (the reference: a float32 exp, then a float32 power)
+    const float t = powf((float)(1.0 + zc), 0.7f);
+    const float e1 = expf(t * INV1 * p1[c]);
+    const float e2 = expf(t * INV2 * p2[c]);
+    const float e3 = expf(t * INV3 * p3[c]);
     const double ptot = mix(zc, coef_of_z(p, zc), (double)e1, (double)e2, (double)e3);
     if (!(rnd1 < p.norm * ptot)) return;
   }

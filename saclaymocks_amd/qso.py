@@ -174,54 +174,73 @@ def _guess_table(nodes, n_bins):
     return np.clip(np.searchsorted(nodes, edges, side="right") - 1, 0, len(nodes) - 2).astype(np.int32), lo, step
 
 
+class Drawer:
+    """smk_qso_draw with the set-up's tables uploaded once."""
+
+    def __init__(self, setup, capacity=None):
+        self.dev = torch.device("cuda", torch.cuda.current_device())
+        self._keep = []
+        s = setup
+        self.setup = s
+        prm = SmkQsoParams()
+        prm.nx, prm.ny, prm.nz = s.nx, s.ny, s.nz
+        prm.randoms, prm.rsd = int(s.randoms), int(s.rsd and not s.randoms)
+        prm.zfix_on, prm.zfix = int(s.zfix is not None), float(s.zfix or 0.0)
+        prm.dx, prm.dy, prm.dz = s.dx, s.dy, s.dz
+        prm.x_axis, prm.y_axis, prm.z_axis = self._t(s.x_axis), self._t(s.y_axis), self._t(s.z_axis)
+        prm.h = s.h
+        chi, zz = s.cosmo.chi, s.cosmo.z
+        guess, lo, step = _guess_table(chi, 4 * len(chi))
+        prm.r2z_chi, prm.r2z_z, prm.r2z_n = self._t(chi), self._t(zz), len(chi)
+        prm.r2z_guess, prm.r2z_ng = self._t(guess, torch.int32), len(guess)
+        prm.r2z_lo, prm.r2z_step = lo, step
+        prm.coef_z, prm.coef_c, prm.coef_n = self._t(s.coef_z), self._t(s.coef_c), len(s.coef_z)
+        prm.dn_cell, prm.dn_n, prm.dn_z0, prm.dn_dz = self._t(s.dn_cell), len(s.dn_cell), s.dn_z0, s.dn_dz
+        prm.sigma = (_d * 3)(*s.sigma)
+        prm.norm, prm.density_max = s.norm, s.density_max
+        prm.zmin, prm.zmax = s.z_min, s.z_max
+        prm.ra0, prm.dec0, prm.dra, prm.ddec = np.radians(s.ra0), np.radians(s.dec0), s.dra, s.ddec
+        prm.dg_z, prm.dg_v, prm.dg_n, prm.dgrowth0 = self._t(s.dg_z), self._t(s.dg_v), len(s.dg_z), s.dgrowth0
+        self.prm = prm
+        self.count = torch.zeros(1, dtype=torch.int64, device=self.dev)
+        self.capacity = int(capacity) if capacity is not None else max(1024, int(4 * s.n_expected) + 1024)
+        self.records = torch.empty((self.capacity, RECORD.itemsize), dtype=torch.uint8, device=self.dev)
+
+    def _t(self, a, dtype=torch.float64):
+        x = torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).to(self.dev)
+        self._keep.append(x)
+        return x.data_ptr()
+
+    def launch(self, p1, p2, p3, vel=None, seed=0, stream=None):
+        """One smk_qso_draw into the record buffer (asynchronous); returns the
+        device count."""
+        s = self.setup
+        for a in (p1, p2, p3):
+            if a is not None and (a.dtype != torch.float32 or tuple(a.shape) != (s.nx, s.ny, s.nz)
+                                  or not a.is_contiguous()):
+                raise ValueError("fields must be contiguous float32 [nx][ny][nz] tensors")
+        if self.prm.rsd and vel is None:
+            raise ValueError("rsd needs the velocity boxes")
+        vx, vy, vz = vel if vel is not None else (None, None, None)
+        _lib.check(_lib.lib().smk_qso_draw(ctypes.byref(self.prm), _lib.ptr(p1), _lib.ptr(p2), _lib.ptr(p3),
+                                           _lib.ptr(vx), _lib.ptr(vy), _lib.ptr(vz),
+                                           ctypes.c_uint64(int(seed) & (2 ** 64 - 1)), _lib.ptr(self.records),
+                                           self.capacity, _lib.ptr(self.count), _lib.stream_ptr(stream)),
+                   "smk_qso_draw")
+        return self.count
+
+    def __call__(self, p1, p2, p3, vel=None, seed=0, stream=None):
+        while True:
+            n = int(self.launch(p1, p2, p3, vel, seed, stream).item())
+            if n <= self.capacity:
+                break
+            self.capacity = n + 1024
+            self.records = torch.empty((self.capacity, RECORD.itemsize), dtype=torch.uint8, device=self.dev)
+        out = np.frombuffer(self.records[:n].cpu().numpy().tobytes(), dtype=RECORD).copy()
+        return out[np.argsort(out["key"], kind="stable")]
+
+
 def draw(setup, p1, p2, p3, vel=None, seed=0, capacity=None, stream=None):
     """smk_qso_draw over the slab (float32 CUDA tensors [nx][ny][nz]); returns
     a structured numpy array of RECORD, sorted in the reference's order."""
-    dev = torch.device("cuda", torch.cuda.current_device())
-    keep = []
-
-    def t(a, dtype=torch.float64):
-        x = torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).to(dev)
-        keep.append(x)
-        return x.data_ptr()
-
-    s = setup
-    prm = SmkQsoParams()
-    prm.nx, prm.ny, prm.nz = s.nx, s.ny, s.nz
-    prm.randoms, prm.rsd = int(s.randoms), int(s.rsd and not s.randoms)
-    prm.zfix_on, prm.zfix = int(s.zfix is not None), float(s.zfix or 0.0)
-    prm.dx, prm.dy, prm.dz = s.dx, s.dy, s.dz
-    prm.x_axis, prm.y_axis, prm.z_axis = t(s.x_axis), t(s.y_axis), t(s.z_axis)
-    prm.h = s.h
-    chi, zz = s.cosmo.chi, s.cosmo.z
-    guess, lo, step = _guess_table(chi, 4 * len(chi))
-    prm.r2z_chi, prm.r2z_z, prm.r2z_n = t(chi), t(zz), len(chi)
-    prm.r2z_guess, prm.r2z_ng, prm.r2z_lo, prm.r2z_step = t(guess, torch.int32), len(guess), lo, step
-    prm.coef_z, prm.coef_c, prm.coef_n = t(s.coef_z), t(s.coef_c), len(s.coef_z)
-    prm.dn_cell, prm.dn_n, prm.dn_z0, prm.dn_dz = t(s.dn_cell), len(s.dn_cell), s.dn_z0, s.dn_dz
-    prm.sigma = (_d * 3)(*s.sigma)
-    prm.norm, prm.density_max = s.norm, s.density_max
-    prm.zmin, prm.zmax = s.z_min, s.z_max
-    prm.ra0, prm.dec0, prm.dra, prm.ddec = np.radians(s.ra0), np.radians(s.dec0), s.dra, s.ddec
-    prm.dg_z, prm.dg_v, prm.dg_n, prm.dgrowth0 = t(s.dg_z), t(s.dg_v), len(s.dg_z), s.dgrowth0
-    for a in (p1, p2, p3):
-        if a is not None and (a.dtype != torch.float32 or tuple(a.shape) != (s.nx, s.ny, s.nz)
-                              or not a.is_contiguous()):
-            raise ValueError("fields must be contiguous float32 [nx][ny][nz] tensors")
-    vx, vy, vz = vel if vel is not None else (None, None, None)
-    if prm.rsd and vel is None:
-        raise ValueError("rsd needs the velocity boxes")
-    count = torch.zeros(1, dtype=torch.int64, device=dev)
-    cap = int(capacity) if capacity is not None else max(1024, int(4 * s.n_expected) + 1024)
-    L = _lib.lib()
-    while True:
-        rec = torch.empty((cap, RECORD.itemsize), dtype=torch.uint8, device=dev)
-        _lib.check(L.smk_qso_draw(ctypes.byref(prm), _lib.ptr(p1), _lib.ptr(p2), _lib.ptr(p3), _lib.ptr(vx),
-                                  _lib.ptr(vy), _lib.ptr(vz), ctypes.c_uint64(int(seed) & (2 ** 64 - 1)),
-                                  _lib.ptr(rec), cap, _lib.ptr(count), _lib.stream_ptr(stream)), "smk_qso_draw")
-        n = int(count.item())
-        if n <= cap:
-            break
-        cap = n + 1024
-    out = np.frombuffer(rec[:n].cpu().numpy().tobytes(), dtype=RECORD).copy()
-    return out[np.argsort(out["key"], kind="stable")]
+    return Drawer(setup, capacity)(p1, p2, p3, vel, seed, stream)
