@@ -396,6 +396,28 @@ int smk_qso_draw(const smk_qso_params* p, const float* p1, const float* p2, cons
                  const float* vy, const float* vz, uint64_t seed, smk_qso_record* records, int64_t capacity,
                  int64_t* count, void* stream);
 
+/* ------------------------------------------------------------ P(k) tables */
+/* A cubic spline as piecewise cubics: on [x[i], x[i+1]) (positive length),
+ * P(k) = ((c[i][0] t + c[i][1]) t + c[i][2]) t + c[i][3], t = k - x[i];
+ * guess[b] is an interval with x[guess[b]] <= g_lo + b * g_step (a start for
+ * the search).  All arrays are device arrays of doubles (guess: int32). */
+typedef struct smk_pk_spline {
+  const double* x;     /* [n_int + 1] breakpoints, ascending */
+  const double* c;     /* [n_int][4] */
+  int32_t n_int;
+  const int32_t* guess;
+  int32_t n_guess;
+  double g_lo, g_step;
+} smk_pk_spline;
+
+/* interpolate_pk.py:17-26, 63-149: table[i][j][l] = float32
+ * sqrt(float32(max(P(k), 0)) / vcell), k = sqrt(kx[i]^2 + ky[j]^2 + kz[l]^2)
+ * in float32 (correctly rounded, as numpy), P the spline at k in FP64.
+ *   kx, ky, kz   device float32 axes [nxs], [ny], [nzh]
+ *   table        device float32 [nxs][ny][nzh] */
+int smk_pk_table(const smk_pk_spline* spline, const float* kx, const float* ky, const float* kz, int nxs, int ny,
+                 int nzh, float vcell, float* table, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

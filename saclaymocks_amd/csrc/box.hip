@@ -233,7 +233,7 @@ __global__ __launch_bounds__(T*(N / E)) __attribute__((amdgpu_waves_per_eu(OCC >
         const int i = p1 + P * e;
         const float kx = kxv[i];
         // kk = kx*kx + ky*ky + kz*kz in float32, numpy evaluation order
-        const float kk = __fadd_rn(__fadd_rn(__fmul_rn(kx, kx), __fmul_rn(ky, ky)), __fmul_rn(kz, kz));
+        const float kk = add_rn(add_rn(mul_rn(kx, kx), mul_rn(ky, ky)), mul_rn(kz, kz));
         const float k = __builtin_amdgcn_sqrtf(kk);
         const float sp = __builtin_amdgcn_sqrtf(pk_analytic(k, a) * inv_vcell);
         const float2 d = cscale(v[e], sp * a.norm);
@@ -269,7 +269,7 @@ __global__ __launch_bounds__(T*(N / E)) __attribute__((amdgpu_waves_per_eu(OCC >
         for (int e = 0; e < E; ++e) {
           const int i = p3 + P * e;
           const float kx = kxv2[i];
-          const float kk = __fadd_rn(__fadd_rn(__fmul_rn(kx, kx), __fmul_rn(ky, ky)), __fmul_rn(kz, kz));
+          const float kk = add_rn(add_rn(mul_rn(kx, kx), mul_rn(ky, ky)), mul_rn(kz, kz));
           v[e] = aux_apply(v[e], sel, kx, ky, kz, kk, i == 0 && iky == 0 && ikz == 0, 1.0f / (float)N);
         }
         line_fft<N, E, true, XCH, SWS, SWB>(v, lds, t, p, a.tw, sw);

@@ -18,6 +18,25 @@
 
 namespace smk {
 
+// ------------------------------------------------------- unfused arithmetic
+// One IEEE operation each, which the compiler may not fuse with a neighbour
+// into an FMA.  HIP's __fadd_rn / __dmul_rn and friends are plain operators,
+// and hipcc contracts a product feeding a sum into one FMA even across those
+// calls; these are for results that must match numpy's (or numba's) unfused
+// evaluation bit for bit: k of a mode, pixel positions, cell indices.
+#define SMK_UNFUSED(T, NAME, OP)                                  \
+  __device__ __forceinline__ T NAME(T a, T b) {                   \
+    _Pragma("clang fp contract(off)") return a OP b;              \
+  }
+SMK_UNFUSED(float, add_rn, +)
+SMK_UNFUSED(float, sub_rn, -)
+SMK_UNFUSED(float, mul_rn, *)
+SMK_UNFUSED(double, add_rn, +)
+SMK_UNFUSED(double, sub_rn, -)
+SMK_UNFUSED(double, mul_rn, *)
+SMK_UNFUSED(double, div_rn, /)
+#undef SMK_UNFUSED
+
 // ---------------------------------------------------------------- complex ops
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }

@@ -11,7 +11,10 @@
 // (fixed thread-to-data assignment, fixed tree order).
 #include <math.h>
 
+#include "fft_core.h"
 #include "smk_internal.h"
+
+using namespace smk;
 
 namespace {
 
@@ -82,7 +85,7 @@ __global__ __launch_bounds__(FIT_THREADS) void k_reduce_by_set(const int32_t* __
 
 // numpy.fft.rfftfreq(n) * 2 * k_ny: (j * (1/n)) * 2 * (pi / pixel)
 __device__ __forceinline__ double kfreq(int j, int n, double k_ny) {
-  return __dmul_rn(__dmul_rn(__dmul_rn((double)j, 1.0 / (double)n), 2.0), k_ny);
+  return mul_rn(mul_rn(mul_rn((double)j, 1.0 / (double)n), 2.0), k_ny);
 }
 
 // scipy binned_statistic bin: edges[b] <= k < edges[b+1], last bin closed

@@ -63,14 +63,14 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
   const int i = (int)(pix - sk.off[s]);
   const int64_t g = (int64_t)sk.g0[s] + i;
   // R_g = rmin + g*dr (make_spectra.py:312), no contraction
-  const double R = __dadd_rn(sk.rmin, __dmul_rn((double)g, sk.dr));
+  const double R = add_rn(sk.rmin, mul_rn((double)g, sk.dr));
   double X, Y, Z;
   if (sk.geometry == SMK_GEOM_RADIAL) {
     // Xvec = Rvec * X_QSO / R_QSO (make_spectra.py:450-452)
     const double qr = sk.qr[s];
-    X = __ddiv_rn(__dmul_rn(R, sk.qx[s]), qr);
-    Y = __ddiv_rn(__dmul_rn(R, sk.qy[s]), qr);
-    Z = __ddiv_rn(__dmul_rn(R, sk.qz[s]), qr);
+    X = div_rn(mul_rn(R, sk.qx[s]), qr);
+    Y = div_rn(mul_rn(R, sk.qy[s]), qr);
+    Z = div_rn(mul_rn(R, sk.qz[s]), qr);
   } else {
     X = sk.qx[s];
     Y = sk.qy[s];
@@ -79,9 +79,9 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
   const smk_box_view& b = a.box;
   const double D = b.cell;
   // make_spectra.py:47-49: int((X + LX/2)/DX), int((Z + LZ/2 - R0)/DZ)
-  const int64_t ix = (int64_t)__ddiv_rn(__dadd_rn(X, a.hx), D);
-  const int64_t iy = (int64_t)__ddiv_rn(__dadd_rn(Y, a.hy), D);
-  const int64_t iz = (int64_t)__ddiv_rn(__dsub_rn(__dadd_rn(Z, a.hz), b.r0), D);
+  const int64_t ix = (int64_t)div_rn(add_rn(X, a.hx), D);
+  const int64_t iy = (int64_t)div_rn(add_rn(Y, a.hy), D);
+  const int64_t iz = (int64_t)div_rn(sub_rn(add_rn(Z, a.hz), b.r0), D);
   if (a.cells) {
     a.cells[3 * pix + 0] = (int32_t)ix;
     a.cells[3 * pix + 1] = (int32_t)iy;
@@ -101,27 +101,27 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
     const float* fld[7] = {b.delta, radial ? b.eta[0] : b.eta[2], b.eta[1], b.eta[2], b.eta[3], b.eta[4], b.eta[5]};
     if (a.interp == SMK_INTERP_GAUSS343) {
       // cell centres (i+0.5)D - L/2 (+R0 along Z), weights exp(-dr^2/(2 D^2))
-      const double cx = __dsub_rn(__dmul_rn((double)ix + 0.5, D), a.hx);
-      const double cy = __dsub_rn(__dmul_rn((double)iy + 0.5, D), a.hy);
-      const double cz = __dadd_rn(__dsub_rn(__dmul_rn((double)iz + 0.5, D), a.hz), b.r0);
+      const double cx = sub_rn(mul_rn((double)ix + 0.5, D), a.hx);
+      const double cy = sub_rn(mul_rn((double)iy + 0.5, D), a.hy);
+      const double cz = add_rn(sub_rn(mul_rn((double)iz + 0.5, D), a.hz), b.r0);
       const double sig2 = 2.0 * D * D;
       double wsum = 0.0;
       for (int da = -3; da <= 3; ++da) {
-        const double ex = __dsub_rn(__dadd_rn(da * D, cx), X);
+        const double ex = sub_rn(add_rn(da * D, cx), X);
         int64_t jx = ix + da;
         if (b.periodic) jx = wrap(jx, nx); else if (jx < 0 || jx >= nx) outside = true;
         // a cell inside the box but not among the resident planes
         if (jx >= 0 && jx < nx && (jx < x0 || jx >= x0 + nxs)) off_slab = true;
         for (int db = -3; db <= 3; ++db) {
-          const double ey = __dsub_rn(__dadd_rn(db * D, cy), Y);
+          const double ey = sub_rn(add_rn(db * D, cy), Y);
           int64_t jy = iy + db;
           if (b.periodic) jy = wrap(jy, ny); else if (jy < 0 || jy >= ny) outside = true;
           for (int dc = -3; dc <= 3; ++dc) {
-            const double ez = __dsub_rn(__dadd_rn(dc * D, cz), Z);
+            const double ez = sub_rn(add_rn(dc * D, cz), Z);
             int64_t jz = iz + dc;
             if (b.periodic) jz = wrap(jz, nz); else if (jz < 0 || jz >= nz) outside = true;
             if (outside || off_slab) continue;
-            const double r2 = __dadd_rn(__dadd_rn(__dmul_rn(ex, ex), __dmul_rn(ey, ey)), __dmul_rn(ez, ez));
+            const double r2 = add_rn(add_rn(mul_rn(ex, ex), mul_rn(ey, ey)), mul_rn(ez, ez));
             const double w = exp(-r2 / sig2);
             wsum += w;
             const int64_t flat = ((jx - x0) * ny + jy) * nz + jz;
@@ -132,9 +132,9 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
       for (int f = 0; f < nf; ++f) acc[f] /= wsum;
     } else {
       // trilinear on cell-centred values; indices wrap in a periodic box
-      const double ux = __ddiv_rn(__dadd_rn(X, a.hx), D) - 0.5;
-      const double uy = __ddiv_rn(__dadd_rn(Y, a.hy), D) - 0.5;
-      const double uz = __ddiv_rn(__dsub_rn(__dadd_rn(Z, a.hz), b.r0), D) - 0.5;
+      const double ux = div_rn(add_rn(X, a.hx), D) - 0.5;
+      const double uy = div_rn(add_rn(Y, a.hy), D) - 0.5;
+      const double uz = div_rn(sub_rn(add_rn(Z, a.hz), b.r0), D) - 0.5;
       const double fx0 = floor(ux), fy0 = floor(uy), fz0 = floor(uz);
       const double fx = ux - fx0, fy = uy - fy0, fz = uz - fz0;
       const int64_t i0x = (int64_t)fx0, i0y = (int64_t)fy0, i0z = (int64_t)fz0;
