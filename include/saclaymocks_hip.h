@@ -1,7 +1,8 @@
 /*
  * saclaymocks_hip.h - C ABI of libsaclay_hip.so, the MI355X (gfx950) kernels
- * behind the SaclayMocks box stage (make_boxes.py) and forest stage
- * (make_spectra.py + merge_spectra.py) and the FGPA fit (fit_az.py).
+ * behind the SaclayMocks box stage (make_boxes.py, interpolate_pk.py), the
+ * quasar draw (draw_qso.py), the forest stage (make_spectra.py +
+ * merge_spectra.py) and the FGPA fit (fit_az.py).
  *
  * Conventions
  *   - every entry point returns 0 (SMK_OK) or a negative SMK_ERR_* code and
