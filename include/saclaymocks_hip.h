@@ -419,6 +419,22 @@ typedef struct smk_pk_spline {
 int smk_pk_table(const smk_pk_spline* spline, const float* kx, const float* ky, const float* kz, int nxs, int ny,
                  int nzh, float vcell, float* table, void* stream);
 
+/* p1d_missing.py:12-37,120-140 (P_RSD, D_Prats, PW2cut) and
+ * powerspectrum.py:37-49 (P_1D_RSD): for every k_par[i],
+ *   out[i] = trapz_j( k_perp[j] * P3(k_par[i], k_perp[j]) ) / (2 pi)
+ * over the ascending grid k_perp (trapezoid rule, numpy.trapz), with
+ * k = sqrt(k_par^2 + k_perp^2), mu = k_par / max(k, 1e-15), P(k) = max(spline(k), 0)
+ * and P3 = (1 + beta mu^2)^2 F(k, mu) P(k), F by `model`:
+ *   0  1                                            (linear, P_RSD)
+ *   1  exp((q1 D2 + q2 D2^2)(1 - (k/kv)^av mu^bv) - (k/kp)^2),
+ *      D2 = k^3 P(k) G^2 / (2 pi^2)                 (P_RSD_Prats)
+ *   2  exp(-dx^2 k^2) for k < kny, else 0           (PW2cut)
+ * prats = {q1, q2, kp, kv, av, bv, G}; dx, kny used by model 2.
+ * One block per k_par, FP64, deterministic.  out: device double [n_par]. */
+int smk_p1d_rsd_rows(const smk_pk_spline* spline, const double* k_par, int n_par, const double* k_perp, int n_perp,
+                     double beta, int model, const double* prats, double dx, double kny, double* out,
+                     void* stream);
+
 #ifdef __cplusplus
 }
 #endif

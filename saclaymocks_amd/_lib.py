@@ -71,6 +71,7 @@ _SIGNATURES = {
     "smk_fgpa_flux": (_i, [_vp, _vp, _vp, _vp, _c.c_int64, _vp, _vp, _vp]),
     "smk_qso_draw": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _vp, _c.c_int64, _vp, _vp]),
     "smk_pk_table": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _c.c_float, _vp, _vp]),
+    "smk_p1d_rsd_rows": (_i, [_vp, _vp, _i, _vp, _i, _d, _i, _vp, _d, _d, _vp, _vp]),
     "smk_reduce_by_set": (_i, [_vp, _vp, _i, _i, _vp, _vp]),
     "smk_p1d_binned": (_i, [_vp, _vp, _vp, _d, _i, _vp, _i, _i, _vp, _vp, _vp, _vp, _vp]),
     "smk_p1d_adjoint": (_i, [_vp, _vp, _vp, _vp, _c.c_int64, _vp, _vp, _d, _i, _vp, _i, _i, _vp, _vp, _vp, _vp,
