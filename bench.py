@@ -170,6 +170,62 @@ def qso_bench(delta, eta, shape, cell, steps, warmup, nslice=16):
                          "limiter": "VALU: Philox, FP64 table lookups, a power and three exponentials per cell"}}
 
 
+P1D_BETA, P1D_DX, P1D_ZREF, P1D_DK = 1.6, 2.19, 2.2, 0.001
+
+
+def p1d_bench(steps, warmup):
+    """p1d_missing's 2-D integrals at the reference's default grid (k_max = 20,
+    dk = 0.001: 20000 k_par rows x 20000 k_perp points), the D_NL and the W^2
+    cut spectra of beta = 1.6, zref = 2.2, voxel 2.19, on the PlanckDR12 spline."""
+    import torch
+    from saclaymocks_amd import p1d, pk, util
+    beta, dx, zref = P1D_BETA, P1D_DX, P1D_ZREF
+    kk = np.arange(20.0 / P1D_DK) * P1D_DK
+    kn, pn = pk.p0_nodes(1.0)
+    spl = pk.DeviceSpline(kn, pn, np.sqrt(2.0) * kk[-1] * 1.001)
+    kt = torch.from_numpy(kk).cuda()
+    out = torch.empty_like(kt)
+    params, g = p1d.prats_params(zref), util.fgrowth(zref)
+
+    def both():
+        p1d.launch_rows(spl, kt, kt, beta, p1d.PRATS_D, out, params=params, growth=g)
+        p1d.launch_rows(spl, kt, kt, beta, p1d.W2_CUT, out, dx=dx)
+
+    for _ in range(warmup):
+        both()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        both()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    pts = 2.0 * len(kk) * len(kk)
+    return {"workload": "p1d_missing k_perp integrals, 20000 x 20000 points, D_NL and W^2-cut spectra (FP64)",
+            "ms_both_models": ms, "points_per_s": pts / (ms * 1e-3),
+            "limiter": "FP64 VALU: per point a square root, a division, a spline segment lookup and, for D_NL, a log and two exponentials"}
+
+
+def cpu_baseline_p1d(rows=100):
+    """The oracle's numpy rows (the reference's formulation of the same
+    integrals) on `rows` k_par rows of each model, one core."""
+    from oracle import p1d_oracle as po
+    from saclaymocks_amd import p1d, pk, util
+    kk = np.arange(20.0 / P1D_DK) * P1D_DK
+    kn, pn = pk.p0_nodes(1.0)
+    pfun = po.spline_p(kn, pn)
+    sel = kk[np.linspace(0, len(kk) - 1, rows).astype(int)]
+    t0 = time.perf_counter()
+    po.p1d_rsd(sel, kk, po.p3d_rsd(sel, kk, pfun, P1D_BETA, 1, growth=util.fgrowth(P1D_ZREF),
+                                   params=p1d.prats_params(P1D_ZREF)))
+    po.p1d_rsd(sel, kk, po.p3d_rsd(sel, kk, pfun, P1D_BETA, 2, dx=P1D_DX, kny=np.pi / P1D_DX))
+    pps = 2.0 * rows * len(kk) / (time.perf_counter() - t0)
+    return {"value": pps, "unit": "points/s", "cores": 1, "kind": "port",
+            "sample": f"{rows} k_par rows of each model x 20000 k_perp points",
+            "full_grid_s_extrapolated": 2.0 * len(kk) * len(kk) / pps}
+
+
 def fit_bench(delta, eta, shape, cell, per_bin, maxfev):
     """15-parameter FGPA fit (5 z bins of 4000 skewers), plain derivative-free fit
     vs adjoint-gradient fit on the same device forward model."""
@@ -233,6 +289,7 @@ def main():
     ap.add_argument("--skip-forest", action="store_true")
     ap.add_argument("--skip-fit", action="store_true")
     ap.add_argument("--skip-qso", action="store_true")
+    ap.add_argument("--skip-p1d", action="store_true")
     ap.add_argument("--forest-skewers", type=int, default=200000)
     ap.add_argument("--fit-per-bin", type=int, default=4000)
     ap.add_argument("--fit-maxfev", type=int, default=10000)
@@ -349,10 +406,14 @@ def main():
                                      args.forest_skewers)
     if not args.skip_qso:
         out["qso"] = qso_bench(delta, eta, (nx, ny, nz), args.cell, args.steps, args.warmup)
+    if not args.skip_p1d:
+        out["p1d_missing"] = p1d_bench(args.steps, args.warmup)
     if not args.skip_fit:
         out["fit"] = fit_bench(delta, eta, (nx, ny, nz), args.cell, args.fit_per_bin, args.fit_maxfev)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cell)
+        if "p1d_missing" in out:
+            out["p1d_missing"]["cpu_baseline"] = cpu_baseline_p1d()
     print(json.dumps(out), flush=True)
 
 

@@ -60,16 +60,26 @@ struct RsdArgs {
   double* out;
 };
 
-// k_perp * P3(k_par, k_perp) of p1d_missing.py:12-37, 49-52
-__device__ __forceinline__ double rsd_integrand(const RsdArgs& a, double kpar, double kperp) {
+// k_perp * P3(k_par, k_perp) of p1d_missing.py:12-37, 49-52; lkpar = log(k_par)
+template <int MODEL>
+__device__ __forceinline__ double rsd_integrand(const RsdArgs& a, double kpar, double lkpar, double kperp) {
   const double k = sqrt(kpar * kpar + kperp * kperp);
   const double mu = kpar / fmax(k, 1e-15);
   const double p = fmax(spline_eval(a.sp, k), 0.0);
   double f = 1.0;
-  if (a.model == 1) {
+  if (MODEL == 1) {
     const double d2 = k * k * k * p * a.G * a.G / 2.0 / (M_PI * M_PI);
-    f = exp((a.q1 * d2 + a.q2 * d2 * d2) * (1.0 - pow(k / a.kv, a.av) * pow(mu, a.bv)) - (k / a.kp) * (k / a.kp));
-  } else if (a.model == 2) {
+    // (k / k_v)^a_v mu^b_v as one exponential of logs, mu = k_par / k; the
+    // reference's powers where a base is 0
+    double pw;
+    if (kpar > 0.0) {
+      const double lk = log(k);
+      pw = exp(a.av * (lk - log(a.kv)) + a.bv * (lkpar - lk));
+    } else {
+      pw = pow(k / a.kv, a.av) * pow(mu, a.bv);
+    }
+    f = exp((a.q1 * d2 + a.q2 * d2 * d2) * (1.0 - pw) - (k / a.kp) * (k / a.kp));
+  } else if (MODEL == 2) {
     const double w = exp(-a.dx * a.dx * k * k / 2.0);
     f = k < a.kny ? w * w : 0.0;
   }
@@ -77,14 +87,22 @@ __device__ __forceinline__ double rsd_integrand(const RsdArgs& a, double kpar, d
   return kperp * r * r * f * p;
 }
 
+// One block per k_par row.  The trapezoid sum over k_perp is taken as
+// sum_j f_j (x_{j+1} - x_{j-1}) / 2 (half intervals at the ends), so each
+// point is evaluated once.
+template <int MODEL>
 __global__ __launch_bounds__(256) void k_p1d_rsd_rows(RsdArgs a) {
   __shared__ double red[4];
   const int i = blockIdx.x;
   const double kpar = a.k_par[i];
+  const double lkpar = kpar > 0.0 ? log(kpar) : 0.0;
+  const int n = a.n_perp;
   double acc = 0.0;
-  for (int j = threadIdx.x; j + 1 < a.n_perp; j += 256) {
-    const double x0 = a.k_perp[j], x1 = a.k_perp[j + 1];
-    acc += (x1 - x0) * (rsd_integrand(a, kpar, x0) + rsd_integrand(a, kpar, x1)) / 2.0;
+  for (int j = threadIdx.x; j < n; j += 256) {
+    const double x = a.k_perp[j];
+    const double hi = j + 1 < n ? a.k_perp[j + 1] : x;
+    const double lo = j > 0 ? a.k_perp[j - 1] : x;
+    acc += (hi - lo) * rsd_integrand<MODEL>(a, kpar, lkpar, x) / 2.0;
   }
   // wave then block reduction in a fixed order
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
@@ -118,7 +136,10 @@ extern "C" int smk_p1d_rsd_rows(const smk_pk_spline* spline, const double* k_par
   a.dx = dx;
   a.kny = kny;
   a.out = out;
-  hipLaunchKernelGGL(k_p1d_rsd_rows, dim3(n_par), dim3(256), 0, (hipStream_t)stream, a);
+  const hipStream_t st = (hipStream_t)stream;
+  if (model == 0) hipLaunchKernelGGL(k_p1d_rsd_rows<0>, dim3(n_par), dim3(256), 0, st, a);
+  else if (model == 1) hipLaunchKernelGGL(k_p1d_rsd_rows<1>, dim3(n_par), dim3(256), 0, st, a);
+  else hipLaunchKernelGGL(k_p1d_rsd_rows<2>, dim3(n_par), dim3(256), 0, st, a);
   SMK_LAUNCH_CHECK();
   return SMK_OK;
 }

@@ -77,15 +77,21 @@ def rsd_rows(spline, k_par, k_perp, beta, model, params=None, growth=None, dx=No
     kp = torch.from_numpy(np.ascontiguousarray(k_par, dtype=np.float64)).to(dev)
     kq = torch.from_numpy(np.ascontiguousarray(k_perp, dtype=np.float64)).to(dev)
     out = torch.empty(len(k_par), dtype=torch.float64, device=dev)
+    launch_rows(spline, kp, kq, beta, model, out, params, growth, dx, stream)
+    return out.cpu().numpy()
+
+
+def launch_rows(spline, k_par, k_perp, beta, model, out, params=None, growth=None, dx=None, stream=None):
+    """rsd_rows on device-resident float64 tensors: enqueues smk_p1d_rsd_rows
+    and returns without synchronising."""
     prats = None
     if model == PRATS_D:
         prats = (ctypes.c_double * 7)(*params, growth)
     kny = np.pi / dx if model == W2_CUT else 0.0
-    _lib.check(_lib.lib().smk_p1d_rsd_rows(ctypes.byref(spline.struct), _lib.ptr(kp), len(k_par), _lib.ptr(kq),
-                                           len(k_perp), ctypes.c_double(beta), model, prats,
+    _lib.check(_lib.lib().smk_p1d_rsd_rows(ctypes.byref(spline.struct), _lib.ptr(k_par), k_par.numel(),
+                                           _lib.ptr(k_perp), k_perp.numel(), ctypes.c_double(beta), model, prats,
                                            ctypes.c_double(dx or 0.0), ctypes.c_double(kny), _lib.ptr(out),
                                            _lib.stream_ptr(stream)), "smk_p1d_rsd_rows")
-    return out.cpu().numpy()
 
 
 def missing(beta, dx=2.19, zref=2.2, kmax=20.0, dk=0.001, rsd=True, linear=False, pkfile=None):
