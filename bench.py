@@ -290,6 +290,8 @@ def main():
     ap.add_argument("--skip-fit", action="store_true")
     ap.add_argument("--skip-qso", action="store_true")
     ap.add_argument("--skip-p1d", action="store_true")
+    ap.add_argument("--variant", action="append", default=[], metavar="PASS=ID",
+                    help="pin a pass's kernel variant (e.g. x_spectral=1) instead of autotuning; repeatable")
     ap.add_argument("--forest-skewers", type=int, default=200000)
     ap.add_argument("--fit-per-bin", type=int, default=4000)
     ap.add_argument("--fit-maxfev", type=int, default=10000)
@@ -321,7 +323,10 @@ def main():
     delta = torch.empty((nx, ny, nz), dtype=torch.float32, device=dev)
     eta = torch.empty((nx, ny, nz), dtype=torch.float32, device=dev)
     tune = None
-    if not args.no_autotune:
+    for kv in args.variant:
+        kind, vid = kv.split("=")
+        plan.set_variant(kind, int(vid))
+    if not args.no_autotune and not args.variant:
         # the FFTW_MEASURE analogue: pick each pass's kernel variant on these buffers
         # (outside the timed region, as the reference's planning is)
         t_tune = time.perf_counter()

@@ -39,11 +39,13 @@ class BoxPlan:
         _lib.check(L.smk_box_plan_create(ctypes.byref(h), int(nx), int(ny), int(nz), float(cell)),
                    "smk_box_plan_create")
         self._h = h
+        self._destroy = L.smk_box_plan_destroy
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and h.value:
-            _lib.lib().smk_box_plan_destroy(h)
+        destroy = getattr(self, "_destroy", None)
+        if h is not None and h.value and destroy is not None:
+            destroy(h)   # bound at creation: module globals may be gone at interpreter exit
             self._h = None
 
     @property

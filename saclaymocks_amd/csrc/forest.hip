@@ -41,8 +41,8 @@ struct GatherArgs {
   double hx, hy, hz;  // LX/2, LY/2, LZ/2
 };
 
-__device__ __forceinline__ int find_skewer(const int64_t* __restrict__ off, int n, int64_t pix) {
-  int lo = 0, hi = n - 1;
+// last skewer s in [lo, hi] with off[s] <= pix (skips empty skewers)
+__device__ __forceinline__ int find_skewer(const int64_t* __restrict__ off, int lo, int hi, int64_t pix) {
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
     if (off[mid] <= pix) lo = mid; else hi = mid - 1;
@@ -50,16 +50,52 @@ __device__ __forceinline__ int find_skewer(const int64_t* __restrict__ off, int 
   return lo;
 }
 
+// find_skewer for a whole wave: a 64-ary search, one load per lane and step
+// (log64 n steps).  Every lane of the wave must call it; off[lo] <= pix.
+__device__ __forceinline__ int wave_find_skewer(const int64_t* __restrict__ off, int lo, int hi, int64_t pix,
+                                                int lane) {
+  while (lo < hi) {
+    const int64_t step = ((int64_t)hi - lo + 64) / 64;
+    const int64_t c = (int64_t)lo + (int64_t)lane * step;
+    const uint64_t m = __ballot(c <= hi && off[c] <= pix);
+    const int last = __popcll(m) - 1;  // lane 0 holds: off[lo] <= pix
+    lo += (int)(last * step);
+    const int64_t h = (int64_t)lo + step - 1;
+    hi = h < hi ? (int)h : hi;
+  }
+  return lo;
+}
+
+// i mod n in [0, n); indices within one box length of the box need no division
 __device__ __forceinline__ int wrap(int64_t i, int n) {
-  int64_t r = i % n;
+  if (i >= 0 && i < n) return (int)i;
+  if (i >= -(int64_t)n && i < 2 * (int64_t)n) return (int)(i < 0 ? i + n : i - n);
+  const int64_t r = i % n;
   return (int)(r < 0 ? r + n : r);
 }
 
 __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
-  const int64_t pix = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (pix >= a.npix_total) return;
+  const int64_t pix0 = (int64_t)blockIdx.x * blockDim.x;
+  const int64_t pix = pix0 + threadIdx.x;
   const smk_skewers& sk = a.sk;
-  const int s = find_skewer(sk.off, sk.n, pix);
+  // the skewers of this block's pixel range, searched once by the first wave
+  // (pix0 < npix_total): the first one, then the last by a 64-wide probe
+  __shared__ int s_rng[2];
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    const int64_t plast = pix0 + blockDim.x - 1 < a.npix_total ? pix0 + blockDim.x - 1 : a.npix_total - 1;
+    const int s0 = wave_find_skewer(sk.off, 0, sk.n - 1, pix0, lane);
+    const int64_t c = (int64_t)s0 + lane;
+    const int cnt = __popcll(__ballot(c < sk.n && sk.off[c] <= plast));
+    const int s1 = cnt < 64 ? s0 + cnt - 1 : wave_find_skewer(sk.off, s0 + 63, sk.n - 1, plast, lane);
+    if (lane == 0) {
+      s_rng[0] = s0;
+      s_rng[1] = s1;
+    }
+  }
+  __syncthreads();
+  if (pix >= a.npix_total) return;
+  const int s = find_skewer(sk.off, s_rng[0], s_rng[1], pix);
   const int i = (int)(pix - sk.off[s]);
   const int64_t g = (int64_t)sk.g0[s] + i;
   // R_g = rmin + g*dr (make_spectra.py:312), no contraction
@@ -89,6 +125,15 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
   }
   const bool in_forest = i >= sk.imin[s] && i < sk.imax[s];
   const bool radial = sk.geometry == SMK_GEOM_RADIAL;
+  // FGPA inputs, loaded before the field gather so their latency overlaps it
+  const int64_t row = sk.pset ? (int64_t)sk.pset[s] : g;
+  const bool row_ok = row >= 0 && row < a.abcg_rows;
+  float4 prow = make_float4(0.f, 0.f, 0.f, 0.f);
+  float ds = 0.0f;
+  if (a.flux) {
+    if (row_ok) prow = reinterpret_cast<const float4*>(a.abcg)[row];
+    if (a.delta_s) ds = a.delta_s[pix];
+  }
   float dl = -1000000.0f, eta = 0.0f;
   bool outside = false, off_slab = false;
   if (in_forest) {
@@ -177,16 +222,13 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
   if (a.delta_l) a.delta_l[pix] = dl;
   if (a.eta_par) a.eta_par[pix] = eta;
   if (a.flux) {
-    const int64_t row = sk.pset ? (int64_t)sk.pset[s] : g;
-    if (row < 0 || row >= a.abcg_rows) {
+    if (!row_ok) {
       a.flux[pix] = __int_as_float(0x7fc00000);
       if (a.status) atomicOr(&a.status[s], 2);
     } else {
-      const float4 p = reinterpret_cast<const float4*>(a.abcg)[row];
-      const float ds = a.delta_s ? a.delta_s[pix] : 0.0f;
       // F = exp(-a exp(b G (delta + c eta)))
-      const float gfield = dl + ds + p.z * eta;
-      a.flux[pix] = expf(-p.x * expf(p.y * p.w * gfield));
+      const float gfield = dl + ds + prow.z * eta;
+      a.flux[pix] = expf(-prow.x * expf(prow.y * prow.w * gfield));
     }
   }
 }

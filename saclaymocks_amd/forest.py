@@ -139,11 +139,13 @@ class ForestPlan:
         h = ctypes.c_void_p()
         _lib.check(_lib.lib().smk_forest_plan_create(ctypes.byref(h)), "smk_forest_plan_create")
         self._h = h
+        self._destroy = _lib.lib().smk_forest_plan_destroy
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and h.value:
-            _lib.lib().smk_forest_plan_destroy(h)
+        destroy = getattr(self, "_destroy", None)
+        if h is not None and h.value and destroy is not None:
+            destroy(h)   # bound at creation: module globals may be gone at interpreter exit
             self._h = None
 
 
