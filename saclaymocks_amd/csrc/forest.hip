@@ -74,16 +74,11 @@ __device__ __forceinline__ int wrap(int64_t i, int n) {
   return (int)(r < 0 ? r + n : r);
 }
 
-__global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
-  const int64_t pix0 = (int64_t)blockIdx.x * blockDim.x;
-  const int64_t pix = pix0 + threadIdx.x;
-  const smk_skewers& sk = a.sk;
-  // the skewers of this block's pixel range, searched once by the first wave
-  // (pix0 < npix_total): the first one, then the last by a 64-wide probe
-  __shared__ int s_rng[2];
+// The block's skewer range [s_rng[0], s_rng[1]] for pixels [pix0, plast]
+// (pix0 < npix_total), searched by the first wave; every thread must call it.
+__device__ __forceinline__ void block_skewers(const smk_skewers& sk, int64_t pix0, int64_t plast, int* s_rng) {
   if (threadIdx.x < 64) {
     const int lane = threadIdx.x;
-    const int64_t plast = pix0 + blockDim.x - 1 < a.npix_total ? pix0 + blockDim.x - 1 : a.npix_total - 1;
     const int s0 = wave_find_skewer(sk.off, 0, sk.n - 1, pix0, lane);
     const int64_t c = (int64_t)s0 + lane;
     const int cnt = __popcll(__ballot(c < sk.n && sk.off[c] <= plast));
@@ -94,6 +89,15 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
     }
   }
   __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
+  const int64_t pix0 = (int64_t)blockIdx.x * blockDim.x;
+  const int64_t pix = pix0 + threadIdx.x;
+  const smk_skewers& sk = a.sk;
+  // the skewers of this block's pixel range, searched once by the first wave
+  __shared__ int s_rng[2];
+  block_skewers(sk, pix0, pix0 + blockDim.x - 1 < a.npix_total ? pix0 + blockDim.x - 1 : a.npix_total - 1, s_rng);
   if (pix >= a.npix_total) return;
   const int s = find_skewer(sk.off, s_rng[0], s_rng[1], pix);
   const int i = (int)(pix - sk.off[s]);
@@ -229,6 +233,145 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
       // F = exp(-a exp(b G (delta + c eta)))
       const float gfield = dl + ds + prow.z * eta;
       a.flux[pix] = expf(-prow.x * expf(prow.y * prow.w * gfield));
+    }
+  }
+}
+
+// k_gather for plane-parallel skewers with trilinear interpolation, PPT
+// pixels per thread (pix0 + threadIdx.x + 256 k): positions, cells and
+// corner indices of all of a thread's pixels first, then all their field
+// loads at once, then the sums, so that the loads' latency overlaps.  Same
+// arithmetic, in the same order, as k_gather.
+template <int PPT>
+__global__ __launch_bounds__(256) void k_gather_plane_tri(GatherArgs a) {
+  const int64_t pix0 = (int64_t)blockIdx.x * (256 * PPT);
+  const smk_skewers& sk = a.sk;
+  __shared__ int s_rng[2];
+  {
+    const int64_t pl = pix0 + 256 * PPT - 1;
+    block_skewers(sk, pix0, pl < a.npix_total ? pl : a.npix_total - 1, s_rng);
+  }
+  const smk_box_view& b = a.box;
+  const int nx = b.nx, ny = b.ny, nz = b.nz;
+  const int64_t x0 = b.nxs > 0 ? b.x0 : 0, nxs = b.nxs > 0 ? b.nxs : nx;
+  const bool has_eta = b.eta[2] != nullptr;
+  const double D = b.cell;
+
+  int s_[PPT];
+  bool act[PPT], inf[PPT], bad[PPT], outs[PPT], offs[PPT];
+  double fx_[PPT], fy_[PPT], fz_[PPT];
+  int64_t rx_[PPT][2];
+  int jy_[PPT][2], jz_[PPT][2];
+  float4 prow[PPT];
+  float dsv[PPT];
+  bool row_ok[PPT];
+#pragma unroll
+  for (int k = 0; k < PPT; ++k) {
+    const int64_t pix = pix0 + threadIdx.x + 256 * k;
+    act[k] = pix < a.npix_total;
+    const int s = act[k] ? find_skewer(sk.off, s_rng[0], s_rng[1], pix) : s_rng[0];
+    s_[k] = s;
+    const int i = (int)(pix - sk.off[s]);
+    const int64_t g = (int64_t)sk.g0[s] + i;
+    const double X = sk.qx[s], Y = sk.qy[s];
+    // R_g = rmin + g*dr (make_spectra.py:312), no contraction
+    const double Z = add_rn(sk.rmin, mul_rn((double)g, sk.dr));
+    const double qx = add_rn(X, a.hx), qy = add_rn(Y, a.hy), qz = sub_rn(add_rn(Z, a.hz), b.r0);
+    // make_spectra.py:47-49
+    const int64_t ix = (int64_t)div_rn(qx, D), iy = (int64_t)div_rn(qy, D), iz = (int64_t)div_rn(qz, D);
+    if (a.cells && act[k]) {
+      a.cells[3 * pix + 0] = (int32_t)ix;
+      a.cells[3 * pix + 1] = (int32_t)iy;
+      a.cells[3 * pix + 2] = (int32_t)iz;
+    }
+    inf[k] = act[k] && i >= sk.imin[s] && i < sk.imax[s];
+    const int64_t row = sk.pset ? (int64_t)sk.pset[s] : g;
+    row_ok[k] = row >= 0 && row < a.abcg_rows;
+    prow[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    dsv[k] = 0.0f;
+    if (a.flux && act[k]) {
+      if (row_ok[k]) prow[k] = reinterpret_cast<const float4*>(a.abcg)[row];
+      if (a.delta_s) dsv[k] = a.delta_s[pix];
+    }
+    // trilinear on cell-centred values; indices wrap in a periodic box
+    const double ux = div_rn(qx, D) - 0.5, uy = div_rn(qy, D) - 0.5, uz = div_rn(qz, D) - 0.5;
+    const double fx0 = floor(ux), fy0 = floor(uy), fz0 = floor(uz);
+    fx_[k] = ux - fx0;
+    fy_[k] = uy - fy0;
+    fz_[k] = uz - fz0;
+    bool outside = false, off_slab = false;
+    int64_t jx[2], jy[2], jz[2];
+#pragma unroll
+    for (int o = 0; o < 2; ++o) {
+      jx[o] = (int64_t)fx0 + o;
+      jy[o] = (int64_t)fy0 + o;
+      jz[o] = (int64_t)fz0 + o;
+      if (b.periodic) {
+        jx[o] = wrap(jx[o], nx);
+        jy[o] = wrap(jy[o], ny);
+        jz[o] = wrap(jz[o], nz);
+      } else if (jx[o] < 0 || jx[o] >= nx || jy[o] < 0 || jy[o] >= ny || jz[o] < 0 || jz[o] >= nz) {
+        outside = true;
+      }
+      if (jx[o] >= 0 && jx[o] < nx && (jx[o] < x0 || jx[o] >= x0 + nxs)) off_slab = true;
+    }
+    outs[k] = outside;
+    offs[k] = off_slab;
+    bad[k] = outside || off_slab || !inf[k];
+#pragma unroll
+    for (int o = 0; o < 2; ++o) {
+      // a pixel whose sum is not used loads cell 0 (always resident)
+      rx_[k][o] = bad[k] ? 0 : (jx[o] - x0) * ny;
+      jy_[k][o] = bad[k] ? 0 : (int)jy[o];
+      jz_[k][o] = bad[k] ? 0 : (int)jz[o];
+    }
+  }
+  float vd[PPT][8], ve[PPT][8];
+#pragma unroll
+  for (int k = 0; k < PPT; ++k) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int64_t flat = (rx_[k][c >> 2] + jy_[k][(c >> 1) & 1]) * nz + jz_[k][c & 1];
+      vd[k][c] = b.delta[flat];
+      ve[k][c] = has_eta ? b.eta[2][flat] : 0.0f;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < PPT; ++k) {
+    if (!act[k]) continue;
+    const int64_t pix = pix0 + threadIdx.x + 256 * k;
+    const int s = s_[k];
+    float dl = -1000000.0f, eta = 0.0f;
+    if (inf[k]) {
+      if (outs[k] || offs[k]) {
+        dl = __int_as_float(0x7fc00000);
+        eta = dl;
+        if (a.status) atomicOr(&a.status[s], (outs[k] ? 1 : 0) | (offs[k] ? 4 : 0));
+      } else {
+        const double fx = fx_[k], fy = fy_[k], fz = fz_[k];
+        double acc0 = 0.0, acc1 = 0.0;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          const int ox = c >> 2, oy = (c >> 1) & 1, oz = c & 1;
+          const double w = (ox ? fx : 1.0 - fx) * (oy ? fy : 1.0 - fy) * (oz ? fz : 1.0 - fz);
+          acc0 += w * (double)vd[k][c];
+          acc1 += w * (double)ve[k][c];
+        }
+        dl = (float)acc0;
+        if (has_eta) eta = (float)acc1;
+      }
+    }
+    if (a.delta_l) a.delta_l[pix] = dl;
+    if (a.eta_par) a.eta_par[pix] = eta;
+    if (a.flux) {
+      if (!row_ok[k]) {
+        a.flux[pix] = __int_as_float(0x7fc00000);
+        if (a.status) atomicOr(&a.status[s], 2);
+      } else {
+        // F = exp(-a exp(b G (delta + c eta)))
+        const float gfield = dl + dsv[k] + prow[k].z * eta;
+        a.flux[pix] = expf(-prow[k].x * expf(prow[k].y * prow[k].w * gfield));
+      }
     }
   }
 }
@@ -457,8 +600,14 @@ extern "C" int smk_forest_gather(const smk_skewers* sk, const smk_box_view* box,
   a.hy = box->cell * box->ny / 2;
   a.hz = box->cell * box->nz / 2;
   if (a.npix_total <= 0) return SMK_OK;
-  const uint64_t nblk = (uint64_t)(a.npix_total + 255) / 256;
-  hipLaunchKernelGGL(k_gather, dim3((uint32_t)nblk), dim3(256), 0, (hipStream_t)stream, a);
+  if (interp == SMK_INTERP_TRILINEAR && sk->geometry == SMK_GEOM_PLANE) {
+    constexpr int PPT = 2;
+    const uint64_t nblk = (uint64_t)(a.npix_total + 256 * PPT - 1) / (256 * PPT);
+    hipLaunchKernelGGL(k_gather_plane_tri<PPT>, dim3((uint32_t)nblk), dim3(256), 0, (hipStream_t)stream, a);
+  } else {
+    const uint64_t nblk = (uint64_t)(a.npix_total + 255) / 256;
+    hipLaunchKernelGGL(k_gather, dim3((uint32_t)nblk), dim3(256), 0, (hipStream_t)stream, a);
+  }
   SMK_LAUNCH_CHECK();
   return SMK_OK;
 }

@@ -72,6 +72,39 @@ def test_plane_gather_parity(cuda, interp):
     assert out["status"].sum().item() == 0
 
 
+@pytest.mark.parametrize("interp", ["gauss343", "trilinear"])
+def test_plane_gather_many_short_and_empty_skewers(cuda, interp):
+    """More than 64 skewers inside one block's pixel range, empty ones among
+    them: the block's skewer search takes its long branch."""
+    from saclaymocks_amd import forest
+    print(f"[forest short skewers] interp={interp}", flush=True)
+    shape, cell, r0 = (24, 20, 64), 2.19, 3000.0
+    rng = np.random.default_rng(5)
+    c = _plane_case(rng, shape, cell, r0, 700)
+    c["npix"] = rng.integers(0, 7, 700).astype(np.int32)
+    c["npix"][rng.random(700) < 0.3] = 0
+    c["imin"] = np.zeros(700, dtype=np.int32)
+    c["imax"] = c["npix"].copy()
+    d, e = _field(shape, 2), _field(shape, 3)
+    box = forest.BoxView.plane(torch.from_numpy(d).to(cuda), torch.from_numpy(e).to(cuda), cell=cell, r0=r0,
+                               periodic=True)
+    b = forest.SkewerBatch(forest.GEOM_PLANE, c["g0"], c["npix"], c["rmin"], c["dr"], c["qx"], c["qy"],
+                           imin=c["imin"], imax=c["imax"])
+    out = forest.gather(b, box, interp=interp, want=("delta_l", "eta_par"), cells=True)
+    X, Y, Z, _ = _plane_positions(c)
+    L, D = c["L"], (cell, cell, cell)
+    ix, iy, iz = fo.cell_index(X, Y, Z, L, D, r0)
+    cells = out["cells"].cpu().numpy()
+    assert np.array_equal(cells[:, 0], ix) and np.array_equal(cells[:, 1], iy) and np.array_equal(cells[:, 2], iz)
+    if interp == "gauss343":
+        (rd, re), _ = fo.gauss343([d, e], X, Y, Z, L, D, r0, periodic=True)
+    else:
+        rd, re = fo.trilinear([d, e], X, Y, Z, L, D, r0)
+    assert np.abs(out["delta_l"].cpu().numpy() - rd).max() < 2e-6
+    assert np.abs(out["eta_par"].cpu().numpy() - re).max() < 2e-6
+    assert out["status"].sum().item() == 0
+
+
 def test_radial_gather_parity_reference_geometry(cuda):
     from saclaymocks_amd import forest, util
     print("[forest radial gather gauss343]", flush=True)
