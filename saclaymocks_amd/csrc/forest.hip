@@ -488,14 +488,31 @@ __global__ __launch_bounds__(T* (M / elems_per_thread(M))) void k_missing_power(
 __global__ void k_classify(smk_skewers sk, const int32_t* __restrict__ fset, int32_t n_sets,
                            int32_t* __restrict__ lists, int32_t* __restrict__ counts, int32_t* __restrict__ bad) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= sk.n) return;
-  if (fset && (fset[s] < 0 || fset[s] >= n_sets)) { atomicOr(bad, 2); return; }
-  const int n = sk.npix[s];
-  int c = 0, nz = 256;
-  while (nz < n + 50 && c < N_CLASS) { nz <<= 1; ++c; }
-  if (c >= N_CLASS) { atomicOr(bad, 1); return; }
-  const int pos = atomicAdd(&counts[c], 1);
-  lists[(int64_t)c * sk.n + pos] = s;
+  int c = -1;  // length class, -1: none
+  if (s < sk.n) {
+    if (fset && (fset[s] < 0 || fset[s] >= n_sets)) {
+      atomicOr(bad, 2);
+    } else {
+      const int n = sk.npix[s];
+      int nz = 256;
+      c = 0;
+      while (nz < n + 50 && c < N_CLASS) { nz <<= 1; ++c; }
+      if (c >= N_CLASS) { atomicOr(bad, 1); c = -1; }
+    }
+  }
+  // one atomic per class and wave (the order within a class list is free:
+  // each skewer's missing power depends only on its own index)
+  const int lane = threadIdx.x & 63;
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  for (int k = 0; k < N_CLASS; ++k) {
+    const uint64_t m = __ballot(c == k);
+    if (m == 0) continue;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&counts[k], __popcll(m));
+    base = __shfl(base, leader, 64);
+    if (c == k) lists[(int64_t)k * sk.n + base + __popcll(m & below)] = s;
+  }
 }
 
 __global__ void k_zero_ints(int32_t* p, int n) {
