@@ -106,13 +106,14 @@ __global__ __launch_bounds__(256) void k_qso_draw(smk_qso_params p, const float*
                                                   const float* __restrict__ vx, const float* __restrict__ vy,
                                                   const float* __restrict__ vz, uint64_t seed,
                                                   smk_qso_record* __restrict__ rec, int64_t cap,
-                                                  unsigned long long* __restrict__ count, uint64_t ncells) {
-  const uint64_t c = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  if (c >= ncells) return;
-  const int mz = (int)(c % (uint64_t)p.nz);
-  const uint64_t rest = c / (uint64_t)p.nz;
-  const int iy = (int)(rest % (uint64_t)p.ny);
-  const int ix = (int)(rest / (uint64_t)p.ny);
+                                                  unsigned long long* __restrict__ count) {
+  // block (column ix * ny + iy, 256-cell chunk of z): the column's indices
+  // are block-uniform (scalar), the cell index c = column * nz + mz
+  const uint32_t col = blockIdx.x;
+  const int mz = (int)(blockIdx.y * 256 + threadIdx.x);
+  if (mz >= p.nz) return;
+  const int ix = (int)(col / (uint32_t)p.ny), iy = (int)(col % (uint32_t)p.ny);
+  const uint64_t c = (uint64_t)col * (uint64_t)p.nz + (uint64_t)mz;
   const double xc = p.x_axis[ix], yc = p.y_axis[iy], zcell = p.z_axis[mz];
   const double rc = sqrt(xc * xc + yc * yc + zcell * zcell);
   const double zc = z_of_r(p, rc / p.h);
@@ -207,11 +208,11 @@ extern "C" int smk_qso_draw(const smk_qso_params* p, const float* p1, const floa
   unsigned long long* cnt = reinterpret_cast<unsigned long long*>(count);
   hipLaunchKernelGGL(k_clear_count, dim3(1), dim3(64), 0, s, cnt);
   SMK_LAUNCH_CHECK();
-  const uint64_t ncells = (uint64_t)p->nx * (uint64_t)p->ny * (uint64_t)p->nz;
-  const uint64_t nblk = (ncells + 255) / 256;
-  SMK_REQUIRE(nblk <= 0x7fffffffull, SMK_ERR_ARG, "slab too large for one launch");
-  hipLaunchKernelGGL(k_qso_draw, dim3((uint32_t)nblk), dim3(256), 0, s, *p, p1, p2, p3, vx, vy, vz, seed, records,
-                     capacity, cnt, ncells);
+  const uint64_t ncol = (uint64_t)p->nx * (uint64_t)p->ny;
+  const uint32_t nzb = (uint32_t)((p->nz + 255) / 256);
+  SMK_REQUIRE(ncol <= 0x7fffffffull && nzb <= 65535u, SMK_ERR_ARG, "slab too large for one launch");
+  hipLaunchKernelGGL(k_qso_draw, dim3((uint32_t)ncol, nzb), dim3(256), 0, s, *p, p1, p2, p3, vx, vy, vz, seed,
+                     records, capacity, cnt);
   SMK_LAUNCH_CHECK();
   return SMK_OK;
 }
