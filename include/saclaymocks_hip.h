@@ -256,6 +256,14 @@ int smk_forest_gather(const smk_skewers* sk, const smk_box_view* box, int interp
 typedef struct smk_forest_plan smk_forest_plan;
 int smk_forest_plan_create(smk_forest_plan** plan); /* FFT tables for every class */
 int smk_forest_plan_destroy(smk_forest_plan* plan);
+/* Graph mode, for capturing a forest step in a HIP graph (no counterpart in
+ * the reference, whose steps run eagerly): with seed_dev non-NULL, the
+ * missing-power calls of this plan read the seed from *seed_dev (a device
+ * uint64) when their kernels run, so a replay takes the seed the caller has
+ * just written, and they do not read the length-class checks back to the host
+ * (a capture may not synchronise): run the batch once eagerly first, which
+ * checks it.  NULL restores the normal mode. */
+int smk_forest_plan_set_graph_mode(smk_forest_plan* plan, const uint64_t* seed_dev);
 size_t smk_forest_workspace_bytes(int n_skewers);   /* class bucketing scratch */
 /*   filt_off    HOST int64[6]
  *   workspace   smk_forest_workspace_bytes(sk->n) bytes

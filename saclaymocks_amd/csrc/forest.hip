@@ -396,9 +396,9 @@ template <int M, int T>
 __global__ __launch_bounds__(T* (M / elems_per_thread(M))) void k_missing_power(
     smk_skewers sk, const int32_t* __restrict__ list, const int32_t* __restrict__ count,
     const float* __restrict__ filt_all, const int32_t* __restrict__ fset, int64_t set_stride,
-    const float* __restrict__ scale, uint64_t seed, const float* __restrict__ noise,
-    const int64_t* __restrict__ noise_off, const float2* __restrict__ tw, const float2* __restrict__ twr,
-    float* __restrict__ delta_s) {
+    const float* __restrict__ scale, uint64_t seed_arg, const uint64_t* __restrict__ seed_dev,
+    const float* __restrict__ noise, const int64_t* __restrict__ noise_off, const float2* __restrict__ tw,
+    const float2* __restrict__ twr, float* __restrict__ delta_s) {
   constexpr int E = elems_per_thread(M);
   constexpr int P = M / E;
   constexpr int LL = lds_line_len(M);
@@ -406,6 +406,8 @@ __global__ __launch_bounds__(T* (M / elems_per_thread(M))) void k_missing_power(
   const int ncls = *count;
   const int base = blockIdx.x * T;
   if (base >= ncls) return;  // uniform across the block
+  // the seed of a graph replay is read from device memory
+  const uint64_t seed = seed_dev ? *seed_dev : seed_arg;
   const int tid = threadIdx.x;
   const int p = tid % P, t = tid / P;
   const bool valid = base + t < ncls;
@@ -528,13 +530,13 @@ constexpr int mp_T(int m) {
 template <int M>
 hipError_t launch_mp(const smk_skewers& sk, const int32_t* list, const int32_t* count, const float* filt,
                      const int32_t* fset, int64_t set_stride, const float* scale, uint64_t seed,
-                     const float* noise, const int64_t* noise_off, const float2* tw, const float2* twr,
-                     float* delta_s, hipStream_t s) {
+                     const uint64_t* seed_dev, const float* noise, const int64_t* noise_off, const float2* tw,
+                     const float2* twr, float* delta_s, hipStream_t s) {
   constexpr int T = mp_T(M);
   const int nblk = (sk.n + T - 1) / T;
   if (nblk == 0) return hipSuccess;
   hipLaunchKernelGGL((k_missing_power<M, T>), dim3(nblk), dim3(T * (M / elems_per_thread(M))), 0, s, sk, list,
-                     count, filt, fset, set_stride, scale, seed, noise, noise_off, tw, twr, delta_s);
+                     count, filt, fset, set_stride, scale, seed, seed_dev, noise, noise_off, tw, twr, delta_s);
   return hipGetLastError();
 }
 
@@ -552,6 +554,7 @@ std::vector<float2> twiddles_f(int n, double scale) {
 struct smk_forest_plan {
   float2* tw[N_CLASS];   // W_M^m, M = 128 << c
   float2* twr[N_CLASS];  // W_{2M}^k
+  const uint64_t* seed_dev;  // graph mode: the seed in device memory, no host read-back
 };
 
 extern "C" int smk_forest_plan_create(smk_forest_plan** plan) {
@@ -571,6 +574,12 @@ extern "C" int smk_forest_plan_create(smk_forest_plan** plan) {
     }
   }
   *plan = p;
+  return SMK_OK;
+}
+
+extern "C" int smk_forest_plan_set_graph_mode(smk_forest_plan* p, const uint64_t* seed_dev) {
+  SMK_REQUIRE(p != nullptr, SMK_ERR_ARG, "plan is NULL");
+  p->seed_dev = seed_dev;
   return SMK_OK;
 }
 
@@ -653,15 +662,19 @@ extern "C" int smk_forest_missing_power_sets(smk_forest_plan* plan, const smk_sk
                      counts + N_CLASS);
   SMK_LAUNCH_CHECK();
   const int64_t* foff = filt_off;
-  int32_t bad = 0;
-  SMK_HIP_CHECK(hipMemcpyAsync(&bad, counts + N_CLASS, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  SMK_HIP_CHECK(hipStreamSynchronize(s));
-  SMK_REQUIRE((bad & 2) == 0, SMK_ERR_RANGE, "a filter set index is outside [0, %d)", (int)n_sets);
-  SMK_REQUIRE(bad == 0, SMK_ERR_RANGE, "a skewer needs an FFT longer than %d", 256 << (N_CLASS - 1));
+  if (plan->seed_dev == nullptr) {
+    // (in graph mode the batch was checked by an eager call; a skewer that
+    // fails the checks is left out of every class list and gets no delta_s)
+    int32_t bad = 0;
+    SMK_HIP_CHECK(hipMemcpyAsync(&bad, counts + N_CLASS, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    SMK_HIP_CHECK(hipStreamSynchronize(s));
+    SMK_REQUIRE((bad & 2) == 0, SMK_ERR_RANGE, "a filter set index is outside [0, %d)", (int)n_sets);
+    SMK_REQUIRE(bad == 0, SMK_ERR_RANGE, "a skewer needs an FFT longer than %d", 256 << (N_CLASS - 1));
+  }
   const int64_t stride = filt_set ? set_stride : 0;
 #define SMK_MP(C, M)                                                                                            \
   SMK_HIP_CHECK(launch_mp<M>(*sk, lists + (int64_t)C * sk->n, counts + C, filt + foff[C], filt_set, stride,     \
-                             scale, seed, noise, noise_off, plan->tw[C], plan->twr[C], delta_s, s))
+                             scale, seed, plan->seed_dev, noise, noise_off, plan->tw[C], plan->twr[C], delta_s, s))
   SMK_MP(0, 128);
   SMK_MP(1, 256);
   SMK_MP(2, 512);

@@ -140,6 +140,17 @@ class ForestPlan:
         _lib.check(_lib.lib().smk_forest_plan_create(ctypes.byref(h)), "smk_forest_plan_create")
         self._h = h
         self._destroy = _lib.lib().smk_forest_plan_destroy
+        self._seed_dev = None
+
+    def set_graph_mode(self, seed_dev=None):
+        """Graph mode (smk_forest_plan_set_graph_mode): seed_dev, an int64 CUDA
+        tensor of one element, holds the missing-power seed read by the
+        kernels; None restores the normal mode."""
+        if seed_dev is not None and (seed_dev.dtype != torch.int64 or seed_dev.numel() != 1 or not seed_dev.is_cuda):
+            raise ValueError("seed_dev must be a one-element int64 CUDA tensor")
+        _lib.check(_lib.lib().smk_forest_plan_set_graph_mode(self._h, _lib.ptr(seed_dev)),
+                   "smk_forest_plan_set_graph_mode")
+        self._seed_dev = seed_dev
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -180,14 +191,15 @@ def fft_len_for(n):
 
 
 def missing_power(plan, batch, filt, filt_off, seed=0, noise=None, noise_off=None, out=None, stream=None,
-                  filt_set=None, set_stride=0, n_sets=1, scale=None):
+                  filt_set=None, set_stride=0, n_sets=1, scale=None, workspace=None):
     """delta_s per pixel (float32 CUDA tensor).  noise None -> device Fourier draw.
     filt_set (int32 per skewer) picks one of n_sets stacked filter sets
     (missing_power_filter_sets); scale (float32 per pixel) multiplies delta_s."""
     dev = _dev()
     if out is None:
         out = torch.empty(batch.npix_total, dtype=torch.float32, device=dev)
-    ws = torch.empty(int(_lib.lib().smk_forest_workspace_bytes(batch.n)), dtype=torch.uint8, device=dev)
+    ws = workspace if workspace is not None else \
+        torch.empty(int(_lib.lib().smk_forest_workspace_bytes(batch.n)), dtype=torch.uint8, device=dev)
     noff = None
     if noise is not None:
         noff = noise_off if isinstance(noise_off, torch.Tensor) else _t(noise_off, torch.int64, dev)
@@ -226,17 +238,71 @@ def gather(batch, box, interp="gauss343", abcg=None, delta_s=None, want=("delta_
             out[k] = torch.empty(n, dtype=torch.float32, device=dev)
     if cells:
         out["cells"] = torch.empty((n, 3), dtype=torch.int32, device=dev)
-    status = torch.zeros(batch.n, dtype=torch.int32, device=dev)
+    out["status"] = torch.zeros(batch.n, dtype=torch.int32, device=dev)
+    gather_into(batch, box, out, interp, abcg, delta_s, stream)
+    return out
+
+
+def gather_into(batch, box, out, interp="gauss343", abcg=None, delta_s=None, stream=None):
+    """gather() into caller-owned buffers: out maps "delta_l", "eta_par",
+    "flux", "cells" (any subset) and "status" (int32 per skewer, zeroed by the
+    caller) to CUDA tensors.  Allocates nothing."""
     sk = batch.struct()
     bv = box.struct()
     rows = int(abcg.shape[0]) if abcg is not None else 0
     rc = _lib.lib().smk_forest_gather(ctypes.byref(sk), ctypes.byref(bv), INTERP[interp], _lib.ptr(abcg),
                                       rows, _lib.ptr(delta_s), _lib.ptr(out.get("delta_l")),
                                       _lib.ptr(out.get("eta_par")), _lib.ptr(out.get("flux")),
-                                      _lib.ptr(out.get("cells")), _lib.ptr(status), _lib.stream_ptr(stream))
+                                      _lib.ptr(out.get("cells")), _lib.ptr(out["status"]), _lib.stream_ptr(stream))
     _lib.check(rc, "smk_forest_gather")
-    out["status"] = status
-    return out
+
+
+class CapturedStep:
+    """One forest step of a fixed batch (device missing power, gather, FGPA)
+    captured as a HIP graph (torch.cuda.CUDAGraph) and replayed with a new
+    seed per call.  It owns a forest plan in graph mode (the seed is read from
+    a device tensor); the batch is run once eagerly first, which checks it.
+    Outputs are the same tensors at every call: copy them before the next."""
+
+    def __init__(self, batch, box, filt, filt_off, abcg=None, interp="trilinear",
+                 want=("delta_l", "eta_par", "flux"), filt_set=None, set_stride=0, n_sets=1, scale=None):
+        dev = _dev()
+        n = batch.npix_total
+        plan = ForestPlan()
+        self.plan, self.batch = plan, batch
+        self.seed = torch.zeros(1, dtype=torch.int64, device=dev)
+        self.delta_s = torch.empty(n, dtype=torch.float32, device=dev)
+        self.workspace = torch.empty(int(_lib.lib().smk_forest_workspace_bytes(batch.n)), dtype=torch.uint8,
+                                     device=dev)
+        self.out = {k: torch.empty(n, dtype=torch.float32, device=dev) for k in want
+                    if k != "flux" or abcg is not None}
+        self.out["status"] = torch.empty(batch.n, dtype=torch.int32, device=dev)
+
+        def run():
+            self.out["status"].zero_()
+            missing_power(plan, batch, filt, filt_off, out=self.delta_s, workspace=self.workspace,
+                          filt_set=filt_set, set_stride=set_stride, n_sets=n_sets, scale=scale)
+            gather_into(batch, box, self.out, interp, abcg, self.delta_s)
+
+        plan.set_graph_mode(None)
+        run()                                  # eager: checks the batch
+        torch.cuda.synchronize()
+        plan.set_graph_mode(self.seed)
+        side = torch.cuda.Stream(device=dev)
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            run()                              # warm-up on a side stream before capture
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            run()
+
+    def __call__(self, seed):
+        """Replay with this seed; returns {name: tensor} (and "delta_s")."""
+        self.seed.fill_(int(seed) & (2 ** 63 - 1))
+        self.graph.replay()
+        return dict(self.out, delta_s=self.delta_s)
 
 
 def fgpa_rows(a, b, c, G):
