@@ -119,6 +119,20 @@ def forest_bench(delta, eta, shape, cell, steps, warmup, n_skewers):
     ms = e0.elapsed_time(e1) / steps
     f = out["flux"]
     assert int(out["status"].sum().item()) == 0
+    # the same step captured as a HIP graph and replayed with a new seed per step
+    cap = forest.CapturedStep(batch, box, filt, foff, abcg=rows, interp="trilinear", want=("flux",))
+    for i in range(warmup):
+        cap(100 + i)
+    torch.cuda.synchronize()
+    g0, g1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    g0.record()
+    for i in range(steps):
+        cap(i)
+    g1.record()
+    torch.cuda.synchronize()
+    ms_graph = g0.elapsed_time(g1) / steps
+    assert torch.equal(cap(steps - 1)["flux"], f)
+    del cap
     npx = batch.npix_total
     # algorithmic bytes: 4 z-columns x cells spanned x 2 fields x 4 B per skewer,
     # 4 B of F per pixel, 32 B of descriptor per skewer
@@ -126,7 +140,8 @@ def forest_bench(delta, eta, shape, cell, steps, warmup, n_skewers):
     alg = float(np.sum(4 * span * 2 * 4) + 4 * npx + 32 * batch.n)
     return {"workload": f"{batch.n} plane-parallel skewers, 300-1500 px of 0.2 Mpc/h, trilinear gather of "
                         "delta and eta_zz, missing power 0.05 exp(-(k/1.8)^2), FGPA at z_QSO",
-            "pixels": npx, "skewers": batch.n, "ms_per_step": ms, "pixels_per_s": npx * steps / dt,
+            "pixels": npx, "skewers": batch.n, "ms_per_step": ms, "ms_per_step_graph": ms_graph,
+            "pixels_per_s": npx * steps / dt,
             "mean_flux": float(f.double().mean().item()),
             "roofline": {"bound": "hbm", "alg_bytes": alg, "achieved": alg / (ms * 1e-3) / 1e9,
                          "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS}}
