@@ -94,6 +94,8 @@ __device__ __forceinline__ void block_skewers(const smk_skewers& sk, int64_t pix
   __syncthreads();
 }
 
+// NF fields: 1 (delta), 2 (delta, eta_zz) or 7 (delta and the six eta_ij)
+template <int NF>
 __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
   const int64_t pix0 = (int64_t)blockIdx.x * blockDim.x;
   const int64_t pix = pix0 + threadIdx.x;
@@ -148,15 +150,33 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
     // resident x planes [x0, x0 + nxs); the field pointers point at plane x0
     const int64_t x0 = b.nxs > 0 ? b.x0 : 0, nxs = b.nxs > 0 ? b.nxs : nx;
     // fields: 0 = delta, 1..6 = eta_xx..eta_yz (radial) or 1 = eta_zz (plane)
-    const int nf = b.eta[2] == nullptr ? 1 : (radial ? 7 : 2);
+    constexpr int nf = NF;
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     const float* fld[7] = {b.delta, radial ? b.eta[0] : b.eta[2], b.eta[1], b.eta[2], b.eta[3], b.eta[4], b.eta[5]};
     if (a.interp == SMK_INTERP_GAUSS343) {
-      // cell centres (i+0.5)D - L/2 (+R0 along Z), weights exp(-dr^2/(2 D^2))
+      // cell centres (i+0.5)D - L/2 (+R0 along Z), weights exp(-dr^2/(2 D^2)),
+      // taken as the product of the three axes' factors (21 exponentials, not 343)
       const double cx = sub_rn(mul_rn((double)ix + 0.5, D), a.hx);
       const double cy = sub_rn(mul_rn((double)iy + 0.5, D), a.hy);
       const double cz = add_rn(sub_rn(mul_rn((double)iz + 0.5, D), a.hz), b.r0);
       const double sig2 = 2.0 * D * D;
+      double wy[7], wz[7];
+      int64_t jyv[7], jzv[7];
+      for (int d = 0; d < 7; ++d) {
+        const double ey = sub_rn(add_rn((d - 3) * D, cy), Y);
+        const double ez = sub_rn(add_rn((d - 3) * D, cz), Z);
+        int64_t jy = iy + d - 3, jz = iz + d - 3;
+        if (b.periodic) {
+          jy = wrap(jy, ny);
+          jz = wrap(jz, nz);
+        } else if (jy < 0 || jy >= ny || jz < 0 || jz >= nz) {
+          outside = true;
+        }
+        wy[d] = exp(-mul_rn(ey, ey) / sig2);
+        wz[d] = exp(-mul_rn(ez, ez) / sig2);
+        jyv[d] = jy;
+        jzv[d] = jz;
+      }
       double wsum = 0.0;
       for (int da = -3; da <= 3; ++da) {
         const double ex = sub_rn(add_rn(da * D, cx), X);
@@ -164,19 +184,17 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
         if (b.periodic) jx = wrap(jx, nx); else if (jx < 0 || jx >= nx) outside = true;
         // a cell inside the box but not among the resident planes
         if (jx >= 0 && jx < nx && (jx < x0 || jx >= x0 + nxs)) off_slab = true;
-        for (int db = -3; db <= 3; ++db) {
-          const double ey = sub_rn(add_rn(db * D, cy), Y);
-          int64_t jy = iy + db;
-          if (b.periodic) jy = wrap(jy, ny); else if (jy < 0 || jy >= ny) outside = true;
-          for (int dc = -3; dc <= 3; ++dc) {
-            const double ez = sub_rn(add_rn(dc * D, cz), Z);
-            int64_t jz = iz + dc;
-            if (b.periodic) jz = wrap(jz, nz); else if (jz < 0 || jz >= nz) outside = true;
-            if (outside || off_slab) continue;
-            const double r2 = add_rn(add_rn(mul_rn(ex, ex), mul_rn(ey, ey)), mul_rn(ez, ez));
-            const double w = exp(-r2 / sig2);
+        if (outside || off_slab) continue;
+        const double wx = exp(-mul_rn(ex, ex) / sig2);
+        for (int db = 0; db < 7; ++db) {
+          const double wxy = wx * wy[db];
+          const int64_t row = ((jx - x0) * ny + jyv[db]) * nz;
+#pragma unroll
+          for (int dc = 0; dc < 7; ++dc) {
+            const double w = wxy * wz[dc];
             wsum += w;
-            const int64_t flat = ((jx - x0) * ny + jy) * nz + jz;
+            const int64_t flat = row + jzv[dc];
+#pragma unroll
             for (int f = 0; f < nf; ++f) acc[f] += w * (double)fld[f][flat];
           }
         }
@@ -635,7 +653,10 @@ extern "C" int smk_forest_gather(const smk_skewers* sk, const smk_box_view* box,
     hipLaunchKernelGGL(k_gather_plane_tri<PPT>, dim3((uint32_t)nblk), dim3(256), 0, (hipStream_t)stream, a);
   } else {
     const uint64_t nblk = (uint64_t)(a.npix_total + 255) / 256;
-    hipLaunchKernelGGL(k_gather, dim3((uint32_t)nblk), dim3(256), 0, (hipStream_t)stream, a);
+    const int nf = box->eta[2] == nullptr ? 1 : (sk->geometry == SMK_GEOM_RADIAL ? 7 : 2);
+    if (nf == 1) hipLaunchKernelGGL(k_gather<1>, dim3((uint32_t)nblk), dim3(256), 0, (hipStream_t)stream, a);
+    else if (nf == 2) hipLaunchKernelGGL(k_gather<2>, dim3((uint32_t)nblk), dim3(256), 0, (hipStream_t)stream, a);
+    else hipLaunchKernelGGL(k_gather<7>, dim3((uint32_t)nblk), dim3(256), 0, (hipStream_t)stream, a);
   }
   SMK_LAUNCH_CHECK();
   return SMK_OK;
