@@ -61,7 +61,9 @@ __device__ __forceinline__ int wave_find_skewer(const int64_t* __restrict__ off,
     const int64_t step = ((int64_t)hi - lo + 64) / 64;
     const int64_t c = (int64_t)lo + (int64_t)lane * step;
     const uint64_t m = __ballot(c <= hi && off[c] <= pix);
-    const int last = __popcll(m) - 1;  // lane 0 holds: off[lo] <= pix
+    // lane 0 holds (off[lo] <= pix) for packed offsets starting at 0; the
+    // clamp keeps the search in range for any offsets
+    const int last = __popcll(m) > 0 ? __popcll(m) - 1 : 0;
     lo += (int)(last * step);
     const int64_t h = (int64_t)lo + step - 1;
     hi = h < hi ? (int)h : hi;
@@ -85,7 +87,8 @@ __device__ __forceinline__ void block_skewers(const smk_skewers& sk, int64_t pix
     const int s0 = wave_find_skewer(sk.off, 0, sk.n - 1, pix0, lane);
     const int64_t c = (int64_t)s0 + lane;
     const int cnt = __popcll(__ballot(c < sk.n && sk.off[c] <= plast));
-    const int s1 = cnt < 64 ? s0 + cnt - 1 : wave_find_skewer(sk.off, s0 + 63, sk.n - 1, plast, lane);
+    const int s1 = cnt < 64 ? s0 + (cnt > 0 ? cnt - 1 : 0)
+                            : wave_find_skewer(sk.off, s0 + 63, sk.n - 1, plast, lane);
     if (lane == 0) {
       s_rng[0] = s0;
       s_rng[1] = s1;
