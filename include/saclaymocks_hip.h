@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define SMK_ABI_VERSION 2
+#define SMK_ABI_VERSION 3
 
 #define SMK_OK 0
 #define SMK_ERR_ARG (-1)         /* bad argument (null pointer, size, enum) */
@@ -202,6 +202,7 @@ typedef struct smk_box_view {
    * at plane x0 and hold planes [x0, x0 + nxs) of the nx-plane box; a cell
    * outside them is never read (status bit 2).  nxs = 0: the whole box. */
   int32_t x0, nxs;
+  const float* vel[3];  /* vx, vy, vz: the velocity boxes of v_par (RADIAL); NULL when unused */
 } smk_box_view;
 
 /* A batch of skewers on the global pixel grid R_g = rmin + g * dr (Mpc/h),
@@ -233,6 +234,10 @@ typedef struct smk_skewers {
  *               or, with pset NULL, the global pixel index g
  *   delta_s     NULL or float32 per pixel (smk_forest_missing_power)
  * Outputs (NULL to skip, float32 per pixel): delta_l, eta_par, flux;
+ *   v_par       the smoothed velocity along the line of sight, (vx X + vy Y
+ *               + vz Z) / R (make_spectra.py:130-134, the -dla skewers); RADIAL
+ *               skewers with the six eta and three velocity boxes; 0 outside
+ *               the forest;
  *   cells       int32 [pixels][3], containing cell (ix, iy, iz), make_spectra.py:47-49
  *   status      int32 per skewer (zeroed by the caller): bit 0 set if a kernel
  *               cell left a non-periodic box (the reference skips such a QSO,
@@ -240,8 +245,8 @@ typedef struct smk_skewers {
  *               range (flux NaN); bit 2 if a kernel cell lay outside the
  *               resident x slab (smk_box_view.x0, nxs). */
 int smk_forest_gather(const smk_skewers* sk, const smk_box_view* box, int interp, const float* abcg,
-                      int64_t abcg_rows, const float* delta_s, float* delta_l, float* eta_par, float* flux,
-                      int32_t* cells, int32_t* status, void* stream);
+                      int64_t abcg_rows, const float* delta_s, float* delta_l, float* eta_par, float* v_par,
+                      float* flux, int32_t* cells, int32_t* status, void* stream);
 
 /* Missing small-scale power, merge_spectra.py:303-322: for a skewer of n
  * pixels, nz = 256 * 2^m >= n + 50; delta_s = irfft(rfft(w) * filt)[:n] with

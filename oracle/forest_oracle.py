@@ -12,6 +12,8 @@ Pinned by (tests/test_oracle_forest.py):
     loop over the 343 cells on a tiny box, and the cell index of a point at a
     known cell centre;
   - trilinear: exact on linear fields;
+  - v_par_radial: constant velocity fields smoothed by gauss343 give v . x_hat;
+    velo_par_scale is 1 at z = 0 and (1 + z) D'(z)/D'(0) H(0)/H(z) by hand;
   - missing_power: closed form with a single-frequency noise vector;
   - fgpa: a = 0 gives F = 1, b = 0 gives exp(-a), delta = -1e6 gives F = 1;
   - p1d_1spectrum: brute-force DFT and Parseval;
@@ -57,6 +59,10 @@ class Cosmo:
 
     def r_2_z(self, r):
         return np.interp(r, self.chi, self.z)
+
+    def dist_hubble(self, z):
+        """util.py:828: c / H(z), linear in z between the nodes."""
+        return np.interp(z, self.z, C_KMS / self.hubble)
 
 
 def fgrowth(z, Om0=OMEGA_M, unnormed=False):
@@ -178,6 +184,19 @@ def eta_par_radial(eta6, X, Y, Z):
     exx, eyy, ezz, exy, exz, eyz = eta6
     RR = X ** 2 + Y ** 2 + Z ** 2
     return (X * exx * X + Y * eyy * Y + Z * ezz * Z + 2 * X * exy * Y + 2 * X * exz * Z + 2 * Y * eyz * Z) / RR
+
+
+def v_par_radial(v3, X, Y, Z):
+    """make_spectra.py:130-134: x_hat . v from the three smoothed velocities."""
+    vx, vy, vz = v3
+    return (vx * X + vy * Y + vz * Z) / np.sqrt(X ** 2 + Y ** 2 + Z ** 2)
+
+
+def velo_par_scale(z, dist_hubble, dgrowth):
+    """make_spectra.py:510: (1 + z) h_of_z(z) / h_of_z(0) dD/dz(z) / dD/dz(0),
+    h_of_z being the Hubble distance c / H(z) (util.py:828) and dgrowth the
+    dD/dz table."""
+    return (1 + z) * dist_hubble(z) / dist_hubble(0.0) * dgrowth(z) / dgrowth(0.0)
 
 
 # ----------------------------------------------------------- missing power

@@ -64,7 +64,7 @@ _SIGNATURES = {
     "smk_forest_plan_create": (_i, [_c.POINTER(_vp)]),
     "smk_forest_plan_destroy": (_i, [_vp]),
     "smk_forest_workspace_bytes": (_sz, [_i]),
-    "smk_forest_gather": (_i, [_vp, _vp, _i, _vp, _c.c_int64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "smk_forest_gather": (_i, [_vp, _vp, _i, _vp, _c.c_int64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "smk_forest_missing_power": (_i, [_vp, _vp, _vp, _c.POINTER(_c.c_int64), _u64, _vp, _vp, _vp, _vp, _vp]),
     "smk_forest_plan_set_graph_mode": (_i, [_vp, _vp]),
     "smk_forest_missing_power_sets": (_i, [_vp, _vp, _vp, _c.POINTER(_c.c_int64), _c.c_int64, _c.c_int32, _vp,

@@ -13,8 +13,10 @@ eta_ij-<ix>.fits with -rsd); the planes stay resident on the device and all
 QSOs of the slice are gathered in one launch (smk_forest_gather,
 343-cell Gaussian weights of ComputeWeight/computeRho, :39-139).  A QSO whose
 kernel leaves the box is skipped with the reference's warning (:490-494).
-Not supported: -dla (velocity skewers); -dmax other than 3 (the kernel's
-7^3 stencil).
+With -rsd and -dla the velocity boxes vx, vy, vz-<j>.fits are read too and a
+VELO_PAR extension follows ETA_PAR: the smoothed line-of-sight velocity
+(:130-134) times (1 + z) h_of_z(z) / h_of_z(0) dD/dz(z) / dD/dz(0) per pixel
+(:509-511).  Not supported: -dmax other than 3 (the kernel's 7^3 stencil).
 """
 import argparse
 import os
@@ -44,7 +46,7 @@ def parse_args(argv=None):
     p.add_argument("-NQSOfile", type=int, default=-1, help="number of QSO files, default = N")
     p.add_argument("-NQSO", type=int, default=-1, help="cut at NQSO, default = -1, no cut")
     p.add_argument("-rsd", default="True", help="If True, rsd are added, default True")
-    p.add_argument("-dla", default="False", help="velocity skewers (not supported here)")
+    p.add_argument("-dla", default="False", help="with -rsd, also the velocity skewers VELO_PAR")
     p.add_argument("-dgrowthfile", default=None, help="dD/dz file (used by -dla only)")
     p.add_argument("-ncpu", type=int, default=16, help="host threads for file reading")
     return p.parse_args(argv)
@@ -77,8 +79,8 @@ def read_qsos(prefix, i_slice, n_slice, nqsofile):
 def run(args):
     t_init = time.time()
     rsd = util.str2bool(args.rsd)
-    if util.str2bool(args.dla):
-        raise NotImplementedError("-dla True (velocity skewers) is not supported by this drop-in")
+    velo = rsd and util.str2bool(args.dla)      # the reference keeps v_par only with rsd (:505-511)
+    dgrowth = dgrowth_table(args.dgrowthfile) if util.str2bool(args.dla) else None
     if args.dmax != 3:
         raise NotImplementedError("the gather kernel uses the 7^3-cell stencil of -dmax 3")
     h0 = boxfiles.read_header(os.path.join(args.boxdir, "box"))
@@ -102,10 +104,14 @@ def run(args):
     if rsd:
         etas = [boxfiles.read_planes(os.path.join(args.boxdir, k), dev, args.ncpu, x0=x0, x1=x1)[0]
                 for k in ETA_KINDS]
+    vels = None
+    if velo:
+        vels = [boxfiles.read_planes(os.path.join(args.boxdir, k), dev, args.ncpu, x0=x0, x1=x1)[0]
+                for k in ("vx", "vy", "vz")]
     print(f"slab planes [{x0}, {x1}) read. {time.time() - t0:.1f} s")
     cosmo = util.Cosmo()
     R0 = util.H * cosmo.r_comoving(util.Z0)
-    box = forest.BoxView(delta, etas, cell=cell, r0=R0, periodic=False, x0=x0, nx=nhdu)
+    box = forest.BoxView(delta, etas, cell=cell, r0=R0, periodic=False, x0=x0, nx=nhdu, vel=vels)
 
     nqsofile = n_slice if args.NQSOfile < 0 else args.NQSOfile
     qsos, ra0, dec0 = read_qsos(args.QSOfile, i_slice, n_slice, nqsofile)
@@ -127,7 +133,8 @@ def run(args):
     t1 = time.time()
     batch = forest.SkewerBatch(forest.GEOM_RADIAL, rows["g0"], rows["npix"], grid["rmin"], grid["dr"], rows["qx"],
                                rows["qy"], rows["qz"], rows["qr"], imin=rows["imin"], imax=rows["imax"])
-    out = forest.gather(batch, box, interp="gauss343", want=("delta_l", "eta_par") if rsd else ("delta_l",))
+    want = ("delta_l", "eta_par", "v_par") if velo else (("delta_l", "eta_par") if rsd else ("delta_l",))
+    out = forest.gather(batch, box, interp="gauss343", want=want)
     status = out["status"].cpu().numpy()
     dl = batch.split(out["delta_l"])
     eta = batch.split(out["eta_par"]) if rsd else None
@@ -141,6 +148,13 @@ def run(args):
         good.append(s)
     lam = [grid["lambda_vec"][rows["k0"][s]:rows["k0"][s] + rows["npix"][s]] for s in good]
     red = [cosmo.r_2_z(grid["R_vec"][rows["k0"][s]:rows["k0"][s] + rows["npix"][s]] / util.H) for s in good]
+    vpar = None
+    if velo:
+        # :510, in float64 on the float64 v_par as the reference
+        vp = batch.split(out["v_par"])
+        h0z, d0 = cosmo.dist_hubble(0.0), dgrowth.interp(0.0)
+        vpar = [vp[s].astype(np.float64) * (1 + z) * cosmo.dist_hubble(z) / h0z * dgrowth.interp(z) / d0
+                for s, z in zip(good, red)]
     maxsize = max((len(x) for x in lam), default=0)
     hlist = [{"name": "z0", "value": util.Z0, "comment": "redshift of box center"},
              {"name": "pixel", "value": args.pixel}, {"name": "Npixel", "value": int(grid["npixeltot"])},
@@ -169,11 +183,29 @@ def run(args):
             f.write(padded([dl[good[j]] for j in msk], -2e6), extname="DELTA_L")
             if rsd:
                 f.write(padded([eta[good[j]] for j in msk], -2e6), extname="ETA_PAR")
+                if velo:
+                    f.write(padded([vpar[j] for j in msk], -2e6), extname="VELO_PAR")
             f.write(padded([red[j] for j in msk], -1.0), extname="REDSHIFT")
         nw += len(msk)
     print(nw, "QSO written")
     print(f"Slice {i_slice} done. Took {time.time() - t_init:.1f}s")
     return 0
+
+
+def dgrowth_table(path=None):
+    """dD/dz versus z (make_spectra.py:170-180): the packaged copy of
+    etc/dgrowth.fits, or a FITS table with columns Z, dD/dz whose header OM
+    must equal Omega_M."""
+    if path is None:
+        import json
+        with open(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "data",
+                               "dgrowth_z.json")) as f:
+            d = json.load(f)
+        return util.InterpTable(d["z"], d["dD/dz"])
+    hdu = fitsio_lite.read_all(path)[1]
+    if hdu.header["OM"] != util.OMEGA_M:
+        raise ValueError(f"Omega_M_0 ({util.OMEGA_M}) != OM in {path} ({hdu.header['OM']})")
+    return util.InterpTable(hdu.data["Z"], hdu.data["dD/dz"])
 
 
 def main(argv=None):

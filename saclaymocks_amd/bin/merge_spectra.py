@@ -5,8 +5,10 @@ Same command line (merge_spectra.py:21-42).  Reads every spectra-<j>-<i>.fits.gz
 of -inDir whose last field is -i (:128-138), concatenates the pieces of each
 THING_ID in ascending lambda (:250-262), keeps the spectra of the nominal
 length Npixel (:341-351), and writes spectra_merged-<pix>-<i>.fits.gz per
-HEALPix pixel (nside, nest; :385-406): METADATA, LAMBDA, FLUX and with
---store-g also DELTA_L, GROWTHF, VELO_PAR (zeros), ETA_PAR, Z, DELTA_S.
+HEALPix pixel (nside, nest; :385-406): METADATA, LAMBDA, FLUX; with -dla
+DELTA_L, GROWTHF and VELO_PAR (the pieces' VELO_PAR merged like the other
+columns, zeros without -rsd, :354-360, 394-397); with --store-g also DELTA_L,
+GROWTHF, VELO_PAR (zeros without -dla), ETA_PAR, Z, DELTA_S.
 
 Device work, for all merged spectra of the HDU at once:
   - delta_s: the missing-power GRF of :303-322 (smk_forest_missing_power[_sets])
@@ -22,7 +24,7 @@ a cubic spline (:104-109; --store-g is then forced as in :105); else the
 QSO's mean forest redshift z_eff, linear in k, and delta_s scaled by
 sigma(z) / sigma(z_eff) (:111-116, :316-322).  Without -p1dfile (the
 reference's default etc/pkmiss_interp.fits.gz is not distributed) the
-analytic A exp(-(k/k1)^2) of -p1dA / -p1dk1.  -dla is not supported.
+analytic A exp(-(k/k1)^2) of -p1dA / -p1dk1.
 """
 import argparse
 import glob
@@ -58,7 +60,7 @@ def parse_args(argv=None):
     p.add_argument("-nest", default="True", help="If True, healpix scheme is nest. Default True")
     p.add_argument("-rsd", default="True", help="If True, rsd are added, default True")
     p.add_argument("-addnoise", default="True", help="If True, small scales are added, default True")
-    p.add_argument("-dla", default="False", help="(not supported)")
+    p.add_argument("-dla", default="False", help="If True, store delta and growth skewers, default False")
     p.add_argument("-zfix", type=float, default=None, help="redshift of the FGPA parameters (default: per pixel)")
     p.add_argument("--fit-p1d", default="False", help="If True, P1D_miss(k) from a k table (forces --store-g)")
     p.add_argument("--store-g", default="False", help="If True, store delta_l, delta_s and eta_par")
@@ -130,11 +132,12 @@ def read_pieces(in_dir, hdu):
         if header is None:
             header = meta.header
         eta = hdus["ETA_PAR"].data if "ETA_PAR" in hdus else None
+        velo = hdus["VELO_PAR"].data if "VELO_PAR" in hdus else None
         for q in range(len(meta.data)):
             m = hdus["LAMBDA"].data[q] > 0
             pieces.append({"meta": meta.data[q], "lam": hdus["LAMBDA"].data[q][m],
                            "delta": hdus["DELTA_L"].data[q][m], "eta": None if eta is None else eta[q][m],
-                           "z": hdus["REDSHIFT"].data[q][m]})
+                           "velo": None if velo is None else velo[q][m], "z": hdus["REDSHIFT"].data[q][m]})
     return pieces, header
 
 
@@ -144,8 +147,7 @@ def run(args):
     add_noise = util.str2bool(args.addnoise)
     store_g = util.str2bool(args.store_g) or util.str2bool(args.fit_p1d)   # :105
     nest = util.str2bool(args.nest)
-    if util.str2bool(args.dla):
-        raise NotImplementedError("-dla True is not supported by this drop-in")
+    dla = util.str2bool(args.dla)
     seed = args.seed
     if seed is None:
         seed = int(np.random.randint(2**31 - 1))
@@ -186,6 +188,9 @@ def run(args):
                "delta": np.concatenate([p["delta"] for p in ps])[order],
                "z": np.concatenate([p["z"] for p in ps])[order]}
         rec["eta"] = np.concatenate([p["eta"] for p in ps])[order] if rsd else np.zeros_like(rec["delta"])
+        if dla and rsd and any(p["velo"] is None for p in ps):
+            raise ValueError(f"-dla -rsd: a piece of THING_ID {tid} has no VELO_PAR (make_spectra -dla True)")
+        rec["velo"] = np.concatenate([p["velo"] for p in ps])[order] if (dla and rsd) else np.zeros_like(rec["delta"])
         merged.append(rec)
     print(f"{len(pieces)} initial forests, {len(merged)} merged spectra")
     if not merged:
@@ -266,10 +271,11 @@ def run(args):
             f.write(meta, names=META_NAMES, header=hlist, extname="METADATA")
             f.write(np.float32(lam_grid), extname="LAMBDA")
             f.write(np.float32(flux[sel]), extname="FLUX")
-            if store_g:
+            if dla or store_g:
                 f.write(np.float32([merged[j]["delta"] for j in sel]), extname="DELTA_L")
                 f.write(np.float32(growth), extname="GROWTHF")
-                f.write(np.zeros((len(sel), npixeltot), np.float32), extname="VELO_PAR")
+                f.write(np.float32([merged[j]["velo"] for j in sel]), extname="VELO_PAR")
+            if store_g:
                 f.write(np.float32([merged[j]["eta"] for j in sel]), extname="ETA_PAR")
                 f.write(np.float32(zz), extname="Z")
                 f.write(np.float32(ds[sel]), extname="DELTA_S")

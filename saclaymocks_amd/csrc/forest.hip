@@ -37,6 +37,7 @@ struct GatherArgs {
   const float* delta_s;
   float* delta_l;
   float* eta_par;
+  float* v_par;
   float* flux;
   int32_t* cells;
   int32_t* status;
@@ -97,7 +98,8 @@ __device__ __forceinline__ void block_skewers(const smk_skewers& sk, int64_t pix
   __syncthreads();
 }
 
-// NF fields: 1 (delta), 2 (delta, eta_zz) or 7 (delta and the six eta_ij)
+// NF fields: 1 (delta), 2 (delta, eta_zz), 7 (delta and the six eta_ij) or
+// 10 (those and the three velocities)
 template <int NF>
 __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
   const int64_t pix0 = (int64_t)blockIdx.x * blockDim.x;
@@ -146,7 +148,7 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
     if (row_ok) prow = reinterpret_cast<const float4*>(a.abcg)[row];
     if (a.delta_s) ds = a.delta_s[pix];
   }
-  float dl = -1000000.0f, eta = 0.0f;
+  float dl = -1000000.0f, eta = 0.0f, vpar = 0.0f;
   bool outside = false, off_slab = false;
   if (in_forest) {
     const int nx = b.nx, ny = b.ny, nz = b.nz;
@@ -154,8 +156,9 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
     const int64_t x0 = b.nxs > 0 ? b.x0 : 0, nxs = b.nxs > 0 ? b.nxs : nx;
     // fields: 0 = delta, 1..6 = eta_xx..eta_yz (radial) or 1 = eta_zz (plane)
     constexpr int nf = NF;
-    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-    const float* fld[7] = {b.delta, radial ? b.eta[0] : b.eta[2], b.eta[1], b.eta[2], b.eta[3], b.eta[4], b.eta[5]};
+    double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const float* fld[10] = {b.delta, radial ? b.eta[0] : b.eta[2], b.eta[1], b.eta[2], b.eta[3], b.eta[4],
+                            b.eta[5], b.vel[0], b.vel[1], b.vel[2]};
     if (a.interp == SMK_INTERP_GAUSS343) {
       // cell centres (i+0.5)D - L/2 (+R0 along Z), weights exp(-dr^2/(2 D^2)),
       // taken as the product of the three axes' factors (21 exponentials, not 343)
@@ -231,6 +234,7 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
     if (outside || off_slab) {
       dl = __int_as_float(0x7fc00000);
       eta = dl;
+      vpar = dl;
       if (a.status) atomicOr(&a.status[s], (outside ? 1 : 0) | (off_slab ? 4 : 0));
     } else {
       dl = (float)acc[0];
@@ -241,6 +245,8 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
           const double e = (X * acc[1] * X + Y * acc[2] * Y + Z * acc[3] * Z + 2 * X * acc[4] * Y +
                             2 * X * acc[5] * Z + 2 * Y * acc[6] * Z) / RR;
           eta = (float)e;
+          // make_spectra.py:134: v_par = (vx X + vy Y + vz Z) / sqrt(RR)
+          if (nf == 10) vpar = (float)((acc[7] * X + acc[8] * Y + acc[9] * Z) / sqrt(RR));
         } else {
           eta = (float)acc[1];
         }
@@ -249,6 +255,7 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
   }
   if (a.delta_l) a.delta_l[pix] = dl;
   if (a.eta_par) a.eta_par[pix] = eta;
+  if (a.v_par) a.v_par[pix] = vpar;
   if (a.flux) {
     if (!row_ok) {
       a.flux[pix] = __int_as_float(0x7fc00000);
@@ -619,8 +626,8 @@ extern "C" size_t smk_forest_workspace_bytes(int n_skewers) {
 }
 
 extern "C" int smk_forest_gather(const smk_skewers* sk, const smk_box_view* box, int interp, const float* abcg,
-                                 int64_t abcg_rows, const float* delta_s, float* delta_l, float* eta_par, float* flux,
-                                 int32_t* cells, int32_t* status, void* stream) {
+                                 int64_t abcg_rows, const float* delta_s, float* delta_l, float* eta_par,
+                                 float* v_par, float* flux, int32_t* cells, int32_t* status, void* stream) {
   SMK_REQUIRE(sk != nullptr && box != nullptr, SMK_ERR_ARG, "NULL skewers/box");
   SMK_REQUIRE(box->delta != nullptr, SMK_ERR_ARG, "box delta is NULL");
   SMK_REQUIRE(box->nx > 0 && box->ny > 0 && box->nz > 0 && box->cell > 0, SMK_ERR_ARG, "bad box geometry");
@@ -632,6 +639,9 @@ extern "C" int smk_forest_gather(const smk_skewers* sk, const smk_box_view* box,
   SMK_REQUIRE(sk->geometry != SMK_GEOM_RADIAL ||
                   (box->eta[0] == nullptr) == (box->eta[2] == nullptr),
               SMK_ERR_ARG, "radial eta_par needs all six eta boxes");
+  SMK_REQUIRE(v_par == nullptr || (sk->geometry == SMK_GEOM_RADIAL && box->eta[0] != nullptr &&
+                                   box->vel[0] != nullptr && box->vel[1] != nullptr && box->vel[2] != nullptr),
+              SMK_ERR_ARG, "v_par needs radial skewers, the six eta boxes and the three velocity boxes");
   if (sk->n <= 0) return SMK_OK;
   GatherArgs a{};
   a.sk = *sk;
@@ -642,6 +652,7 @@ extern "C" int smk_forest_gather(const smk_skewers* sk, const smk_box_view* box,
   a.delta_s = delta_s;
   a.delta_l = delta_l;
   a.eta_par = eta_par;
+  a.v_par = v_par;
   a.flux = flux;
   a.cells = cells;
   a.status = status;
@@ -656,9 +667,11 @@ extern "C" int smk_forest_gather(const smk_skewers* sk, const smk_box_view* box,
     hipLaunchKernelGGL(k_gather_plane_tri<PPT>, dim3((uint32_t)nblk), dim3(256), 0, (hipStream_t)stream, a);
   } else {
     const uint64_t nblk = (uint64_t)(a.npix_total + 255) / 256;
-    const int nf = box->eta[2] == nullptr ? 1 : (sk->geometry == SMK_GEOM_RADIAL ? 7 : 2);
+    const bool radial = sk->geometry == SMK_GEOM_RADIAL;
+    const int nf = box->eta[2] == nullptr ? 1 : (radial ? (v_par ? 10 : 7) : 2);
     if (nf == 1) hipLaunchKernelGGL(k_gather<1>, dim3((uint32_t)nblk), dim3(256), 0, (hipStream_t)stream, a);
     else if (nf == 2) hipLaunchKernelGGL(k_gather<2>, dim3((uint32_t)nblk), dim3(256), 0, (hipStream_t)stream, a);
+    else if (nf == 10) hipLaunchKernelGGL(k_gather<10>, dim3((uint32_t)nblk), dim3(256), 0, (hipStream_t)stream, a);
     else hipLaunchKernelGGL(k_gather<7>, dim3((uint32_t)nblk), dim3(256), 0, (hipStream_t)stream, a);
   }
   SMK_LAUNCH_CHECK();

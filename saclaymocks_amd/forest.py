@@ -26,7 +26,7 @@ class SmkBoxView(ctypes.Structure):
     _fields_ = [("delta", ctypes.c_void_p), ("eta", ctypes.c_void_p * 6),
                 ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
                 ("periodic", ctypes.c_int32), ("cell", ctypes.c_double), ("r0", ctypes.c_double),
-                ("x0", ctypes.c_int32), ("nxs", ctypes.c_int32)]
+                ("x0", ctypes.c_int32), ("nxs", ctypes.c_int32), ("vel", ctypes.c_void_p * 3)]
 
 
 class SmkSkewers(ctypes.Structure):
@@ -49,9 +49,10 @@ def _t(a, dtype, dev):
 class BoxView:
     """Resident boxes (torch float32 CUDA tensors [nx][ny][nz]) + box frame."""
 
-    def __init__(self, delta, eta=None, cell=2.19, r0=0.0, periodic=True, x0=0, nx=None):
-        """delta/eta: [nxs][ny][nz] tensors holding x planes [x0, x0 + nxs) of an
-        nx-plane box (default: the whole box, nx = nxs, x0 = 0)."""
+    def __init__(self, delta, eta=None, cell=2.19, r0=0.0, periodic=True, x0=0, nx=None, vel=None):
+        """delta/eta/vel: [nxs][ny][nz] tensors holding x planes [x0, x0 + nxs) of
+        an nx-plane box (default: the whole box, nx = nxs, x0 = 0); vel = (vx,
+        vy, vz) for the v_par skewers."""
         self.delta = delta
         self.x0 = int(x0)
         self.nx_box = int(nx) if nx is not None else int(delta.shape[0])
@@ -60,8 +61,11 @@ class BoxView:
         self.eta = list(eta) if eta is not None else [None] * 6
         if len(self.eta) != 6:
             raise ValueError("eta must list 6 boxes (xx, yy, zz, xy, xz, yz); None where unused")
+        self.vel = list(vel) if vel is not None else [None] * 3
+        if len(self.vel) != 3:
+            raise ValueError("vel must list 3 boxes (vx, vy, vz)")
         self.cell, self.r0, self.periodic = float(cell), float(r0), bool(periodic)
-        for t in [delta] + [e for e in self.eta if e is not None]:
+        for t in [delta] + [e for e in self.eta + self.vel if e is not None]:
             if not (t.is_cuda and t.dtype == torch.float32 and t.is_contiguous() and t.shape == delta.shape):
                 raise ValueError("box fields must be contiguous float32 CUDA tensors of one shape")
 
@@ -75,6 +79,8 @@ class BoxView:
         v.delta = self.delta.data_ptr()
         for i, e in enumerate(self.eta):
             v.eta[i] = e.data_ptr() if e is not None else None
+        for i, e in enumerate(self.vel):
+            v.vel[i] = e.data_ptr() if e is not None else None
         v.nx, v.ny, v.nz = self.nx_box, ny, nz
         whole = self.x0 == 0 and nxs == self.nx_box
         v.x0, v.nxs = (0, 0) if whole else (self.x0, nxs)
@@ -233,7 +239,7 @@ def gather(batch, box, interp="gauss343", abcg=None, delta_s=None, want=("delta_
     dev = _dev()
     n = batch.npix_total
     out = {}
-    for k in ("delta_l", "eta_par", "flux"):
+    for k in ("delta_l", "eta_par", "v_par", "flux"):
         if k in want and (k != "flux" or abcg is not None):
             out[k] = torch.empty(n, dtype=torch.float32, device=dev)
     if cells:
@@ -244,7 +250,7 @@ def gather(batch, box, interp="gauss343", abcg=None, delta_s=None, want=("delta_
 
 
 def gather_into(batch, box, out, interp="gauss343", abcg=None, delta_s=None, stream=None):
-    """gather() into caller-owned buffers: out maps "delta_l", "eta_par",
+    """gather() into caller-owned buffers: out maps "delta_l", "eta_par", "v_par",
     "flux", "cells" (any subset) and "status" (int32 per skewer, zeroed by the
     caller) to CUDA tensors.  Allocates nothing."""
     sk = batch.struct()
@@ -252,7 +258,8 @@ def gather_into(batch, box, out, interp="gauss343", abcg=None, delta_s=None, str
     rows = int(abcg.shape[0]) if abcg is not None else 0
     rc = _lib.lib().smk_forest_gather(ctypes.byref(sk), ctypes.byref(bv), INTERP[interp], _lib.ptr(abcg),
                                       rows, _lib.ptr(delta_s), _lib.ptr(out.get("delta_l")),
-                                      _lib.ptr(out.get("eta_par")), _lib.ptr(out.get("flux")),
+                                      _lib.ptr(out.get("eta_par")), _lib.ptr(out.get("v_par")),
+                                      _lib.ptr(out.get("flux")),
                                       _lib.ptr(out.get("cells")), _lib.ptr(out["status"]), _lib.stream_ptr(stream))
     _lib.check(rc, "smk_forest_gather")
 

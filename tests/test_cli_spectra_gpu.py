@@ -55,6 +55,13 @@ def _read_box(d, name):
     return np.concatenate([np.asarray(p, dtype=np.float32).reshape(-1, NY, NZ) for p in planes])
 
 
+def _read_files(d, name):
+    """All <name>-<i>.fits of a box, whatever the planes per file."""
+    from saclaymocks_amd import fitsio_lite
+    paths = sorted(glob.glob(os.path.join(d, f"{name}-*.fits")), key=lambda p: int(p.rsplit("-", 1)[1][:-5]))
+    return np.concatenate([np.asarray(fitsio_lite.read(p, 0), dtype=np.float32).reshape(-1, NY, NZ) for p in paths])
+
+
 def _p1d_file(path):
     """z-dependent P1D_miss file: extensions z, k, pk [nz][nk], sigma."""
     from saclaymocks_amd import fitsio_lite
@@ -157,3 +164,58 @@ def test_spectra_pipeline_matches_oracle(cuda, tmp_path):
                 assert np.abs(got - flux).max() < 2e-5, (tid, np.abs(got - flux).max())
                 assert np.all(got[~forest] == 1.0)
     assert seen == len(rows)
+
+
+def test_velocity_skewers_match_oracle(cuda, tmp_path):
+    """make_spectra -rsd True -dla True writes VELO_PAR, the Gaussian-smoothed
+    line-of-sight velocity times (1 + z) h_of_z(z)/h_of_z(0) D'(z)/D'(0), and
+    merge_spectra -dla True carries it, with DELTA_L and GROWTHF, into the
+    merged files."""
+    import json
+    from saclaymocks_amd import fitsio_lite
+    from saclaymocks_amd.bin import make_boxes, make_spectra, merge_spectra
+    boxdir, qdir = str(tmp_path / "boxes"), str(tmp_path / "qso")
+    sdir, mdir = str(tmp_path / "spectra"), str(tmp_path / "merged")
+    print("[velocity make_boxes]", flush=True)
+    assert make_boxes.main(["-NX", str(NX), "-NY", str(NY), "-NZ", str(NZ), "-seed", "4", "-outDir", boxdir]) == 0
+    cos = fo.Cosmo()
+    R0 = fo.H * cos.r_comoving(fo.Z0)
+    rows = _qsos(R0)
+    _write_qsos(qdir, rows)
+    print("[velocity make_spectra -dla]", flush=True)
+    for i in range(NSLICE):
+        assert make_spectra.main(["-QSOfile", os.path.join(qdir, "QSO-"), "-boxdir", boxdir, "-outDir", sdir,
+                                  "-i", str(i), "-N", str(NSLICE), "-zmin", str(ZMIN), "-zmax", str(ZMAX),
+                                  "-dla", "True"]) == 0
+    print("[velocity merge_spectra -dla]", flush=True)
+    assert merge_spectra.main(["-inDir", sdir, "-outDir", mdir, "-i", "0", "-seed", str(SEED), "-dla", "True"]) == 0
+    vel = [_read_files(boxdir, k) for k in ("vx", "vy", "vz")]
+    L, D = (CELL * NX, CELL * NY, CELL * NZ), (CELL,) * 3
+    Rmin, Rmax = fo.H * cos.r_comoving(ZMIN), fo.H * cos.r_comoving(ZMAX)
+    R_all = Rmin + np.arange(int((Rmax - Rmin) / PIXEL + 0.5)) * PIXEL
+    lam_all = fo.LYA * (1 + cos.r_2_z(R_all / fo.H))
+    keep = lam_all > 3476.0
+    R_vec = R_all[keep]
+    zpix = cos.r_2_z(R_vec / fo.H)
+    with open(os.path.join(os.path.dirname(fo.__file__), "..", "saclaymocks_amd", "data", "dgrowth_z.json")) as f:
+        d = json.load(f)
+    dg = lambda z: np.interp(z, np.asarray(d["z"]), np.asarray(d["dD/dz"]))
+    scale = fo.velo_par_scale(zpix, cos.dist_hubble, dg)
+    seen = 0
+    for path in sorted(glob.glob(os.path.join(mdir, "spectra_merged-*-0.fits.gz"))):
+        hd = {h.name: h for h in fitsio_lite.read_all(path)}
+        assert {"DELTA_L", "GROWTHF", "VELO_PAR"} <= set(hd)
+        meta = hd["METADATA"].data
+        for q, tid in enumerate(meta["THING_ID"]):
+            r = next(r for r in rows if r[4] == tid)
+            R_Q = fo.H * cos.r_comoving(r[2])
+            X, Y, Z = fo.compute_xyz2(np.radians(r[0]), np.radians(r[1]), R_Q, 0.0, 0.0)
+            x, y, z = R_vec * X / R_Q, R_vec * Y / R_Q, R_vec * Z / R_Q
+            vals, _ = fo.gauss343(vel, x, y, z, L, D, R0)
+            forest = lam_all[keep] < fo.LYA * (1 + r[2])
+            want = np.where(forest, fo.v_par_radial(vals, x, y, z), 0.0) * scale
+            got = hd["VELO_PAR"].data[q]
+            assert np.abs(got - want).max() < 2e-6 * max(1.0, np.abs(want).max()), tid
+            assert np.any(got != 0) == bool(np.any(forest))     # one QSO has no forest pixel
+            seen += 1
+    assert seen == sum(1 for r in rows if r[5] == 0)

@@ -364,3 +364,54 @@ def test_trilinear_respects_a_strict_box_and_a_slab(cuda):
     assert out["status"].cpu().tolist() == [0, 4]
     full = forest.gather(b, periodic, interp="trilinear", want=("delta_l",))
     assert torch.equal(out["delta_l"][:50], full["delta_l"][:50])
+
+
+def test_radial_v_par_parity(cuda):
+    """v_par of radial skewers (make_spectra.py:130-134) against the oracle, and
+    the argument checks of the velocity skewers."""
+    from saclaymocks_amd import forest, util
+    print("[forest radial v_par]", flush=True)
+    shape, cell = (64, 64, 64), 20.0
+    cos = util.Cosmo()
+    R0 = util.H * cos.r_comoving(util.Z0)
+    L = [cell * s for s in shape]
+    zmax = float(cos.r_2_z((R0 + L[2] / 2 - 4 * cell) / util.H))
+    zmin = float(cos.r_2_z((R0 - L[2] / 2 + 4 * cell) / util.H))
+    rng = np.random.default_rng(8)
+    nq = 12
+    ang = np.degrees(0.25 * (L[0] / 2) / R0)
+    ra, dec = rng.uniform(-ang, ang, nq), rng.uniform(-ang, ang, nq)
+    zq = rng.uniform(0.5 * (zmin + zmax), zmax, nq)
+    rows, grid, kept = forest.radial_skewers(ra, dec, zq, 0.0, 0.0, shape, cell, zmin=zmin, zmax=zmax, pixel=0.2)
+    fields = [_field(shape, 30 + i) for i in range(10)]
+    tens = [torch.from_numpy(f).to(cuda) for f in fields]
+    box = forest.BoxView(tens[0], tens[1:7], cell=cell, r0=R0, periodic=False, vel=tens[7:])
+    b = forest.SkewerBatch(forest.GEOM_RADIAL, rows["g0"], rows["npix"], grid["rmin"], grid["dr"], rows["qx"],
+                           rows["qy"], rows["qz"], rows["qr"], imin=rows["imin"], imax=rows["imax"])
+    for interp in ("gauss343", "trilinear"):
+        out = forest.gather(b, box, interp=interp, want=("delta_l", "eta_par", "v_par"))
+        X, Y, Z, msk = [], [], [], []
+        for s in range(b.n):
+            R = grid["rmin"] + (rows["g0"][s] + np.arange(rows["npix"][s])) * grid["dr"]
+            X.append(R * rows["qx"][s] / rows["qr"][s])
+            Y.append(R * rows["qy"][s] / rows["qr"][s])
+            Z.append(R * rows["qz"][s] / rows["qr"][s])
+            i = np.arange(rows["npix"][s])
+            msk.append((i >= rows["imin"][s]) & (i < rows["imax"][s]))
+        X, Y, Z, msk = map(np.concatenate, (X, Y, Z, msk))
+        if interp == "gauss343":
+            vals, _ = fo.gauss343(fields[7:], X[msk], Y[msk], Z[msk], L, (cell,) * 3, R0)
+        else:
+            vals = fo.trilinear(fields[7:], X[msk], Y[msk], Z[msk], L, (cell,) * 3, R0)
+        want = fo.v_par_radial(vals, X[msk], Y[msk], Z[msk])
+        got = out["v_par"].cpu().numpy()
+        assert np.abs(got[msk] - want).max() < 2e-6, interp
+        assert np.all(got[~msk] == 0)
+        assert out["status"].sum().item() == 0
+    # v_par needs radial skewers with the eta and velocity boxes
+    no_vel = forest.BoxView(tens[0], tens[1:7], cell=cell, r0=R0, periodic=False)
+    with pytest.raises(ValueError):
+        forest.gather(b, no_vel, interp="gauss343", want=("v_par",))
+    plane = forest.SkewerBatch(forest.GEOM_PLANE, [40], [50], -L[2] / 2 + R0 + 0.05, 0.2, [0.0], [0.0])
+    with pytest.raises(ValueError):
+        forest.gather(plane, box, interp="gauss343", want=("v_par",))

@@ -71,6 +71,34 @@ def test_gauss343_constant_and_bruteforce():
         assert abs(v[p] - num / den) < 1e-12
 
 
+def test_v_par_of_constant_velocity_fields():
+    """Constant velocity boxes smooth to themselves, so v_par is v . x_hat."""
+    shape = (12, 10, 14)
+    D, R0 = 2.0, 100.0
+    L = tuple(D * n for n in shape)
+    v = (0.5, -1.25, 2.0)
+    fields = [np.full(shape, c, np.float32) for c in v]
+    X = np.array([0.3, -1.1, 0.0]); Y = np.array([0.2, 1.7, 0.0]); Z = np.array([R0 + 0.4, R0 - 2.2, R0])
+    vals, _ = fo.gauss343(fields, X, Y, Z, L, (D, D, D), R0)
+    got = fo.v_par_radial(vals, X, Y, Z)
+    R = np.sqrt(X ** 2 + Y ** 2 + Z ** 2)
+    assert np.allclose(got, (v[0] * X + v[1] * Y + v[2] * Z) / R, rtol=0, atol=1e-13)
+    assert got[2] == pytest.approx(2.0)          # on the z axis v_par is vz
+
+
+def test_velo_par_scale_by_hand():
+    from saclaymocks_amd import util
+    cos = fo.Cosmo()
+    dg = lambda z: 1.0 / (1.0 + np.asarray(z)) ** 2        # any dD/dz
+    assert fo.velo_par_scale(0.0, cos.dist_hubble, dg) == pytest.approx(1.0)
+    z = 2.3
+    # c / H(z) with H(z) = H0 sqrt(Om (1+z)^3 + 1 - Om) (flat, no radiation)
+    hz = lambda zz: np.sqrt(fo.OMEGA_M * (1 + zz) ** 3 + 1 - fo.OMEGA_M)
+    want = (1 + z) * (hz(0.0) / hz(z)) * dg(z) / dg(0.0)
+    assert fo.velo_par_scale(z, cos.dist_hubble, dg) == pytest.approx(want, rel=1e-5)
+    assert fo.velo_par_scale(z, util.Cosmo().dist_hubble, dg) == pytest.approx(want, rel=1e-5)
+
+
 def test_cell_index_known_point():
     D, R0 = 2.19, 3000.0
     shape = (64, 64, 64)
