@@ -233,6 +233,40 @@ def p1d_bench(steps, warmup):
             "limiter": "FP64 VALU: per point a square root, a division, a spline segment lookup and, for D_NL, a log and two exponentials"}
 
 
+def cpu_baseline_forest(cell, n_skewers=5000):
+    """The oracle's numpy forest step (trilinear gather of delta and eta_zz,
+    missing power by rfft / irfft per skewer, FGPA) on n_skewers of the bench's
+    skewer distribution over a 128 x 128 x 1536 host box, one core."""
+    from oracle import forest_oracle as fo
+    from saclaymocks_amd import util, workloads
+    shape = (128, 128, 1536)
+    rng = np.random.default_rng(5)
+    d = rng.standard_normal(shape, dtype=np.float32)
+    e = rng.standard_normal(shape, dtype=np.float32)
+    zq = np.clip(rng.normal(2.4, 0.4, n_skewers), 1.8, 3.6)
+    lengths = rng.integers(300, 1501, n_skewers)
+    h = workloads.quasar_skewers(shape, cell, n_skewers, zq, lengths, 8, 0.2)
+    L = tuple(cell * n for n in shape)
+    npix = np.asarray(h["npix"])
+    X = np.repeat(np.asarray(h["x"], dtype=np.float64), npix)
+    Y = np.repeat(np.asarray(h["y"], dtype=np.float64), npix)
+    g = np.concatenate([g0 + np.arange(k) for g0, k in zip(h["g0"], npix)])
+    Z = h["fr"]["rmin"] + g * 0.2
+    G = util.growthf(np.repeat(zq, npix))
+    t0 = time.perf_counter()
+    dl, eta = fo.trilinear([d, e], X, Y, Z, L, (cell,) * 3, h["fr"]["R0"])
+    off = np.concatenate([[0], np.cumsum(npix)])
+    ds = np.empty(len(X))
+    for s, k in enumerate(npix):
+        nz = fo.fft_len_for(int(k))
+        ds[off[s]:off[s + 1]] = fo.missing_power(rng.standard_normal(nz), int(k), 0.2, workloads.p1d_missing)
+    flux = fo.fgpa(dl + ds, eta, G, 0.1, 1.5, 1.0)
+    dt = time.perf_counter() - t0
+    assert np.all(np.isfinite(flux))
+    return {"value": len(X) / dt, "unit": "pixels/s", "cores": 1, "kind": "port",
+            "sample": f"{n_skewers} skewers, {len(X)} pixels (numpy trilinear, rfft missing power, FGPA; {dt:.1f} s)"}
+
+
 def cpu_baseline_p1d(rows=100):
     """The oracle's numpy rows (the reference's formulation of the same
     integrals) on `rows` k_par rows of each model, one core."""
@@ -445,6 +479,8 @@ def main():
         out["cpu_baseline"] = cpu_baseline(args.cell)
         if "p1d_missing" in out:
             out["p1d_missing"]["cpu_baseline"] = cpu_baseline_p1d()
+        if "forest" in out:
+            out["forest"]["cpu_baseline"] = cpu_baseline_forest(args.cell)
     print(json.dumps(out), flush=True)
 
 
