@@ -267,6 +267,26 @@ def cpu_baseline_forest(cell, n_skewers=5000):
             "sample": f"{n_skewers} skewers, {len(X)} pixels (numpy trilinear, rfft missing power, FGPA; {dt:.1f} s)"}
 
 
+def cpu_baseline_qso(cell, planes=2):
+    """The oracle's numpy draw (draw_qso.py's per-cell conditions, Philox
+    uniforms, positions and RSD redshifts) over `planes` x planes of the
+    2560 x 2560 x 1536 box at its centre, N(0, 1) fields, one core."""
+    from oracle import qso_oracle as qo
+    from saclaymocks_amd import qso
+    nx, ny, nz = 2560, 2560, 1536
+    nslice = nx // planes
+    setup = qso.DrawSetup(planes, ny, nz, nx, cell, cell, cell, nslice // 2, nslice, sigma=(1.0, 1.0, 1.0),
+                          dra=20.0, ddec=20.0)
+    rng = np.random.default_rng(6)
+    f = [rng.standard_normal((planes, ny, nz), dtype=np.float32) for _ in range(3)]
+    t0 = time.perf_counter()
+    out = qo.draw(f[0], f[1], f[2], (f[1], f[1], f[1]), setup.params(), 11)
+    dt = time.perf_counter() - t0
+    cells = planes * ny * nz
+    return {"value": cells / dt, "unit": "cells/s", "cores": 1, "kind": "port",
+            "sample": f"{planes} x planes of {ny}x{nz} cells, {len(out['key'])} QSOs ({dt:.1f} s)"}
+
+
 def cpu_baseline_p1d(rows=100):
     """The oracle's numpy rows (the reference's formulation of the same
     integrals) on `rows` k_par rows of each model, one core."""
@@ -481,6 +501,8 @@ def main():
             out["p1d_missing"]["cpu_baseline"] = cpu_baseline_p1d()
         if "forest" in out:
             out["forest"]["cpu_baseline"] = cpu_baseline_forest(args.cell)
+        if "qso" in out:
+            out["qso"]["cpu_baseline"] = cpu_baseline_qso(args.cell)
     print(json.dumps(out), flush=True)
 
 
