@@ -69,6 +69,20 @@ def measured_traffic(pass_name, vid, shape):
                                                      "source": os.path.relpath(TRAFFIC_PROFILE, ROOT)}
 
 
+def cpu_model():
+    """The host CPU model (lscpu's "Model name") and the cores this process may use."""
+    name = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": name, "affinity": len(os.sched_getaffinity(0))}
+
+
 def cpu_baseline(cell, seconds=12.0):
     """FP64 oracle of the same job on a 256^3 sample, timed on this host."""
     import numpy as np
@@ -88,7 +102,7 @@ def cpu_baseline(cell, seconds=12.0):
     cells = n * shape[0] * shape[1] * shape[2]
     return {"value": cells / dt, "unit": "cells/s", "cores": cores, "kind": "port",
             "sample": f"{n} x 256^3 box: numpy N(0,1) noise, scipy.fft FP64 rfftn, "
-                      f"x sqrt(P/Vcell), irfftn delta + eta_zz ({dt:.1f} s)"}
+                      f"x sqrt(P/Vcell), irfftn delta + eta_zz ({dt:.1f} s)", "host": cpu_model()}
 
 
 def forest_bench(delta, eta, shape, cell, steps, warmup, n_skewers):
@@ -479,6 +493,8 @@ def main():
                      "kernel": dom, "kernel_ms": dom_ms,
                      "alg_bytes_per_cell": PASS_BYTES_PER_CELL[dom], "traffic_pmc": traffic_info},
         "job": {"model_bytes_per_cell": MODEL_BYTES_PER_CELL,
+                "compulsory_bytes_per_cell": 8,
+                "compulsory_frac": 8 * cells / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBPS,
                 "model_GBps": MODEL_BYTES_PER_CELL * cells / (ms_per_step * 1e-3) / 1e9,
                 "model_frac": MODEL_BYTES_PER_CELL * cells / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBPS,
                 "schedule": sched, "timed_calls": ncalls, "sigma_delta_fp64": sigma,
