@@ -247,7 +247,7 @@ def p1d_bench(steps, warmup):
             "limiter": "FP64 VALU: per point a square root, a division, a spline segment lookup and, for D_NL, a log and two exponentials"}
 
 
-def cpu_baseline_forest(cell, n_skewers=5000):
+def cpu_baseline_forest(cell, n_skewers=20000):
     """The oracle's numpy forest step (trilinear gather of delta and eta_zz,
     missing power by rfft / irfft per skewer, FGPA) on n_skewers of the bench's
     skewer distribution over a 128 x 128 x 1536 host box, one core."""
@@ -281,7 +281,7 @@ def cpu_baseline_forest(cell, n_skewers=5000):
             "sample": f"{n_skewers} skewers, {len(X)} pixels (numpy trilinear, rfft missing power, FGPA; {dt:.1f} s)"}
 
 
-def cpu_baseline_qso(cell, planes=2):
+def cpu_baseline_qso(cell, planes=8):
     """The oracle's numpy draw (draw_qso.py's per-cell conditions, Philox
     uniforms, positions and RSD redshifts) over `planes` x planes of the
     2560 x 2560 x 1536 box at its centre, N(0, 1) fields, one core."""
