@@ -125,11 +125,13 @@ __global__ __launch_bounds__(256) void k_qso_draw(smk_qso_params p, const float*
   } else {
     // exp(p)**a_k(z), float32 as the reference's in-place float32 arrays
     // (:219-223, :236-238), evaluated as exp(a_k p) with a_k = ((1+z)/(1+z_k))^0.7
-    // from one power per cell (the reference: a float32 exp, then a float32 power)
-    const float t = powf((float)(1.0 + zc), 0.7f);
-    const float e1 = expf(t * INV1 * p1[c]);
-    const float e2 = expf(t * INV2 * p2[c]);
-    const float e3 = expf(t * INV3 * p3[c]);
+    // from one power per cell (the reference: a float32 exp, then a float32
+    // power), on the hardware exp2 / log2 (a few 1e-7 relative: ptot only
+    // decides the draw against a uniform)
+    const float t = __builtin_amdgcn_exp2f(0.7f * __builtin_amdgcn_logf((float)(1.0 + zc)));
+    const float e1 = __expf(t * INV1 * p1[c]);
+    const float e2 = __expf(t * INV2 * p2[c]);
+    const float e3 = __expf(t * INV3 * p3[c]);
     const double ptot = mix(zc, coef_of_z(p, zc), (double)e1, (double)e2, (double)e3);
     if (!(rnd1 < p.norm * ptot)) return;
   }
