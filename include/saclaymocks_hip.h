@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define SMK_ABI_VERSION 3
+#define SMK_ABI_VERSION 4
 
 #define SMK_OK 0
 #define SMK_ERR_ARG (-1)         /* bad argument (null pointer, size, enum) */
@@ -225,7 +225,17 @@ typedef struct smk_skewers {
   const int64_t* off;
   const int32_t* pset;  /* FGPA parameter row per skewer; NULL: row = global index g */
   const uint64_t* noise_id; /* missing-power noise stream per skewer (e.g. THING_ID) */
+  /* NULL, or ceil(npix_total / SMK_GATHER_BLOCK_PIXELS) + 1 entries: entry b
+   * is the skewer of pixel min(b * SMK_GATHER_BLOCK_PIXELS, npix_total - 1),
+   * the last s with off[s] <= that pixel.  smk_forest_gather then reads each
+   * block's skewer range from it instead of searching off[] on the device
+   * (entries are clamped to [0, n - 1]; a wrong table gives wrong pixels,
+   * never an access outside the arrays). */
+  const int32_t* blk_s0;
 } smk_skewers;
+
+/* Pixels per block of smk_forest_gather (the blk_s0 table's stride). */
+#define SMK_GATHER_BLOCK_PIXELS 256
 
 /* Gather along each line of sight (make_spectra.py:90-139) and, when `flux`
  * is non-NULL, the FGPA (merge_spectra.py:335-339, util.py:421-433):

@@ -81,8 +81,18 @@ __device__ __forceinline__ int wrap(int64_t i, int n) {
 }
 
 // The block's skewer range [s_rng[0], s_rng[1]] for pixels [pix0, plast]
-// (pix0 < npix_total), searched by the first wave; every thread must call it.
+// (pix0 < npix_total): from the batch's per-block table when it has one and
+// the block is one table stride, else searched by the first wave; every
+// thread must call it.
 __device__ __forceinline__ void block_skewers(const smk_skewers& sk, int64_t pix0, int64_t plast, int* s_rng) {
+  if (sk.blk_s0 != nullptr && blockDim.x == SMK_GATHER_BLOCK_PIXELS && pix0 == (int64_t)blockIdx.x * blockDim.x) {
+    if (threadIdx.x < 2) {
+      const int v = sk.blk_s0[blockIdx.x + threadIdx.x];
+      s_rng[threadIdx.x] = v < 0 ? 0 : (v >= sk.n ? sk.n - 1 : v);
+    }
+    __syncthreads();
+    return;
+  }
   if (threadIdx.x < 64) {
     const int lane = threadIdx.x;
     const int s0 = wave_find_skewer(sk.off, 0, sk.n - 1, pix0, lane);

@@ -35,7 +35,9 @@ class SmkSkewers(ctypes.Structure):
                 ("qx", ctypes.c_void_p), ("qy", ctypes.c_void_p), ("qz", ctypes.c_void_p),
                 ("qr", ctypes.c_void_p), ("g0", ctypes.c_void_p), ("npix", ctypes.c_void_p),
                 ("imin", ctypes.c_void_p), ("imax", ctypes.c_void_p), ("off", ctypes.c_void_p),
-                ("pset", ctypes.c_void_p), ("noise_id", ctypes.c_void_p)]
+                ("pset", ctypes.c_void_p), ("noise_id", ctypes.c_void_p), ("blk_s0", ctypes.c_void_p)]
+
+GATHER_BLOCK_PIXELS = 256      # SMK_GATHER_BLOCK_PIXELS
 
 
 def _dev():
@@ -120,6 +122,14 @@ class SkewerBatch:
         nid = np.arange(n, dtype=np.uint64) if noise_id is None else np.asarray(noise_id, dtype=np.uint64)
         self.noise_id = torch.as_tensor(nid.view(np.int64)).to(dev)
         self.g0_host = np.asarray(g0, dtype=np.int64)
+        # the gather's per-block skewer table: the skewer of every
+        # GATHER_BLOCK_PIXELS-th pixel (the last s with off[s] <= pixel)
+        if self.npix_total:
+            nblk = -(-self.npix_total // GATHER_BLOCK_PIXELS)
+            first = np.minimum(np.arange(nblk + 1, dtype=np.int64) * GATHER_BLOCK_PIXELS, self.npix_total - 1)
+            self.blk_s0 = _t(np.searchsorted(off, first, side="right") - 1, torch.int32, dev)
+        else:
+            self.blk_s0 = None
 
     def struct(self):
         s = SmkSkewers()
@@ -130,6 +140,7 @@ class SkewerBatch:
                                                                        self.off))
         s.pset = self.pset.data_ptr() if self.pset is not None else None
         s.noise_id = self.noise_id.data_ptr()
+        s.blk_s0 = self.blk_s0.data_ptr() if self.blk_s0 is not None else None
         return s
 
     def split(self, x):

@@ -25,7 +25,7 @@ def test_python_signatures_cover_header():
 
 def test_version_and_lengths():
     L = _lib.lib()
-    assert L.smk_abi_version() == 3
+    assert L.smk_abi_version() == 4
     for n in (16, 24, 40, 64, 128, 256, 768, 1536, 2560):
         assert L.smk_fft_length_supported(n) == 1
     for n in (7, 11, 4096, 3):

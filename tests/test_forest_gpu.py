@@ -103,6 +103,12 @@ def test_plane_gather_many_short_and_empty_skewers(cuda, interp):
     assert np.abs(out["delta_l"].cpu().numpy() - rd).max() < 2e-6
     assert np.abs(out["eta_par"].cpu().numpy() - re).max() < 2e-6
     assert out["status"].sum().item() == 0
+    # the per-block skewer table and the device search give the same pixels
+    assert b.blk_s0 is not None
+    b.blk_s0 = None
+    out2 = forest.gather(b, box, interp=interp, want=("delta_l", "eta_par"), cells=True)
+    for k in ("delta_l", "eta_par", "cells"):
+        assert torch.equal(out[k], out2[k]), k
 
 
 def test_radial_gather_parity_reference_geometry(cuda):
