@@ -377,47 +377,79 @@ __global__ __launch_bounds__(T*(M / E)) void k_z_r2c(const float* __restrict__ n
   }
 }
 
-// z pencils, C2R (unnormalised, times `scale`).  Z'_k = (X_k + conj X_{M-k})
-// + i W_{2M}^{-k} (X_k - conj X_{M-k}); the inverse M-point FFT of Z' is
-// x_{2n} + i x_{2n+1}.  Imaginary parts of X_0 and X_M are ignored, as a C2R
-// transform does.
-template <int M, int E, int T>
+// Z'_k = (X_k + conj X_{M-k}) + i W_{2M}^{-k} (X_k - conj X_{M-k}) of the
+// C2R below for point k, with xc = X_{M-k} (X_M at k = 0); the imaginary
+// parts of X_0 and X_M are ignored, as a C2R transform does.
+__device__ __forceinline__ float2 c2r_fold(float2 xk, float2 xc, int k, const float2* __restrict__ twr1) {
+  if (k == 0) { xk.y = 0.0f; xc.y = 0.0f; }
+  const float2 sm = cadd(xk, cconj(xc));
+  const float2 d = csub(xk, cconj(xc));
+  const float2 wd = cmul(cconj(twr1[k]), d);
+  return make_float2(sm.x - wd.y, sm.y + wd.x);  // sm + i*wd
+}
+
+// z pencils, C2R (unnormalised, times `scale`): the inverse M-point FFT of
+// Z' (c2r_fold) is x_{2n} + i x_{2n+1}.  XCH 0 exchanges float2 values
+// through LDS; XCH 1 the real, then the imaginary parts through a float line
+// (half the LDS per pencil, so up to twice the resident waves, for twice the
+// barriers).
+template <int M, int E, int T, int XCH>
 __global__ __launch_bounds__(T*(M / E)) void k_z_c2r(const float2* __restrict__ K, float* __restrict__ out,
                                                      const float2* __restrict__ tw, const float2* __restrict__ twr,
                                                      uint64_t n_pencils, float scale, int* __restrict__ status) {
   constexpr int P = M / E;
   constexpr int LL = lds_line_len(M);
-  __shared__ float2 lds[T * LL];
   __shared__ float2 xnyq[T];
   const int tid = threadIdx.x;
   const int p = tid % P, t = tid / P;
   const uint64_t L = (uint64_t)blockIdx.x * T + t;
   const bool valid = L < n_pencils;
-  float2* line = lds + t * LL;
   const float2* src = K + L * (uint64_t)(M + 1);
   float2 v[E];
   // one coalesced load per point; the mirror point X_{M-k} comes from LDS
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = valid ? src[p + P * e] : make_float2(0.f, 0.f);
   if (p == 0) xnyq[t] = valid ? src[M] : make_float2(0.f, 0.f);
-  lds_scatter<M, P, E>(v, line, p);
-  __syncthreads();
   const int z1 = opaque_zero();
   const int p1 = p + z1;
   const float2* twr1 = twr + z1;
+  if constexpr (XCH == 0) {
+    __shared__ float2 lds[T * LL];
+    float2* line = lds + t * LL;
+    lds_scatter<M, P, E>(v, line, p);
+    __syncthreads();
 #pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int k = p1 + P * e;
-    float2 xk = v[e];
-    float2 xc = (k == 0) ? xnyq[t] : line[lds_pos<M>(M - k)];
-    if (k == 0) { xk.y = 0.0f; xc.y = 0.0f; }
-    const float2 sm = cadd(xk, cconj(xc));
-    const float2 d = csub(xk, cconj(xc));
-    const float2 wd = cmul(cconj(twr1[k]), d);
-    v[e] = make_float2(sm.x - wd.y, sm.y + wd.x);  // sm + i*wd
+    for (int e = 0; e < E; ++e) {
+      const int k = p1 + P * e;
+      v[e] = c2r_fold(v[e], (k == 0) ? xnyq[t] : line[lds_pos<M>(M - k)], k, twr1);
+    }
+    __syncthreads();
+    fft_line_e<M, E, true>(v, line, p, tw);
+  } else {
+    __shared__ float ldf[T * LL];
+    float* lf = ldf + t * LL;
+    float xr[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) lf[lds_pos<M>(p + P * e)] = v[e].x;
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = p1 + P * e;
+      xr[e] = (k == 0) ? xnyq[t].x : lf[lds_pos<M>(M - k)];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) lf[lds_pos<M>(p + P * e)] = v[e].y;
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = p1 + P * e;
+      const float xi = (k == 0) ? xnyq[t].y : lf[lds_pos<M>(M - k)];
+      v[e] = c2r_fold(v[e], make_float2(xr[e], xi), k, twr1);
+    }
+    __syncthreads();
+    fft_line_split<M, E, true>(v, lf, p, tw);
   }
-  __syncthreads();
-  fft_line_e<M, E, true>(v, line, p, tw);
   int flags = 0;
   if (valid) {
     float2* dst = reinterpret_cast<float2*>(out + L * (uint64_t)(2 * M));
@@ -601,12 +633,17 @@ uint64_t strided_blocks_max(int n, int mode, uint64_t n_outer, uint32_t n_inner)
 }
 
 // Pencil-pass variants: 0 = the default register tile, 1 = half of it
-// (twice the threads per pencil; m >= 256 only).
-constexpr int SMK_NPVAR = 2;
+// (twice the threads per pencil; m >= 256 only), 2 = the default tile with
+// the split LDS exchange (C2R only, m >= 256; the half tile with it measured
+// slower at m = 768: 18.2 ms against 17.7 for variant 0 and 17.0 for 2).
+constexpr int SMK_NPVAR = 3;
 constexpr int pencil_E(int m, int vid) {
-  return vid == 0 ? elems_per_thread(m) : (vid == 1 && m >= 256) ? half_tile(m) : 0;
+  return vid == 0 ? elems_per_thread(m) : m < 256 ? 0 : vid == 1 ? half_tile(m) : vid == 2 ? elems_per_thread(m) : 0;
 }
-bool pencil_variant_ok(int m, int vid) { return vid >= 0 && vid < SMK_NPVAR && pencil_E(m, vid) != 0; }
+constexpr int pencil_XCH(int vid) { return vid == 2 ? 1 : 0; }
+bool pencil_variant_ok(int m, int vid, bool c2r) {
+  return vid >= 0 && vid < SMK_NPVAR && pencil_E(m, vid) != 0 && (c2r || pencil_XCH(vid) == 0);
+}
 
 template <int M, int VID>
 hipError_t launch_z_r2c_v(const float* noise, uint64_t seed, float2* K, const float2* tw, const float2* twr,
@@ -634,7 +671,7 @@ hipError_t launch_z_c2r_v(const float2* K, float* out, const float2* tw, const f
   } else {
     constexpr int T = pencil_T(M, E);
     if ((np + T - 1) / T > 0x7fffffffull) return hipErrorInvalidConfiguration;
-    hipLaunchKernelGGL((k_z_c2r<M, E, T>), dim3((uint32_t)((np + T - 1) / T)), dim3(T * (M / E)), 0, s, K, out,
+    hipLaunchKernelGGL((k_z_c2r<M, E, T, pencil_XCH(VID)>), dim3((uint32_t)((np + T - 1) / T)), dim3(T * (M / E)), 0, s, K, out,
                        tw, twr, np, scale, status);
     return hipGetLastError();
   }
@@ -646,8 +683,9 @@ hipError_t launch_z_r2c(int m, int vid, const float* noise, uint64_t seed, float
   switch (m) {
 #define SMK_CASE(N)                                                                         \
   case N:                                                                                   \
-    return vid == 0 ? launch_z_r2c_v<N, 0>(noise, seed, K, tw, twr, p0, np, trans, ny, s)   \
-                    : launch_z_r2c_v<N, 1>(noise, seed, K, tw, twr, p0, np, trans, ny, s);
+    return vid == 0   ? launch_z_r2c_v<N, 0>(noise, seed, K, tw, twr, p0, np, trans, ny, s) \
+           : vid == 1 ? launch_z_r2c_v<N, 1>(noise, seed, K, tw, twr, p0, np, trans, ny, s) \
+                      : hipErrorInvalidValue;
     SMK_FFT_LENGTHS(SMK_CASE)
 #undef SMK_CASE
     default: return hipErrorInvalidValue;
@@ -659,8 +697,10 @@ hipError_t launch_z_c2r(int m, int vid, const float2* K, float* out, const float
   switch (m) {
 #define SMK_CASE(N)                                                                         \
   case N:                                                                                   \
-    return vid == 0 ? launch_z_c2r_v<N, 0>(K, out, tw, twr, np, scale, status, s)           \
-                    : launch_z_c2r_v<N, 1>(K, out, tw, twr, np, scale, status, s);
+    return vid == 0   ? launch_z_c2r_v<N, 0>(K, out, tw, twr, np, scale, status, s) \
+           : vid == 1 ? launch_z_c2r_v<N, 1>(K, out, tw, twr, np, scale, status, s) \
+           : vid == 2 ? launch_z_c2r_v<N, 2>(K, out, tw, twr, np, scale, status, s) \
+                      : hipErrorInvalidValue;
     SMK_FFT_LENGTHS(SMK_CASE)
 #undef SMK_CASE
     default: return hipErrorInvalidValue;
@@ -1178,7 +1218,7 @@ PassKind pass_kind(const smk_box_plan* p, int kind) {
 bool variant_valid(const smk_box_plan* p, int kind, int vid) {
   const PassKind k = pass_kind(p, kind);
   if (k.type == 0) return strided_variant_ok(k.n, k.mode, vid);
-  if (k.type == 1) return pencil_variant_ok(k.n, vid);
+  if (k.type == 1) return pencil_variant_ok(k.n, vid, k.mode == MODE_INV);
   return false;
 }
 
