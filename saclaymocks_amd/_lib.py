@@ -11,6 +11,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsaclay_hip.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "saclaymocks_hip.h")
 
+# layout of the structs passed by pointer (smk_skewers, smk_box_view): must
+# equal SMK_ABI_VERSION of include/saclaymocks_hip.h and the library's
+ABI_VERSION = 4
+
 SMK_OK = 0
 SMK_ERR_ARG = -1
 SMK_ERR_OOM = -2
@@ -101,6 +105,9 @@ def lib():
         f = getattr(L, name)
         f.restype = res
         f.argtypes = args
+    if L.smk_abi_version() != ABI_VERSION:
+        raise ImportError(f"{LIB_PATH} has ABI {L.smk_abi_version()}, this binding expects {ABI_VERSION}: "
+                          "rebuild it with `python -m saclaymocks_amd.build`")
     _lib = L
     return L
 

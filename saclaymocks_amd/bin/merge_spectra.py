@@ -28,7 +28,6 @@ analytic A exp(-(k/k1)^2) of -p1dA / -p1dk1.
 """
 import argparse
 import glob
-import json
 import os
 import sys
 import time
@@ -39,7 +38,6 @@ import torch
 from .. import _lib, fitsio_lite, forest, healpix, util
 
 META_NAMES = ["RA", "DEC", "Z_noRSD", "Z", "HDU", "THING_ID", "PLATE", "MJD", "FIBERID", "PMF"]
-HERE = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def parse_args(argv=None):
@@ -72,9 +70,7 @@ def parse_args(argv=None):
 
 def _param_tables(path):
     if path is None:
-        with open(os.path.join(HERE, "data", "fgpa_params_z.json")) as f:
-            d = json.load(f)
-        return {c: util.InterpTable(d["z"], d[c]) for c in ("a", "b", "c")}
+        return util.fgpa_param_tables()
     t = fitsio_lite.read(path, 1)
     return {c: util.InterpTable(t["z"].astype(float), t[c].astype(float)) for c in ("a", "b", "c")}
 

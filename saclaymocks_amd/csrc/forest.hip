@@ -81,11 +81,13 @@ __device__ __forceinline__ int wrap(int64_t i, int n) {
 }
 
 // The block's skewer range [s_rng[0], s_rng[1]] for pixels [pix0, plast]
-// (pix0 < npix_total): from the batch's per-block table when it has one and
-// the block is one table stride, else searched by the first wave; every
-// thread must call it.
-__device__ __forceinline__ void block_skewers(const smk_skewers& sk, int64_t pix0, int64_t plast, int* s_rng) {
-  if (sk.blk_s0 != nullptr && blockDim.x == SMK_GATHER_BLOCK_PIXELS && pix0 == (int64_t)blockIdx.x * blockDim.x) {
+// (pix0 < npix_total), the block covering `block_pixels` pixels: from the
+// batch's per-block table when it has one and the block is exactly one table
+// stride, else searched by the first wave; every thread must call it.
+__device__ __forceinline__ void block_skewers(const smk_skewers& sk, int64_t pix0, int64_t plast, int* s_rng,
+                                              int64_t block_pixels) {
+  if (sk.blk_s0 != nullptr && block_pixels == SMK_GATHER_BLOCK_PIXELS &&
+      pix0 == (int64_t)blockIdx.x * SMK_GATHER_BLOCK_PIXELS) {
     if (threadIdx.x < 2) {
       const int v = sk.blk_s0[blockIdx.x + threadIdx.x];
       s_rng[threadIdx.x] = v < 0 ? 0 : (v >= sk.n ? sk.n - 1 : v);
@@ -117,7 +119,8 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
   const smk_skewers& sk = a.sk;
   // the skewers of this block's pixel range, searched once by the first wave
   __shared__ int s_rng[2];
-  block_skewers(sk, pix0, pix0 + blockDim.x - 1 < a.npix_total ? pix0 + blockDim.x - 1 : a.npix_total - 1, s_rng);
+  block_skewers(sk, pix0, pix0 + blockDim.x - 1 < a.npix_total ? pix0 + blockDim.x - 1 : a.npix_total - 1, s_rng,
+                (int64_t)blockDim.x);
   if (pix >= a.npix_total) return;
   const int s = find_skewer(sk.off, s_rng[0], s_rng[1], pix);
   const int i = (int)(pix - sk.off[s]);
@@ -290,7 +293,7 @@ __global__ __launch_bounds__(256) void k_gather_plane_tri(GatherArgs a) {
   __shared__ int s_rng[2];
   {
     const int64_t pl = pix0 + 256 * PPT - 1;
-    block_skewers(sk, pix0, pl < a.npix_total ? pl : a.npix_total - 1, s_rng);
+    block_skewers(sk, pix0, pl < a.npix_total ? pl : a.npix_total - 1, s_rng, (int64_t)256 * PPT);
   }
   const smk_box_view& b = a.box;
   const int nx = b.nx, ny = b.ny, nz = b.nz;

@@ -1,14 +1,12 @@
-"""Regenerate fgpa_params_z.json from the reference's etc/params.fits.
+"""Regenerate fgpa_params_z.npy from the reference's etc/params.fits.
 
-The table samples a(z), b(z), c(z) (merge_spectra.py:58-75 reads them with
-util.InterpFitsTable) every 0.002 in z over [1.8, 3.6], so that machines
-without the reference (the GPU hosts) can interpolate them.  Linear
-interpolation of this table differs from linear interpolation of the
-10000-row original by < 4e-4 relative in a, < 4e-6 in b and < 5e-7 in c.
+The file holds the whole table, all 10000 rows of (z, a, b, c) in float64,
+as merge_spectra.py:58-75 reads it with util.InterpFitsTable
+(util.py:300-314), so that machines without the reference (the GPU hosts)
+interpolate exactly the same nodes.
 
 Usage: python -m saclaymocks_amd.data.make_fgpa_table /path/to/etc/params.fits
 """
-import json
 import os
 import sys
 
@@ -16,17 +14,18 @@ import numpy as np
 
 from saclaymocks_amd import fitsio_lite
 
+DST = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fgpa_params_z.npy")
+
+
+def table_from_fits(path):
+    """float64 [n][4] array of the PARS HDU's z, a, b, c columns."""
+    t = fitsio_lite.read(path, "PARS")
+    return np.stack([t[c].astype(np.float64) for c in ("z", "a", "b", "c")], axis=1)
+
 
 def main(path):
-    t = fitsio_lite.read(path, "PARS")
-    z = np.round(np.arange(1.80, 3.60001, 0.002), 3)
-    out = {"source": "etc/params.fits (SaclayMocks), columns z, a, b, c", "z": z.tolist()}
-    for col in ("a", "b", "c"):
-        out[col] = np.interp(z, t["z"].astype(float), t[col].astype(float)).tolist()
-    dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fgpa_params_z.json")
-    with open(dst, "w") as f:
-        json.dump(out, f, indent=0)
-    print("wrote", dst)
+    np.save(DST, table_from_fits(path), allow_pickle=False)
+    print("wrote", DST)
 
 
 if __name__ == "__main__":

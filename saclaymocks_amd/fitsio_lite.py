@@ -121,9 +121,28 @@ def _table_dtype(h):
     return np.dtype(fields)
 
 
+def _logical_columns(h):
+    """Names of the TFORM 'L' columns: one byte per value, 'T' true, 'F'
+    false, 0 null (FITS standard 4.0, section 7.3.3.1)."""
+    out = []
+    for i in range(1, h["TFIELDS"] + 1):
+        if str(h["TFORM%d" % i]).strip().lstrip("0123456789").startswith("L"):
+            out.append(str(h.get("TTYPE%d" % i, "col%d" % i)).strip())
+    return out
+
+
 def _decode(h, raw):
     if h.get("XTENSION", "").strip() == "BINTABLE":
-        return np.frombuffer(raw, dtype=_table_dtype(h), count=h["NAXIS2"]).copy()
+        rec = np.frombuffer(raw, dtype=_table_dtype(h), count=h["NAXIS2"]).copy()
+        logical = _logical_columns(h)
+        if logical:
+            dt = np.dtype([(n, ("?", rec.dtype[n].shape) if n in logical else rec.dtype[n])
+                           for n in rec.dtype.names])
+            out = np.zeros(len(rec), dtype=dt)
+            for n in rec.dtype.names:
+                out[n] = (rec[n] == ord("T")) if n in logical else rec[n]
+            rec = out
+        return rec
     if h.get("NAXIS", 0) > 0:
         shape = tuple(h["NAXIS%d" % i] for i in range(h["NAXIS"], 0, -1))
         data = np.frombuffer(raw, dtype=_BITPIX[h["BITPIX"]]).reshape(shape)
@@ -286,7 +305,7 @@ def _tform(dt):
     if k == "S":
         return "%dA" % base.itemsize, "S%d" % base.itemsize
     if k == "U":
-        return "%dA" % base.itemsize // 4, None
+        return "%dA" % (base.itemsize // 4), None
     m = {("f", 8): "D", ("f", 4): "E", ("i", 8): "K", ("i", 4): "J", ("i", 2): "I", ("u", 1): "B",
          ("b", 1): "L", ("i", 1): "B"}
     code = m.get((k, base.itemsize))
@@ -318,6 +337,8 @@ def _table_hdu(rec, header, extname):
         col = rec[n]
         if col.dtype.kind == "U":
             col = np.char.encode(col, "ascii")
+        elif col.dtype.kind == "b":
+            col = np.where(col, ord("T"), ord("F")).astype(np.int8)
         out[n] = col
     cards = [_card("XTENSION", "BINTABLE"), _card("BITPIX", 8), _card("NAXIS", 2),
              _card("NAXIS1", out.dtype.itemsize), _card("NAXIS2", len(out)), _card("PCOUNT", 0),

@@ -318,7 +318,10 @@ class CapturedStep:
 
     def __call__(self, seed):
         """Replay with this seed; returns {name: tensor} (and "delta_s")."""
-        self.seed.fill_(int(seed) & (2 ** 63 - 1))
+        # the device reads the seed as uint64: store its two's-complement
+        # int64, as the eager path passes int(seed) & (2**64 - 1)
+        v = int(seed) & (2 ** 64 - 1)
+        self.seed.fill_(v - 2 ** 64 if v >= 2 ** 63 else v)
         self.graph.replay()
         return dict(self.out, delta_s=self.delta_s)
 

@@ -118,3 +118,13 @@ class InterpTable:
             raise ValueError(f"ERROR: Redshift {redshift} is out of range ! "
                              f"Please enter a redshift between {self.zmin} and {self.zmax}")
         return np.interp(r, self.z, self.y)
+
+
+def fgpa_param_tables():
+    """a(z), b(z), c(z) of the reference's etc/params.fits (all 10000 rows,
+    shipped as data/fgpa_params_z.npy), each an InterpTable as
+    merge_spectra.py:58-75 builds them with util.InterpFitsTable."""
+    import os
+    t = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "fgpa_params_z.npy"),
+                allow_pickle=False)
+    return {c: InterpTable(t[:, 0], t[:, i]) for i, c in enumerate(("a", "b", "c"), start=1)}

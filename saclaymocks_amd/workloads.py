@@ -6,24 +6,19 @@ the box spans Z in [R0 - LZ/2, R0 + LZ/2) with R0 = h chi(z0), so a quasar at
 redshift z sits at Z = h chi(z) and its forest is the n pixels of the global
 0.2 Mpc/h grid just below it.
 """
-import json
-import os
 
 import numpy as np
 import torch
 
 from . import forest, util
 
-HERE = os.path.dirname(os.path.abspath(__file__))
 P1D_MISS_DEFAULT = (0.05, 1.8)     # P1D_miss(k) = A exp(-(k/k1)^2), Mpc/h
 FIT_Z_BINS = (1.8, 2.2, 2.6, 3.0, 3.6)
 
 
 def fgpa_tables():
-    """a(z), b(z), c(z) from the sampled etc/params.fits table (util.InterpTable)."""
-    with open(os.path.join(HERE, "data", "fgpa_params_z.json")) as f:
-        d = json.load(f)
-    return {c: util.InterpTable(d["z"], d[c]) for c in ("a", "b", "c")}
+    """a(z), b(z), c(z) from the full etc/params.fits table (util.InterpTable)."""
+    return util.fgpa_param_tables()
 
 
 def p1d_missing(k, A=P1D_MISS_DEFAULT[0], k1=P1D_MISS_DEFAULT[1]):

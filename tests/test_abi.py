@@ -25,7 +25,9 @@ def test_python_signatures_cover_header():
 
 def test_version_and_lengths():
     L = _lib.lib()
-    assert L.smk_abi_version() == 4
+    assert L.smk_abi_version() == _lib.ABI_VERSION
+    with open(_lib.HEADER) as f:
+        assert f"#define SMK_ABI_VERSION {_lib.ABI_VERSION}\n" in f.read()
     for n in (16, 24, 40, 64, 128, 256, 768, 1536, 2560):
         assert L.smk_fft_length_supported(n) == 1
     for n in (7, 11, 4096, 3):

@@ -39,10 +39,10 @@ def _problem(cuda, seed=0, ns=3, per=40, m=1, lo=200, hi=900, save_modes=True):
     return model, (d64, e64, pset, growth, edges, 0.2, targets, m), theta_true
 
 
-@pytest.mark.parametrize("m", [1, 2])
-def test_forward_parity(cuda, m):
-    model, args, theta = _problem(cuda, m=m)
-    th = theta * np.array([1.1, 0.95, 1.05] * 3)
+@pytest.mark.parametrize("m, ns", [(1, 3), (2, 3), (1, 5)])
+def test_forward_parity(cuda, m, ns):
+    model, args, theta = _problem(cuda, m=m, ns=ns)
+    th = theta * np.array([1.1, 0.95, 1.05] * ns)
     fmean, bins = model.forward(th)
     d64, e64, pset, growth, edges, pixel, targets, mm = args
     rf, rb, _ = fto.forward(d64, e64, pset, th, growth, edges, pixel, mm)
@@ -76,11 +76,12 @@ def test_autograd_function_backward(cuda):
     assert np.allclose(th.grad.numpy(), want, rtol=1e-7, atol=1e-9 * np.abs(want).max())
 
 
-def test_gradient_fit_recovers_parameters(cuda):
+@pytest.mark.parametrize("ns", [2, 5])
+def test_gradient_fit_recovers_parameters(cuda, ns):
     from saclaymocks_amd import fit
-    model, args, theta = _problem(cuda, ns=2, per=60)
-    th0 = theta * np.array([1.3, 0.85, 1.2] * 2)
-    bounds = [(1e-3, 3.0), (0.1, 4.0), (0.0, 4.0)] * 2
+    model, args, theta = _problem(cuda, ns=ns, per=60)
+    th0 = theta * np.array([1.3, 0.85, 1.2] * ns)
+    bounds = [(1e-3, 3.0), (0.1, 4.0), (0.0, 4.0)] * ns
     res = fit.fit_gradient(model, th0, bounds)
     assert res["chi2"] < 1e-3
     assert np.allclose(res["x"], theta, rtol=2e-2, atol=2e-2)

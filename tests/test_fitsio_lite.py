@@ -1,10 +1,13 @@
 """The FITS codec against its own written files and the FITS standard layout."""
 import gzip
+import os
 
 import numpy as np
 import pytest
 
 from saclaymocks_amd import fitsio_lite as fl
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def test_image_round_trip_and_layout(tmp_path):
@@ -51,3 +54,105 @@ def test_tables_images_gzip_and_selective_read(tmp_path):
     assert t["A"].tolist() == [1.0, 2.0] and t["B"].tolist() == [3, 4]
     with pytest.raises(KeyError):
         fl.read(p, "NOPE")
+
+
+def _raw_hdus(raw):
+    """(header cards, data offset) of each HDU, found by scanning for END in
+    2880-byte blocks: a decoder independent of the codec's own."""
+    out, pos = [], 0
+    while pos < len(raw):
+        cards = []
+        while True:
+            block = raw[pos:pos + 2880]
+            pos += 2880
+            done = False
+            for i in range(36):
+                c = block[80 * i:80 * i + 80].decode("ascii")
+                if c.startswith("END" + " " * 77):
+                    done = True
+                    break
+                if c.strip():
+                    cards.append(c)
+            if done:
+                break
+        kv = {c[:8].strip(): c[10:].split("/")[0].strip().strip("'").strip() for c in cards if c[8:10] == "= "}
+        n = 0
+        if int(kv["NAXIS"]) > 0:
+            n = abs(int(kv["BITPIX"])) // 8
+            for i in range(1, int(kv["NAXIS"]) + 1):
+                n *= int(kv["NAXIS%d" % i])
+        out.append((kv, pos, n))
+        pos += (n + 2879) // 2880 * 2880
+    return out
+
+
+@pytest.mark.parametrize("name, cols", [("dgrowth.fits", ("Z", "dD/dz")), ("PlanckDR12.fits", ("K", "PK", "PKSB"))])
+def test_reads_files_written_by_fitsio(name, cols, tmp_path):
+    """The reference's etc/ tables (written by fitsio): the codec's columns
+    equal a plain big-endian parse of the data bytes, and writing the table
+    back gives the same table cards and the same data bytes."""
+    path = os.path.join(GOLDEN, name)
+    raw = open(path, "rb").read()
+    hdus = _raw_hdus(raw)
+    kv, off, n = hdus[1]
+    assert kv["XTENSION"] == "BINTABLE" and int(kv["TFIELDS"]) == len(cols)
+    t = fl.read(path, 1)
+    assert t.dtype.names == cols
+    flat = np.frombuffer(raw[off:off + n], dtype=">f8").reshape(int(kv["NAXIS2"]), len(cols))
+    for i, c in enumerate(cols):
+        assert np.array_equal(t[c], flat[:, i])
+    h = fl.read_header(path, 1)
+    for key in ("NAXIS1", "NAXIS2", "TFIELDS"):
+        assert h[key] == int(kv[key])
+    out = str(tmp_path / name)
+    with fl.FITS(out) as f:
+        f.write(t, extname=h.get("EXTNAME"))
+    raw2 = open(out, "rb").read()
+    kv2, off2, n2 = _raw_hdus(raw2)[1]
+    for key in ["NAXIS1", "NAXIS2", "TFIELDS"] + ["TTYPE%d" % i for i in range(1, len(cols) + 1)] + \
+            ["TFORM%d" % i for i in range(1, len(cols) + 1)]:
+        assert kv2[key] == kv[key], key
+    assert raw2[off2:off2 + n2] == raw[off:off + n]
+
+
+def test_dgrowth_table_is_the_reference_grid():
+    t = fl.read(os.path.join(GOLDEN, "dgrowth.fits"), 1)
+    assert len(t) == 5000 and t["Z"][0] == 0.0 and abs(t["Z"][-1] - 4.999) < 1e-12
+    assert np.allclose(np.diff(t["Z"]), 0.001)
+    # dD/dz < 0 everywhere (growth decreases with redshift)
+    assert np.all(t["dD/dz"] < 0)
+
+
+def test_logical_and_unicode_columns_round_trip(tmp_path):
+    p = str(tmp_path / "t.fits")
+    rec = np.zeros(4, dtype=[("FLAG", "?"), ("PMF", "U8"), ("V", "f4")])
+    rec["FLAG"] = [True, False, False, True]
+    rec["PMF"] = ["1-2-3", "a", "", "xyzw"]
+    rec["V"] = [1, 2, 3, 4]
+    with fl.FITS(p) as f:
+        f.write(rec, extname="T")
+    raw = open(p, "rb").read()
+    kv, off, n = _raw_hdus(raw)[1]
+    assert kv["TFORM1"] == "L" and kv["TFORM2"] == "5A"  # longest string
+    # FITS logical values are the bytes 'T' and 'F'
+    row = int(kv["NAXIS1"])
+    assert [raw[off + i * row] for i in range(4)] == [ord(c) for c in "TFFT"]
+    t = fl.read(p, "T")
+    assert t["FLAG"].dtype == np.bool_ and t["FLAG"].tolist() == [True, False, False, True]
+    assert t["PMF"].tolist() == [b"1-2-3", b"a", b"", b"xyzw"]
+
+
+def test_full_fgpa_parameter_table():
+    """The shipped a, b, c(z) table is every row of etc/params.fits."""
+    from saclaymocks_amd import util
+    tabs = util.fgpa_param_tables()
+    assert len(tabs["a"].z) == 10000 and tabs["a"].zmin == 1.8 and tabs["a"].zmax == 3.6
+    ref = "/root/reference/etc/params.fits"
+    if not os.path.exists(ref):
+        pytest.skip("reference parameter table not present")
+    t = fl.read(ref, "PARS")
+    for c in ("a", "b", "c"):
+        assert np.array_equal(tabs[c].z, t["z"]) and np.array_equal(tabs[c].y, t[c])
+    # interpolation between nodes: the reference's interp1d is linear
+    z = np.array([2.4, 3.0])
+    assert np.allclose(tabs["a"].interp(z), [0.012764, 0.127], rtol=2e-3)

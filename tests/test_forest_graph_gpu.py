@@ -41,7 +41,7 @@ def test_captured_step_replays_match_eager(cuda, interp):
     print(f"[forest graph] interp={interp}", flush=True)
     plan = forest.ForestPlan()
     step = forest.CapturedStep(batch, box, filt, foff, abcg=rows, interp=interp)
-    seeds = [5, 9, 5, 123456789]
+    seeds = [5, 9, 5, 123456789, -1, 2 ** 63 + 5]
     prev = None
     for s in seeds:
         want = _eager(forest, plan, batch, box, filt, foff, rows, s, interp)   # eager work between replays

@@ -1,0 +1,200 @@
+"""The benchmark configurations at their full sizes, checked through
+size-independent properties and sampled oracle parity:
+
+  - the 2560 x 2560 x 1536 delta + eta_zz box (one fused job): status bits,
+    Parseval between the device's FP64 k-space sum and the real-space
+    variance of the float32 output, the expected variance and the isotropy
+    ratios on the same k grid (within cosmic variance), and every x slab of
+    the 40 GB outputs populated alike (64-bit offsets);
+  - the 200 000-skewer uneven batch (300 to 1500 pixels): 2000 sampled
+    skewers against the oracle's trilinear gather, device-mode missing power
+    and FGPA;
+  - the 20 000-skewer, 5-bin FGPA fit problem: forward and adjoint gradient
+    of all 15 parameters against the oracle.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import box_oracle as bo
+from oracle import fit_oracle as fto
+from oracle import forest_oracle as fo
+
+pytestmark = pytest.mark.gpu
+
+C2 = (2560, 2560, 1536)
+CELL = 2.19
+
+
+def _grid_moments(shape, cell, dev, pk=bo.PK_DEFAULT):
+    """FP64 sums over the rfft grid of w_k s_k, w_k s_k f_k and w_k s_k f_k^2
+    with s = P(k)/Vcell and f = kz^2/kk (the eta_zz factor; kk = 1 at k = 0),
+    evaluated plane by plane on the device.  Dividing by N gives the expected
+    var(delta), cov(delta, eta_zz) and var(eta_zz) of the job."""
+    nx, ny, nz = shape
+    kx, ky, kz = (torch.from_numpy(np.asarray(a, dtype=np.float64).reshape(-1)).to(dev)
+                  for a in bo.k_axes(nx, ny, nz, cell))
+    kxy = (kx[:, None] ** 2 + ky[None, :] ** 2)
+    A, k0, n, R = pk
+    vcell = float(np.float32(cell ** 3))
+    out = torch.zeros(3, dtype=torch.float64, device=dev)
+    for j in range(nz // 2 + 1):
+        w = 1.0 if j in (0, nz // 2) else 2.0
+        kk = kxy + kz[j] ** 2
+        k = torch.sqrt(kk)
+        x = k / k0
+        s = A * x / (1.0 + x ** n) * torch.exp(-(k * R) ** 2) / vcell
+        kkp = torch.where(kk > 0, kk, torch.ones_like(kk))
+        f = kz[j] ** 2 / kkp
+        out[0] += w * s.sum()
+        out[1] += w * (s * f).sum()
+        out[2] += w * (s * f * f).sum()
+    return (out / (nx * ny * nz)).cpu().numpy()
+
+
+def test_grid_moments_evaluator_matches_the_oracle(cuda):
+    """The device evaluator above equals the oracle's FP64 k-space variance
+    and eta_zz factors on a shape the oracle handles."""
+    shape = (64, 48, 40)
+    m = _grid_moments(shape, CELL, cuda)
+    assert m[0] == pytest.approx(bo.kspace_variance(*shape, CELL), rel=1e-6)
+    s = bo.sqrt_p_over_vcell(*shape, CELL) ** 2
+    f = bo.aux_factor("eta_zz", *shape, CELL)
+    w = np.full(shape[2] // 2 + 1, 2.0)
+    w[0] = w[-1] = 1.0
+    N = np.prod(shape)
+    assert m[1] == pytest.approx(float((s * f * w).sum() / N), rel=1e-6)
+    assert m[2] == pytest.approx(float((s * f * f * w).sum() / N), rel=1e-6)
+
+
+def _slab_sums(a, b, nslab):
+    """FP64 sum(a*a), sum(b*b), sum(a*b) per x slab (chunked: no full-size
+    FP64 copy)."""
+    nx = a.shape[0]
+    bounds = np.linspace(0, nx, nslab + 1).astype(int)
+    out = np.zeros((nslab, 3))
+    for i in range(nslab):
+        for x in range(bounds[i], bounds[i + 1], 32):
+            x1 = min(x + 32, bounds[i + 1])
+            ad, bd = a[x:x1].double(), b[x:x1].double()
+            out[i] += [float((ad * ad).sum()), float((bd * bd).sum()), float((ad * bd).sum())]
+    return out
+
+
+def test_c2_full_box_statistics(cuda):
+    from saclaymocks_amd.box import BoxPlan
+    print(f"[full-size box] shape={C2}", flush=True)
+    torch.cuda.empty_cache()
+    plan = BoxPlan(*C2, cell=CELL)
+    d, e, var, st = plan.grf_box(seed=20240601, aux="eta_zz")
+    torch.cuda.synchronize()
+    assert st.cpu().tolist() == [2, 2]                 # no NaN, not all zero
+    N = float(np.prod(C2))
+    sums = _slab_sums(d, e, 8)
+    vd, ve, cde = sums.sum(axis=0) / N
+    # Parseval: the FP64 k-space sum of the device spectrum against the
+    # real-space variance of the float32 output (mean is 0: P(0) = 0)
+    assert abs(vd / var.item() - 1) < 1e-5
+    # every x slab holds its share of the variance (a slab left unwritten or
+    # written at a wrapped 32-bit offset would be off by O(1))
+    per = sums[:, 0] / (N / 8)
+    assert np.all(np.abs(per / vd - 1) < 0.02), per / vd
+    # expected moments on the same grid; one realisation differs from them by
+    # cosmic variance, ~1e-4 relative here
+    m = _grid_moments(C2, CELL, cuda)
+    assert vd == pytest.approx(m[0], rel=2e-3)
+    assert cde / vd == pytest.approx(m[1] / m[0], abs=2e-3)
+    assert ve / vd == pytest.approx(m[2] / m[0], abs=2e-3)
+    # and those are the isotropic values 1/3 and 1/5 to the grid's precision
+    assert m[1] / m[0] == pytest.approx(1 / 3, abs=5e-3)
+    assert m[2] / m[0] == pytest.approx(1 / 5, abs=5e-3)
+    del d, e, plan
+    torch.cuda.empty_cache()
+
+
+def test_c3_full_batch_sampled_parity(cuda):
+    """The uneven 200 000-skewer batch in one step (device missing power,
+    trilinear gather, FGPA at z_QSO), 2000 skewers checked against the oracle."""
+    from saclaymocks_amd import forest, workloads
+    from saclaymocks_amd.box import BoxPlan
+    shape = (256, 256, 1536)
+    print(f"[full-size forest] 200000 skewers on box {shape}", flush=True)
+    plan = BoxPlan(*shape, cell=CELL)
+    d, e, _, _ = plan.grf_box(seed=77, aux="eta_zz")
+    fr = workloads.box_frame(shape, CELL)
+    box = forest.BoxView.plane(d, e, cell=CELL, r0=fr["R0"], periodic=True)
+    batch, rows, h = workloads.uneven_quasar_batch(shape, CELL, n=200000)
+    assert batch.n == 200000 and 300 <= batch.npix_host.min() and batch.npix_host.max() <= 1500
+    fplan = forest.ForestPlan()
+    filt, foff = forest.missing_power_filters(0.2, fo.p1d_missing_analytic)
+    seed = 31337
+    ds = forest.missing_power(fplan, batch, filt, foff, seed=seed)
+    out = forest.gather(batch, box, interp="trilinear", abcg=rows, delta_s=ds,
+                        want=("delta_l", "eta_par", "flux"))
+    torch.cuda.synchronize()
+    assert int(out["status"].sum().item()) == 0
+    rng = np.random.default_rng(5)
+    pick = np.sort(rng.choice(batch.n, 2000, replace=False))
+    pick[-1] = batch.n - 1                              # the last pixels of the batch
+    dn, en = d.cpu().numpy(), e.cpu().numpy()
+    L = [CELL * s for s in shape]
+    rows_h = rows.cpu().numpy()
+    nid = batch.noise_id.cpu().numpy().view(np.uint64)
+    got = {k: out[k].cpu().numpy() for k in ("delta_l", "eta_par", "flux")}
+    got_ds = ds.cpu().numpy()
+    err = {"dl": 0.0, "eta": 0.0, "ds": 0.0, "F": 0.0}
+    for s in pick:
+        o, n = int(batch.off_host[s]), int(batch.npix_host[s])
+        g = h["g0"][s] + np.arange(n)
+        Z = fr["rmin"] + g * 0.2
+        X, Y = np.full(n, h["x"][s]), np.full(n, h["y"][s])
+        rd, re = fo.trilinear([dn, en], X, Y, Z, L, (CELL,) * 3, fr["R0"])
+        want_ds = fo.missing_power_device(seed, nid[s], n, 0.2, fo.p1d_missing_analytic)
+        a, b, c, G = (float(v) for v in rows_h[s])
+        F = fo.fgpa(rd + got_ds[o:o + n].astype(np.float64), re, G, a, b, c)
+        err["dl"] = max(err["dl"], float(np.abs(got["delta_l"][o:o + n] - rd).max()))
+        err["eta"] = max(err["eta"], float(np.abs(got["eta_par"][o:o + n] - re).max()))
+        err["ds"] = max(err["ds"], float(np.abs(got_ds[o:o + n] - want_ds).max() / np.abs(want_ds).max()))
+        err["F"] = max(err["F"], float(np.abs(got["flux"][o:o + n] - F).max()))
+    print("[full-size forest] max errors", err, flush=True)
+    assert err["dl"] < 2e-6 and err["eta"] < 2e-6
+    assert err["ds"] < 1e-5
+    assert err["F"] < 2e-6
+
+
+def test_c4_fit_problem_all_sets(cuda):
+    """20 000 skewers in the 5 redshift bins: the device forward (mean flux,
+    binned P1D) and the adjoint gradient of all 15 parameters against the
+    oracle on the same float32 fields."""
+    from saclaymocks_amd import fit, forest, workloads
+    from saclaymocks_amd.box import BoxPlan
+    shape = (256, 256, 1536)
+    print("[full-size fit] 20000 skewers, 5 sets", flush=True)
+    plan = BoxPlan(*shape, cell=CELL)
+    d, e, _, _ = plan.grf_box(seed=91, aux="eta_zz")
+    fr = workloads.box_frame(shape, CELL)
+    box = forest.BoxView.plane(d, e, cell=CELL, r0=fr["R0"], periodic=True)
+    batch, delta, eta, growth, truth = workloads.fit_bins_problem(box, shape, CELL, per_bin=4000, seed=5)
+    assert batch.n == 20000
+    edges = workloads.p1d_edges()
+    model = fit.P1DFitModel(batch, delta, eta, growth, edges, pixel=0.2)
+    d64 = [x.astype(np.float64) for x in batch.split(delta)]
+    e64 = [x.astype(np.float64) for x in batch.split(eta)]
+    pset = batch.pset.cpu().numpy()
+    theta = truth.reshape(-1)
+    fm, bins, _ = fto.forward(d64, e64, pset, theta, growth, edges, 0.2, 1)
+    c = bins[..., 0]
+    my = np.where(c > 0, bins[..., 2] / np.where(c > 0, c, 1), np.nan)
+    targets = (workloads.mean_flux_target(workloads.FIT_Z_BINS), np.full(5, 0.005), my, 0.05 * np.abs(my) + 1e-4)
+    model.set_targets(*targets)
+    th = theta * np.array([1.15, 0.97, 1.04] * 5)
+    gfm, gbins = model.forward(th)
+    rf, rb, _ = fto.forward(d64, e64, pset, th, growth, edges, 0.2, 1)
+    assert np.allclose(gfm, rf, rtol=1e-12)
+    assert np.array_equal(gbins[..., 0], rb[..., 0])
+    assert np.allclose(gbins[..., 2], rb[..., 2], rtol=1e-9, atol=1e-12 * rb[..., 2].max())
+    val, g = model.value_and_grad(th)
+    args = (d64, e64, pset, growth, edges, 0.2, targets, 1)
+    want = fto.adjoint_grad(th, *args)
+    assert abs(val / fto.chi2(th, *args) - 1) < 1e-10
+    assert np.allclose(g, want, rtol=1e-7, atol=1e-9 * np.abs(want).max())

@@ -34,6 +34,93 @@ def test_growth():
     assert fo.fgrowth(10.0) * 11 == pytest.approx(fo.fgrowth(20.0) * 21, rel=2e-2)
 
 
+def test_growth_einstein_de_sitter_closed_form():
+    """Om0 = 1: D(z)/D(0) = a = 1/(1+z) exactly (unnormalised g = 1)."""
+    z = np.array([0.0, 0.3, 1.0, 2.4, 5.0, 20.0])
+    assert np.allclose(fo.fgrowth(z, Om0=1.0), 1.0 / (1.0 + z), rtol=1e-14)
+    assert fo.fgrowth(0.0, Om0=1.0, unnormed=True) == pytest.approx(1.0, rel=1e-14)
+
+
+def test_growth_carroll_press_turner_published_value():
+    """Carroll, Press & Turner (1992, ARA&A 30, 499, eq. 29): the growth
+    suppression g(Om = 0.3, OL = 0.7) = 0.779 at z = 0 (their fig. 3).  A slip
+    in the Lambda terms (Om**(4/7), OL/70) moves it by several per cent."""
+    assert fo.fgrowth(0.0, Om0=0.3, unnormed=True) == pytest.approx(0.779, abs=2e-3)
+
+
+def test_growth_against_exact_linear_growth_integral():
+    """Flat LCDM linear growth (Heath 1977): D(z) ~ E(z) int_z^inf (1+z')/E(z')^3
+    dz' with E^2 = Om (1+z)^3 + 1 - Om; CPT approximates the normalised ratio to
+    better than 1 %.  The same check fails by > 3 % if OL/70 becomes OL/7."""
+    Om0 = fo.OMEGA_M
+
+    def E(z):
+        return np.sqrt(Om0 * (1 + z) ** 3 + 1 - Om0)
+
+    def D(z):
+        return E(z) * integrate.quad(lambda x: (1 + x) / E(x) ** 3, z, np.inf, epsabs=0, epsrel=1e-12)[0]
+
+    d0 = D(0.0)
+    for z in (0.5, 1.0, 1.8, 2.4, 3.6, 5.0):
+        assert fo.fgrowth(z) == pytest.approx(D(z) / d0, rel=1e-2)
+
+    # the same bound rejects a planted slip in the Lambda term of the formula
+    def fgrowth_bad(z, Om0=Om0):
+        Om = 1.0 / (1.0 + (1.0 - Om0) / (Om0 * (1.0 + z) ** 3))
+        Ol = 1.0 - Om
+        return 2.5 * Om / (1 + z) / (Om ** (4.0 / 7.0) - Ol + (1.0 + Om / 2.0) * (1.0 + Ol / 7.0))
+
+    bad = fgrowth_bad(2.4) / fgrowth_bad(0.0)
+    assert abs(bad / (D(2.4) / d0) - 1) > 3e-2
+
+
+def test_product_fgrowth_matches_oracle():
+    from saclaymocks_amd import util
+    z = np.linspace(0.0, 5.0, 51)
+    assert np.array_equal(util.fgrowth(z), fo.fgrowth(z))
+
+
+def _rand_sym(rng):
+    a = rng.standard_normal((3, 3))
+    return a + a.T
+
+
+def _eta6(t):
+    """(xx, yy, zz, xy, xz, yz) of a symmetric 3x3 tensor, the order of the
+    six smoothed fields in make_spectra.py:121-126."""
+    return (t[0, 0], t[1, 1], t[2, 2], t[0, 1], t[0, 2], t[1, 2])
+
+
+def test_eta_par_radial_isotropic_and_rank_one():
+    rng = np.random.default_rng(7)
+    pos = rng.standard_normal((20, 3)) * 500.0
+    X, Y, Z = pos.T
+    c = 0.731
+    iso = (np.full(20, c), np.full(20, c), np.full(20, c), np.zeros(20), np.zeros(20), np.zeros(20))
+    assert np.allclose(fo.eta_par_radial(iso, X, Y, Z), c, rtol=1e-14)
+    n = np.array([0.3, -1.2, 0.7])
+    t = np.outer(n, n)
+    got = fo.eta_par_radial(tuple(np.full(20, v) for v in _eta6(t)), X, Y, Z)
+    xh = pos / np.linalg.norm(pos, axis=1)[:, None]
+    assert np.allclose(got, (xh @ n) ** 2, rtol=1e-12)
+
+
+def test_eta_par_radial_on_the_z_axis_and_under_rotation():
+    rng = np.random.default_rng(8)
+    t = _rand_sym(rng)
+    e6 = _eta6(t)
+    for zz in (10.0, 4000.0):
+        assert fo.eta_par_radial(e6, 0.0, 0.0, zz) == pytest.approx(t[2, 2], rel=1e-14)
+    # x_hat . T . x_hat is invariant under x -> R x, T -> R T R^T
+    q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+    for _ in range(10):
+        x = rng.standard_normal(3) * 300.0
+        v0 = fo.eta_par_radial(e6, *x)
+        v1 = fo.eta_par_radial(_eta6(q @ t @ q.T), *(q @ x))
+        assert v1 == pytest.approx(v0, rel=1e-11, abs=1e-13)
+        assert v0 == pytest.approx(x @ t @ x / (x @ x), rel=1e-12)
+
+
 def test_compute_xyz2_frame():
     rng = np.random.default_rng(0)
     ra, dec = rng.uniform(0, 2 * np.pi, 50), rng.uniform(-1.2, 1.2, 50)
