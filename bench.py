@@ -17,6 +17,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 import argparse
 import json
+import re
 import os
 import sys
 import time
@@ -106,70 +107,91 @@ def cpu_baseline(cell, seconds=12.0):
 
 
 def forest_bench(delta, eta, shape, cell, steps, warmup, n_skewers):
-    """200k uneven skewers (300-1500 pixels) on the resident box -> forest pixels/s."""
+    """200k uneven skewers (300-1500 pixels) on the resident box -> forest pixels/s.
+    The step is the one-launch smk_forest_step; the same step as separate
+    missing-power and gather launches (delta_s through HBM) is timed beside it."""
     import torch
     from saclaymocks_amd import forest, workloads
     box = forest.BoxView.plane(delta, eta, cell=cell, r0=workloads.box_frame(shape, cell)["R0"], periodic=True)
     batch, rows, _ = workloads.uneven_quasar_batch(shape, cell, n=n_skewers)
     plan = forest.ForestPlan()
     filt, foff = forest.missing_power_filters(0.2, workloads.p1d_missing)
-    ds = torch.empty(batch.npix_total, dtype=torch.float32, device=delta.device)
+    npx = batch.npix_total
+    dev = delta.device
+    out = {"flux": torch.empty(npx, dtype=torch.float32, device=dev),
+           "status": torch.zeros(batch.n, dtype=torch.int32, device=dev)}
+    ws = torch.empty(int(forest._lib.lib().smk_forest_step_workspace_bytes()), dtype=torch.uint8, device=dev)
+    max_npix = int(batch.npix_host.max())
 
-    def step(seed):
+    def fused(seed):
+        return forest.step(plan, batch, box, filt, foff, rows, seed=seed, max_npix=max_npix, out=out, workspace=ws)
+
+    ds = torch.empty(npx, dtype=torch.float32, device=dev)
+
+    def separate(seed):
         forest.missing_power(plan, batch, filt, foff, seed=seed, out=ds)
         return forest.gather(batch, box, interp="trilinear", abcg=rows, delta_s=ds, want=("flux",))
 
-    for i in range(warmup):
-        step(100 + i)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    e0.record()
-    for i in range(steps):
-        out = step(i)
-    e1.record()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    ms = e0.elapsed_time(e1) / steps
-    f = out["flux"]
-    assert int(out["status"].sum().item()) == 0
-    # the same step captured as a HIP graph and replayed with a new seed per step
-    cap = forest.CapturedStep(batch, box, filt, foff, abcg=rows, interp="trilinear", want=("flux",))
-    for i in range(warmup):
-        cap(100 + i)
-    torch.cuda.synchronize()
-    g0, g1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    g0.record()
-    for i in range(steps):
-        cap(i)
-    g1.record()
-    torch.cuda.synchronize()
-    ms_graph = g0.elapsed_time(g1) / steps
+    def timed(fn):
+        for i in range(warmup):
+            fn(100 + i)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record()
+        for i in range(steps):
+            r = fn(i)
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps, time.perf_counter() - t0, r
+
+    ms, dt, res = timed(fused)
+    f = res["flux"].clone()
+    assert int(res["status"].sum().item()) == 0
+    ms_sep, _, res2 = timed(separate)
+    sep_diff = float((res2["flux"] - f).abs().max())
+    # the fused step captured as a HIP graph and replayed with a new seed per step
+    cap = forest.CapturedStep(batch, box, filt, foff, abcg=rows, interp="trilinear", want=("flux",), fused=True)
+    ms_graph, _, _ = timed(cap)
     assert torch.equal(cap(steps - 1)["flux"], f)
     del cap
     # the make_spectra drop-in's smoothing: the 343-cell Gaussian kernel instead of trilinear
-    for i in range(warmup):
-        forest.gather(batch, box, interp="gauss343", abcg=rows, delta_s=ds, want=("flux",))
-    torch.cuda.synchronize()
-    g0.record()
-    for i in range(steps):
-        forest.gather(batch, box, interp="gauss343", abcg=rows, delta_s=ds, want=("flux",))
-    g1.record()
-    torch.cuda.synchronize()
-    ms_gauss = g0.elapsed_time(g1) / steps
-    npx = batch.npix_total
+    ms_gauss, _, _ = timed(lambda i: forest.gather(batch, box, interp="gauss343", abcg=rows, delta_s=ds,
+                                                   want=("flux",)))
     # algorithmic bytes: 4 z-columns x cells spanned x 2 fields x 4 B per skewer,
-    # 4 B of F per pixel, 32 B of descriptor per skewer
+    # 4 B of F per pixel, 32 B of descriptor per skewer (delta_s stays on chip)
     span = np.ceil(batch.npix_host * 0.2 / cell) + 1
     alg = float(np.sum(4 * span * 2 * 4) + 4 * npx + 32 * batch.n)
     return {"workload": f"{batch.n} plane-parallel skewers, 300-1500 px of 0.2 Mpc/h, trilinear gather of "
                         "delta and eta_zz, missing power 0.05 exp(-(k/1.8)^2), FGPA at z_QSO",
             "pixels": npx, "skewers": batch.n, "ms_per_step": ms, "ms_per_step_graph": ms_graph,
+            "ms_per_step_separate_launches": ms_sep, "separate_vs_fused_max_abs_dflux": sep_diff,
             "ms_gather_gauss343": ms_gauss,
             "pixels_per_s": npx * steps / dt,
             "mean_flux": float(f.double().mean().item()),
-            "roofline": {"bound": "hbm", "alg_bytes": alg, "achieved": alg / (ms * 1e-3) / 1e9,
-                         "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS}}
+            "roofline": {"bound": "valu", "alg_bytes": alg, "achieved": alg / (ms * 1e-3) / 1e9,
+                         "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS,
+                         "valu": forest_valu_roofline(ms)}}
+
+
+def forest_valu_roofline(ms):
+    """VALU issue of the fused forest step against the MI355X issue peak, from
+    the SQ_INSTS_VALU count of profiles/forest_step_200k_valu_counters.md (None
+    if that profile is absent): one wave64 VALU instruction per 4 cycles per
+    SIMD, 4 SIMDs x 256 CUs at 2.4 GHz = 6.14e11 wave-instructions/s."""
+    path = os.path.join(ROOT, "profiles", "forest_step_200k_valu_counters.md")
+    if not os.path.exists(path):
+        return None
+    txt = open(path).read()
+    m = re.search(r"```json\n(.*?)```", txt, re.S)
+    if not m:
+        return None
+    doc = json.loads(m.group(1))
+    insts = float(doc["valu_insts_per_step"])
+    peak = 256 * 4 / 4 * 2.4e9
+    achieved = insts / (ms * 1e-3)
+    return {"valu_insts_per_step": insts, "achieved": achieved, "peak": peak, "unit": "wave-instr/s",
+            "frac": achieved / peak, "source": "profiles/forest_step_200k_valu_counters.md"}
 
 
 def qso_bench(delta, eta, shape, cell, steps, warmup, nslice=16):

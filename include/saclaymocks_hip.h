@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define SMK_ABI_VERSION 4
+#define SMK_ABI_VERSION 5
 
 #define SMK_OK 0
 #define SMK_ERR_ARG (-1)         /* bad argument (null pointer, size, enum) */
@@ -138,7 +138,7 @@ int smk_box_get_schedule(const smk_box_plan* plan);
 #define SMK_PASS_X_FWD 5
 #define SMK_PASS_X_INV 6
 #define SMK_PASS_COUNT 7
-#define SMK_BOX_NVAR 14
+#define SMK_BOX_NVAR 16
 
 /* SMK_BOX_NVAR of the library (for bindings that cannot read the header). */
 int smk_box_variant_count(void);
@@ -286,6 +286,34 @@ size_t smk_forest_workspace_bytes(int n_skewers);   /* class bucketing scratch *
 int smk_forest_missing_power(smk_forest_plan* plan, const smk_skewers* sk, const float* filt,
                              const int64_t* filt_off, uint64_t seed, const float* noise,
                              const int64_t* noise_off, void* workspace, float* delta_s, void* stream);
+
+/* The forest step of one batch in one persistent launch: for plane-parallel
+ * skewers, the device-drawn missing power (smk_forest_missing_power_sets,
+ * merge_spectra.py:303-322), trilinear delta and eta_zz along the line of
+ * sight (smk_forest_gather) and the FGPA (merge_spectra.py:335-339), with
+ * delta_s kept on chip.  Each block takes skewers from a work queue in the
+ * workspace (smk_forest_step_workspace_bytes(), cleared by the call).
+ *   filt, filt_off, set_stride, n_sets, filt_set, scale, seed: as
+ *               smk_forest_missing_power_sets (device-drawn modes only; the
+ *               plan's graph mode applies)
+ *   abcg        float32 [abcg_rows][4] FGPA rows, as smk_forest_gather
+ *   max_npix    an upper bound of skewers.npix (sizes the on-chip buffers);
+ *               a longer skewer is not processed (status bit 3, flux NaN)
+ *   flux        float32 per pixel (output); delta_l, eta_par, delta_s: NULL
+ *               or float32 per pixel (outputs, for checks)
+ *   status      int32 per skewer, zeroed by the caller: the bits of
+ *               smk_forest_gather, and bit 3 for a skewer not processed
+ * Box: box.delta and, if non-NULL, box.eta[2] (eta_zz); periodic or not,
+ * whole or an x slab.  Returns SMK_ERR_RANGE if max_npix needs nz > 8192 or,
+ * outside graph mode, if a filter set index is out of range (one host sync).
+ * Replaces, for the plane-parallel bench workload, the per-skewer loop of
+ * merge_spectra.py:282-339 over the spectra of make_spectra.py:429-522. */
+size_t smk_forest_step_workspace_bytes(void);
+int smk_forest_step(smk_forest_plan* plan, const smk_skewers* sk, const smk_box_view* box, const float* filt,
+                    const int64_t* filt_off, int64_t set_stride, int32_t n_sets, const int32_t* filt_set,
+                    const float* scale, uint64_t seed, const float* abcg, int64_t abcg_rows, int32_t max_npix,
+                    float* flux, float* delta_l, float* eta_par, float* delta_s, int32_t* status, void* workspace,
+                    void* stream);
 
 /* Missing power with a P1D_miss table chosen per skewer, merge_spectra.py:311-322
  * without --fit-p1d: P1D_miss(z, k) is the table at the redshift nearest the

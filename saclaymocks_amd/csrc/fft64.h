@@ -1,0 +1,260 @@
+// FP64 FFTs for the fit kernels on gfx950: a register-resident Stockham FFT
+// of M = 2^m points (P threads, E = M / P points each, LDS exchange between
+// radix stages) and, on top of it, Bluestein's chirp-z algorithm for a DFT
+// of any length N with 2N - 1 <= M:
+//   X_k = sum_n y_n exp(-2 pi i k n / N)
+//       = w_k sum_n (y_n w_n) conj(w_{k-n}),   w_n = exp(-i pi n^2 / N),
+// i.e. a circular convolution of length M done with three M-point FFTs
+// (L. Bluestein, IEEE Trans. Audio Electroacoust. 18, 451, 1970).
+// The chirp's phase n^2 mod 2N is formed exactly in integers, so w_n is
+// accurate to FP64 rounding for any n.
+//
+// This file is device-only; it is included by fit.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace smk {
+namespace f64 {
+
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
+}
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+// times -i (forward) or +i (inverse)
+template <bool INV> __device__ __forceinline__ double2 mul_mi(double2 a) {
+  return INV ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
+}
+
+template <int R, bool INV> struct DFT;
+template <bool INV> struct DFT<2, INV> {
+  static __device__ __forceinline__ void run(double2* a) {
+    const double2 t = a[0];
+    a[0] = cadd(t, a[1]);
+    a[1] = csub(t, a[1]);
+  }
+};
+template <bool INV> struct DFT<4, INV> {
+  static __device__ __forceinline__ void run(double2* a) {
+    const double2 t0 = cadd(a[0], a[2]), t1 = csub(a[0], a[2]);
+    const double2 t2 = cadd(a[1], a[3]), t3 = mul_mi<INV>(csub(a[1], a[3]));
+    a[0] = cadd(t0, t2);
+    a[2] = csub(t0, t2);
+    a[1] = cadd(t1, t3);
+    a[3] = csub(t1, t3);
+  }
+};
+template <bool INV> struct DFT<8, INV> {
+  static __device__ __forceinline__ void run(double2* a) {
+    const double r = 0.70710678118654752440084436210484904;
+    double2 e[4] = {a[0], a[2], a[4], a[6]};
+    double2 o[4] = {a[1], a[3], a[5], a[7]};
+    DFT<4, INV>::run(e);
+    DFT<4, INV>::run(o);
+    double2 o1, o3;
+    if (INV) {
+      o1 = make_double2(r * (o[1].x - o[1].y), r * (o[1].x + o[1].y));
+      o3 = make_double2(-r * (o[3].x + o[3].y), r * (o[3].x - o[3].y));
+    } else {
+      o1 = make_double2(r * (o[1].x + o[1].y), r * (o[1].y - o[1].x));
+      o3 = make_double2(r * (o[3].y - o[3].x), -r * (o[3].x + o[3].y));
+    }
+    const double2 o2 = mul_mi<INV>(o[2]);
+    a[0] = cadd(e[0], o[0]);
+    a[4] = csub(e[0], o[0]);
+    a[1] = cadd(e[1], o1);
+    a[5] = csub(e[1], o1);
+    a[2] = cadd(e[2], o2);
+    a[6] = csub(e[2], o2);
+    a[3] = cadd(e[3], o3);
+    a[7] = csub(e[3], o3);
+  }
+};
+// 16 = 4 x 4: four 4-point DFTs on stride-4 inputs, twiddles W16^(q r), four
+// 4-point DFTs across (six-step decomposition inside the registers)
+template <bool INV> struct DFT<16, INV> {
+  static __device__ __forceinline__ void run(double2* a) {
+    const double c1 = 0.92387953251128675612818318939678829;  // cos(pi/8)
+    const double s1 = 0.38268343236508977172845998403039887;  // sin(pi/8)
+    const double h = 0.70710678118654752440084436210484904;
+    double2 t[4][4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      double2 u[4] = {a[q], a[q + 4], a[q + 8], a[q + 12]};
+      DFT<4, INV>::run(u);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) t[q][r] = u[r];
+    }
+    // W16^(q r) for q, r in 1..3 (sign of the imaginary part by direction)
+    const double sg = INV ? 1.0 : -1.0;
+    const double2 w[4][4] = {
+        {{1, 0}, {1, 0}, {1, 0}, {1, 0}},
+        {{1, 0}, {c1, sg * s1}, {h, sg * h}, {s1, sg * c1}},
+        {{1, 0}, {h, sg * h}, {0, sg}, {-h, sg * h}},
+        {{1, 0}, {s1, sg * c1}, {-h, sg * h}, {-c1, -sg * s1}}};
+#pragma unroll
+    for (int q = 1; q < 4; ++q)
+#pragma unroll
+      for (int r = 1; r < 4; ++r) t[q][r] = cmul(t[q][r], w[q][r]);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      double2 u[4] = {t[0][r], t[1][r], t[2][r], t[3][r]};
+      DFT<4, INV>::run(u);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) a[r + 4 * q] = u[q];
+    }
+  }
+};
+
+// radix of stage S of an M-point FFT with E points per thread: the largest
+// of 16, 8, 4, 2 dividing both E and what remains of M
+__host__ __device__ constexpr int stage_radix(int M, int E, int S) {
+  int rem = M;
+  int r = 0;
+  for (int s = 0; s <= S; ++s) {
+    r = 0;
+    const int cand[4] = {16, 8, 4, 2};
+    for (int i = 0; i < 4; ++i)
+      if (E % cand[i] == 0 && rem % cand[i] == 0) { r = cand[i]; break; }
+    if (r == 0) return 0;
+    rem /= r;
+  }
+  return r;
+}
+__host__ __device__ constexpr int n_stages(int M, int E) {
+  int rem = M, s = 0;
+  while (rem > 1) {
+    const int r = stage_radix(M, E, s);
+    if (r == 0) return -1;
+    rem /= r;
+    ++s;
+  }
+  return s;
+}
+// LDS slot of point `pos`: one pad slot per 32 points (double2 = 4 banks)
+__device__ __forceinline__ int slot(int pos) { return pos + (pos >> 5); }
+__host__ __device__ constexpr int line_len(int M) { return M + M / 32 + 1; }
+
+// Stockham stage: butterfly j = p + P b takes points j + r M/R (register
+// slots b + r B), twiddles W_M^(r (j % Ns) M/(Ns R)), outputs to points
+// (j / Ns) Ns R + j % Ns + r Ns (the last stage keeps them in registers:
+// slot b + r B holds point j + r M/R).  twM: W_TW^k, k < TW, TW >= M.
+template <int M, int E, int R, int Ns, bool INV, bool LAST>
+__device__ __forceinline__ void stage(double2 (&v)[E], double2* __restrict__ line, int p,
+                                      const double2* __restrict__ twM, int TW) {
+  constexpr int P = M / E;
+  constexpr int B = E / R;
+  const int tws = TW / M;  // stride of W_M in the table
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
+    const int j = p + P * b;
+    double2 a[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) a[r] = v[b + r * B];
+    if (Ns > 1) {
+      const int jj = j & (Ns - 1);
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        double2 w = twM[(int64_t)r * jj * (M / (Ns * R)) * tws];
+        if (INV) w.y = -w.y;
+        a[r] = cmul(a[r], w);
+      }
+    }
+    DFT<R, INV>::run(a);
+    if (LAST) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b + r * B] = a[r];
+    } else {
+      const int d0 = (j / Ns) * Ns * R + (j & (Ns - 1));
+#pragma unroll
+      for (int r = 0; r < R; ++r) line[slot(d0 + r * Ns)] = a[r];
+    }
+  }
+}
+
+template <int M, int E, bool INV, int S, int Ns>
+__device__ __forceinline__ void stages(double2 (&v)[E], double2* __restrict__ line, int p,
+                                       const double2* __restrict__ twM, int TW) {
+  constexpr int NS = n_stages(M, E);
+  static_assert(NS > 0, "unsupported FP64 FFT size");
+  constexpr int P = M / E;
+  if constexpr (S < NS) {
+    constexpr int R = stage_radix(M, E, S);
+    constexpr bool LAST = S + 1 == NS;
+    stage<M, E, R, Ns, INV, LAST>(v, line, p, twM, TW);
+    if constexpr (!LAST) {
+      __syncthreads();
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = line[slot(p + P * e)];
+      __syncthreads();
+      stages<M, E, INV, S + 1, Ns * R>(v, line, p, twM, TW);
+    }
+  }
+}
+
+// In-register FFT of one M-point line over the M / E threads p of the block
+// (every thread of the block must call it: it holds block barriers).  On
+// entry and exit thread p holds points p + P e.  Unnormalised.
+template <int M, int E, bool INV>
+__device__ __forceinline__ void fft(double2 (&v)[E], double2* __restrict__ line, int p,
+                                    const double2* __restrict__ twM, int TW) {
+  stages<M, E, INV, 0, 1>(v, line, p, twM, TW);
+}
+
+// w_n = exp(-i pi n^2 / N), n^2 mod 2N exact (n < 2^31 / N)
+__device__ __forceinline__ double2 chirp(int64_t n, int N) {
+  const int64_t q = (n * n) % (2 * (int64_t)N);
+  double s, c;
+  sincospi(-(double)q / (double)N, &s, &c);
+  return make_double2(c, s);
+}
+
+// Bluestein DFT of length N (2N - 1 <= M) over a block of M / E threads:
+// in(n) gives y_n for n < N (called once per n by one thread); emit(k, X_k)
+// is called for k < nout (nout <= N) by the thread that holds it.  `line`
+// is M-point LDS scratch (line_len(M) double2), `ch` N-point LDS scratch for
+// the chirp; every thread of the block must call it.
+template <int M, int E, class In, class Emit>
+__device__ __forceinline__ void bluestein(int N, int nout, In in, Emit emit, double2* __restrict__ line,
+                                          double2* __restrict__ ch, const double2* __restrict__ twM, int TW) {
+  constexpr int P = M / E;
+  const int p = threadIdx.x;
+  for (int n = p; n < N; n += P) ch[n] = chirp(n, N);
+  __syncthreads();
+  double2 vb[E], va[E];
+  // B = FFT(b), b_m = conj(w_m) for |m| < N (circularly), 0 elsewhere
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int m = p + P * e;
+    const int mm = m < N ? m : (m > M - N ? M - m : -1);
+    vb[e] = mm >= 0 ? cconj(ch[mm]) : make_double2(0.0, 0.0);
+  }
+  fft<M, E, false>(vb, line, p, twM, TW);
+  __syncthreads();
+  // A = FFT(a), a_n = y_n w_n for n < N
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int n = p + P * e;
+    va[e] = n < N ? cmul(in(n), ch[n]) : make_double2(0.0, 0.0);
+  }
+  fft<M, E, false>(va, line, p, twM, TW);
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < E; ++e) va[e] = cmul(va[e], vb[e]);
+  fft<M, E, true>(va, line, p, twM, TW);
+  const double inv = 1.0 / (double)M;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int k = p + P * e;
+    if (k < nout) {
+      const double2 c = cmul(va[e], ch[k]);
+      emit(k, make_double2(c.x * inv, c.y * inv));
+    }
+  }
+  __syncthreads();
+}
+
+}  // namespace f64
+}  // namespace smk

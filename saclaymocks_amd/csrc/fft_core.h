@@ -139,6 +139,63 @@ template <bool INV> struct DFT<8, INV> {
   }
 };
 
+// ------------------------------------------------ compile-time unit roots
+// exp(-2 pi i num / den) in double at compile time (argument reduced to
+// [-pi/4, pi/4], Taylor series to 1e-17), rounded once to float: the
+// rotation constants of fft_stage's class-2 butterflies.
+namespace detail {
+constexpr double kPi = 3.14159265358979323846264338327950288;
+constexpr double taylor_sin(double x) {
+  double term = x, sum = x;
+  for (int i = 1; i < 14; ++i) {
+    term *= -x * x / ((2.0 * i) * (2.0 * i + 1.0));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double taylor_cos(double x) {
+  double term = 1.0, sum = 1.0;
+  for (int i = 1; i < 14; ++i) {
+    term *= -x * x / ((2.0 * i - 1.0) * (2.0 * i));
+    sum += term;
+  }
+  return sum;
+}
+// cos and sin of 2 pi m / d for integers m, d > 0, via the octant of m / d
+constexpr void sincos_frac(long m, long d, double& c, double& s) {
+  m %= d;
+  if (m < 0) m += d;
+  // q = the octant: angle = 2 pi m / d = q pi/4 + r with |r| <= pi/8
+  const long q = (8 * m + d / 2) / d;
+  const double r = 2.0 * kPi * ((double)m / (double)d) - (double)q * (kPi / 4.0);
+  const double cr = taylor_cos(r), sr = taylor_sin(r);
+  const double h = 0.70710678118654752440084436210484904;
+  double cq = 1.0, sq = 0.0;
+  switch (q % 8) {
+    case 0: cq = 1.0; sq = 0.0; break;
+    case 1: cq = h; sq = h; break;
+    case 2: cq = 0.0; sq = 1.0; break;
+    case 3: cq = -h; sq = h; break;
+    case 4: cq = -1.0; sq = 0.0; break;
+    case 5: cq = -h; sq = -h; break;
+    case 6: cq = 0.0; sq = -1.0; break;
+    default: cq = h; sq = -h; break;
+  }
+  c = cq * cr - sq * sr;
+  s = sq * cr + cq * sr;
+}
+}  // namespace detail
+
+struct CF {
+  float x, y;
+};
+// W_den^num = exp(-2 pi i num / den)
+constexpr CF unit_root(long num, long den) {
+  double c = 0.0, s = 0.0;
+  detail::sincos_frac(num, den, c, s);
+  return CF{(float)c, (float)-s};
+}
+
 // --------------------------------------------------------- compile-time plan
 // Every radix must divide E so that each thread owns a whole number of
 // butterflies in every stage.  Power-of-two radices come first (largest
@@ -147,14 +204,15 @@ template <bool INV> struct DFT<8, INV> {
 // compile-time offsets (see fft_stage).
 struct RadixPlan {
   int n, e, p, nstage;
-  int r[8];
+  int r[16];
 };
 
+// nstage = -1 when n does not factor into radices dividing e within 16 stages
 __host__ __device__ constexpr RadixPlan make_plan(int n, int e) {
-  RadixPlan pl{n, e, n / e, 0, {0, 0, 0, 0, 0, 0, 0, 0}};
+  RadixPlan pl{n, e, n / e, 0, {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
   int rem = n;
   const int cand[5] = {8, 4, 2, 5, 3};
-  while (rem > 1 && pl.nstage < 8) {
+  while (rem > 1 && pl.nstage < 16) {
     int pick = 0;
     for (int i = 0; i < 5; ++i) {
       if (e % cand[i] == 0 && rem % cand[i] == 0) { pick = cand[i]; break; }
@@ -163,12 +221,13 @@ __host__ __device__ constexpr RadixPlan make_plan(int n, int e) {
     pl.r[pl.nstage++] = pick;
     rem /= pick;
   }
+  if (rem > 1) pl.nstage = -1;
   return pl;
 }
 
-// Elements per thread for each supported line length (chosen to keep the
-// number of LDS exchanges at 2-4 and the register tile at <= 40 points).
-__host__ __device__ constexpr int elems_per_thread(int n) {
+// Register tile (elements per thread) of the lengths whose box passes were
+// measured and tuned (2 to 4 LDS exchanges, at most 40 points per thread).
+__host__ __device__ constexpr int tuned_elems(int n) {
   switch (n) {
     case 8: return 8;
     case 12: return 12;
@@ -198,6 +257,48 @@ __host__ __device__ constexpr int elems_per_thread(int n) {
     case 4096: return 16;
     default: return 0;
   }
+}
+
+__host__ __device__ constexpr bool smooth235(int n) {
+  if (n < 1) return false;
+  while (n % 2 == 0) n /= 2;
+  while (n % 3 == 0) n /= 3;
+  while (n % 5 == 0) n /= 5;
+  return n == 1;
+}
+
+// Any other 2/3/5-smooth length n <= 4096: E = (3 if 3 | n) (5 if 5 | n)
+// 2^k, so that every radix of the plan divides E; among those with a plan,
+// prefer 32 <= n/E <= 256 threads per line, then E <= 40, then the fewest
+// stages, then the largest E.
+__host__ __device__ constexpr int general_elems(int n) {
+  const int odd = (n % 3 == 0 ? 3 : 1) * (n % 5 == 0 ? 5 : 1);
+  int best = 0, bs0 = 9, bs1 = 9, bs2 = 99;
+  for (int p2 = 1; p2 <= 64; p2 *= 2) {
+    const int e = odd * p2;
+    if (n % e != 0 || e > 64) continue;
+    const RadixPlan pl = make_plan(n, e);
+    if (pl.nstage <= 0) continue;
+    const int P = n / e;
+    const int s0 = (P >= 32 && P <= 256) ? 0 : 1, s1 = e <= 40 ? 0 : 1, s2 = pl.nstage;
+    const bool better = s0 < bs0 || (s0 == bs0 && (s1 < bs1 || (s1 == bs1 && (s2 < bs2 || (s2 == bs2 && e > best)))));
+    if (best == 0 || better) {
+      best = e;
+      bs0 = s0;
+      bs1 = s1;
+      bs2 = s2;
+    }
+  }
+  return best;
+}
+
+// Elements per thread for every supported line length (0: unsupported):
+// the tuned tile, or the general rule for the other 2/3/5-smooth lengths
+// from 8 to 4096.
+__host__ __device__ constexpr int elems_per_thread(int n) {
+  const int t = tuned_elems(n);
+  if (t) return t;
+  return (n >= 8 && n <= 4096 && smooth235(n)) ? general_elems(n) : 0;
 }
 
 // LDS line stride (in float2): one pad slot every 32 points breaks the
@@ -265,6 +366,21 @@ template <int N, int P, int R, int Ns> struct StageIdx {
     return KIND == 2 ? ((P * b) / Ns) * Ns * R + (P * b) % Ns : b * P * R;
   }
   static constexpr int offJ(int b) { return KIND == 2 ? (P * b) % Ns : 0; }
+};
+
+// Rotation constants of a class-2 stage: c[b][r] = W_N^(r offJ(b) step).
+template <int N, int P, int R, int Ns, int E> struct RotTab {
+  struct Tab {
+    CF c[E / R][R];
+  };
+  static constexpr Tab make() {
+    Tab t{};
+    using I = StageIdx<N, P, R, Ns>;
+    for (int b = 0; b < E / R; ++b)
+      for (int r = 0; r < R; ++r) t.c[b][r] = unit_root((long)r * I::offJ(b) * I::step, N);
+    return t;
+  }
+  static constexpr Tab tab = make();
 };
 
 // LDS slots of a stage's outputs: output r of butterfly b goes to point
@@ -352,8 +468,18 @@ __device__ __forceinline__ void fft_stage(float2 (&v)[E], float2* __restrict__ l
       if (b == 0) {
 #pragma unroll
         for (int r = 1; r < R; ++r) a[r] = cmul(a[r], w0[r]);
+      } else if constexpr (I::KIND == 2) {
+        // W^(r (p + offJ(b)) step) = w0[r] * W_N^(r offJ(b) step): butterfly
+        // 0's twiddles rotated by a compile-time constant (no per-butterfly
+        // loads, whose results the compiler would keep live in registers)
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          const CF c = RotTab<N, P, R, Ns, E>::tab.c[b][r];
+          const float2 cw = make_float2(c.x, INV ? -c.y : c.y);
+          a[r] = cmul(cmul(a[r], cw), w0[r]);
+        }
       } else {
-        const uint32_t jb = I::KIND == 2 ? p + (uint32_t)I::offJ(b) : (p + (uint32_t)(P * b)) % Ns;
+        const uint32_t jb = (p + (uint32_t)(P * b)) % Ns;
 #pragma unroll
         for (int r = 1; r < R; ++r) {
           float2 w = tw[r * jb * I::step];

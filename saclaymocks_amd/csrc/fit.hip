@@ -4,13 +4,19 @@
 // chi2), powerspectrum.py:204-238 (P1D_1spectrum, ComputeP1D) and
 // util.py:29-37,46-75 (regroup, MakeProfileHisto).
 //
-// One block per skewer (grid-stride).  The P1D of a forest of arbitrary
-// length N is a direct DFT evaluated in FP64 by Horner's rule (N
-// multiply-adds per mode, one mode per thread), so any N is exact to FP64
-// rounding, as numpy's rfft is.  All reductions are FP64 and deterministic
-// (fixed thread-to-data assignment, fixed tree order).
+// One block per skewer (grid-stride).  The P1D of a forest of N <= 2048
+// points is the DFT at that exact length by Bluestein's algorithm on FP64
+// power-of-two FFTs (fft64.h, O(N log N)); longer forests take a direct DFT
+// evaluated in FP64 by Horner's rule (N multiply-adds per mode, one mode per
+// thread).  Either is exact to FP64 rounding at any N, as numpy's rfft is.
+// All reductions are FP64 and deterministic (fixed thread-to-data
+// assignment, fixed tree order).
 #include <math.h>
 
+#include <mutex>
+#include <vector>
+
+#include "fft64.h"
 #include "fft_core.h"
 #include "smk_internal.h"
 
@@ -18,12 +24,51 @@ using namespace smk;
 
 namespace {
 
-constexpr int FIT_THREADS = 256;
+constexpr int FIT_THREADS = 512;
 constexpr int FIT_NMAX = 8192;  // longest (regrouped) forest handled by the P1D kernels
+constexpr int BLUE_NMAX = 2048; // longest forest of the Bluestein path (M <= 4096)
+constexpr int BLUE_MMAX = 4096;
 
-// dynamic LDS of the P1D kernels for forests of at most nmax (even) points
-size_t p1d_lds_bytes(int nmax) { return (size_t)nmax * 8 + (size_t)(nmax / 2 + 1) * 12; }
-size_t adj_lds_bytes(int nmax) { return (size_t)nmax * 8 + (size_t)(nmax / 2 + 1) * 16; }
+// dynamic LDS of the P1D kernels for forests of at most nmax (even) points:
+// x, then the binning arrays (forward) or the adjoint's coefficients, with
+// the Bluestein FFT line aliased over them
+int blue_m(int nmax) {
+  const int n = nmax < BLUE_NMAX ? nmax : BLUE_NMAX;
+  int M = 1024;
+  while (M < 2 * n - 1) M *= 2;
+  return M;
+}
+size_t blue_line_bytes(int nmax) { return (size_t)f64::line_len(blue_m(nmax)) * 16; }
+size_t blue_chirp_bytes(int nmax) { return (size_t)(nmax < BLUE_NMAX ? nmax : BLUE_NMAX) * 16; }
+size_t align16(size_t b) { return (b + 15) & ~(size_t)15; }
+// forward: [x (Horner) or the Bluestein line][pk][bins][chirp]
+size_t p1d_x_bytes(int nmax) {
+  const size_t a = (size_t)nmax * 8, b = blue_line_bytes(nmax);
+  return align16(a > b ? a : b);
+}
+size_t p1d_chirp_off(int nmax) { return align16(p1d_x_bytes(nmax) + (size_t)(nmax / 2 + 1) * 12); }
+size_t p1d_lds_bytes(int nmax) { return p1d_chirp_off(nmax) + blue_chirp_bytes(nmax); }
+// adjoint: [x][cj][Bluestein line][chirp]
+size_t adj_line_off(int nmax) { return align16((size_t)nmax * 8 + (size_t)(nmax / 2 + 1) * 16); }
+size_t adj_lds_bytes(int nmax) { return adj_line_off(nmax) + blue_line_bytes(nmax) + blue_chirp_bytes(nmax); }
+
+// one transform size per function, so that each gets its own register
+// allocation instead of the union of all of them
+template <int M, class In, class Emit>
+__device__ __noinline__ void bluestein_at(int N, int nout, In in, Emit emit, double2* line, double2* ch,
+                                          const double2* tw) {
+  f64::bluestein<M, M / FIT_THREADS>(N, nout, in, emit, line, ch, tw, BLUE_MMAX);
+}
+
+// Bluestein DFT of length N <= BLUE_NMAX with M the smallest power of two
+// >= max(2N - 1, 1024): X_k for k < nout via emit(k, X_k)
+template <class In, class Emit>
+__device__ __forceinline__ void dft_bluestein(int N, int nout, In in, Emit emit, double2* line, double2* ch,
+                                              const double2* tw) {
+  if (2 * N - 1 <= 1024) bluestein_at<1024>(N, nout, in, emit, line, ch, tw);
+  else if (2 * N - 1 <= 2048) bluestein_at<2048>(N, nout, in, emit, line, ch, tw);
+  else bluestein_at<4096>(N, nout, in, emit, line, ch, tw);
+}
 
 __device__ __forceinline__ double block_reduce_sum(double v, double* red) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -186,13 +231,18 @@ struct P1dArgs {
   int nmax;
   double2* xj;           // NULL or the modes X_j of every skewer, at xj + xoff[s]
   const int64_t* xoff;
+  const double2* tw;     // W_4096^k, k < 4096 (FP64), for the Bluestein FFTs
+  size_t x_bytes;        // p1d_x_bytes(nmax)
+  size_t chirp_off;      // p1d_chirp_off(nmax)
 };
 
 __global__ __launch_bounds__(FIT_THREADS) void k_p1d(P1dArgs a) {
   extern __shared__ double dyn[];
-  double* x = dyn;
-  double* pk = dyn + a.nmax;
+  double* x = dyn;                                    // Horner input
+  double2* bline = reinterpret_cast<double2*>(dyn);   // Bluestein FFT line (aliases x)
+  double* pk = reinterpret_cast<double*>(reinterpret_cast<char*>(dyn) + a.x_bytes);
   int* bins = reinterpret_cast<int*>(pk + a.nmax / 2 + 1);
+  double2* ch = reinterpret_cast<double2*>(reinterpret_cast<char*>(dyn) + a.chirp_off);
   const smk_skewers& sk = a.sk;
   for (int s = blockIdx.x; s < sk.n; s += gridDim.x) {
     double* part = a.partial + (int64_t)s * a.nb * 4;
@@ -205,18 +255,37 @@ __global__ __launch_bounds__(FIT_THREADS) void k_p1d(P1dArgs a) {
       continue;
     }
     const double inv = 1.0 / a.fmean[sk.pset[s]];
-    load_delta_f(a.flux, sk.off[s], i0, i1, a.regroup, inv, x);
-    __syncthreads();
     const double px = a.pixel * a.regroup;
     const double k_ny = M_PI / px;
     const int nj = nq > 0 ? nq / 2 + 1 : 0;
     double2* xj = a.xj ? a.xj + a.xoff[s] : nullptr;
-    dft_all<false>(x, nullptr, nq, nq, nj, [&](int j, double re, double im) {
-      pk[j] = (re * re + im * im) * px / nq;
-      bins[j] = find_bin(a.edges, a.nb, kfreq(j, nq, k_ny));
-      if (xj) xj[j] = make_double2(re, im);
-    });
-    __syncthreads();
+    if (nq <= BLUE_NMAX && nq > 0) {
+      // delta_F read straight from the flux (regrouped), the DFT by Bluestein
+      const int64_t o = sk.off[s] + i0;
+      const int m = a.regroup;
+      dft_bluestein(nq, nj,
+                    [&](int q) {
+                      double acc = 0.0;
+                      for (int r = 0; r < m; ++r) acc += a.flux[o + (int64_t)q * m + r];
+                      return make_double2((acc / m) * inv - 1.0, 0.0);
+                    },
+                    [&](int j, double2 X) {
+                      pk[j] = (X.x * X.x + X.y * X.y) * px / nq;
+                      bins[j] = find_bin(a.edges, a.nb, kfreq(j, nq, k_ny));
+                      if (xj) xj[j] = X;
+                    },
+                    bline, ch, a.tw);
+      __syncthreads();
+    } else {
+      load_delta_f(a.flux, sk.off[s], i0, i1, a.regroup, inv, x);
+      __syncthreads();
+      dft_all<false>(x, nullptr, nq, nq, nj, [&](int j, double re, double im) {
+        pk[j] = (re * re + im * im) * px / nq;
+        bins[j] = find_bin(a.edges, a.nb, kfreq(j, nq, k_ny));
+        if (xj) xj[j] = make_double2(re, im);
+      });
+      __syncthreads();
+    }
     for (int b = threadIdx.x; b < a.nb; b += FIT_THREADS) {
       double c = 0.0, sk_ = 0.0, sp = 0.0, sp2 = 0.0;
       for (int j = 0; j < nj; ++j) {
@@ -252,6 +321,9 @@ struct AdjArgs {
   int nmax;
   const double2* xj;     // NULL: recompute the modes; else those saved by k_p1d
   const int64_t* xoff;
+  const double2* tw;     // W_4096^k (FP64), for the Bluestein FFTs
+  size_t line_off;       // adj_line_off(nmax): byte offset of the Bluestein line
+  int line_m;            // its length (the largest M of the batch)
 };
 
 // Adjoint of the binned P1D through delta_F = F/<F> - 1 (at fixed <F>) and
@@ -260,6 +332,9 @@ __global__ __launch_bounds__(FIT_THREADS) void k_p1d_adjoint(AdjArgs a) {
   extern __shared__ double dyn[];
   double* x = dyn;
   double2* cj = reinterpret_cast<double2*>(dyn + a.nmax);
+  double2* bline = reinterpret_cast<double2*>(reinterpret_cast<char*>(dyn) + a.line_off);
+  double2* ch = reinterpret_cast<double2*>(reinterpret_cast<char*>(dyn) + a.line_off) +
+                f64::line_len(a.line_m);
   __shared__ double red[FIT_THREADS / 64];
   const smk_skewers& sk = a.sk;
   for (int s = blockIdx.x; s < sk.n; s += gridDim.x) {
@@ -285,6 +360,7 @@ __global__ __launch_bounds__(FIT_THREADS) void k_p1d_adjoint(AdjArgs a) {
       const int b = find_bin(a.edges, a.nb, kfreq(j, nq, k_ny));
       return b >= 0 ? gps[b] * 2.0 * px / nq : 0.0;
     };
+    const bool blue = nq > 0 && nq <= BLUE_NMAX;
     if (a.xj) {
       const double2* xs = a.xj + a.xoff[s];
       for (int j = threadIdx.x; j < nj; j += FIT_THREADS) {
@@ -292,6 +368,13 @@ __global__ __launch_bounds__(FIT_THREADS) void k_p1d_adjoint(AdjArgs a) {
         const double2 X = xs[j];
         cj[j] = make_double2(g * X.x, -g * X.y);   // g conj(X_j)
       }
+    } else if (blue) {
+      dft_bluestein(nq, nj, [&](int n) { return make_double2(x[n], 0.0); },
+                    [&](int j, double2 X) {
+                      const double g = coef(j);
+                      cj[j] = make_double2(g * X.x, -g * X.y);
+                    },
+                    bline, ch, a.tw);
     } else {
       dft_all<false>(x, nullptr, nq, nq, nj, [&](int j, double re, double im) {
         const double g = coef(j);
@@ -299,14 +382,23 @@ __global__ __launch_bounds__(FIT_THREADS) void k_p1d_adjoint(AdjArgs a) {
       });
     }
     __syncthreads();
-    // A_q = Re sum_j cj[j] W^{jq}: Horner over j with w = W^q, q < nq.  The
-    // thread that owns q accumulates A_q F'_q and then stores A_q over x[q]
-    // (x is not read by any other thread from here on).
+    // A_q = Re sum_j cj[j] W^{jq}, q < nq (a DFT of length nq of cj padded
+    // with zeros).  The thread that owns q accumulates A_q F'_q and then
+    // stores A_q over x[q] (x is not read by any other thread from here on).
     double saf = 0.0;
-    dft_all<true>(nullptr, cj, nj, nq, nq, [&](int q, double re, double im) {
-      saf += re * (x[q] + 1.0) * fm;
-      x[q] = re;
-    });
+    if (blue) {
+      dft_bluestein(nq, nq, [&](int n) { return n < nj ? cj[n] : make_double2(0.0, 0.0); },
+                    [&](int q, double2 X) {
+                      saf += X.x * (x[q] + 1.0) * fm;
+                      x[q] = X.x;
+                    },
+                    bline, ch, a.tw);
+    } else {
+      dft_all<true>(nullptr, cj, nj, nq, nq, [&](int q, double re, double im) {
+        saf += re * (x[q] + 1.0) * fm;
+        x[q] = re;
+      });
+    }
     __syncthreads();
     const double* A = x;
     const double4 p = prow(a.abcg, a.rows, set);
@@ -339,6 +431,33 @@ __global__ __launch_bounds__(FIT_THREADS) void k_p1d_adjoint(AdjArgs a) {
 }
 
 int grid_for(int n) { return n < 65536 ? (n > 0 ? n : 1) : 65536; }
+
+// W_4096^k, k < 4096, FP64, one table per device (built on first use)
+const double2* fp64_twiddles(hipError_t* err) {
+  static std::mutex mu;
+  static const double2* tabs[64] = {};
+  int dev = 0;
+  *err = hipGetDevice(&dev);
+  if (*err != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+  std::lock_guard<std::mutex> lk(mu);
+  if (tabs[dev] == nullptr) {
+    std::vector<double2> t(BLUE_MMAX);
+    for (int k = 0; k < BLUE_MMAX; ++k) {
+      const double ang = -2.0 * M_PI * (double)k / (double)BLUE_MMAX;
+      t[k] = make_double2(cos(ang), sin(ang));
+    }
+    void* d = nullptr;
+    *err = hipMalloc(&d, BLUE_MMAX * sizeof(double2));
+    if (*err != hipSuccess) return nullptr;
+    *err = hipMemcpy(d, t.data(), BLUE_MMAX * sizeof(double2), hipMemcpyHostToDevice);
+    if (*err != hipSuccess) {
+      (void)hipFree(d);
+      return nullptr;
+    }
+    tabs[dev] = (const double2*)d;
+  }
+  return tabs[dev];
+}
 
 }  // namespace
 
@@ -378,7 +497,11 @@ extern "C" int smk_p1d_binned(const smk_skewers* sk, const double* flux, const d
   nmax += nmax & 1;
   if (nmax < 2) nmax = 2;
   SMK_REQUIRE(modes == nullptr || modes_off != nullptr, SMK_ERR_ARG, "modes need modes_off");
-  P1dArgs a{*sk, flux, fmean, pixel, regroup, edges, nbins, partial, status, nmax, (double2*)modes, modes_off};
+  hipError_t te = hipSuccess;
+  const double2* tw = fp64_twiddles(&te);
+  SMK_REQUIRE(tw != nullptr, SMK_ERR_HIP, "FP64 twiddle table: %s", hipGetErrorString(te));
+  P1dArgs a{*sk, flux, fmean, pixel, regroup, edges, nbins, partial, status, nmax, (double2*)modes, modes_off,
+            tw, p1d_x_bytes(nmax), p1d_chirp_off(nmax)};
   const size_t lds = p1d_lds_bytes(nmax);
   SMK_HIP_CHECK(hipFuncSetAttribute((const void*)k_p1d, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   hipLaunchKernelGGL(k_p1d, dim3(grid_for(sk->n)), dim3(FIT_THREADS), lds, (hipStream_t)stream, a);
@@ -402,8 +525,11 @@ extern "C" int smk_p1d_adjoint(const smk_skewers* sk, const float* delta, const 
   nmax += nmax & 1;
   if (nmax < 2) nmax = 2;
   SMK_REQUIRE(modes == nullptr || modes_off != nullptr, SMK_ERR_ARG, "modes need modes_off");
+  hipError_t te = hipSuccess;
+  const double2* tw = fp64_twiddles(&te);
+  SMK_REQUIRE(tw != nullptr, SMK_ERR_HIP, "FP64 twiddle table: %s", hipGetErrorString(te));
   AdjArgs a{*sk,  delta, eta,   abcg, abcg_rows, flux, fmean, pixel, regroup, edges,
-            nbins, gp,   out, nmax, (const double2*)modes, modes_off};
+            nbins, gp,   out, nmax, (const double2*)modes, modes_off, tw, adj_line_off(nmax), blue_m(nmax)};
   const size_t lds = adj_lds_bytes(nmax);
   SMK_HIP_CHECK(
       hipFuncSetAttribute((const void*)k_p1d_adjoint, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

@@ -421,16 +421,24 @@ __global__ __launch_bounds__(256) void k_gather_plane_tri(GatherArgs a) {
 }
 
 // ------------------------------------------------------------ missing power
-__device__ __forceinline__ float2 mp_mode(uint64_t seed, uint64_t id, int k, int M, const float* __restrict__ filt) {
-  // Fourier mode k of real N(0,1) white noise of length 2M: interior modes
-  // have Re, Im ~ N(0, M); k = 0 and k = M are real with variance 2M
-  const U4 u = philox4x32_10_c4((uint32_t)k, (uint32_t)id, (uint32_t)(id >> 32), 0x6d697373u, seed);
-  float n0, n1;
-  box_muller(u.x, u.y, n0, n1);
-  const float f = filt[k];
+// Fourier modes of real N(0,1) white noise of length 2M: interior modes have
+// Re, Im ~ N(0, M); k = 0 and k = M are real with variance 2M.  Modes 2j and
+// 2j + 1 share one Philox call (counter j): words 0, 1 give mode 2j, words
+// 2, 3 mode 2j + 1, each through Box-Muller.
+constexpr uint32_t MP_TAG = 0x6d697373u;
+
+__device__ __forceinline__ float2 mp_scale_mode(float n0, float n1, int k, int M, float f) {
   if (k == 0 || k == M) return make_float2(n0 * sqrtf(2.0f * M) * f, 0.0f);
   const float sm = sqrtf((float)M) * f;
   return make_float2(n0 * sm, n1 * sm);
+}
+
+__device__ __forceinline__ float2 mp_mode(uint64_t seed, uint64_t id, int k, int M, const float* __restrict__ filt) {
+  const U4 u = philox4x32_10_c4((uint32_t)k >> 1, (uint32_t)id, (uint32_t)(id >> 32), MP_TAG, seed);
+  float n0, n1;
+  if (k & 1) box_muller_hw(u.z, u.w, n0, n1);
+  else box_muller_hw(u.x, u.y, n0, n1);
+  return mp_scale_mode(n0, n1, k, M, filt[k]);
 }
 
 template <int M, int T>
@@ -558,9 +566,266 @@ __global__ void k_classify(smk_skewers sk, const int32_t* __restrict__ fset, int
   }
 }
 
+// ------------------------------------------------------ fused forest step
+// One persistent launch for a batch of plane-parallel skewers: each block
+// takes skewers from a device work queue and, per skewer, (1) draws the
+// missing-power Fourier modes into LDS (one Philox call per two modes),
+// (2) folds them for the C2R transform and inverse-FFTs them in LDS, so that
+// delta_s stays in LDS, (3) interpolates delta and eta_zz trilinearly along
+// the line of sight and applies the FGPA.  The x and y terms of the
+// interpolation (cells, weights, the four z columns) are computed once per
+// skewer; along z the position advances by dr/D per pixel from one division
+// per skewer.  Float32 weights and sums (the result is stored as float32).
+constexpr int STEP_NT = 256;
+
+struct StepArgs {
+  smk_skewers sk;
+  smk_box_view box;
+  const float* filt;
+  int64_t foff[N_CLASS];
+  const int32_t* fset;
+  int64_t set_stride;
+  int32_t n_sets;
+  const float* scale;
+  const float* abcg;
+  int64_t abcg_rows;
+  uint64_t seed;
+  const uint64_t* seed_dev;
+  const float2* tw[N_CLASS];
+  const float2* twr[N_CLASS];
+  float* flux;
+  float* delta_l;
+  float* eta_par;
+  float* delta_s;
+  int32_t* status;
+  int32_t* queue;
+  int32_t max_class;
+  double hx, hy, hz;
+};
+
+// register tile of the fused missing-power FFT of M points: all STEP_NT
+// threads on one line from M = 512 up; shorter lines are transformed by
+// STEP_NT / (M / E) thread groups, each on its own LDS line
+constexpr int step_E(int M) { return M >= 512 ? M / STEP_NT : 2; }
+constexpr int step_groups(int M) { return STEP_NT / (M / step_E(M)); }
+constexpr int step_lds_float2(int M) {
+  const int lines = step_groups(M) * lds_line_len(M);
+  return lines > M + 1 ? lines : M + 1;
+}
+constexpr int step_lds_max(int c) {
+  int m = 0;
+  for (int i = 0; i <= c; ++i) {
+    const int v = step_lds_float2(128 << i);
+    m = v > m ? v : m;
+  }
+  return m;
+}
+
+// delta_s of skewer s (n pixels) into dsf[0, n) (LDS floats); length class
+// C, M = 128 << C
+template <int C>
+__device__ __forceinline__ void step_missing_power(const StepArgs& a, int s, int n, uint64_t seed,
+                                                   float2* __restrict__ sh) {
+  constexpr int M = 128 << C;
+  constexpr int E = step_E(M);
+  constexpr int P = M / E;
+  constexpr int LL = lds_line_len(M);
+  static_assert(P * step_groups(M) == STEP_NT, "thread groups must tile the block");
+  const int tid = threadIdx.x;
+  const float* __restrict__ filt = a.filt + a.foff[C] + (a.fset ? (int64_t)a.fset[s] * a.set_stride : 0);
+  const uint64_t id = a.sk.noise_id ? a.sk.noise_id[s] : (uint64_t)s;
+  // (1) modes k = 0..M, two per Philox call
+  for (int j = tid; 2 * j <= M; j += STEP_NT) {
+    const U4 u = philox4x32_10_c4((uint32_t)j, (uint32_t)id, (uint32_t)(id >> 32), MP_TAG, seed);
+    float n0, n1;
+    box_muller_hw(u.x, u.y, n0, n1);
+    sh[2 * j] = mp_scale_mode(n0, n1, 2 * j, M, filt[2 * j]);
+    if (2 * j + 1 <= M) {
+      box_muller_hw(u.z, u.w, n0, n1);
+      sh[2 * j + 1] = mp_scale_mode(n0, n1, 2 * j + 1, M, filt[2 * j + 1]);
+    }
+  }
+  __syncthreads();
+  // (2) C2R fold: Z'_k = (Y_k + conj Y_{M-k}) + i W_2M^-k (Y_k - conj Y_{M-k})
+  const int g = tid / P, p = tid % P;
+  const float2* __restrict__ twr = a.twr[C];
+  float2 v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int k = p + P * e;
+    const float2 yk = sh[k];
+    const float2 yc = cconj(sh[M - k]);
+    const float2 sm = cadd(yk, yc);
+    const float2 d = csub(yk, yc);
+    const float2 wd = cmul(cconj(twr[k]), d);
+    v[e] = make_float2(sm.x - wd.y, sm.y + wd.x);
+  }
+  __syncthreads();
+  // (3) inverse FFT: x_2k + i x_2k+1, k = p + P e
+  fft_line_e<M, E, true>(v, sh + g * LL, p, a.tw[C]);
+  __syncthreads();
+  if (g == 0) {
+    float* dsf = reinterpret_cast<float*>(sh);
+    const float inv = 1.0f / (2.0f * M);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int j = 2 * (p + P * e);
+      if (j < n) dsf[j] = v[e].x * inv;
+      if (j + 1 < n) dsf[j + 1] = v[e].y * inv;
+    }
+  }
+  __syncthreads();
+}
+
+// MAXC: the longest length class of the batch (classes above it are flagged)
+template <int MAXC>
+__global__ __launch_bounds__(STEP_NT) void k_forest_step(StepArgs a) {
+  extern __shared__ float2 sh[];
+  __shared__ int s_next;
+  const int tid = threadIdx.x;
+  const smk_skewers& sk = a.sk;
+  const smk_box_view& b = a.box;
+  const uint64_t seed = a.seed_dev ? *a.seed_dev : a.seed;
+  const int nx = b.nx, ny = b.ny, nz = b.nz;
+  const int64_t x0 = b.nxs > 0 ? b.x0 : 0, nxs = b.nxs > 0 ? b.nxs : nx;
+  const float* __restrict__ fd = b.delta;
+  const float* __restrict__ fe = b.eta[2];
+  const double D = b.cell;
+  for (;;) {
+    if (tid == 0) s_next = atomicAdd(a.queue, 1);
+    __syncthreads();
+    // the same value in every lane: scalar registers for all per-skewer terms
+    const int s = __builtin_amdgcn_readfirstlane(s_next);
+    __syncthreads();
+    if (s >= sk.n) break;
+    const int n = sk.npix[s];
+    int c = 0;
+    for (int nzc = 256; nzc < n + 50 && c < N_CLASS; nzc <<= 1) ++c;
+    const int64_t off = sk.off[s];
+    if (c > a.max_class || (a.fset && (a.fset[s] < 0 || a.fset[s] >= a.n_sets))) {
+      // longer than the caller's max_npix bound, or a filter set outside the
+      // tables (possible only in graph mode, which skips the host check):
+      // flagged, flux NaN
+      if (tid == 0 && a.status) atomicOr(&a.status[s], 8);
+      for (int i = tid; i < n; i += STEP_NT) a.flux[off + i] = __int_as_float(0x7fc00000);
+      continue;
+    }
+    if (c == 0) step_missing_power<0>(a, s, n, seed, sh);
+    else if (c == 1) step_missing_power<1>(a, s, n, seed, sh);
+    else if (c == 2) step_missing_power<2>(a, s, n, seed, sh);
+    else if (c == 3 || MAXC <= 3) step_missing_power<(MAXC < 3 ? MAXC : 3)>(a, s, n, seed, sh);
+    else if (c == 4 || MAXC <= 4) step_missing_power<(MAXC < 4 ? MAXC : 4)>(a, s, n, seed, sh);
+    else step_missing_power<MAXC>(a, s, n, seed, sh);
+    const float* dsf = reinterpret_cast<const float*>(sh);
+    // per-skewer x and y terms (make_spectra.py:47-49 positions; trilinear on
+    // cell-centred values, periodic indices in a periodic box)
+    const double ux = div_rn(add_rn(sk.qx[s], a.hx), D) - 0.5;
+    const double uy = div_rn(add_rn(sk.qy[s], a.hy), D) - 0.5;
+    const double fx0 = floor(ux), fy0 = floor(uy);
+    const double fx = ux - fx0, fy = uy - fy0;
+    bool outside = false, off_slab = false;
+    int64_t jx[2], jy[2];
+#pragma unroll
+    for (int o = 0; o < 2; ++o) {
+      jx[o] = (int64_t)fx0 + o;
+      jy[o] = (int64_t)fy0 + o;
+      if (b.periodic) {
+        jx[o] = wrap(jx[o], nx);
+        jy[o] = wrap(jy[o], ny);
+      } else if (jx[o] < 0 || jx[o] >= nx || jy[o] < 0 || jy[o] >= ny) {
+        outside = true;
+      }
+      if (jx[o] >= 0 && jx[o] < nx && (jx[o] < x0 || jx[o] >= x0 + nxs)) off_slab = true;
+    }
+    const bool xy_bad = outside || off_slab;
+    // the four z columns (ox, oy) and their weights
+    int64_t col[4];
+    float wxy[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int ox = q >> 1, oy = q & 1;
+      col[q] = xy_bad ? 0 : ((jx[ox] - x0) * ny + jy[oy]) * nz;
+      wxy[q] = (float)((ox ? fx : 1.0 - fx) * (oy ? fy : 1.0 - fy));
+    }
+    // z of pixel i: uz = ((rmin + g dr) + LZ/2 - R0)/D - 0.5 with g = g0 + i,
+    // advanced by dr/D per pixel from pixel 0
+    const int64_t g0 = sk.g0[s];
+    const double z0 = add_rn(sk.rmin, mul_rn((double)g0, sk.dr));
+    const double uz0 = div_rn(sub_rn(add_rn(z0, a.hz), b.r0), D) - 0.5;
+    const double duz = sk.dr / D;
+    const int imin = sk.imin[s], imax = sk.imax[s];
+    const bool per_pixel_row = sk.pset == nullptr;
+    const int64_t srow = per_pixel_row ? 0 : (int64_t)sk.pset[s];
+    float4 prow = make_float4(0.f, 0.f, 0.f, 0.f);
+    bool row_ok = per_pixel_row || (srow >= 0 && srow < a.abcg_rows);
+    if (!per_pixel_row && row_ok) prow = reinterpret_cast<const float4*>(a.abcg)[srow];
+    int flags = 0;
+    for (int i = tid; i < n; i += STEP_NT) {
+      float ds = dsf[i];
+      if (a.scale) ds *= a.scale[off + i];
+      float dl = -1000000.0f, eta = 0.0f;
+      if (i >= imin && i < imax) {
+        const double uz = fma((double)i, duz, uz0);
+        const double fz0 = floor(uz);
+        const float fz = (float)(uz - fz0);
+        int64_t jz0 = (int64_t)fz0, jz1 = jz0 + 1;
+        bool zout = false;
+        if (b.periodic) {
+          jz0 = wrap(jz0, nz);
+          jz1 = wrap(jz1, nz);
+        } else if (jz0 < 0 || jz1 >= nz) {
+          zout = true;
+        }
+        if (xy_bad || zout) {
+          dl = __int_as_float(0x7fc00000);
+          eta = dl;
+          flags |= (outside || zout ? 1 : 0) | (off_slab ? 4 : 0);
+        } else {
+          float a0 = 0.f, a1 = 0.f, e0 = 0.f, e1 = 0.f;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            a0 = fmaf(wxy[q], fd[col[q] + jz0], a0);
+            a1 = fmaf(wxy[q], fd[col[q] + jz1], a1);
+            if (fe) {
+              e0 = fmaf(wxy[q], fe[col[q] + jz0], e0);
+              e1 = fmaf(wxy[q], fe[col[q] + jz1], e1);
+            }
+          }
+          dl = fmaf(fz, a1 - a0, a0);
+          eta = fmaf(fz, e1 - e0, e0);
+        }
+      }
+      if (a.delta_l) a.delta_l[off + i] = dl;
+      if (a.eta_par) a.eta_par[off + i] = eta;
+      if (a.delta_s) a.delta_s[off + i] = ds;
+      float4 pr = prow;
+      bool rok = row_ok;
+      if (per_pixel_row) {
+        const int64_t row = g0 + i;
+        rok = row >= 0 && row < a.abcg_rows;
+        if (rok) pr = reinterpret_cast<const float4*>(a.abcg)[row];
+      }
+      if (!rok) {
+        a.flux[off + i] = __int_as_float(0x7fc00000);
+        flags |= 2;
+      } else {
+        // F = exp(-a exp(b G (delta + c eta)))
+        const float gfield = dl + ds + pr.z * eta;
+        a.flux[off + i] = expf(-pr.x * expf(pr.y * pr.w * gfield));
+      }
+    }
+    if (flags && a.status) atomicOr(&a.status[s], flags);
+  }
+}
+
 __global__ void k_zero_ints(int32_t* p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
+}
+
+__global__ void k_check_sets(const int32_t* __restrict__ fset, int n, int32_t n_sets, int32_t* bad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && (fset[i] < 0 || fset[i] >= n_sets)) atomicOr(bad, 1);
 }
 
 constexpr int mp_T(int m) {
@@ -740,4 +1005,87 @@ extern "C" int smk_forest_missing_power(smk_forest_plan* plan, const smk_skewers
                                         const int64_t* noise_off, void* workspace, float* delta_s, void* stream) {
   return smk_forest_missing_power_sets(plan, sk, filt, filt_off, 0, 1, nullptr, nullptr, seed, noise, noise_off,
                                        workspace, delta_s, stream);
+}
+
+extern "C" size_t smk_forest_step_workspace_bytes(void) { return 256; }
+
+extern "C" int smk_forest_step(smk_forest_plan* plan, const smk_skewers* sk, const smk_box_view* box,
+                               const float* filt, const int64_t* filt_off, int64_t set_stride, int32_t n_sets,
+                               const int32_t* filt_set, const float* scale, uint64_t seed, const float* abcg,
+                               int64_t abcg_rows, int32_t max_npix, float* flux, float* delta_l, float* eta_par,
+                               float* delta_s, int32_t* status, void* workspace, void* stream) {
+  SMK_REQUIRE(plan != nullptr && sk != nullptr && box != nullptr, SMK_ERR_ARG, "NULL plan/skewers/box");
+  SMK_REQUIRE(filt != nullptr && filt_off != nullptr, SMK_ERR_ARG, "NULL filter tables");
+  SMK_REQUIRE(abcg != nullptr && flux != nullptr, SMK_ERR_ARG, "the step needs FGPA parameters and a flux output");
+  SMK_REQUIRE(workspace != nullptr && ((uintptr_t)workspace & 3) == 0, SMK_ERR_ARG, "bad workspace");
+  SMK_REQUIRE(box->delta != nullptr, SMK_ERR_ARG, "box delta is NULL");
+  SMK_REQUIRE(box->nx > 0 && box->ny > 0 && box->nz > 0 && box->cell > 0, SMK_ERR_ARG, "bad box geometry");
+  SMK_REQUIRE(box->nxs >= 0 && box->x0 >= 0 && (box->nxs == 0 || box->x0 + box->nxs <= box->nx), SMK_ERR_ARG,
+              "bad resident x slab [%d, %d + %d) of a %d-plane box", box->x0, box->x0, box->nxs, box->nx);
+  SMK_REQUIRE(sk->geometry == SMK_GEOM_PLANE, SMK_ERR_ARG, "the fused step takes plane-parallel skewers");
+  SMK_REQUIRE(filt_set == nullptr || (n_sets >= 1 && set_stride >= 0), SMK_ERR_ARG,
+              "filter sets need n_sets >= 1 and set_stride >= 0");
+  SMK_REQUIRE(max_npix >= 0 && max_npix + 50 <= (256 << (N_CLASS - 1)), SMK_ERR_RANGE,
+              "max_npix %d needs an FFT longer than %d", (int)max_npix, 256 << (N_CLASS - 1));
+  if (sk->n <= 0) return SMK_OK;
+  hipStream_t s = (hipStream_t)stream;
+  StepArgs a{};
+  a.sk = *sk;
+  a.box = *box;
+  a.filt = filt;
+  for (int c = 0; c < N_CLASS; ++c) {
+    a.foff[c] = filt_off[c];
+    a.tw[c] = plan->tw[c];
+    a.twr[c] = plan->twr[c];
+  }
+  a.fset = filt_set;
+  a.set_stride = filt_set ? set_stride : 0;
+  a.n_sets = n_sets;
+  a.scale = scale;
+  a.abcg = abcg;
+  a.abcg_rows = abcg_rows;
+  a.seed = seed;
+  a.seed_dev = plan->seed_dev;
+  a.flux = flux;
+  a.delta_l = delta_l;
+  a.eta_par = eta_par;
+  a.delta_s = delta_s;
+  a.status = status;
+  a.queue = (int32_t*)workspace;
+  int c = 0;
+  for (int nzc = 256; nzc < max_npix + 50; nzc <<= 1) ++c;
+  a.max_class = c;
+  a.hx = box->cell * box->nx / 2;
+  a.hy = box->cell * box->ny / 2;
+  a.hz = box->cell * box->nz / 2;
+  // a filter-set index out of range reads outside the tables: checked here
+  // in normal mode (one host sync), as smk_forest_missing_power_sets does
+  if (filt_set != nullptr && plan->seed_dev == nullptr) {
+    int32_t* bad = a.queue + 1;
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, a.queue, 2);
+    SMK_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_check_sets, dim3((sk->n + 255) / 256), dim3(256), 0, s, filt_set, sk->n, n_sets, bad);
+    SMK_LAUNCH_CHECK();
+    int32_t h = 0;
+    SMK_HIP_CHECK(hipMemcpyAsync(&h, bad, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    SMK_HIP_CHECK(hipStreamSynchronize(s));
+    SMK_REQUIRE(h == 0, SMK_ERR_RANGE, "a filter set index is outside [0, %d)", (int)n_sets);
+  }
+  // kernel instance: classes up to 3 (nz <= 2048, n <= 1998) or all six;
+  // skewers of a class above max_npix's are flagged (a.max_class)
+  const bool small = c <= 3;
+  const size_t lds = (size_t)step_lds_max(small ? 3 : N_CLASS - 1) * sizeof(float2);
+  int dev = 0, ncu = 0, per_cu = 0;
+  SMK_HIP_CHECK(hipGetDevice(&dev));
+  SMK_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  SMK_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu, small ? k_forest_step<3> : k_forest_step<N_CLASS - 1>, STEP_NT, lds));
+  const int64_t want = (int64_t)ncu * (per_cu > 0 ? per_cu : 1);
+  const int nblk = (int)(want < sk->n ? want : sk->n);
+  hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, a.queue, 1);
+  SMK_LAUNCH_CHECK();
+  if (small) hipLaunchKernelGGL(k_forest_step<3>, dim3(nblk), dim3(STEP_NT), lds, s, a);
+  else hipLaunchKernelGGL(k_forest_step<N_CLASS - 1>, dim3(nblk), dim3(STEP_NT), lds, s, a);
+  SMK_LAUNCH_CHECK();
+  return SMK_OK;
 }

@@ -275,31 +275,88 @@ def gather_into(batch, box, out, interp="gauss343", abcg=None, delta_s=None, str
     _lib.check(rc, "smk_forest_gather")
 
 
+def step(plan, batch, box, filt, filt_off, abcg, seed=0, filt_set=None, set_stride=0, n_sets=1, scale=None,
+         max_npix=None, out=None, want=("flux",), workspace=None, stream=None):
+    """The whole forest step of a plane-parallel batch in one launch
+    (smk_forest_step): device missing power, trilinear delta and eta_zz, FGPA.
+    Returns {name: packed per-pixel tensor} for `want` (any of "flux",
+    "delta_l", "eta_par", "delta_s") and "status" (int32 per skewer).  `out`
+    (a dict of caller-owned tensors with the same keys, "status" zeroed by the
+    caller) makes the call allocate nothing."""
+    dev = _dev()
+    n = batch.npix_total
+    if out is None:
+        out = {k: torch.empty(n, dtype=torch.float32, device=dev) for k in set(want) | {"flux"}}
+        out["status"] = torch.zeros(batch.n, dtype=torch.int32, device=dev)
+    ws = workspace if workspace is not None else \
+        torch.empty(int(_lib.lib().smk_forest_step_workspace_bytes()), dtype=torch.uint8, device=dev)
+    fs = None if filt_set is None else (filt_set if isinstance(filt_set, torch.Tensor)
+                                        else _t(filt_set, torch.int32, dev))
+    sc = None if scale is None else (scale if isinstance(scale, torch.Tensor) else _t(scale, torch.float32, dev))
+    if sc is not None and (sc.dtype != torch.float32 or sc.numel() != n):
+        raise ValueError("scale must be float32 with one value per pixel")
+    if fs is not None and (fs.dtype != torch.int32 or fs.numel() != batch.n):
+        raise ValueError("filt_set must be int32 with one value per skewer")
+    if abcg.dtype != torch.float32 or abcg.dim() != 2 or abcg.shape[1] != 4:
+        raise ValueError("abcg must be float32 [rows][4]")
+    mx = int(batch.npix_host.max()) if max_npix is None and batch.n else int(max_npix or 0)
+    sk = batch.struct()
+    bv = box.struct()
+    rc = _lib.lib().smk_forest_step(plan._h, ctypes.byref(sk), ctypes.byref(bv), _lib.ptr(filt), filt_off,
+                                    int(set_stride), int(n_sets), _lib.ptr(fs), _lib.ptr(sc),
+                                    ctypes.c_uint64(int(seed) & (2**64 - 1)), _lib.ptr(abcg), int(abcg.shape[0]), mx,
+                                    _lib.ptr(out["flux"]), _lib.ptr(out.get("delta_l")), _lib.ptr(out.get("eta_par")),
+                                    _lib.ptr(out.get("delta_s")), _lib.ptr(out["status"]), _lib.ptr(ws),
+                                    _lib.stream_ptr(stream))
+    _lib.check(rc, "smk_forest_step")
+    return out
+
+
 class CapturedStep:
     """One forest step of a fixed batch (device missing power, gather, FGPA)
     captured as a HIP graph (torch.cuda.CUDAGraph) and replayed with a new
     seed per call.  It owns a forest plan in graph mode (the seed is read from
     a device tensor); the batch is run once eagerly first, which checks it.
-    Outputs are the same tensors at every call: copy them before the next."""
+    Outputs are the same tensors at every call: copy them before the next.
+    fused=True captures the one-launch step (smk_forest_step: plane-parallel
+    skewers, trilinear, FGPA rows required); otherwise the missing power and
+    the gather are separate launches, with delta_s in between."""
 
     def __init__(self, batch, box, filt, filt_off, abcg=None, interp="trilinear",
-                 want=("delta_l", "eta_par", "flux"), filt_set=None, set_stride=0, n_sets=1, scale=None):
+                 want=("delta_l", "eta_par", "flux"), filt_set=None, set_stride=0, n_sets=1, scale=None,
+                 fused=False):
         dev = _dev()
         n = batch.npix_total
         plan = ForestPlan()
         self.plan, self.batch = plan, batch
         self.seed = torch.zeros(1, dtype=torch.int64, device=dev)
         self.delta_s = torch.empty(n, dtype=torch.float32, device=dev)
-        self.workspace = torch.empty(int(_lib.lib().smk_forest_workspace_bytes(batch.n)), dtype=torch.uint8,
-                                     device=dev)
+        if fused:
+            if interp != "trilinear" or batch.geometry != GEOM_PLANE or abcg is None:
+                raise ValueError("the fused step takes plane-parallel trilinear skewers with FGPA rows")
+            self.workspace = torch.empty(int(_lib.lib().smk_forest_step_workspace_bytes()), dtype=torch.uint8,
+                                         device=dev)
+        else:
+            self.workspace = torch.empty(int(_lib.lib().smk_forest_workspace_bytes(batch.n)), dtype=torch.uint8,
+                                         device=dev)
         self.out = {k: torch.empty(n, dtype=torch.float32, device=dev) for k in want
                     if k != "flux" or abcg is not None}
+        if fused:
+            self.out.setdefault("flux", torch.empty(n, dtype=torch.float32, device=dev))
         self.out["status"] = torch.empty(batch.n, dtype=torch.int32, device=dev)
+        fs = None if filt_set is None else _t(filt_set, torch.int32, dev)
+        sc = None if scale is None else _t(scale, torch.float32, dev)
+        max_npix = int(batch.npix_host.max()) if batch.n else 0
 
         def run():
             self.out["status"].zero_()
+            if fused:
+                step(plan, batch, box, filt, filt_off, abcg, filt_set=fs, set_stride=set_stride, n_sets=n_sets,
+                     scale=sc, max_npix=max_npix, out=dict(self.out, delta_s=self.delta_s),
+                     workspace=self.workspace)
+                return
             missing_power(plan, batch, filt, filt_off, out=self.delta_s, workspace=self.workspace,
-                          filt_set=filt_set, set_stride=set_stride, n_sets=n_sets, scale=scale)
+                          filt_set=fs, set_stride=set_stride, n_sets=n_sets, scale=sc)
             gather_into(batch, box, self.out, interp, abcg, self.delta_s)
 
         plan.set_graph_mode(None)

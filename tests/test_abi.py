@@ -28,10 +28,14 @@ def test_version_and_lengths():
     assert L.smk_abi_version() == _lib.ABI_VERSION
     with open(_lib.HEADER) as f:
         assert f"#define SMK_ABI_VERSION {_lib.ABI_VERSION}\n" in f.read()
-    for n in (16, 24, 40, 64, 128, 256, 768, 1536, 2560):
-        assert L.smk_fft_length_supported(n) == 1
-    for n in (7, 11, 4096, 3):
-        assert L.smk_fft_length_supported(n) == 0
+    # every 2/3/5-smooth length from 8 to 4096, nothing else
+    for n in range(1, 5000):
+        m = n
+        for q in (2, 3, 5):
+            while m % q == 0:
+                m //= q
+        want = 1 if (m == 1 and 8 <= n <= 4096) else 0
+        assert L.smk_fft_length_supported(n) == want, n
 
 
 def test_plan_rejects_bad_sizes_without_gpu():

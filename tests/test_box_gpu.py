@@ -118,3 +118,32 @@ def test_null_and_nan_guards(cuda):
     nan[3, 4, 5] = float("nan")
     with pytest.raises(ValueError, match="NaN"):
         plan.grf_box(noise=nan)
+
+
+# 2/3/5-smooth lengths beyond the tuned ones, with two or more odd-radix
+# stages (1152 = 2^7 3^2, 1800 = 2^3 3^2 5^2, 2400 = 2^5 3 5^2, 3072 = 2^10 3,
+# 2187 = 3^7, 375 = 3 5^3) on each axis, z through nz/2
+GENERAL_SHAPES = [(1152, 8, 16), (8, 1800, 16), (12, 10, 2400), (3072, 8, 16), (8, 8, 6144), (2187, 9, 18),
+                  (15, 375, 50)]
+
+
+@pytest.mark.parametrize("shape", GENERAL_SHAPES)
+def test_grf_box_parity_general_lengths(cuda, shape):
+    from saclaymocks_amd import _lib
+    from saclaymocks_amd.box import BoxPlan
+    print(f"[grf_box general length] shape={shape}", flush=True)
+    L = _lib.lib()
+    assert L.smk_fft_length_supported(shape[0]) and L.smk_fft_length_supported(shape[1])
+    assert L.smk_fft_length_supported(shape[2] // 2)
+    w = _noise(shape, 17)
+    plan = BoxPlan(*shape, cell=2.19)
+    d, e, var, st = plan.grf_box(noise=torch.from_numpy(w).to(cuda), aux="eta_zz")
+    torch.cuda.synchronize()
+    ref = bo.grf_box(w.astype(np.float64), 2.19, aux="eta_zz")
+    assert relinf(d.cpu().numpy(), ref["delta"]) < TOL
+    assert relinf(e.cpu().numpy(), ref["eta_zz"]) < TOL
+    assert abs(var.item() / ref["var"] - 1) < 1e-5
+    assert st.cpu().tolist() == [2, 2]
+    # the drop-in steps in the numpy layout at the same lengths
+    bk = plan.draw_boxk(noise=torch.from_numpy(w).to(cuda)).cpu().numpy()
+    assert relinf(bk, np.fft.rfftn(w.astype(np.float64))) < 1e-5

@@ -35,16 +35,26 @@ def _eager(forest, plan, batch, box, filt, foff, rows, seed, interp):
     return out
 
 
-@pytest.mark.parametrize("interp", ["trilinear", "gauss343"])
+def _eager_fused(forest, plan, batch, box, filt, foff, rows, seed):
+    return forest.step(plan, batch, box, filt, foff, rows, seed=seed, want=("flux", "delta_l", "eta_par", "delta_s"))
+
+
+@pytest.mark.parametrize("interp", ["trilinear", "gauss343", "fused"])
 def test_captured_step_replays_match_eager(cuda, interp):
-    forest, batch, box, filt, foff, rows = _case(cuda, n=300 if interp == "trilinear" else 60)
+    forest, batch, box, filt, foff, rows = _case(cuda, n=60 if interp == "gauss343" else 300)
     print(f"[forest graph] interp={interp}", flush=True)
     plan = forest.ForestPlan()
-    step = forest.CapturedStep(batch, box, filt, foff, abcg=rows, interp=interp)
+    if interp == "fused":
+        step = forest.CapturedStep(batch, box, filt, foff, abcg=rows, interp="trilinear", fused=True)
+    else:
+        step = forest.CapturedStep(batch, box, filt, foff, abcg=rows, interp=interp)
     seeds = [5, 9, 5, 123456789, -1, 2 ** 63 + 5]
     prev = None
     for s in seeds:
-        want = _eager(forest, plan, batch, box, filt, foff, rows, s, interp)   # eager work between replays
+        if interp == "fused":
+            want = _eager_fused(forest, plan, batch, box, filt, foff, rows, s)
+        else:
+            want = _eager(forest, plan, batch, box, filt, foff, rows, s, interp)   # eager work between replays
         torch.cuda.synchronize()
         got = step(s)
         torch.cuda.synchronize()

@@ -232,6 +232,13 @@ def test_device_noise_modes_are_white_noise_modes():
     w = np.fft.irfft(y, n=nz)
     assert abs(w.var() - 1) < 0.05 and abs(w.mean()) < 0.05
     assert abs(np.mean(np.abs(y[1:-1]) ** 2) / nz - 1) < 0.05
+    # the two modes drawn from one Philox call (2j, 2j+1) are uncorrelated:
+    # 8 streams give 16376 pairs, sample correlations ~ N(0, 0.0078^2)
+    ys = [fo.device_noise_modes(12345, 100 + i, nz) for i in range(8)]
+    ev = np.concatenate([v[2:-1:2] for v in ys])
+    od = np.concatenate([v[3::2] for v in ys])
+    for part in (np.real, np.imag):
+        assert abs(np.corrcoef(part(ev), part(od))[0, 1]) < 0.035
     # different streams are uncorrelated
     w2 = np.fft.irfft(fo.device_noise_modes(12345, 78, nz), n=nz)
     assert abs(np.mean(w * w2)) < 0.05
