@@ -138,7 +138,7 @@ int smk_box_get_schedule(const smk_box_plan* plan);
 #define SMK_PASS_X_FWD 5
 #define SMK_PASS_X_INV 6
 #define SMK_PASS_COUNT 7
-#define SMK_BOX_NVAR 16
+#define SMK_BOX_NVAR 17
 
 /* SMK_BOX_NVAR of the library (for bindings that cannot read the header). */
 int smk_box_variant_count(void);

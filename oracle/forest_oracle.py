@@ -210,32 +210,38 @@ MP_STREAM_TAG = 0x6D697373   # Philox counter word 3 of the missing-power modes
 
 def device_noise_modes(seed, noise_id, nz):
     """The Fourier modes of N(0,1) white noise of length nz = 2M that the
-    device draws instead of numpy's normal draw of merge_spectra.py:310-311:
-    modes 2j and 2j+1 share one Philox4x32-10 call (counter = (j, id_lo,
-    id_hi, tag), key = seed); mode k takes Box-Muller of words (0, 1) for
-    even k and (2, 3) for odd k -> (n0, n1).  Interior modes are
-    (n0 + i n1) sqrt(M) (Re, Im ~ N(0, M), as rfft of white noise); k = 0 and
-    k = M are real, n0 sqrt(2M)."""
+    device draws instead of numpy's normal draw of merge_spectra.py:310-311,
+    M/2 Philox4x32-10 calls per skewer (counter = (c, id_lo, id_hi, tag), key
+    = seed), each giving two Box-Muller pairs, (w0, w1) -> (n0, n1) and
+    (w2, w3) -> (n2, n3):
+      call 0:      mode 0 = n0 sqrt(2M), mode M = n1 sqrt(2M) (both real),
+                   mode 1 = (n2 + i n3) sqrt(M);
+      call c >= 1: mode 2c = (n0 + i n1) sqrt(M), mode 2c+1 = (n2 + i n3) sqrt(M).
+    Interior modes have Re, Im ~ N(0, M) and the real ones variance 2M, as
+    the rfft of white noise."""
     from oracle import philox
     M = nz // 2
-    k = np.arange(M + 1, dtype=np.uint64)
-    j = (k >> np.uint64(1)).astype(np.uint32)
+    c = np.arange(M // 2, dtype=np.uint32)
     nid = np.uint64(noise_id)
     seed = np.uint64(seed)
     lo = np.uint32(nid & np.uint64(0xFFFFFFFF))
     hi = np.uint32(nid >> np.uint64(32))
-    w = philox.philox4x32_10(j, np.full(M + 1, lo), np.full(M + 1, hi),
-                             np.full(M + 1, MP_STREAM_TAG, np.uint32), np.uint32(seed & np.uint64(0xFFFFFFFF)),
+    w = philox.philox4x32_10(c, np.full(M // 2, lo), np.full(M // 2, hi),
+                             np.full(M // 2, MP_STREAM_TAG, np.uint32), np.uint32(seed & np.uint64(0xFFFFFFFF)),
                              np.uint32(seed >> np.uint64(32)))
-    odd = (k & np.uint64(1)).astype(bool)
-    wa = np.where(odd, w[2], w[0])
-    wb = np.where(odd, w[3], w[1])
-    r = np.sqrt(-2.0 * np.log(philox.u01(wa)))
-    th = 2.0 * np.pi * philox.u01(wb)
-    n0, n1 = r * np.cos(th), r * np.sin(th)
-    y = (n0 + 1j * n1) * np.sqrt(M)
+
+    def bm(ua, ub):
+        r = np.sqrt(-2.0 * np.log(philox.u01(ua)))
+        th = 2.0 * np.pi * philox.u01(ub)
+        return r * np.cos(th), r * np.sin(th)
+
+    n0, n1 = bm(w[0], w[1])
+    n2, n3 = bm(w[2], w[3])
+    y = np.zeros(M + 1, dtype=np.complex128)
+    y[2::2][:M // 2 - 1] = (n0[1:] + 1j * n1[1:]) * np.sqrt(M)
+    y[1::2][:M // 2] = (n2 + 1j * n3) * np.sqrt(M)
     y[0] = n0[0] * np.sqrt(2.0 * M)
-    y[M] = n0[M] * np.sqrt(2.0 * M)
+    y[M] = n1[0] * np.sqrt(2.0 * M)
     return y
 
 

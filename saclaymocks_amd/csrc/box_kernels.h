@@ -390,6 +390,100 @@ __global__ __launch_bounds__(T*(N / E)) __attribute__((amdgpu_waves_per_eu(OCC))
   }
 }
 
+// Spectral x pass, variant 16: as k_xspec (three line FFTs, aux spectrum in
+// registers), but delta's inverse FFT runs in place in LDS (fft_line_dif,
+// decimation in frequency), so that while it runs the registers hold only
+// the aux tile and one butterfly; delta is then stored from the
+// digit-reversed line.  Needs P = N / E to be a product of the plan's
+// leading radices (DifRev::J >= 0).
+template <int N, int E, int T, int SWS>
+__global__ __launch_bounds__(T*(N / E)) __attribute__((amdgpu_waves_per_eu(2))) void k_xspec_dif(StridedArgs a) {
+  constexpr int P = N / E;
+  constexpr int LL = lds_line_len(N);
+  constexpr int NT = T * P;
+  constexpr int SWB = lds_padded(N) ? 0 : ilog2(T);
+  using Rev = DifRev<N, E>;
+  static_assert(Rev::J >= 0, "threads per line must be a product of leading radices");
+  using LM = LdsMap<N, SWS, SWB>;
+  __shared__ float2 lds[T * LL];
+
+  const uint32_t tid = threadIdx.x;
+  const uint32_t t = tid % T, p = tid / T;
+  const uint32_t sw = SWB ? (t << SWS) : 0u;
+  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t o = tile / a.n_iblk;
+  const uint32_t inner0 = (tile % a.n_iblk) * T;
+  const uint32_t inner = inner0 + t;
+  const bool valid = inner < a.n_inner;
+  const uint32_t tl = valid ? t : 0u;
+  const float2* src = a.in + (uint64_t)o * a.in_so + (uint64_t)inner0 * a.in_si;
+  const uint32_t src_b = (tl * (uint32_t)a.in_si + p * (uint32_t)a.in_sn) * 8u;
+  const uint64_t in_step = (uint64_t)P * a.in_sn;
+
+  float2 v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = ld_at(src + (uint64_t)e * in_step, src_b);
+
+  line_fft<N, E, false, 0, SWS, SWB>(v, lds, t, p, a.tw, sw);
+
+  const uint32_t iky = a.kz_outer ? inner : o;
+  const uint32_t ikz = (a.kz_outer ? o : inner) + a.kz0;
+  const float ky = valid ? a.kyv[iky] : 0.0f;
+  const float kz = valid ? a.kzv[ikz] : 0.0f;
+  const float ky2 = mul_rn(ky, ky), kz2 = mul_rn(kz, kz);
+  const double wz = (ikz == 0 || ikz == a.nz_nyq) ? 1.0 : 2.0;
+  const AuxSel sel = aux_sel(a.aux_kind, a.aux_coef);
+  const bool has_aux = a.aux_kind != SMK_AUX_NONE;
+  const bool origin_line = iky == 0 && ikz == 0;
+  double acc = 0.0;
+  float2 w[E];
+  {
+    const uint32_t kb = p * 4u;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const uint32_t i = p + (uint32_t)(P * e);
+      const float kx = ld_at(a.kxv + P * e, kb);
+      // kk = kx*kx + ky*ky + kz*kz in float32, numpy evaluation order
+      const float kk = add_rn(add_rn(mul_rn(kx, kx), ky2), kz2);
+      const float l2x = 0.5f * __builtin_amdgcn_logf(kk) - a.pk_l2k0;
+      const float xn = __builtin_amdgcn_exp2f(a.pk_n * l2x);
+      const float ex = 0.5f * l2x - 0.5f * __builtin_amdgcn_logf(1.0f + xn) - a.pk_c2 * kk;
+      const float sp = a.pk_cm * __builtin_amdgcn_exp2f(ex);
+      const float2 d = cscale(v[e], sp);
+      acc += (double)d.x * (double)d.x + (double)d.y * (double)d.y;
+      v[e] = d;
+      w[e] = aux_apply(d, sel, kx, ky, kz, kk, origin_line && i == 0, 1.0f);
+    }
+  }
+  acc = valid ? wz * acc : 0.0;
+  const double bs = block_sum<NT>(acc, reinterpret_cast<double*>(lds));
+  if (threadIdx.x == 0) a.partial[tile] = bs;
+
+  const uint64_t out_base = (uint64_t)o * a.out_so + (uint64_t)inner0 * a.out_si;
+  const uint32_t dst_b = (t * (uint32_t)a.out_si + p * (uint32_t)a.out_sk) * 8u;
+  const uint64_t out_step = (uint64_t)P * a.out_sk;
+
+  // delta: in place in this line of the block's LDS, then stored point by
+  // point from its digit-reversed slot
+  float2* line = lds + t * LL;
+  fft_line_dif<N, E, true, SWS, SWB>(v, line, p, a.tw, sw);
+  if (valid && a.write_delta) {
+    float2* dst = a.out + out_base;
+    const uint32_t lp = Rev::lane(p);
+#pragma unroll
+    for (int e = 0; e < E; ++e) st_at(dst + (uint64_t)e * out_step, dst_b, line[LM::slot(lp + Rev::slot(e), sw)]);
+  }
+  __syncthreads();
+  if (has_aux) {
+    line_fft<N, E, true, 0, SWS, SWB>(w, lds, t, p, a.tw, sw);
+    if (valid) {
+      float2* adst = a.aux + out_base;
+#pragma unroll
+      for (int e = 0; e < E; ++e) st_at(adst + (uint64_t)e * out_step, dst_b, w[e]);
+    }
+  }
+}
+
 // z pencils, R2C.  M = nz/2 complex points z_n = x_{2n} + i x_{2n+1}; after
 // the M-point FFT, X_k = Ze_k + W_{2M}^k Zo_k with
 // Ze_k = (Z_k + conj Z_{M-k})/2, Zo_k = -i (Z_k - conj Z_{M-k})/2.
@@ -580,6 +674,19 @@ constexpr int SMK_NVAR = SMK_BOX_NVAR;
 
 constexpr bool plan_ok(int n, int e) { return e > 0 && n % e == 0 && make_plan(n, e).nstage > 0; }
 
+// the in-place DIF variant needs n / e threads per line to be a product of
+// the plan's leading radices
+constexpr bool dif_ok(int n, int e) {
+  if (!plan_ok(n, e)) return false;
+  const RadixPlan pl = make_plan(n, e);
+  int prod = 1;
+  for (int s = 0; s <= pl.nstage; ++s) {
+    if (prod == n / e) return true;
+    if (s < pl.nstage) prod *= pl.r[s];
+  }
+  return false;
+}
+
 constexpr int lines_per_block(int n, int e, int xch, int max_threads) {
   const int bytes = lds_line_len(n) * (xch ? 4 : 8);
   int T = 8;
@@ -606,6 +713,9 @@ constexpr int half_tile(int n) {
 //  14  E0, k_xspec: aux in a second register tile (3 FFTs), scalar-base
 //      addressing, log2-form spectrum, SWS 2 / 0 (spectral mode only)
 //  15  as 14, registers capped for 2 waves per SIMD (spills what is left)
+//  16  E0, k_xspec_dif: delta's inverse FFT in place in LDS, the aux
+//      spectrum in registers (3 FFTs); lengths whose threads per line are a
+//      product of leading radices
 // E0 = elems_per_thread(n), E1 = half_tile(n).
 constexpr SVar strided_variant(int n, int mode, int vid) {
   const int e0 = elems_per_thread(n);
@@ -635,6 +745,7 @@ constexpr SVar strided_variant(int n, int mode, int vid) {
     case 13: return (spec && e1 && t1 >= 2) ? SVar{e1, t1 / 2, 0, 2, swz ? 2 : 0, 4} : none;
     case 14: return spec ? SVar{e0, t0, 0, 4, swz ? 2 : 0, 1} : none;
     case 15: return spec ? SVar{e0, t0, 0, 4, swz ? 2 : 0, 2} : none;
+    case 16: return (spec && dif_ok(n, e0)) ? SVar{e0, t0, 0, 5, swz ? 2 : 0, 0} : none;
     default: return none;
   }
 }
@@ -649,7 +760,10 @@ hipError_t launch_strided_nv(const StridedArgs& a, uint64_t n_outer, hipStream_t
     b.n_iblk = (a.n_inner + sv.t - 1) / sv.t;
     const uint64_t nblk = n_outer * b.n_iblk;
     if (nblk > 0x7fffffffull) return hipErrorInvalidConfiguration;
-    if constexpr (sv.tiles == 4) {
+    if constexpr (sv.tiles == 5) {
+      hipLaunchKernelGGL((k_xspec_dif<N, sv.e, sv.t, sv.sws>), dim3((uint32_t)nblk), dim3(sv.t * (N / sv.e)), 0, s,
+                         b);
+    } else if constexpr (sv.tiles == 4) {
       hipLaunchKernelGGL((k_xspec<N, sv.e, sv.t, sv.sws, sv.occ>), dim3((uint32_t)nblk), dim3(sv.t * (N / sv.e)), 0,
                          s, b);
     } else {
@@ -679,6 +793,7 @@ hipError_t launch_strided_n(int vid, const StridedArgs& a, uint64_t n_outer, hip
     case 13: return launch_strided_nv<N, MODE, 13>(a, n_outer, s);
     case 14: return launch_strided_nv<N, MODE, 14>(a, n_outer, s);
     case 15: return launch_strided_nv<N, MODE, 15>(a, n_outer, s);
+    case 16: return launch_strided_nv<N, MODE, 16>(a, n_outer, s);
     default: return hipErrorInvalidValue;
   }
 }

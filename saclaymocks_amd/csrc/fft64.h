@@ -214,30 +214,37 @@ __device__ __forceinline__ double2 chirp(int64_t n, int N) {
 // Bluestein DFT of length N (2N - 1 <= M) over a block of M / E threads:
 // in(n) gives y_n for n < N (called once per n by one thread); emit(k, X_k)
 // is called for k < nout (nout <= N) by the thread that holds it.  `line`
-// is M-point LDS scratch (line_len(M) double2), `ch` N-point LDS scratch for
-// the chirp; every thread of the block must call it.
+// is M-point LDS scratch (line_len(M) double2); every thread of the block
+// must call it.  Thread p computes the chirp w_n of its own points n = p + P e
+// once and keeps it in registers; the wrapped half of b (m > M - N needs
+// w_{M-m}, a point of another thread) goes through the line.
 template <int M, int E, class In, class Emit>
 __device__ __forceinline__ void bluestein(int N, int nout, In in, Emit emit, double2* __restrict__ line,
-                                          double2* __restrict__ ch, const double2* __restrict__ twM, int TW) {
+                                          const double2* __restrict__ twM, int TW) {
   constexpr int P = M / E;
   const int p = threadIdx.x;
-  for (int n = p; n < N; n += P) ch[n] = chirp(n, N);
+  double2 ch[E], vb[E], va[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int n = p + P * e;
+    ch[e] = n < N ? chirp(n, N) : make_double2(0.0, 0.0);
+    line[slot(n)] = ch[e];
+  }
   __syncthreads();
-  double2 vb[E], va[E];
   // B = FFT(b), b_m = conj(w_m) for |m| < N (circularly), 0 elsewhere
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int m = p + P * e;
-    const int mm = m < N ? m : (m > M - N ? M - m : -1);
-    vb[e] = mm >= 0 ? cconj(ch[mm]) : make_double2(0.0, 0.0);
+    vb[e] = m < N ? cconj(ch[e]) : (m > M - N ? cconj(line[slot(M - m)]) : make_double2(0.0, 0.0));
   }
+  __syncthreads();
   fft<M, E, false>(vb, line, p, twM, TW);
   __syncthreads();
   // A = FFT(a), a_n = y_n w_n for n < N
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int n = p + P * e;
-    va[e] = n < N ? cmul(in(n), ch[n]) : make_double2(0.0, 0.0);
+    va[e] = n < N ? cmul(in(n), ch[e]) : make_double2(0.0, 0.0);
   }
   fft<M, E, false>(va, line, p, twM, TW);
   __syncthreads();
@@ -249,7 +256,7 @@ __device__ __forceinline__ void bluestein(int N, int nout, In in, Emit emit, dou
   for (int e = 0; e < E; ++e) {
     const int k = p + P * e;
     if (k < nout) {
-      const double2 c = cmul(va[e], ch[k]);
+      const double2 c = cmul(va[e], ch[e]);
       emit(k, make_double2(c.x * inv, c.y * inv));
     }
   }

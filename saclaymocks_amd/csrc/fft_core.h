@@ -597,6 +597,108 @@ __device__ __forceinline__ void fft_stages_split(float2 (&v)[E], float* __restri
   }
 }
 
+// ------------------------------------------------- in-place DIF FFT in LDS
+// Gentleman-Sande decimation in frequency on one N-point line held in LDS,
+// with the radices of make_plan(N, E) in order: stage s (sub-transform
+// length L = N / (R_1 ... R_{s-1}), stride m = L / R_s) takes the points
+// base + j + r m of butterfly u (base = (u / m) L, j = u % m), computes
+// their R-point DFT and multiplies output q by W_L^(j q) (conjugated for the
+// inverse), in place.  The first stage reads the thread's register tile
+// (natural order: thread p holds points p + P e) and every later one reads
+// and writes LDS, so the data costs no registers beyond one butterfly.  The
+// result is in digit-reversed order: X_k sits at dif_pos(k) = sum_s d_s(k)
+// N / (R_1 ... R_s) with k = d_1 + R_1 (d_2 + R_2 (...)).  Every thread of
+// the block must call it (it holds block barriers).
+template <int N, int E> struct DifRev {
+  static constexpr RadixPlan pl = make_plan(N, E);
+  static constexpr int P = N / E;
+  // J: the number of leading radices whose product is P (-1 if none)
+  static constexpr int J = []() {
+    int prod = 1;
+    for (int s = 0; s < pl.nstage; ++s) {
+      if (prod == P) return s;
+      prod *= pl.r[s];
+    }
+    return prod == P ? pl.nstage : -1;
+  }();
+  static constexpr int stride(int s) {
+    int prod = 1;
+    for (int i = 0; i <= s; ++i) prod *= pl.r[i];
+    return N / prod;
+  }
+  // the digits of lane p (radices 1..J)
+  static __device__ __forceinline__ uint32_t lane(uint32_t p) {
+    uint32_t pos = 0;
+#pragma unroll
+    for (int s = 0; s < J; ++s) {
+      pos += (p % (uint32_t)pl.r[s]) * (uint32_t)stride(s);
+      p /= (uint32_t)pl.r[s];
+    }
+    return pos;
+  }
+  // the digits of slot e (radices J+1..S)
+  static constexpr uint32_t slot(int e) {
+    uint32_t pos = 0;
+    for (int s = J; s < pl.nstage; ++s) {
+      pos += (uint32_t)(e % pl.r[s]) * (uint32_t)stride(s);
+      e /= pl.r[s];
+    }
+    return pos;
+  }
+};
+
+template <int N, int P, int R, int L, bool INV, bool FIRST, int E, int SWS, int SWB>
+__device__ __forceinline__ void dif_stage(float2 (&v)[E], float2* __restrict__ line, uint32_t p_in,
+                                          const float2* tw_in, uint32_t sw) {
+  using M = LdsMap<N, SWS, SWB>;
+  constexpr int m = L / R;
+  constexpr int B = E / R;
+  constexpr int TS = N / L;  // W_L^x = W_N^(x TS)
+  const int z = opaque_zero();
+  const float2* tw = tw_in + z;
+  const uint32_t p = p_in + (uint32_t)z;
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
+    const uint32_t u = p + (uint32_t)(P * b);
+    const uint32_t j = FIRST ? u : u % (uint32_t)m;
+    const uint32_t base = FIRST ? u : (u / (uint32_t)m) * (uint32_t)L + j;
+    float2 a[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) a[r] = FIRST ? v[b + r * B] : line[M::slot(base + (uint32_t)(r * m), sw)];
+    DFT<R, INV>::run(a);
+#pragma unroll
+    for (int q = 1; q < R; ++q) {
+      float2 w = tw[j * (uint32_t)(q * TS)];
+      if (INV) w.y = -w.y;
+      a[q] = cmul(a[q], w);
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) line[M::slot(base + (uint32_t)(q * m), sw)] = a[q];
+  }
+}
+
+template <int N, int E, bool INV, int S, int L, int SWS, int SWB>
+__device__ __forceinline__ void dif_stages(float2 (&v)[E], float2* __restrict__ line, uint32_t p, const float2* tw,
+                                           uint32_t sw) {
+  constexpr RadixPlan pl = make_plan(N, E);
+  constexpr int P = N / E;
+  if constexpr (S < pl.nstage) {
+    constexpr int R = pl.r[S];
+    dif_stage<N, P, R, L, INV, S == 0, E, SWS, SWB>(v, line, p, tw, sw);
+    __syncthreads();
+    dif_stages<N, E, INV, S + 1, L / R, SWS, SWB>(v, line, p, tw, sw);
+  }
+}
+
+// The in-place DIF transform of the register tile v into `line`; on return
+// (after a block barrier) X_k is at slot dif_pos(k) of the line.
+template <int N, int E, bool INV, int SWS = 0, int SWB = 0>
+__device__ __forceinline__ void fft_line_dif(float2 (&v)[E], float2* __restrict__ line, uint32_t p, const float2* tw,
+                                             uint32_t sw = 0) {
+  static_assert(make_plan(N, E).nstage > 0, "unsupported FFT length");
+  dif_stages<N, E, INV, 0, N, SWS, SWB>(v, line, p, tw, sw);
+}
+
 // Full in-register FFT of one line.  All threads of the block must call it
 // (it contains block barriers).  On entry and exit thread p holds points
 // p + P*e.  Forward: X_k = sum_n x_n exp(-2 pi i n k / N); inverse uses

@@ -39,35 +39,33 @@ int blue_m(int nmax) {
   return M;
 }
 size_t blue_line_bytes(int nmax) { return (size_t)f64::line_len(blue_m(nmax)) * 16; }
-size_t blue_chirp_bytes(int nmax) { return (size_t)(nmax < BLUE_NMAX ? nmax : BLUE_NMAX) * 16; }
 size_t align16(size_t b) { return (b + 15) & ~(size_t)15; }
-// forward: [x (Horner) or the Bluestein line][pk][bins][chirp]
 size_t p1d_x_bytes(int nmax) {
   const size_t a = (size_t)nmax * 8, b = blue_line_bytes(nmax);
   return align16(a > b ? a : b);
 }
-size_t p1d_chirp_off(int nmax) { return align16(p1d_x_bytes(nmax) + (size_t)(nmax / 2 + 1) * 12); }
-size_t p1d_lds_bytes(int nmax) { return p1d_chirp_off(nmax) + blue_chirp_bytes(nmax); }
-// adjoint: [x][cj][Bluestein line][chirp]
+// forward: [x (Horner) or the Bluestein line][pk][bins][bin runs]
+size_t p1d_lds_bytes(int nmax, int nb) {
+  return p1d_x_bytes(nmax) + (size_t)(nmax / 2 + 1) * 12 + (size_t)nb * 8;
+}
+// adjoint: [x][cj][Bluestein line]
 size_t adj_line_off(int nmax) { return align16((size_t)nmax * 8 + (size_t)(nmax / 2 + 1) * 16); }
-size_t adj_lds_bytes(int nmax) { return adj_line_off(nmax) + blue_line_bytes(nmax) + blue_chirp_bytes(nmax); }
+size_t adj_lds_bytes(int nmax) { return adj_line_off(nmax) + blue_line_bytes(nmax); }
 
 // one transform size per function, so that each gets its own register
 // allocation instead of the union of all of them
 template <int M, class In, class Emit>
-__device__ __noinline__ void bluestein_at(int N, int nout, In in, Emit emit, double2* line, double2* ch,
-                                          const double2* tw) {
-  f64::bluestein<M, M / FIT_THREADS>(N, nout, in, emit, line, ch, tw, BLUE_MMAX);
+__device__ __noinline__ void bluestein_at(int N, int nout, In in, Emit emit, double2* line, const double2* tw) {
+  f64::bluestein<M, M / FIT_THREADS>(N, nout, in, emit, line, tw, BLUE_MMAX);
 }
 
 // Bluestein DFT of length N <= BLUE_NMAX with M the smallest power of two
 // >= max(2N - 1, 1024): X_k for k < nout via emit(k, X_k)
 template <class In, class Emit>
-__device__ __forceinline__ void dft_bluestein(int N, int nout, In in, Emit emit, double2* line, double2* ch,
-                                              const double2* tw) {
-  if (2 * N - 1 <= 1024) bluestein_at<1024>(N, nout, in, emit, line, ch, tw);
-  else if (2 * N - 1 <= 2048) bluestein_at<2048>(N, nout, in, emit, line, ch, tw);
-  else bluestein_at<4096>(N, nout, in, emit, line, ch, tw);
+__device__ __forceinline__ void dft_bluestein(int N, int nout, In in, Emit emit, double2* line, const double2* tw) {
+  if (2 * N - 1 <= 1024) bluestein_at<1024>(N, nout, in, emit, line, tw);
+  else if (2 * N - 1 <= 2048) bluestein_at<2048>(N, nout, in, emit, line, tw);
+  else bluestein_at<4096>(N, nout, in, emit, line, tw);
 }
 
 __device__ __forceinline__ double block_reduce_sum(double v, double* red) {
@@ -233,7 +231,6 @@ struct P1dArgs {
   const int64_t* xoff;
   const double2* tw;     // W_4096^k, k < 4096 (FP64), for the Bluestein FFTs
   size_t x_bytes;        // p1d_x_bytes(nmax)
-  size_t chirp_off;      // p1d_chirp_off(nmax)
 };
 
 __global__ __launch_bounds__(FIT_THREADS) void k_p1d(P1dArgs a) {
@@ -242,7 +239,8 @@ __global__ __launch_bounds__(FIT_THREADS) void k_p1d(P1dArgs a) {
   double2* bline = reinterpret_cast<double2*>(dyn);   // Bluestein FFT line (aliases x)
   double* pk = reinterpret_cast<double*>(reinterpret_cast<char*>(dyn) + a.x_bytes);
   int* bins = reinterpret_cast<int*>(pk + a.nmax / 2 + 1);
-  double2* ch = reinterpret_cast<double2*>(reinterpret_cast<char*>(dyn) + a.chirp_off);
+  int* bfirst = bins + a.nmax / 2 + 1;
+  int* blast = bfirst + a.nb;
   const smk_skewers& sk = a.sk;
   for (int s = blockIdx.x; s < sk.n; s += gridDim.x) {
     double* part = a.partial + (int64_t)s * a.nb * 4;
@@ -274,7 +272,7 @@ __global__ __launch_bounds__(FIT_THREADS) void k_p1d(P1dArgs a) {
                       bins[j] = find_bin(a.edges, a.nb, kfreq(j, nq, k_ny));
                       if (xj) xj[j] = X;
                     },
-                    bline, ch, a.tw);
+                    bline, a.tw);
       __syncthreads();
     } else {
       load_delta_f(a.flux, sk.off[s], i0, i1, a.regroup, inv, x);
@@ -286,15 +284,29 @@ __global__ __launch_bounds__(FIT_THREADS) void k_p1d(P1dArgs a) {
       });
       __syncthreads();
     }
+    // k_j increases with j, so each bin holds a run of consecutive modes
+    // (bins[] is non-decreasing, then -1 past the last edge): find each
+    // bin's run [bfirst, blast] from the transitions, then sum it in
+    // ascending j (the order of a full scan, so the sums are the same)
+    for (int b = threadIdx.x; b < a.nb; b += FIT_THREADS) {
+      bfirst[b] = 0;
+      blast[b] = -1;
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < nj; j += FIT_THREADS) {
+      const int b = bins[j];
+      if (b < 0) continue;
+      if (j == 0 || bins[j - 1] != b) bfirst[b] = j;
+      if (j == nj - 1 || bins[j + 1] != b) blast[b] = j;
+    }
+    __syncthreads();
     for (int b = threadIdx.x; b < a.nb; b += FIT_THREADS) {
       double c = 0.0, sk_ = 0.0, sp = 0.0, sp2 = 0.0;
-      for (int j = 0; j < nj; ++j) {
-        if (bins[j] == b) {
-          c += 1.0;
-          sk_ += kfreq(j, nq, k_ny);
-          sp += pk[j];
-          sp2 += pk[j] * pk[j];
-        }
+      for (int j = bfirst[b]; j <= blast[b]; ++j) {
+        c += 1.0;
+        sk_ += kfreq(j, nq, k_ny);
+        sp += pk[j];
+        sp2 += pk[j] * pk[j];
       }
       part[4 * b + 0] = c;
       part[4 * b + 1] = sk_;
@@ -323,7 +335,6 @@ struct AdjArgs {
   const int64_t* xoff;
   const double2* tw;     // W_4096^k (FP64), for the Bluestein FFTs
   size_t line_off;       // adj_line_off(nmax): byte offset of the Bluestein line
-  int line_m;            // its length (the largest M of the batch)
 };
 
 // Adjoint of the binned P1D through delta_F = F/<F> - 1 (at fixed <F>) and
@@ -333,8 +344,6 @@ __global__ __launch_bounds__(FIT_THREADS) void k_p1d_adjoint(AdjArgs a) {
   double* x = dyn;
   double2* cj = reinterpret_cast<double2*>(dyn + a.nmax);
   double2* bline = reinterpret_cast<double2*>(reinterpret_cast<char*>(dyn) + a.line_off);
-  double2* ch = reinterpret_cast<double2*>(reinterpret_cast<char*>(dyn) + a.line_off) +
-                f64::line_len(a.line_m);
   __shared__ double red[FIT_THREADS / 64];
   const smk_skewers& sk = a.sk;
   for (int s = blockIdx.x; s < sk.n; s += gridDim.x) {
@@ -374,7 +383,7 @@ __global__ __launch_bounds__(FIT_THREADS) void k_p1d_adjoint(AdjArgs a) {
                       const double g = coef(j);
                       cj[j] = make_double2(g * X.x, -g * X.y);
                     },
-                    bline, ch, a.tw);
+                    bline, a.tw);
     } else {
       dft_all<false>(x, nullptr, nq, nq, nj, [&](int j, double re, double im) {
         const double g = coef(j);
@@ -392,7 +401,7 @@ __global__ __launch_bounds__(FIT_THREADS) void k_p1d_adjoint(AdjArgs a) {
                       saf += X.x * (x[q] + 1.0) * fm;
                       x[q] = X.x;
                     },
-                    bline, ch, a.tw);
+                    bline, a.tw);
     } else {
       dft_all<true>(nullptr, cj, nj, nq, nq, [&](int q, double re, double im) {
         saf += re * (x[q] + 1.0) * fm;
@@ -501,8 +510,8 @@ extern "C" int smk_p1d_binned(const smk_skewers* sk, const double* flux, const d
   const double2* tw = fp64_twiddles(&te);
   SMK_REQUIRE(tw != nullptr, SMK_ERR_HIP, "FP64 twiddle table: %s", hipGetErrorString(te));
   P1dArgs a{*sk, flux, fmean, pixel, regroup, edges, nbins, partial, status, nmax, (double2*)modes, modes_off,
-            tw, p1d_x_bytes(nmax), p1d_chirp_off(nmax)};
-  const size_t lds = p1d_lds_bytes(nmax);
+            tw, p1d_x_bytes(nmax)};
+  const size_t lds = p1d_lds_bytes(nmax, nbins);
   SMK_HIP_CHECK(hipFuncSetAttribute((const void*)k_p1d, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   hipLaunchKernelGGL(k_p1d, dim3(grid_for(sk->n)), dim3(FIT_THREADS), lds, (hipStream_t)stream, a);
   SMK_LAUNCH_CHECK();
@@ -529,7 +538,7 @@ extern "C" int smk_p1d_adjoint(const smk_skewers* sk, const float* delta, const 
   const double2* tw = fp64_twiddles(&te);
   SMK_REQUIRE(tw != nullptr, SMK_ERR_HIP, "FP64 twiddle table: %s", hipGetErrorString(te));
   AdjArgs a{*sk,  delta, eta,   abcg, abcg_rows, flux, fmean, pixel, regroup, edges,
-            nbins, gp,   out, nmax, (const double2*)modes, modes_off, tw, adj_line_off(nmax), blue_m(nmax)};
+            nbins, gp,   out, nmax, (const double2*)modes, modes_off, tw, adj_line_off(nmax)};
   const size_t lds = adj_lds_bytes(nmax);
   SMK_HIP_CHECK(
       hipFuncSetAttribute((const void*)k_p1d_adjoint, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

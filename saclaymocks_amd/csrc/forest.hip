@@ -422,23 +422,26 @@ __global__ __launch_bounds__(256) void k_gather_plane_tri(GatherArgs a) {
 
 // ------------------------------------------------------------ missing power
 // Fourier modes of real N(0,1) white noise of length 2M: interior modes have
-// Re, Im ~ N(0, M); k = 0 and k = M are real with variance 2M.  Modes 2j and
-// 2j + 1 share one Philox call (counter j): words 0, 1 give mode 2j, words
-// 2, 3 mode 2j + 1, each through Box-Muller.
+// Re, Im ~ N(0, M); k = 0 and k = M are real with variance 2M.  M/2 Philox
+// calls per skewer (counter c), each two Box-Muller pairs: call 0 gives
+// modes 0 and M (the first pair, one normal each) and mode 1 (the second);
+// call c >= 1 gives modes 2c (first pair) and 2c + 1 (second pair).
 constexpr uint32_t MP_TAG = 0x6d697373u;
 
-__device__ __forceinline__ float2 mp_scale_mode(float n0, float n1, int k, int M, float f) {
-  if (k == 0 || k == M) return make_float2(n0 * sqrtf(2.0f * M) * f, 0.0f);
-  const float sm = sqrtf((float)M) * f;
-  return make_float2(n0 * sm, n1 * sm);
+__device__ __forceinline__ U4 mp_call(uint64_t seed, uint64_t id, int c) {
+  return philox4x32_10_c4((uint32_t)c, (uint32_t)id, (uint32_t)(id >> 32), MP_TAG, seed);
 }
 
 __device__ __forceinline__ float2 mp_mode(uint64_t seed, uint64_t id, int k, int M, const float* __restrict__ filt) {
-  const U4 u = philox4x32_10_c4((uint32_t)k >> 1, (uint32_t)id, (uint32_t)(id >> 32), MP_TAG, seed);
+  const int c = (k == M || k == 1) ? 0 : k >> 1;
+  const U4 u = mp_call(seed, id, c);
   float n0, n1;
-  if (k & 1) box_muller_hw(u.z, u.w, n0, n1);
-  else box_muller_hw(u.x, u.y, n0, n1);
-  return mp_scale_mode(n0, n1, k, M, filt[k]);
+  if (k == 0 || k == M || (k & 1) == 0) box_muller_hw(u.x, u.y, n0, n1);
+  else box_muller_hw(u.z, u.w, n0, n1);
+  if (k == 0) return make_float2(n0 * sqrtf(2.0f * M) * filt[k], 0.0f);
+  if (k == M) return make_float2(n1 * sqrtf(2.0f * M) * filt[k], 0.0f);
+  const float sm = sqrtf((float)M) * filt[k];
+  return make_float2(n0 * sm, n1 * sm);
 }
 
 template <int M, int T>
@@ -576,7 +579,6 @@ __global__ void k_classify(smk_skewers sk, const int32_t* __restrict__ fset, int
 // interpolation (cells, weights, the four z columns) are computed once per
 // skewer; along z the position advances by dr/D per pixel from one division
 // per skewer.  Float32 weights and sums (the result is stored as float32).
-constexpr int STEP_NT = 256;
 
 struct StepArgs {
   smk_skewers sk;
@@ -603,19 +605,21 @@ struct StepArgs {
   double hx, hy, hz;
 };
 
-// register tile of the fused missing-power FFT of M points: all STEP_NT
-// threads on one line from M = 512 up; shorter lines are transformed by
-// STEP_NT / (M / E) thread groups, each on its own LDS line
-constexpr int step_E(int M) { return M >= 512 ? M / STEP_NT : 2; }
-constexpr int step_groups(int M) { return STEP_NT / (M / step_E(M)); }
-constexpr int step_lds_float2(int M) {
-  const int lines = step_groups(M) * lds_line_len(M);
+// Block of the fused step: one wave per skewer (NT = 64) for batches whose
+// FFTs are at most 2048 points (nz <= 2048, n <= 1998), four waves (NT =
+// 256) for longer ones.  Register tile of the missing-power FFT of M points:
+// all NT threads on one line when M >= 2 NT, else NT / (M / E) thread groups,
+// each transforming the same line in an LDS line of its own.
+constexpr int step_E(int M, int NT) { return M >= 2 * NT ? M / NT : 2; }
+constexpr int step_groups(int M, int NT) { return NT / (M / step_E(M, NT)); }
+constexpr int step_lds_float2(int M, int NT) {
+  const int lines = step_groups(M, NT) * lds_line_len(M);
   return lines > M + 1 ? lines : M + 1;
 }
-constexpr int step_lds_max(int c) {
+constexpr int step_lds_max(int c, int NT) {
   int m = 0;
   for (int i = 0; i <= c; ++i) {
-    const int v = step_lds_float2(128 << i);
+    const int v = step_lds_float2(128 << i, NT);
     m = v > m ? v : m;
   }
   return m;
@@ -623,26 +627,32 @@ constexpr int step_lds_max(int c) {
 
 // delta_s of skewer s (n pixels) into dsf[0, n) (LDS floats); length class
 // C, M = 128 << C
-template <int C>
+template <int C, int NT>
 __device__ __forceinline__ void step_missing_power(const StepArgs& a, int s, int n, uint64_t seed,
                                                    float2* __restrict__ sh) {
   constexpr int M = 128 << C;
-  constexpr int E = step_E(M);
+  constexpr int E = step_E(M, NT);
   constexpr int P = M / E;
   constexpr int LL = lds_line_len(M);
-  static_assert(P * step_groups(M) == STEP_NT, "thread groups must tile the block");
+  static_assert(P * step_groups(M, NT) == NT, "thread groups must tile the block");
   const int tid = threadIdx.x;
   const float* __restrict__ filt = a.filt + a.foff[C] + (a.fset ? (int64_t)a.fset[s] * a.set_stride : 0);
   const uint64_t id = a.sk.noise_id ? a.sk.noise_id[s] : (uint64_t)s;
-  // (1) modes k = 0..M, two per Philox call
-  for (int j = tid; 2 * j <= M; j += STEP_NT) {
-    const U4 u = philox4x32_10_c4((uint32_t)j, (uint32_t)id, (uint32_t)(id >> 32), MP_TAG, seed);
-    float n0, n1;
+  // (1) modes k = 0..M: M/2 Philox calls (see mp_mode)
+  const float sr = sqrtf((float)M), s2 = sqrtf(2.0f * M);
+  for (int c = tid; c < M / 2; c += NT) {
+    const U4 u = mp_call(seed, id, c);
+    float n0, n1, n2, n3;
     box_muller_hw(u.x, u.y, n0, n1);
-    sh[2 * j] = mp_scale_mode(n0, n1, 2 * j, M, filt[2 * j]);
-    if (2 * j + 1 <= M) {
-      box_muller_hw(u.z, u.w, n0, n1);
-      sh[2 * j + 1] = mp_scale_mode(n0, n1, 2 * j + 1, M, filt[2 * j + 1]);
+    box_muller_hw(u.z, u.w, n2, n3);
+    if (c == 0) {
+      sh[0] = make_float2(n0 * s2 * filt[0], 0.0f);
+      sh[M] = make_float2(n1 * s2 * filt[M], 0.0f);
+      sh[1] = make_float2(n2 * sr * filt[1], n3 * sr * filt[1]);
+    } else {
+      const float f0 = filt[2 * c] * sr, f1 = filt[2 * c + 1] * sr;
+      sh[2 * c] = make_float2(n0 * f0, n1 * f0);
+      sh[2 * c + 1] = make_float2(n2 * f1, n3 * f1);
     }
   }
   __syncthreads();
@@ -677,9 +687,11 @@ __device__ __forceinline__ void step_missing_power(const StepArgs& a, int s, int
   __syncthreads();
 }
 
-// MAXC: the longest length class of the batch (classes above it are flagged)
-template <int MAXC>
-__global__ __launch_bounds__(STEP_NT) void k_forest_step(StepArgs a) {
+// MAXC: the longest length class of the batch (classes above it are
+// flagged); NT threads per block
+template <int MAXC, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 64 ? 4 : 1))) void k_forest_step(
+    StepArgs a) {
   extern __shared__ float2 sh[];
   __shared__ int s_next;
   const int tid = threadIdx.x;
@@ -707,15 +719,15 @@ __global__ __launch_bounds__(STEP_NT) void k_forest_step(StepArgs a) {
       // tables (possible only in graph mode, which skips the host check):
       // flagged, flux NaN
       if (tid == 0 && a.status) atomicOr(&a.status[s], 8);
-      for (int i = tid; i < n; i += STEP_NT) a.flux[off + i] = __int_as_float(0x7fc00000);
+      for (int i = tid; i < n; i += NT) a.flux[off + i] = __int_as_float(0x7fc00000);
       continue;
     }
-    if (c == 0) step_missing_power<0>(a, s, n, seed, sh);
-    else if (c == 1) step_missing_power<1>(a, s, n, seed, sh);
-    else if (c == 2) step_missing_power<2>(a, s, n, seed, sh);
-    else if (c == 3 || MAXC <= 3) step_missing_power<(MAXC < 3 ? MAXC : 3)>(a, s, n, seed, sh);
-    else if (c == 4 || MAXC <= 4) step_missing_power<(MAXC < 4 ? MAXC : 4)>(a, s, n, seed, sh);
-    else step_missing_power<MAXC>(a, s, n, seed, sh);
+    if (c == 0) step_missing_power<0, NT>(a, s, n, seed, sh);
+    else if (c == 1) step_missing_power<1, NT>(a, s, n, seed, sh);
+    else if (c == 2) step_missing_power<2, NT>(a, s, n, seed, sh);
+    else if (c == 3 || MAXC <= 3) step_missing_power<(MAXC < 3 ? MAXC : 3), NT>(a, s, n, seed, sh);
+    else if (c == 4 || MAXC <= 4) step_missing_power<(MAXC < 4 ? MAXC : 4), NT>(a, s, n, seed, sh);
+    else step_missing_power<MAXC, NT>(a, s, n, seed, sh);
     const float* dsf = reinterpret_cast<const float*>(sh);
     // per-skewer x and y terms (make_spectra.py:47-49 positions; trilinear on
     // cell-centred values, periodic indices in a periodic box)
@@ -760,7 +772,7 @@ __global__ __launch_bounds__(STEP_NT) void k_forest_step(StepArgs a) {
     bool row_ok = per_pixel_row || (srow >= 0 && srow < a.abcg_rows);
     if (!per_pixel_row && row_ok) prow = reinterpret_cast<const float4*>(a.abcg)[srow];
     int flags = 0;
-    for (int i = tid; i < n; i += STEP_NT) {
+    for (int i = tid; i < n; i += NT) {
       float ds = dsf[i];
       if (a.scale) ds *= a.scale[off + i];
       float dl = -1000000.0f, eta = 0.0f;
@@ -1071,21 +1083,24 @@ extern "C" int smk_forest_step(smk_forest_plan* plan, const smk_skewers* sk, con
     SMK_HIP_CHECK(hipStreamSynchronize(s));
     SMK_REQUIRE(h == 0, SMK_ERR_RANGE, "a filter set index is outside [0, %d)", (int)n_sets);
   }
-  // kernel instance: classes up to 3 (nz <= 2048, n <= 1998) or all six;
-  // skewers of a class above max_npix's are flagged (a.max_class)
+  // kernel instance: one wave per skewer for classes up to 3 (nz <= 2048,
+  // n <= 1998), four waves with all six classes; skewers of a class above
+  // max_npix's are flagged (a.max_class)
   const bool small = c <= 3;
-  const size_t lds = (size_t)step_lds_max(small ? 3 : N_CLASS - 1) * sizeof(float2);
+  constexpr int NT_S = 64, NT_L = 256;
+  const int nt = small ? NT_S : NT_L;
+  const size_t lds = (size_t)(small ? step_lds_max(3, NT_S) : step_lds_max(N_CLASS - 1, NT_L)) * sizeof(float2);
   int dev = 0, ncu = 0, per_cu = 0;
   SMK_HIP_CHECK(hipGetDevice(&dev));
   SMK_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   SMK_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &per_cu, small ? k_forest_step<3> : k_forest_step<N_CLASS - 1>, STEP_NT, lds));
+      &per_cu, small ? k_forest_step<3, NT_S> : k_forest_step<N_CLASS - 1, NT_L>, nt, lds));
   const int64_t want = (int64_t)ncu * (per_cu > 0 ? per_cu : 1);
   const int nblk = (int)(want < sk->n ? want : sk->n);
   hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, a.queue, 1);
   SMK_LAUNCH_CHECK();
-  if (small) hipLaunchKernelGGL(k_forest_step<3>, dim3(nblk), dim3(STEP_NT), lds, s, a);
-  else hipLaunchKernelGGL(k_forest_step<N_CLASS - 1>, dim3(nblk), dim3(STEP_NT), lds, s, a);
+  if (small) hipLaunchKernelGGL((k_forest_step<3, NT_S>), dim3(nblk), dim3(NT_S), lds, s, a);
+  else hipLaunchKernelGGL((k_forest_step<N_CLASS - 1, NT_L>), dim3(nblk), dim3(NT_L), lds, s, a);
   SMK_LAUNCH_CHECK();
   return SMK_OK;
 }
