@@ -315,6 +315,21 @@ int smk_forest_step(smk_forest_plan* plan, const smk_skewers* sk, const smk_box_
                     float* flux, float* delta_l, float* eta_par, float* delta_s, int32_t* status, void* workspace,
                     void* stream);
 
+/* smk_forest_step without a workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops); it
+ * has no counterpart in the reference.  The step's work queue lives in a
+ * static device array of smk_forest_step_noalloc_scratch_bytes() bytes per
+ * GPU (held until the process ends); each call clears what it uses.  All
+ * calls on one GPU share that array, so calls that overlap on one GPU race.
+ * Returns SMK_ERR_RANGE if the step ever needs more scratch than the array
+ * holds; otherwise as smk_forest_step.  Calls no hipMalloc/hipFree. */
+size_t smk_forest_step_noalloc_scratch_bytes(void);
+int smk_forest_step_noalloc(smk_forest_plan* plan, const smk_skewers* sk, const smk_box_view* box,
+                            const float* filt, const int64_t* filt_off, int64_t set_stride, int32_t n_sets,
+                            const int32_t* filt_set, const float* scale, uint64_t seed, const float* abcg,
+                            int64_t abcg_rows, int32_t max_npix, float* flux, float* delta_l, float* eta_par,
+                            float* delta_s, int32_t* status, void* stream);
+
 /* Missing power with a P1D_miss table chosen per skewer, merge_spectra.py:311-322
  * without --fit-p1d: P1D_miss(z, k) is the table at the redshift nearest the
  * QSO's mean forest redshift z_eff (util.py:448-474, InterpP1Dmissing), and

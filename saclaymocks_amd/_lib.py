@@ -75,6 +75,9 @@ _SIGNATURES = {
                                            _vp, _u64, _vp, _vp, _vp, _vp, _vp]),
     "smk_fgpa_flux": (_i, [_vp, _vp, _vp, _vp, _c.c_int64, _vp, _vp, _vp]),
     "smk_forest_step_workspace_bytes": (_sz, []),
+    "smk_forest_step_noalloc_scratch_bytes": (_sz, []),
+    "smk_forest_step_noalloc": (_i, [_vp, _vp, _vp, _vp, _c.POINTER(_c.c_int64), _c.c_int64, _c.c_int32, _vp, _vp,
+                                     _u64, _vp, _c.c_int64, _c.c_int32, _vp, _vp, _vp, _vp, _vp, _vp]),
     "smk_forest_step": (_i, [_vp, _vp, _vp, _vp, _c.POINTER(_c.c_int64), _c.c_int64, _c.c_int32, _vp, _vp, _u64,
                              _vp, _c.c_int64, _c.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "smk_qso_draw": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _vp, _c.c_int64, _vp, _vp]),
