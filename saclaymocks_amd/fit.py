@@ -39,13 +39,10 @@ class P1DFitModel:
         self.eta = None if eta is None else eta.contiguous()
         self.growth = np.asarray(growth, dtype=np.float64)
         self.n_sets = len(self.growth)
-        self.edges_host = np.asarray(edges, dtype=np.float64)
-        self.nb = len(self.edges_host) - 1
         self.pixel = float(pixel)
         self.regroup = int(regroup)
         dev = delta.device
         self.dev = dev
-        self.edges = torch.from_numpy(self.edges_host).to(dev)
         imin = _sync_host(batch.imin).astype(np.int64)
         imax = _sync_host(batch.imax).astype(np.int64)
         pset = _sync_host(batch.pset).astype(np.int64)
@@ -58,8 +55,7 @@ class P1DFitModel:
         self.flux = torch.empty(batch.npix_total, dtype=torch.float64, device=dev)
         self.fsum = torch.empty(n, dtype=torch.float64, device=dev)
         self.fsum_set = torch.empty(self.n_sets, dtype=torch.float64, device=dev)
-        self.partial = torch.empty((n, self.nb, 4), dtype=torch.float64, device=dev)
-        self.bins_set = torch.empty((self.n_sets, self.nb, 4), dtype=torch.float64, device=dev)
+        self.set_edges(edges)
         self.adj = torch.empty((n, 7), dtype=torch.float64, device=dev)
         self.adj_set = torch.empty((self.n_sets, 7), dtype=torch.float64, device=dev)
         self.fmean_dev = torch.empty(self.n_sets, dtype=torch.float64, device=dev)
@@ -73,6 +69,16 @@ class P1DFitModel:
         self.modes_off = torch.from_numpy(moff).to(dev) if save_modes else None
         self.n_eval = 0
         self.n_grad = 0
+
+    def set_edges(self, edges):
+        """New k bin edges (h/Mpc, increasing) for the following evaluations."""
+        self.edges_host = np.asarray(edges, dtype=np.float64)
+        if self.edges_host.ndim != 1 or len(self.edges_host) < 2 or np.any(np.diff(self.edges_host) <= 0):
+            raise ValueError("edges must be an increasing 1-D array of at least two values")
+        self.nb = len(self.edges_host) - 1
+        self.edges = torch.from_numpy(self.edges_host).to(self.dev)
+        self.partial = torch.empty((self.batch.n, self.nb, 4), dtype=torch.float64, device=self.dev)
+        self.bins_set = torch.empty((self.n_sets, self.nb, 4), dtype=torch.float64, device=self.dev)
 
     # ------------------------------------------------------------ forward
     def _abcg(self, theta):
@@ -92,17 +98,30 @@ class P1DFitModel:
         _lib.check(L.smk_reduce_by_set(ctypes.byref(self.sk), _lib.ptr(self.fsum), 1, self.n_sets,
                                        _lib.ptr(self.fsum_set), s), "smk_reduce_by_set")
         fmean = _sync_host(self.fsum_set) / self.count
-        self.fmean_dev.copy_(torch.from_numpy(fmean))
-        _lib.check(L.smk_p1d_binned(ctypes.byref(self.sk), _lib.ptr(self.flux), _lib.ptr(self.fmean_dev),
-                                    self.pixel, self.regroup, _lib.ptr(self.edges), self.nb, self.max_forest,
-                                    _lib.ptr(self.partial), _lib.ptr(self.status), _lib.ptr(self.modes),
-                                    _lib.ptr(self.modes_off), s), "smk_p1d_binned")
-        _lib.check(L.smk_reduce_by_set(ctypes.byref(self.sk), _lib.ptr(self.partial), self.nb * 4, self.n_sets,
-                                       _lib.ptr(self.bins_set), s), "smk_reduce_by_set")
-        bins = _sync_host(self.bins_set)
+        bins = self._binned(self.flux, fmean, self.modes)
         self.n_eval += 1
         self._last = (np.array(theta, dtype=np.float64), abcg, fmean, bins)
         return fmean, bins
+
+    def _binned(self, flux, fmean, modes):
+        L = _lib.lib()
+        s = _lib.stream_ptr()
+        self.fmean_dev.copy_(torch.from_numpy(np.asarray(fmean, dtype=np.float64)))
+        _lib.check(L.smk_p1d_binned(ctypes.byref(self.sk), _lib.ptr(flux), _lib.ptr(self.fmean_dev),
+                                    self.pixel, self.regroup, _lib.ptr(self.edges), self.nb, self.max_forest,
+                                    _lib.ptr(self.partial), _lib.ptr(self.status), _lib.ptr(modes),
+                                    _lib.ptr(self.modes_off if modes is not None else None), s), "smk_p1d_binned")
+        _lib.check(L.smk_reduce_by_set(ctypes.byref(self.sk), _lib.ptr(self.partial), self.nb * 4, self.n_sets,
+                                       _lib.ptr(self.bins_set), s), "smk_reduce_by_set")
+        return _sync_host(self.bins_set)
+
+    def binned_flux(self, flux, fmean):
+        """Binned P1D sums per set of a given flux (float64 CUDA tensor, one
+        value per pixel of the batch) at the given mean flux per set, as
+        compute_p1d(debug=True) bins the mock's own FLUX (fit_az.py:194-198)."""
+        if flux.dtype != torch.float64 or flux.numel() != self.batch.npix_total or not flux.is_cuda:
+            raise ValueError("flux must be a float64 CUDA tensor with one value per pixel of the batch")
+        return self._binned(flux.contiguous(), fmean, None)
 
     @staticmethod
     def profile(bins):

@@ -119,7 +119,7 @@ class SkewerBatch:
         self.imax = _t(npix if imax is None else imax, torch.int32, dev)
         self.off = _t(off, torch.int64, dev)
         self.pset = None if pset is None else _t(pset, torch.int32, dev)
-        nid = np.arange(n, dtype=np.uint64) if noise_id is None else np.asarray(noise_id, dtype=np.uint64)
+        nid = np.arange(n, dtype=np.uint64) if noise_id is None else np.ascontiguousarray(noise_id, dtype=np.uint64)
         self.noise_id = torch.as_tensor(nid.view(np.int64)).to(dev)
         self.g0_host = np.asarray(g0, dtype=np.int64)
         # the gather's per-block skewer table: the skewer of every

@@ -57,6 +57,24 @@ class Cosmo:
     def dist_hubble(self, z):
         return np.interp(z, self.z, C_KMS / self.hubble)
 
+    def hubble_at(self, z):
+        """H(z) in km/s/Mpc, linear between the grid nodes (util.py:826)."""
+        return np.interp(z, self.z, self.hubble)
+
+
+_COSMO_FID = None
+
+
+def kms2mpc(redshift):
+    """util.py:836-851: the factor converting km/s to Mpc/h at `redshift`,
+    H(z) / ((1 + z) h), with H from the fiducial cosmology (Om = 0.31457,
+    Ok = 0, H0 = 100 h)."""
+    global _COSMO_FID
+    if _COSMO_FID is None:
+        _COSMO_FID = Cosmo(Om=OMEGA_M, Ok=OMEGA_K, H0=100.0 * H)
+    z = np.asarray(redshift, dtype=np.float64)
+    return _COSMO_FID.hubble_at(z) / (1.0 + z) / H
+
 
 def fgrowth(z, Om0=OMEGA_M, unnormed=False):
     """util.py:123-134."""
@@ -118,6 +136,79 @@ class InterpTable:
             raise ValueError(f"ERROR: Redshift {redshift} is out of range ! "
                              f"Please enter a redshift between {self.zmin} and {self.zmax}")
         return np.interp(r, self.z, self.y)
+
+
+def _data_file(name):
+    import os
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", name)
+
+
+def read_P1D(redshift, filename=None, mpc=False):
+    """util.py:172-195: the BOSS DR12 P1D (Si III removed) at `redshift`:
+    k, P, sigma_P in s/km and km/s, or in h/Mpc and Mpc/h with mpc=True.
+    Rows are those whose z, rounded to 3 decimals, equals the rounded
+    redshift (2.2 to 4.6 by 0.2); no row raises ValueError.  filename None
+    reads the packaged copy of etc/pk_1d_DR12_13bins_noSi.out, else a text
+    file of the same columns."""
+    import os
+    if filename is None:
+        data = np.load(_data_file("p1d_dr12_noSi.npy"), allow_pickle=False)
+    else:
+        data = np.loadtxt(os.path.expandvars(filename))
+    z = np.round(data[:, 0], 3)
+    msk = np.nonzero(z == np.round(redshift, 3))[0]
+    if len(msk) == 0:
+        raise ValueError(f"You entered a wrong redshift: {redshift}. Here is the list of redshift : {np.unique(z)}")
+    k, pk, err = (np.array(data[msk, c], dtype=np.float64) for c in (1, 2, 3))
+    if mpc:
+        f = kms2mpc(redshift)
+        k, pk, err = k * f, pk / f, err / f
+    return k, pk, err
+
+
+# util.py:234: per-bin amplitude correction smoothing sigma_F(z), bins z = 1.8 + 0.2 i
+_P1D_MODEL_CORR = np.array([1.0037348, 0.99913817, 0.99267796, 0.99769447, 1.00333914, 1.00691257,
+                            1.00574199, 0.99331247, 0.99156929, 1.00622274])
+
+
+def read_P1D_model(redshift, filename=None, mpc=False, z_corr=True):
+    """util.py:227-271: the model P1D (Arinyo-i-Prats fit) that the
+    missing-power iteration aims at.  z = 2.0 and z = 1.8 are extrapolated
+    from the z = 2.2 and 2.4 rows (P22^2 / P24 and P22^3 / P24^2); other
+    redshifts take the rows within 0.01 (2.2 to 3.6).  With z_corr, P is
+    multiplied by corr_i^2, i = round((z - 1.8) / 0.2).  filename None reads
+    the packaged copy of etc/P1DmodelPrats.fits, else a FITS file with the
+    three image HDUs z, k, P."""
+    import os
+    if filename is None:
+        t = np.load(_data_file("p1d_model_prats.npy"), allow_pickle=False)
+        z, k, pk = t[:, 0], t[:, 1], t[:, 2]
+    else:
+        from . import fitsio_lite
+        z, k, pk = (np.asarray(h.data, dtype=np.float64)
+                    for h in fitsio_lite.read_all(os.path.expandvars(filename))[:3])
+    i = int(round((redshift - 1.8) / 0.2))
+    if z_corr:
+        if not 0 <= i < len(_P1D_MODEL_CORR):
+            raise ValueError(f"no P1D model correction for redshift {redshift}")
+        cor = _P1D_MODEL_CORR[i] ** 2
+    else:
+        cor = 1.0
+    if abs(redshift - 2) < 1e-2 or abs(redshift - 1.8) < 1e-2:
+        m1, m2 = np.abs(z - 2.2) < 1e-2, np.abs(z - 2.4) < 1e-2
+        p = 2 if abs(redshift - 2) < 1e-2 else 3
+        k, pk = k[m1], pk[m1] ** p / pk[m2] ** (p - 1)
+    else:
+        m = np.abs(z - redshift) < 1e-2
+        if m.sum() == 0:
+            raise ValueError(f"You entered a wrong redshift: {redshift}. Here is the list of redshifts : "
+                             f"{np.unique(z)}")
+        k, pk = k[m], pk[m]
+    k, pk = np.array(k), np.array(pk)
+    if mpc:
+        f = kms2mpc(redshift)
+        k, pk = k * f, pk / f
+    return k, pk * cor
 
 
 def fgpa_param_tables():
