@@ -84,26 +84,54 @@ def cpu_model():
     return {"model": name, "affinity": len(os.sched_getaffinity(0))}
 
 
-def cpu_baseline(cell, seconds=12.0):
-    """FP64 oracle of the same job on a 256^3 sample, timed on this host."""
-    import numpy as np
+def _box_fp32(w, tabs, workers):
+    """The job as the reference computes it (make_boxes.py:40-108: complex64
+    FFTW): float32 noise, scipy.fft single-precision rfftn, x sqrt(P/Vcell)
+    in float32, irfftn of delta and of eta_zz.  tabs: the two float32 k-space
+    factors, made once per shape (the reference also makes them once)."""
+    import scipy.fft as sfft
+    boxk = sfft.rfftn(w, workers=workers)
+    boxk *= tabs[0]
+    d = sfft.irfftn(boxk, s=w.shape, workers=workers)
+    boxk *= tabs[1]
+    e = sfft.irfftn(boxk, s=w.shape, workers=workers)
+    assert d.dtype == np.float32 and e.dtype == np.float32
+    return d, e
+
+
+def cpu_baseline(cell, seconds=10.0):
+    """The same job on a 256^3 sample, timed on this host with all its cores
+    (at most 16): the FP32 port (the reference's precision) is the value, the
+    FP64 oracle is reported beside it."""
     from oracle import box_oracle
     cores = max(1, min(len(os.sched_getaffinity(0)), 16))
     shape = (256, 256, 256)
-    n = 0
-    t0 = time.perf_counter()
     rng = np.random.default_rng(0)
-    while True:
-        w = rng.standard_normal(shape)
-        box_oracle.grf_box(w, cell, aux="eta_zz", workers=cores)
-        n += 1
-        if time.perf_counter() - t0 >= seconds:
-            break
-    dt = time.perf_counter() - t0
-    cells = n * shape[0] * shape[1] * shape[2]
-    return {"value": cells / dt, "unit": "cells/s", "cores": cores, "kind": "port",
-            "sample": f"{n} x 256^3 box: numpy N(0,1) noise, scipy.fft FP64 rfftn, "
-                      f"x sqrt(P/Vcell), irfftn delta + eta_zz ({dt:.1f} s)", "host": cpu_model()}
+    rates = {}
+    tabs = (box_oracle.sqrt_p_over_vcell(*shape, cell).astype(np.float32),
+            box_oracle.aux_factor("eta_zz", *shape, cell, 1.0).astype(np.float32))
+    for prec in ("fp32", "fp64"):
+        n = 0
+        t0 = time.perf_counter()
+        while True:
+            if prec == "fp32":
+                w = rng.standard_normal(shape, dtype=np.float32)
+                _box_fp32(w, tabs, cores)
+            else:
+                w = rng.standard_normal(shape)
+                box_oracle.grf_box(w, cell, aux="eta_zz", workers=cores)
+            n += 1
+            if time.perf_counter() - t0 >= seconds:
+                break
+        dt = time.perf_counter() - t0
+        rates[prec] = (n * shape[0] * shape[1] * shape[2] / dt, n, dt)
+    v, n, dt = rates["fp32"]
+    return {"value": v, "unit": "cells/s", "cores": cores, "kind": "port",
+            "sample": f"{n} x 256^3 box: numpy float32 N(0,1) noise, scipy.fft FP32 rfftn, x sqrt(P/Vcell), "
+                      f"irfftn delta + eta_zz, {cores} threads ({dt:.1f} s)",
+            "fp64": {"value": rates["fp64"][0], "unit": "cells/s",
+                     "sample": f"{rates['fp64'][1]} x 256^3 box, the FP64 oracle ({rates['fp64'][2]:.1f} s)"},
+            "host": cpu_model()}
 
 
 def forest_bench(delta, eta, shape, cell, steps, warmup, n_skewers):
@@ -270,11 +298,16 @@ def p1d_bench(steps, warmup):
 
 
 def cpu_baseline_forest(cell, n_skewers=20000):
-    """The oracle's numpy forest step (trilinear gather of delta and eta_zz,
-    missing power by rfft / irfft per skewer, FGPA) on n_skewers of the bench's
-    skewer distribution over a 128 x 128 x 1536 host box, one core."""
+    """The forest step on the host with all its cores (at most 16), from the
+    oracle's pieces: the trilinear gather of delta and eta_zz and the FGPA in
+    numpy over pixel chunks on a thread pool, the missing power as one
+    batched scipy.fft rfft / irfft per length class nz (workers = cores), on
+    n_skewers of the bench's skewer distribution over a 128 x 128 x 1536 box."""
+    import scipy.fft as sfft
+    from concurrent.futures import ThreadPoolExecutor
     from oracle import forest_oracle as fo
     from saclaymocks_amd import util, workloads
+    cores = max(1, min(len(os.sched_getaffinity(0)), 16))
     shape = (128, 128, 1536)
     rng = np.random.default_rng(5)
     d = rng.standard_normal(shape, dtype=np.float32)
@@ -289,18 +322,31 @@ def cpu_baseline_forest(cell, n_skewers=20000):
     g = np.concatenate([g0 + np.arange(k) for g0, k in zip(h["g0"], npix)])
     Z = h["fr"]["rmin"] + g * 0.2
     G = util.growthf(np.repeat(zq, npix))
-    t0 = time.perf_counter()
-    dl, eta = fo.trilinear([d, e], X, Y, Z, L, (cell,) * 3, h["fr"]["R0"])
     off = np.concatenate([[0], np.cumsum(npix)])
-    ds = np.empty(len(X))
-    for s, k in enumerate(npix):
-        nz = fo.fft_len_for(int(k))
-        ds[off[s]:off[s + 1]] = fo.missing_power(rng.standard_normal(nz), int(k), 0.2, workloads.p1d_missing)
-    flux = fo.fgpa(dl + ds, eta, G, 0.1, 1.5, 1.0)
-    dt = time.perf_counter() - t0
+    nzs = np.array([fo.fft_len_for(int(k)) for k in npix])
+    noise = {int(nz): rng.standard_normal((int((nzs == nz).sum()), int(nz))) for nz in np.unique(nzs)}
+    bounds = np.linspace(0, len(X), 4 * cores + 1).astype(np.int64)
+    chunks = [slice(int(bounds[i]), int(bounds[i + 1])) for i in range(4 * cores)]
+    with ThreadPoolExecutor(cores) as ex:
+        t0 = time.perf_counter()
+        parts = list(ex.map(lambda sl: fo.trilinear([d, e], X[sl], Y[sl], Z[sl], L, (cell,) * 3, h["fr"]["R0"]),
+                            chunks))
+        dl = np.concatenate([p[0] for p in parts])
+        eta = np.concatenate([p[1] for p in parts])
+        ds = np.empty(len(X))
+        for nz, w in noise.items():
+            k = np.fft.rfftfreq(nz) * 2 * (np.pi / 0.2)
+            filt = np.sqrt(np.maximum(workloads.p1d_missing(k), 0.0) / 0.2)
+            y = sfft.irfft(sfft.rfft(w, axis=1, workers=cores) * filt, n=nz, axis=1, workers=cores)
+            for r, s in enumerate(np.nonzero(nzs == nz)[0]):
+                ds[off[s]:off[s + 1]] = y[r, :npix[s]]
+        flux = np.concatenate(list(ex.map(lambda sl: fo.fgpa(dl[sl] + ds[sl], eta[sl], G[sl], 0.1, 1.5, 1.0),
+                                          chunks)))
+        dt = time.perf_counter() - t0
     assert np.all(np.isfinite(flux))
-    return {"value": len(X) / dt, "unit": "pixels/s", "cores": 1, "kind": "port",
-            "sample": f"{n_skewers} skewers, {len(X)} pixels (numpy trilinear, rfft missing power, FGPA; {dt:.1f} s)"}
+    return {"value": len(X) / dt, "unit": "pixels/s", "cores": cores, "kind": "port",
+            "sample": f"{n_skewers} skewers, {len(X)} pixels: numpy trilinear + FGPA on {cores} threads, missing "
+                      f"power as batched scipy.fft per length class ({dt:.2f} s)"}
 
 
 def cpu_baseline_qso(cell, planes=8):
