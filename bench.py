@@ -173,6 +173,16 @@ def forest_bench(delta, eta, shape, cell, steps, warmup, n_skewers):
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / steps, time.perf_counter() - t0, r
 
+    # the planner's choice between the step's kernel instances (same results)
+    ms_var, flux_var = [], []
+    for v in range(2):
+        plan.set_step_variant(v)
+        m_, _, r_ = timed(fused)
+        ms_var.append(m_)
+        flux_var.append(r_["flux"].clone())
+    assert torch.equal(flux_var[0], flux_var[1])
+    del flux_var
+    plan.set_step_variant(int(np.argmin(ms_var)))
     ms, dt, res = timed(fused)
     f = res["flux"].clone()
     assert int(res["status"].sum().item()) == 0
@@ -192,21 +202,25 @@ def forest_bench(delta, eta, shape, cell, steps, warmup, n_skewers):
     alg = float(np.sum(4 * span * 2 * 4) + 4 * npx + 32 * batch.n)
     return {"workload": f"{batch.n} plane-parallel skewers, 300-1500 px of 0.2 Mpc/h, trilinear gather of "
                         "delta and eta_zz, missing power 0.05 exp(-(k/1.8)^2), FGPA at z_QSO",
-            "pixels": npx, "skewers": batch.n, "ms_per_step": ms, "ms_per_step_graph": ms_graph,
+            "pixels": npx, "skewers": batch.n, "ms_per_step": ms, "ms_per_step_variants": ms_var,
+            "step_variant": int(np.argmin(ms_var)), "ms_per_step_graph": ms_graph,
             "ms_per_step_separate_launches": ms_sep, "separate_vs_fused_max_abs_dflux": sep_diff,
             "ms_gather_gauss343": ms_gauss,
             "pixels_per_s": npx * steps / dt,
             "mean_flux": float(f.double().mean().item()),
-            "roofline": {"bound": "valu", "alg_bytes": alg, "achieved": alg / (ms * 1e-3) / 1e9,
+            # waves wait on memory and barriers most of their life (SQ counters),
+            # so neither HBM nor VALU issue is saturated: both fractions are given
+            "roofline": {"bound": "latency", "alg_bytes": alg, "achieved": alg / (ms * 1e-3) / 1e9,
                          "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS,
-                         "valu": forest_valu_roofline(ms)}}
+                         "valu": forest_valu_roofline(ms, int(np.argmin(ms_var)))}}
 
 
-def forest_valu_roofline(ms):
+def forest_valu_roofline(ms, variant=0):
     """VALU issue of the fused forest step against the MI355X issue peak, from
-    the SQ_INSTS_VALU count of profiles/forest_step_200k_valu_counters.md (None
-    if that profile is absent): one wave64 VALU instruction per 4 cycles per
-    SIMD, 4 SIMDs x 256 CUs at 2.4 GHz = 6.14e11 wave-instructions/s."""
+    the SQ_INSTS_VALU count per step of the kernel instance `variant` in
+    profiles/forest_step_200k_valu_counters.md (None if that profile is
+    absent): one wave64 VALU instruction per 4 cycles per SIMD, 4 SIMDs x 256
+    CUs at 2.4 GHz = 6.14e11 wave-instructions/s."""
     path = os.path.join(ROOT, "profiles", "forest_step_200k_valu_counters.md")
     if not os.path.exists(path):
         return None
@@ -215,7 +229,7 @@ def forest_valu_roofline(ms):
     if not m:
         return None
     doc = json.loads(m.group(1))
-    insts = float(doc["valu_insts_per_step"])
+    insts = float(doc["valu_insts_per_step_by_variant"][variant])
     peak = 256 * 4 / 4 * 2.4e9
     achieved = insts / (ms * 1e-3)
     return {"valu_insts_per_step": insts, "achieved": achieved, "peak": peak, "unit": "wave-instr/s",
@@ -412,6 +426,22 @@ def fit_bench(delta, eta, shape, cell, per_bin, maxfev):
     theta0 = np.tile([0.1, 1.5, 1.0], len(zb))
     bounds = [(1e-7, 3.0), (0.1, 4.0), (0.0, 4.0)] * len(zb)
     model.value_and_grad(theta0)  # warm-up of every kernel
+    # one evaluation (host-synchronised, as the optimisers call it): forward,
+    # forward + adjoint; with the per-length Bluestein tables and without
+    eval_ms = {}
+    nt = fit.P1DFitModel(b, d, e, growth, edges, use_tables=False)
+    for name, mdl in (("tables", model), ("no_tables", nt)):
+        mdl.set_targets(model.f_target, model.f_sigma, model.p_target, model.p_sigma)
+        mdl.value_and_grad(theta0)
+        t0 = time.perf_counter()
+        for _ in range(10):
+            mdl.forward(theta0)
+        t1 = time.perf_counter()
+        for _ in range(10):
+            mdl.value_and_grad(theta0)
+        t2 = time.perf_counter()
+        eval_ms[name] = {"forward": (t1 - t0) * 100.0, "forward_and_adjoint": (t2 - t1) * 100.0}
+    del nt
     plains = {m: fit.fit_plain(model, theta0, bounds, maxfev=maxfev, method=m) for m in ("powell", "fd-lbfgs")}
     # the gradient fit has to reach the better of the plain fits' chi2
     plain = min(plains.values(), key=lambda r: r["chi2"])
@@ -428,6 +458,7 @@ def fit_bench(delta, eta, shape, cell, per_bin, maxfev):
                               "fd-lbfgs": "L-BFGS-B on 2-point finite-difference gradients (migrad-like)"},
             "gradient": {"method": "L-BFGS-B on adjoint gradients", "chi2": grad["chi2"],
                          "wall_s": grad["wall_s"], "n_eval": grad["n_eval"], "status": grad["message"]},
+            "eval_ms": eval_ms,
             "target_chi2": target,
             "reached_plain_chi2": bool(grad["chi2"] <= target),
             "speedup": {m: r["wall_s"] / grad["wall_s"] for m, r in plains.items()}}

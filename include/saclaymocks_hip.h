@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define SMK_ABI_VERSION 5
+#define SMK_ABI_VERSION 6
 
 #define SMK_OK 0
 #define SMK_ERR_ARG (-1)         /* bad argument (null pointer, size, enum) */
@@ -279,6 +279,12 @@ int smk_forest_plan_destroy(smk_forest_plan* plan);
  * (a capture may not synchronise): run the batch once eagerly first, which
  * checks it.  NULL restores the normal mode. */
 int smk_forest_plan_set_graph_mode(smk_forest_plan* plan, const uint64_t* seed_dev);
+/* Kernel instance of smk_forest_step for batches whose FFTs are at most 2048
+ * points (the planner's choice, no counterpart in the reference): 0 (the
+ * default) keeps at least 4 waves per SIMD, 1 at least 3 with no register
+ * spills.  Same results, bit for bit. */
+#define SMK_FOREST_STEP_NVAR 2
+int smk_forest_plan_set_step_variant(smk_forest_plan* plan, int variant);
 size_t smk_forest_workspace_bytes(int n_skewers);   /* class bucketing scratch */
 /*   filt_off    HOST int64[6]
  *   workspace   smk_forest_workspace_bytes(sk->n) bytes
@@ -378,16 +384,40 @@ int smk_reduce_by_set(const smk_skewers* sk, const double* per_skewer, int width
  *   status      NULL or int32 per skewer, 1 if its forest exceeds max_forest
  *   modes       NULL or device complex double: the rfft modes X_j of every
  *               skewer, (N/2 + 1) of them at modes + 2 * modes_off[s], kept
- *               for smk_p1d_adjoint */
+ *               for smk_p1d_adjoint
+ *   tables      NULL, or the Bluestein tables of smk_p1d_tables for the
+ *               batch's forest lengths, with tab_off[s] the offset (in
+ *               complex doubles) of skewer s's length table, < 0 for none;
+ *               the transform is then two FFTs on one register tile instead
+ *               of three, in a kernel per transform size (same results to
+ *               FP64 rounding) */
 int smk_p1d_binned(const smk_skewers* sk, const double* flux, const double* fmean, double pixel, int regroup,
                    const double* edges, int nbins, int max_forest, double* partial, int32_t* status,
-                   double* modes, const int64_t* modes_off, void* stream);
+                   double* modes, const int64_t* modes_off, const double* tables, const int64_t* tab_off,
+                   void* stream);
+
+/* Length of the Bluestein table of a forest of n (regrouped) points, in
+ * complex doubles: n + M (M the smallest power of two >= max(2n - 1, 1024))
+ * for 1 <= n <= 2048, else 0 (such forests take the direct DFT).  A
+ * building block of powerspectrum.ComputeP1D (powerspectrum.py:204-238) on
+ * the device; the reference's numpy rfft needs no table. */
+int64_t smk_p1d_table_len(int n);
+
+/* The Bluestein tables of n_lens forest lengths: lens (device int32) and
+ * offs (device int64, in complex doubles) give length d's table at
+ * tables + 2 * offs[d], smk_p1d_table_len(lens[d]) complex doubles: the
+ * chirp w_m = exp(-i pi m^2 / N), m < N, then FFT_M(b) / M of the chirp
+ * filter b (b_m = conj(w_|m|), circular).  Made once per batch (a fit's
+ * forests do not change between evaluations).  Lengths outside [1, 2048]
+ * are skipped. */
+int smk_p1d_tables(const int32_t* lens, const int64_t* offs, int n_lens, double* tables, void* stream);
 
 /* Adjoint of the binned P1D and of the FGPA, for a loss L(<F>_set, P1D bins).
  *   gp    device double [sets][nbins]: dL/dP for one P_j sample of bin b
  *         (dL/d mean_b divided by the bin count)
  *   modes NULL (recompute) or the modes saved by smk_p1d_binned at the same
  *         parameters
+ *   tables, tab_off  as smk_p1d_binned
  * out[s][7] = (sum_q A_q F'_q, sum_n A_q(n) dF_n/d(a, b, c) / m,
  *              sum_n dF_n/d(a, b, c)) with A_q = dL/d delta_F,q at fixed <F>;
  * the host adds the <F> chain rule per set.  No counterpart in the
@@ -395,7 +425,8 @@ int smk_p1d_binned(const smk_skewers* sk, const double* flux, const double* fmea
 int smk_p1d_adjoint(const smk_skewers* sk, const float* delta, const float* eta, const double* abcg,
                     int64_t abcg_rows, const double* flux, const double* fmean, double pixel, int regroup,
                     const double* edges, int nbins, int max_forest, const double* modes,
-                    const int64_t* modes_off, const double* gp, double* out, void* stream);
+                    const int64_t* modes_off, const double* gp, double* out, const double* tables,
+                    const int64_t* tab_off, void* stream);
 
 /* ---------------------------------------------------------- QSO catalogue */
 /* Quasar draw of one x slice of the lognormal boxes, draw_qso.py:186-462:

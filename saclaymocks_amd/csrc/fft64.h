@@ -263,5 +263,61 @@ __device__ __forceinline__ void bluestein(int N, int nout, In in, Emit emit, dou
   __syncthreads();
 }
 
+// The per-length table of bluestein_tab, written at T: w_n for n < N, then
+// FFT_M(b)_k / M for k < M with b as above (1/M is a power of two, so the
+// products below round exactly as bluestein's).  Every thread of the block
+// must call it.
+template <int M, int E>
+__device__ __forceinline__ void bluestein_table(int N, double2* __restrict__ T, double2* __restrict__ line,
+                                                const double2* __restrict__ twM, int TW) {
+  constexpr int P = M / E;
+  const int p = threadIdx.x;
+  double2 vb[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int m = p + P * e;
+    if (m < N) {
+      const double2 w = chirp(m, N);
+      T[m] = w;
+      vb[e] = cconj(w);
+    } else {
+      vb[e] = m > M - N ? cconj(chirp(M - m, N)) : make_double2(0.0, 0.0);
+    }
+  }
+  fft<M, E, false>(vb, line, p, twM, TW);
+  const double inv = 1.0 / (double)M;
+#pragma unroll
+  for (int e = 0; e < E; ++e) T[N + p + P * e] = make_double2(vb[e].x * inv, vb[e].y * inv);
+}
+
+// Bluestein with the chirp w (N values) and FFT_M(b) / M (M values) read
+// from a table made once per length (bluestein_table): one register tile,
+// two transforms.  in(n) may read LDS that `line` aliases (all of it is read
+// before the first exchange); emit may write there too (the line is free
+// once the last transform's final stage runs in registers).
+template <int M, int E, class In, class Emit>
+__device__ __forceinline__ void bluestein_tab(int N, int nout, In in, Emit emit, double2* __restrict__ line,
+                                              const double2* __restrict__ twM, int TW,
+                                              const double2* __restrict__ wtab, const double2* __restrict__ btab) {
+  constexpr int P = M / E;
+  const int p = threadIdx.x;
+  double2 va[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int n = p + P * e;
+    va[e] = n < N ? cmul(in(n), wtab[n]) : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  fft<M, E, false>(va, line, p, twM, TW);
+#pragma unroll
+  for (int e = 0; e < E; ++e) va[e] = cmul(va[e], btab[p + P * e]);
+  fft<M, E, true>(va, line, p, twM, TW);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int k = p + P * e;
+    if (k < nout) emit(k, cmul(va[e], wtab[k]));
+  }
+}
+
 }  // namespace f64
 }  // namespace smk

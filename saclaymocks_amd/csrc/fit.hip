@@ -9,6 +9,11 @@
 // power-of-two FFTs (fft64.h, O(N log N)); longer forests take a direct DFT
 // evaluated in FP64 by Horner's rule (N multiply-adds per mode, one mode per
 // thread).  Either is exact to FP64 rounding at any N, as numpy's rfft is.
+// With per-length tables (smk_p1d_tables: the chirp and the chirp's
+// transform of every forest length of a batch, made once, since a fit's
+// forests do not change between evaluations) a Bluestein DFT is two FFTs on
+// one register tile instead of three on three tiles; the kernels are then
+// separate instances (TAB = true) with their own, smaller register count.
 // All reductions are FP64 and deterministic (fixed thread-to-data
 // assignment, fixed tree order).
 #include <math.h>
@@ -32,25 +37,32 @@ constexpr int BLUE_MMAX = 4096;
 // dynamic LDS of the P1D kernels for forests of at most nmax (even) points:
 // x, then the binning arrays (forward) or the adjoint's coefficients, with
 // the Bluestein FFT line aliased over them
-int blue_m(int nmax) {
-  const int n = nmax < BLUE_NMAX ? nmax : BLUE_NMAX;
+// transform size of the Bluestein DFT of N points: the smallest power of
+// two >= max(2N - 1, 1024)
+__host__ __device__ inline int blue_m_of(int N) {
   int M = 1024;
-  while (M < 2 * n - 1) M *= 2;
+  while (M < 2 * N - 1) M *= 2;
   return M;
 }
+int blue_m(int nmax) { return blue_m_of(nmax < BLUE_NMAX ? nmax : BLUE_NMAX); }
 size_t blue_line_bytes(int nmax) { return (size_t)f64::line_len(blue_m(nmax)) * 16; }
 size_t align16(size_t b) { return (b + 15) & ~(size_t)15; }
-size_t p1d_x_bytes(int nmax) {
-  const size_t a = (size_t)nmax * 8, b = blue_line_bytes(nmax);
-  return align16(a > b ? a : b);
+size_t max_sz(size_t a, size_t b) { return a > b ? a : b; }
+// forward, region A: the Bluestein line with pk and bins over its start
+// (written once the last transform has left the line), or for Horner the
+// input x, then pk and bins; after region A the edges (nb + 1 doubles) and
+// the bin runs (2 nb ints)
+size_t p1d_a_bytes(int nmax) {
+  const size_t nj = (size_t)(nmax / 2 + 1);
+  return align16(max_sz(blue_line_bytes(nmax), (size_t)nmax * 8 + nj * 12));
 }
-// forward: [x (Horner) or the Bluestein line][pk][bins][bin runs]
-size_t p1d_lds_bytes(int nmax, int nb) {
-  return p1d_x_bytes(nmax) + (size_t)(nmax / 2 + 1) * 12 + (size_t)nb * 8;
+size_t p1d_lds_bytes(int nmax, int nb) { return p1d_a_bytes(nmax) + (size_t)(nb + 1) * 8 + (size_t)nb * 8; }
+// adjoint, generic kernel: [x][cj][Bluestein line (TAB = false)][edges]
+size_t adj_x_bytes(int nmax) { return align16((size_t)nmax * 8); }
+size_t adj_cj_bytes(int nmax) { return align16((size_t)(nmax / 2 + 1) * 16); }
+size_t adj_e_off(int nmax, bool tab) {
+  return adj_x_bytes(nmax) + adj_cj_bytes(nmax) + (tab ? 0 : blue_line_bytes(nmax));
 }
-// adjoint: [x][cj][Bluestein line]
-size_t adj_line_off(int nmax) { return align16((size_t)nmax * 8 + (size_t)(nmax / 2 + 1) * 16); }
-size_t adj_lds_bytes(int nmax) { return adj_line_off(nmax) + blue_line_bytes(nmax); }
 
 // one transform size per function, so that each gets its own register
 // allocation instead of the union of all of them
@@ -68,6 +80,8 @@ __device__ __forceinline__ void dft_bluestein(int N, int nout, In in, Emit emit,
   else bluestein_at<4096>(N, nout, in, emit, line, tw);
 }
 
+// sum over a block of NT threads (red: NT / 64 doubles), fixed order
+template <int NT = FIT_THREADS>
 __device__ __forceinline__ double block_reduce_sum(double v, double* red) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -75,7 +89,7 @@ __device__ __forceinline__ double block_reduce_sum(double v, double* red) {
   if (lane == 0) red[wv] = v;
   __syncthreads();
   double s = 0.0;
-  for (int i = 0; i < FIT_THREADS / 64; ++i) s += red[i];
+  for (int i = 0; i < NT / 64; ++i) s += red[i];
   __syncthreads();
   return s;  // valid on every thread
 }
@@ -144,11 +158,12 @@ __device__ __forceinline__ int find_bin(const double* __restrict__ edges, int nb
 }
 
 // delta_F of the (regrouped) forest into LDS; returns the regrouped length
+template <int NT = FIT_THREADS>
 __device__ __forceinline__ int load_delta_f(const double* __restrict__ flux, int64_t o, int i0, int i1, int m,
                                            double inv_fmean, double* __restrict__ x) {
   const int nf = i1 - i0;
   const int nq = nf / m;
-  for (int q = threadIdx.x; q < nq; q += FIT_THREADS) {
+  for (int q = threadIdx.x; q < nq; q += NT) {
     double acc = 0.0;
     for (int r = 0; r < m; ++r) acc += flux[o + i0 + q * m + r];
     x[q] = (acc / m) * inv_fmean - 1.0;
@@ -230,22 +245,71 @@ struct P1dArgs {
   double2* xj;           // NULL or the modes X_j of every skewer, at xj + xoff[s]
   const int64_t* xoff;
   const double2* tw;     // W_4096^k, k < 4096 (FP64), for the Bluestein FFTs
-  size_t x_bytes;        // p1d_x_bytes(nmax)
+  const double2* tables; // NULL, or the per-length Bluestein tables (bluestein_table layout)
+  const int64_t* tab_off;// with tables: per skewer, its table at tables + tab_off[s]; < 0: none
+  size_t a_bytes;        // byte offset of the edges in the dynamic LDS
 };
 
+// skewer s has a table and a forest in the Bluestein range: the per-size
+// kernels take it, the generic one skips it
+__device__ __forceinline__ bool uses_table(const double2* tables, const int64_t* tab_off, int s, int nq,
+                                           int nmax) {
+  return tables != nullptr && tab_off[s] >= 0 && nq >= 1 && nq <= BLUE_NMAX && nq <= nmax;
+}
+
+// bin runs of one skewer: k_j increases with j, so each bin holds a run of
+// consecutive modes (bins[] is non-decreasing, then -1 past the last edge):
+// find each bin's run [bfirst, blast] from the transitions, then sum it in
+// ascending j (the order of a full scan, so the sums are the same)
+template <int NT>
+__device__ __forceinline__ void bin_runs(const double* pk, const int* bins, int nj, int nq, double k_ny, int nb,
+                                         int* bfirst, int* blast, double* part) {
+  for (int b = threadIdx.x; b < nb; b += NT) {
+    bfirst[b] = 0;
+    blast[b] = -1;
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < nj; j += NT) {
+    const int b = bins[j];
+    if (b < 0) continue;
+    if (j == 0 || bins[j - 1] != b) bfirst[b] = j;
+    if (j == nj - 1 || bins[j + 1] != b) blast[b] = j;
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < nb; b += NT) {
+    double c = 0.0, sk_ = 0.0, sp = 0.0, sp2 = 0.0;
+    for (int j = bfirst[b]; j <= blast[b]; ++j) {
+      c += 1.0;
+      sk_ += kfreq(j, nq, k_ny);
+      sp += pk[j];
+      sp2 += pk[j] * pk[j];
+    }
+    part[4 * b + 0] = c;
+    part[4 * b + 1] = sk_;
+    part[4 * b + 2] = sp;
+    part[4 * b + 3] = sp2;
+  }
+}
+
+// Generic P1D kernel.  TAB = false: every skewer, Bluestein with the chirp
+// formed in place (N <= 2048) or Horner.  TAB = true: the skewers without a
+// usable table only (Horner; those with one go to k_p1d_blue).  Dynamic
+// LDS: region A (the Bluestein line with pk and bins over its start, or the
+// Horner input x, then pk and bins), the edges, the bin runs.
+template <bool TAB>
 __global__ __launch_bounds__(FIT_THREADS) void k_p1d(P1dArgs a) {
   extern __shared__ double dyn[];
-  double* x = dyn;                                    // Horner input
-  double2* bline = reinterpret_cast<double2*>(dyn);   // Bluestein FFT line (aliases x)
-  double* pk = reinterpret_cast<double*>(reinterpret_cast<char*>(dyn) + a.x_bytes);
-  int* bins = reinterpret_cast<int*>(pk + a.nmax / 2 + 1);
-  int* bfirst = bins + a.nmax / 2 + 1;
+  const int njmax = a.nmax / 2 + 1;
+  double* ledges = reinterpret_cast<double*>(reinterpret_cast<char*>(dyn) + a.a_bytes);
+  int* bfirst = reinterpret_cast<int*>(ledges + a.nb + 1);
   int* blast = bfirst + a.nb;
+  for (int i = threadIdx.x; i <= a.nb; i += FIT_THREADS) ledges[i] = a.edges[i];
   const smk_skewers& sk = a.sk;
   for (int s = blockIdx.x; s < sk.n; s += gridDim.x) {
     double* part = a.partial + (int64_t)s * a.nb * 4;
     const int i0 = sk.imin[s], i1 = sk.imax[s];
     const int nq = (i1 - i0) / a.regroup;
+    if (TAB && uses_table(a.tables, a.tab_off, s, nq, a.nmax)) continue;
     __syncthreads();
     if (nq > a.nmax) {
       for (int b = threadIdx.x; b < 4 * a.nb; b += FIT_THREADS) part[b] = 0.0;
@@ -257,62 +321,110 @@ __global__ __launch_bounds__(FIT_THREADS) void k_p1d(P1dArgs a) {
     const double k_ny = M_PI / px;
     const int nj = nq > 0 ? nq / 2 + 1 : 0;
     double2* xj = a.xj ? a.xj + a.xoff[s] : nullptr;
-    if (nq <= BLUE_NMAX && nq > 0) {
-      // delta_F read straight from the flux (regrouped), the DFT by Bluestein
-      const int64_t o = sk.off[s] + i0;
-      const int m = a.regroup;
-      dft_bluestein(nq, nj,
-                    [&](int q) {
-                      double acc = 0.0;
-                      for (int r = 0; r < m; ++r) acc += a.flux[o + (int64_t)q * m + r];
-                      return make_double2((acc / m) * inv - 1.0, 0.0);
-                    },
-                    [&](int j, double2 X) {
-                      pk[j] = (X.x * X.x + X.y * X.y) * px / nq;
-                      bins[j] = find_bin(a.edges, a.nb, kfreq(j, nq, k_ny));
-                      if (xj) xj[j] = X;
-                    },
-                    bline, a.tw);
+    const bool blue = !TAB && nq > 0 && nq <= BLUE_NMAX;
+    double* pk = blue ? dyn : dyn + a.nmax;
+    int* bins = reinterpret_cast<int*>(pk + njmax);
+    if (blue) {
+      if constexpr (!TAB) {
+        // delta_F read straight from the flux (regrouped), the DFT by Bluestein
+        const int64_t o = sk.off[s] + i0;
+        const int m = a.regroup;
+        dft_bluestein(nq, nj,
+                      [&](int q) {
+                        double acc = 0.0;
+                        for (int r = 0; r < m; ++r) acc += a.flux[o + (int64_t)q * m + r];
+                        return make_double2((acc / m) * inv - 1.0, 0.0);
+                      },
+                      [&](int j, double2 X) {
+                        pk[j] = (X.x * X.x + X.y * X.y) * px / nq;
+                        bins[j] = find_bin(ledges, a.nb, kfreq(j, nq, k_ny));
+                        if (xj) xj[j] = X;
+                      },
+                      reinterpret_cast<double2*>(dyn), a.tw);
+      }
       __syncthreads();
     } else {
+      double* x = dyn;
       load_delta_f(a.flux, sk.off[s], i0, i1, a.regroup, inv, x);
       __syncthreads();
       dft_all<false>(x, nullptr, nq, nq, nj, [&](int j, double re, double im) {
         pk[j] = (re * re + im * im) * px / nq;
-        bins[j] = find_bin(a.edges, a.nb, kfreq(j, nq, k_ny));
+        bins[j] = find_bin(ledges, a.nb, kfreq(j, nq, k_ny));
         if (xj) xj[j] = make_double2(re, im);
       });
       __syncthreads();
     }
-    // k_j increases with j, so each bin holds a run of consecutive modes
-    // (bins[] is non-decreasing, then -1 past the last edge): find each
-    // bin's run [bfirst, blast] from the transitions, then sum it in
-    // ascending j (the order of a full scan, so the sums are the same)
-    for (int b = threadIdx.x; b < a.nb; b += FIT_THREADS) {
-      bfirst[b] = 0;
-      blast[b] = -1;
-    }
+    bin_runs<FIT_THREADS>(pk, bins, nj, nq, k_ny, a.nb, bfirst, blast, part);
+  }
+}
+
+// LDS of k_p1d_blue<M>: the line (pk and bins over its start), the edges,
+// the bin runs
+__host__ __device__ constexpr size_t blue_line_bytes_m(int M) { return (((size_t)f64::line_len(M) * 16) + 15) & ~(size_t)15; }
+
+// The P1D of the skewers with a table and a forest of M's size class
+// (blue_m_of(N) == M), M / 8 threads of 8 points each: the DFT by
+// bluestein_tab (two FFTs on one register tile), then the bins.
+template <int M>
+__global__ __launch_bounds__(M / 8) void k_p1d_blue(P1dArgs a) {
+  constexpr int NT = M / 8;
+  extern __shared__ double dyn[];
+  double2* line = reinterpret_cast<double2*>(dyn);
+  double* pk = dyn;
+  int* bins = reinterpret_cast<int*>(pk + (M / 4 + 1));
+  double* ledges = reinterpret_cast<double*>(reinterpret_cast<char*>(dyn) + blue_line_bytes_m(M));
+  int* bfirst = reinterpret_cast<int*>(ledges + a.nb + 1);
+  int* blast = bfirst + a.nb;
+  for (int i = threadIdx.x; i <= a.nb; i += NT) ledges[i] = a.edges[i];
+  const smk_skewers& sk = a.sk;
+  for (int s = blockIdx.x; s < sk.n; s += gridDim.x) {
+    const int i0 = sk.imin[s], i1 = sk.imax[s];
+    const int nq = (i1 - i0) / a.regroup;
+    if (!uses_table(a.tables, a.tab_off, s, nq, a.nmax) || blue_m_of(nq) != M) continue;
     __syncthreads();
-    for (int j = threadIdx.x; j < nj; j += FIT_THREADS) {
-      const int b = bins[j];
-      if (b < 0) continue;
-      if (j == 0 || bins[j - 1] != b) bfirst[b] = j;
-      if (j == nj - 1 || bins[j + 1] != b) blast[b] = j;
-    }
+    double* part = a.partial + (int64_t)s * a.nb * 4;
+    const double inv = 1.0 / a.fmean[sk.pset[s]];
+    const double px = a.pixel * a.regroup;
+    const double k_ny = M_PI / px;
+    const int nj = nq / 2 + 1;
+    double2* xj = a.xj ? a.xj + a.xoff[s] : nullptr;
+    const double2* T = a.tables + a.tab_off[s];
+    const int64_t o = sk.off[s] + i0;
+    const int m = a.regroup;
+    f64::bluestein_tab<M, 8>(
+        nq, nj,
+        [&](int q) {
+          double acc = 0.0;
+          for (int r = 0; r < m; ++r) acc += a.flux[o + (int64_t)q * m + r];
+          return make_double2((acc / m) * inv - 1.0, 0.0);
+        },
+        [&](int j, double2 X) {
+          pk[j] = (X.x * X.x + X.y * X.y) * px / nq;
+          bins[j] = find_bin(ledges, a.nb, kfreq(j, nq, k_ny));
+          if (xj) xj[j] = X;
+        },
+        line, a.tw, BLUE_MMAX, T, T + nq);
     __syncthreads();
-    for (int b = threadIdx.x; b < a.nb; b += FIT_THREADS) {
-      double c = 0.0, sk_ = 0.0, sp = 0.0, sp2 = 0.0;
-      for (int j = bfirst[b]; j <= blast[b]; ++j) {
-        c += 1.0;
-        sk_ += kfreq(j, nq, k_ny);
-        sp += pk[j];
-        sp2 += pk[j] * pk[j];
-      }
-      part[4 * b + 0] = c;
-      part[4 * b + 1] = sk_;
-      part[4 * b + 2] = sp;
-      part[4 * b + 3] = sp2;
-    }
+    bin_runs<NT>(pk, bins, nj, nq, k_ny, a.nb, bfirst, blast, part);
+  }
+}
+
+// the Bluestein table of every length lens[d] at tables + offs[d]
+// (bluestein_table layout); lengths outside [1, BLUE_NMAX] are skipped
+__global__ __launch_bounds__(FIT_THREADS) void k_p1d_tables(const int32_t* __restrict__ lens,
+                                                            const int64_t* __restrict__ offs, int n,
+                                                            double2* __restrict__ tables,
+                                                            const double2* __restrict__ tw) {
+  extern __shared__ double dyn[];
+  double2* line = reinterpret_cast<double2*>(dyn);
+  for (int d = blockIdx.x; d < n; d += gridDim.x) {
+    const int N = lens[d];
+    __syncthreads();
+    if (N < 1 || N > BLUE_NMAX) continue;
+    double2* T = tables + offs[d];
+    if (2 * N - 1 <= 1024) f64::bluestein_table<1024, 1024 / FIT_THREADS>(N, T, line, tw, BLUE_MMAX);
+    else if (2 * N - 1 <= 2048) f64::bluestein_table<2048, 2048 / FIT_THREADS>(N, T, line, tw, BLUE_MMAX);
+    else f64::bluestein_table<4096, 4096 / FIT_THREADS>(N, T, line, tw, BLUE_MMAX);
   }
 }
 
@@ -334,22 +446,66 @@ struct AdjArgs {
   const double2* xj;     // NULL: recompute the modes; else those saved by k_p1d
   const int64_t* xoff;
   const double2* tw;     // W_4096^k (FP64), for the Bluestein FFTs
-  size_t line_off;       // adj_line_off(nmax): byte offset of the Bluestein line
+  const double2* tables; // NULL, or the per-length Bluestein tables
+  const int64_t* tab_off;
+  size_t line_off;       // generic kernel: byte offset of the Bluestein line (after x and cj)
+  size_t e_off;          // generic kernel: byte offset of the edges
 };
+
+// The FGPA part of the adjoint of one skewer, with A_q (q < nq) in A:
+// (sum A F', sum A_q(n) dF_n/d(a, b, c) / m, sum dF_n/d(a, b, c)) into res
+template <int NT>
+__device__ __forceinline__ void adj_fgpa(const AdjArgs& a, int s, int set, int i0, int i1, int nq, double saf,
+                                         const double* A, double* red, double* res) {
+  const int m = a.regroup;
+  const int64_t o = a.sk.off[s];
+  const double4 p = prow(a.abcg, a.rows, set);
+  const double pa = p.x, pb = p.y, pc = p.z, pg = p.w;
+  double t1a = 0.0, t1b = 0.0, t1c = 0.0, t2a = 0.0, t2b = 0.0, t2c = 0.0;
+  for (int i = threadIdx.x; i < i1 - i0; i += NT) {
+    const int q = i / m;
+    const double d = a.delta[o + i0 + i];
+    const double e = a.eta ? (double)a.eta[o + i0 + i] : 0.0;
+    const double gf = d + pc * e;
+    const double T = exp(pb * pg * gf);
+    const double F = exp(-pa * T);
+    const double dfa = -T * F;
+    const double dfb = -pa * T * F * pg * gf;
+    const double dfc = -pa * T * F * pb * pg * e;
+    t2a += dfa; t2b += dfb; t2c += dfc;
+    if (q < nq) {
+      const double Aq = A[q] / m;
+      t1a += Aq * dfa;
+      t1b += Aq * dfb;
+      t1c += Aq * dfc;
+    }
+  }
+  const double v[7] = {saf, t1a, t1b, t1c, t2a, t2b, t2c};
+  for (int k = 0; k < 7; ++k) {
+    const double tot = block_reduce_sum<NT>(v[k], red);
+    if (threadIdx.x == 0) res[k] = tot;
+  }
+}
 
 // Adjoint of the binned P1D through delta_F = F/<F> - 1 (at fixed <F>) and
 // of the FGPA: A_q = dL/d delta_F,q = sum_j gP_j (2 px / N) Re(conj(X_j) W^{jq}).
+// Generic kernel, TAB as k_p1d.  Dynamic LDS: x, cj, the Bluestein line
+// (TAB = false), the edges.
+template <bool TAB>
 __global__ __launch_bounds__(FIT_THREADS) void k_p1d_adjoint(AdjArgs a) {
   extern __shared__ double dyn[];
   double* x = dyn;
-  double2* cj = reinterpret_cast<double2*>(dyn + a.nmax);
+  double2* cj = reinterpret_cast<double2*>(dyn + ((a.nmax + 1) & ~1));
   double2* bline = reinterpret_cast<double2*>(reinterpret_cast<char*>(dyn) + a.line_off);
+  double* ledges = reinterpret_cast<double*>(reinterpret_cast<char*>(dyn) + a.e_off);
   __shared__ double red[FIT_THREADS / 64];
+  for (int i = threadIdx.x; i <= a.nb; i += FIT_THREADS) ledges[i] = a.edges[i];
   const smk_skewers& sk = a.sk;
   for (int s = blockIdx.x; s < sk.n; s += gridDim.x) {
     const int i0 = sk.imin[s], i1 = sk.imax[s];
     const int m = a.regroup;
     const int nq = (i1 - i0) / m;
+    if (TAB && uses_table(a.tables, a.tab_off, s, nq, a.nmax)) continue;
     const int64_t o = sk.off[s];
     double* res = a.out + (int64_t)s * 7;
     __syncthreads();
@@ -366,10 +522,10 @@ __global__ __launch_bounds__(FIT_THREADS) void k_p1d_adjoint(AdjArgs a) {
     const int nj = nq > 0 ? nq / 2 + 1 : 0;
     const double* gps = a.gp + (int64_t)set * a.nb;
     auto coef = [&](int j) -> double {
-      const int b = find_bin(a.edges, a.nb, kfreq(j, nq, k_ny));
+      const int b = find_bin(ledges, a.nb, kfreq(j, nq, k_ny));
       return b >= 0 ? gps[b] * 2.0 * px / nq : 0.0;
     };
-    const bool blue = nq > 0 && nq <= BLUE_NMAX;
+    const bool blue = !TAB && nq > 0 && nq <= BLUE_NMAX;
     if (a.xj) {
       const double2* xs = a.xj + a.xoff[s];
       for (int j = threadIdx.x; j < nj; j += FIT_THREADS) {
@@ -378,12 +534,13 @@ __global__ __launch_bounds__(FIT_THREADS) void k_p1d_adjoint(AdjArgs a) {
         cj[j] = make_double2(g * X.x, -g * X.y);   // g conj(X_j)
       }
     } else if (blue) {
-      dft_bluestein(nq, nj, [&](int n) { return make_double2(x[n], 0.0); },
-                    [&](int j, double2 X) {
-                      const double g = coef(j);
-                      cj[j] = make_double2(g * X.x, -g * X.y);
-                    },
-                    bline, a.tw);
+      if constexpr (!TAB)
+        dft_bluestein(nq, nj, [&](int n) { return make_double2(x[n], 0.0); },
+                      [&](int j, double2 X) {
+                        const double g = coef(j);
+                        cj[j] = make_double2(g * X.x, -g * X.y);
+                      },
+                      bline, a.tw);
     } else {
       dft_all<false>(x, nullptr, nq, nq, nj, [&](int j, double re, double im) {
         const double g = coef(j);
@@ -396,12 +553,13 @@ __global__ __launch_bounds__(FIT_THREADS) void k_p1d_adjoint(AdjArgs a) {
     // stores A_q over x[q] (x is not read by any other thread from here on).
     double saf = 0.0;
     if (blue) {
-      dft_bluestein(nq, nq, [&](int n) { return n < nj ? cj[n] : make_double2(0.0, 0.0); },
-                    [&](int q, double2 X) {
-                      saf += X.x * (x[q] + 1.0) * fm;
-                      x[q] = X.x;
-                    },
-                    bline, a.tw);
+      if constexpr (!TAB)
+        dft_bluestein(nq, nq, [&](int n) { return n < nj ? cj[n] : make_double2(0.0, 0.0); },
+                      [&](int q, double2 X) {
+                        saf += X.x * (x[q] + 1.0) * fm;
+                        x[q] = X.x;
+                      },
+                      bline, a.tw);
     } else {
       dft_all<true>(nullptr, cj, nj, nq, nq, [&](int q, double re, double im) {
         saf += re * (x[q] + 1.0) * fm;
@@ -409,37 +567,106 @@ __global__ __launch_bounds__(FIT_THREADS) void k_p1d_adjoint(AdjArgs a) {
       });
     }
     __syncthreads();
-    const double* A = x;
-    const double4 p = prow(a.abcg, a.rows, set);
-    const double pa = p.x, pb = p.y, pc = p.z, pg = p.w;
-    double t1a = 0.0, t1b = 0.0, t1c = 0.0, t2a = 0.0, t2b = 0.0, t2c = 0.0;
-    for (int i = threadIdx.x; i < i1 - i0; i += FIT_THREADS) {
-      const int q = i / m;
-      const double d = a.delta[o + i0 + i];
-      const double e = a.eta ? (double)a.eta[o + i0 + i] : 0.0;
-      const double gf = d + pc * e;
-      const double T = exp(pb * pg * gf);
-      const double F = exp(-pa * T);
-      const double dfa = -T * F;
-      const double dfb = -pa * T * F * pg * gf;
-      const double dfc = -pa * T * F * pb * pg * e;
-      t2a += dfa; t2b += dfb; t2c += dfc;
-      if (q < nq) {
-        const double Aq = A[q] / m;
-        t1a += Aq * dfa;
-        t1b += Aq * dfb;
-        t1c += Aq * dfc;
+    adj_fgpa<FIT_THREADS>(a, s, set, i0, i1, nq, saf, x, red, res);
+  }
+}
+
+// LDS of k_p1d_adjoint_blue<M>: x (M / 2 doubles), the line with cj over its
+// start (cj is read into registers before the line's first exchange), the
+// edges
+__host__ __device__ constexpr size_t adj_blue_x_bytes(int M) { return (size_t)(M / 2) * 8; }
+
+// The adjoint of the skewers with a table and a forest of M's size class,
+// M / 8 threads of 8 points each.
+template <int M>
+__global__ __launch_bounds__(M / 8) void k_p1d_adjoint_blue(AdjArgs a) {
+  constexpr int NT = M / 8;
+  extern __shared__ double dyn[];
+  double* x = dyn;
+  double2* line = reinterpret_cast<double2*>(reinterpret_cast<char*>(dyn) + adj_blue_x_bytes(M));
+  double2* cj = line;
+  double* ledges =
+      reinterpret_cast<double*>(reinterpret_cast<char*>(dyn) + adj_blue_x_bytes(M) + blue_line_bytes_m(M));
+  __shared__ double red[NT / 64];
+  for (int i = threadIdx.x; i <= a.nb; i += NT) ledges[i] = a.edges[i];
+  const smk_skewers& sk = a.sk;
+  for (int s = blockIdx.x; s < sk.n; s += gridDim.x) {
+    const int i0 = sk.imin[s], i1 = sk.imax[s];
+    const int m = a.regroup;
+    const int nq = (i1 - i0) / m;
+    if (!uses_table(a.tables, a.tab_off, s, nq, a.nmax) || blue_m_of(nq) != M) continue;
+    const int64_t o = sk.off[s];
+    double* res = a.out + (int64_t)s * 7;
+    __syncthreads();
+    const int set = sk.pset[s];
+    const double fm = a.fmean[set];
+    load_delta_f<NT>(a.flux, o, i0, i1, m, 1.0 / fm, x);
+    __syncthreads();
+    const double px = a.pixel * m;
+    const double k_ny = M_PI / px;
+    const int nj = nq / 2 + 1;
+    const double* gps = a.gp + (int64_t)set * a.nb;
+    const double2* T = a.tables + a.tab_off[s];
+    auto coef = [&](int j) -> double {
+      const int b = find_bin(ledges, a.nb, kfreq(j, nq, k_ny));
+      return b >= 0 ? gps[b] * 2.0 * px / nq : 0.0;
+    };
+    if (a.xj) {
+      const double2* xs = a.xj + a.xoff[s];
+      for (int j = threadIdx.x; j < nj; j += NT) {
+        const double g = coef(j);
+        const double2 X = xs[j];
+        cj[j] = make_double2(g * X.x, -g * X.y);   // g conj(X_j)
       }
+    } else {
+      f64::bluestein_tab<M, 8>(nq, nj, [&](int n) { return make_double2(x[n], 0.0); },
+                               [&](int j, double2 X) {
+                                 const double g = coef(j);
+                                 cj[j] = make_double2(g * X.x, -g * X.y);
+                               },
+                               line, a.tw, BLUE_MMAX, T, T + nq);
     }
-    const double v[7] = {saf, t1a, t1b, t1c, t2a, t2b, t2c};
-    for (int k = 0; k < 7; ++k) {
-      const double tot = block_reduce_sum(v[k], red);
-      if (threadIdx.x == 0) res[k] = tot;
-    }
+    __syncthreads();
+    double saf = 0.0;
+    f64::bluestein_tab<M, 8>(nq, nq, [&](int n) { return n < nj ? cj[n] : make_double2(0.0, 0.0); },
+                             [&](int q, double2 X) {
+                               saf += X.x * (x[q] + 1.0) * fm;
+                               x[q] = X.x;
+                             },
+                             line, a.tw, BLUE_MMAX, T, T + nq);
+    __syncthreads();
+    adj_fgpa<NT>(a, s, set, i0, i1, nq, saf, x, red, res);
   }
 }
 
 int grid_for(int n) { return n < 65536 ? (n > 0 ? n : 1) : 65536; }
+
+// propagate a nonzero SMK_ERR_* return
+#define SMK_CALL(expr)          \
+  do {                          \
+    const int smk_rc_ = (expr); \
+    if (smk_rc_) return smk_rc_; \
+  } while (0)
+
+template <int M>
+int launch_p1d_blue(const P1dArgs& a, dim3 grid, size_t tail, hipStream_t st) {
+  const size_t lds = blue_line_bytes_m(M) + tail;
+  SMK_HIP_CHECK(hipFuncSetAttribute((const void*)k_p1d_blue<M>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds));
+  hipLaunchKernelGGL(k_p1d_blue<M>, grid, dim3(M / 8), lds, st, a);
+  SMK_LAUNCH_CHECK();
+  return SMK_OK;
+}
+
+template <int M>
+int launch_adj_blue(const AdjArgs& a, dim3 grid, size_t tail, hipStream_t st) {
+  const size_t lds = adj_blue_x_bytes(M) + blue_line_bytes_m(M) + tail;
+  SMK_HIP_CHECK(hipFuncSetAttribute((const void*)k_p1d_adjoint_blue<M>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(k_p1d_adjoint_blue<M>, grid, dim3(M / 8), lds, st, a);
+  SMK_LAUNCH_CHECK();
+  return SMK_OK;
+}
 
 // W_4096^k, k < 4096, FP64, one table per device (built on first use)
 const double2* fp64_twiddles(hipError_t* err) {
@@ -492,15 +719,37 @@ extern "C" int smk_reduce_by_set(const smk_skewers* sk, const double* per_skewer
   return SMK_OK;
 }
 
+extern "C" int64_t smk_p1d_table_len(int n) {
+  return (n >= 1 && n <= BLUE_NMAX) ? (int64_t)n + blue_m_of(n) : 0;
+}
+
+extern "C" int smk_p1d_tables(const int32_t* lens, const int64_t* offs, int n_lens, double* tables, void* stream) {
+  SMK_REQUIRE(n_lens >= 0, SMK_ERR_ARG, "n_lens < 0");
+  if (n_lens == 0) return SMK_OK;
+  SMK_REQUIRE(lens != nullptr && offs != nullptr && tables != nullptr, SMK_ERR_ARG, "NULL argument");
+  hipError_t te = hipSuccess;
+  const double2* tw = fp64_twiddles(&te);
+  SMK_REQUIRE(tw != nullptr, SMK_ERR_HIP, "FP64 twiddle table: %s", hipGetErrorString(te));
+  const size_t lds = (size_t)f64::line_len(BLUE_MMAX) * 16;
+  SMK_HIP_CHECK(hipFuncSetAttribute((const void*)k_p1d_tables, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds));
+  hipLaunchKernelGGL(k_p1d_tables, dim3(grid_for(n_lens)), dim3(FIT_THREADS), lds, (hipStream_t)stream, lens,
+                     offs, n_lens, (double2*)tables, tw);
+  SMK_LAUNCH_CHECK();
+  return SMK_OK;
+}
+
 extern "C" int smk_p1d_binned(const smk_skewers* sk, const double* flux, const double* fmean, double pixel,
                               int regroup, const double* edges, int nbins, int max_forest, double* partial,
-                              int32_t* status, double* modes, const int64_t* modes_off, void* stream) {
+                              int32_t* status, double* modes, const int64_t* modes_off, const double* tables,
+                              const int64_t* tab_off, void* stream) {
   SMK_REQUIRE(sk != nullptr && flux != nullptr && fmean != nullptr && edges != nullptr && partial != nullptr,
               SMK_ERR_ARG, "NULL argument");
   SMK_REQUIRE(sk->pset != nullptr, SMK_ERR_ARG, "the fit kernels need skewers.pset");
   SMK_REQUIRE(regroup >= 1 && nbins >= 1 && pixel > 0, SMK_ERR_ARG, "bad regroup/nbins/pixel");
   SMK_REQUIRE(max_forest >= 0 && max_forest / regroup <= FIT_NMAX, SMK_ERR_RANGE,
               "forest of %d points exceeds %d", max_forest / regroup, FIT_NMAX);
+  SMK_REQUIRE((tables == nullptr) == (tab_off == nullptr), SMK_ERR_ARG, "tables and tab_off go together");
   if (sk->n <= 0) return SMK_OK;
   int nmax = max_forest / regroup;
   nmax += nmax & 1;
@@ -509,19 +758,37 @@ extern "C" int smk_p1d_binned(const smk_skewers* sk, const double* flux, const d
   hipError_t te = hipSuccess;
   const double2* tw = fp64_twiddles(&te);
   SMK_REQUIRE(tw != nullptr, SMK_ERR_HIP, "FP64 twiddle table: %s", hipGetErrorString(te));
-  P1dArgs a{*sk, flux, fmean, pixel, regroup, edges, nbins, partial, status, nmax, (double2*)modes, modes_off,
-            tw, p1d_x_bytes(nmax)};
+  P1dArgs a{*sk,      flux,      fmean, pixel, regroup, edges, nbins, partial, status, nmax, (double2*)modes,
+            modes_off, tw, (const double2*)tables, tab_off, p1d_a_bytes(nmax)};
+  const dim3 grid(grid_for(sk->n));
+  hipStream_t st = (hipStream_t)stream;
   const size_t lds = p1d_lds_bytes(nmax, nbins);
-  SMK_HIP_CHECK(hipFuncSetAttribute((const void*)k_p1d, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(k_p1d, dim3(grid_for(sk->n)), dim3(FIT_THREADS), lds, (hipStream_t)stream, a);
-  SMK_LAUNCH_CHECK();
+  if (tables) {
+    // the forests without a table (Horner), then one launch per Bluestein
+    // size class; each skips the skewers of the others
+    SMK_HIP_CHECK(hipFuncSetAttribute((const void*)k_p1d<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds));
+    hipLaunchKernelGGL(k_p1d<true>, grid, dim3(FIT_THREADS), lds, st, a);
+    SMK_LAUNCH_CHECK();
+    const int nblue = nmax < BLUE_NMAX ? nmax : BLUE_NMAX;
+    const size_t tail = (size_t)(nbins + 1) * 8 + (size_t)nbins * 8;
+    SMK_CALL(launch_p1d_blue<1024>(a, grid, tail, st));
+    if (blue_m_of(nblue) >= 2048) SMK_CALL(launch_p1d_blue<2048>(a, grid, tail, st));
+    if (blue_m_of(nblue) >= 4096) SMK_CALL(launch_p1d_blue<4096>(a, grid, tail, st));
+  } else {
+    SMK_HIP_CHECK(hipFuncSetAttribute((const void*)k_p1d<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds));
+    hipLaunchKernelGGL(k_p1d<false>, grid, dim3(FIT_THREADS), lds, st, a);
+    SMK_LAUNCH_CHECK();
+  }
   return SMK_OK;
 }
 
 extern "C" int smk_p1d_adjoint(const smk_skewers* sk, const float* delta, const float* eta, const double* abcg,
                                int64_t abcg_rows, const double* flux, const double* fmean, double pixel,
                                int regroup, const double* edges, int nbins, int max_forest, const double* modes,
-                               const int64_t* modes_off, const double* gp, double* out, void* stream) {
+                               const int64_t* modes_off, const double* gp, double* out, const double* tables,
+                               const int64_t* tab_off, void* stream) {
   SMK_REQUIRE(sk != nullptr && delta != nullptr && abcg != nullptr && flux != nullptr && fmean != nullptr &&
                   edges != nullptr && gp != nullptr && out != nullptr,
               SMK_ERR_ARG, "NULL argument");
@@ -529,6 +796,7 @@ extern "C" int smk_p1d_adjoint(const smk_skewers* sk, const float* delta, const 
   SMK_REQUIRE(regroup >= 1 && nbins >= 1 && pixel > 0, SMK_ERR_ARG, "bad regroup/nbins/pixel");
   SMK_REQUIRE(max_forest >= 0 && max_forest / regroup <= FIT_NMAX, SMK_ERR_RANGE,
               "forest of %d points exceeds %d", max_forest / regroup, FIT_NMAX);
+  SMK_REQUIRE((tables == nullptr) == (tab_off == nullptr), SMK_ERR_ARG, "tables and tab_off go together");
   if (sk->n <= 0) return SMK_OK;
   int nmax = max_forest / regroup;
   nmax += nmax & 1;
@@ -537,12 +805,29 @@ extern "C" int smk_p1d_adjoint(const smk_skewers* sk, const float* delta, const 
   hipError_t te = hipSuccess;
   const double2* tw = fp64_twiddles(&te);
   SMK_REQUIRE(tw != nullptr, SMK_ERR_HIP, "FP64 twiddle table: %s", hipGetErrorString(te));
-  AdjArgs a{*sk,  delta, eta,   abcg, abcg_rows, flux, fmean, pixel, regroup, edges,
-            nbins, gp,   out, nmax, (const double2*)modes, modes_off, tw, adj_line_off(nmax)};
-  const size_t lds = adj_lds_bytes(nmax);
-  SMK_HIP_CHECK(
-      hipFuncSetAttribute((const void*)k_p1d_adjoint, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(k_p1d_adjoint, dim3(grid_for(sk->n)), dim3(FIT_THREADS), lds, (hipStream_t)stream, a);
-  SMK_LAUNCH_CHECK();
+  const bool tab = tables != nullptr;
+  const size_t line_off = adj_x_bytes(nmax) + adj_cj_bytes(nmax);
+  const size_t e_off = adj_e_off(nmax, tab);
+  AdjArgs a{*sk,   delta, eta,   abcg, abcg_rows, flux,      fmean, pixel, regroup, edges, nbins, gp, out, nmax,
+            (const double2*)modes, modes_off, tw, (const double2*)tables, tab_off, line_off, e_off};
+  const dim3 grid(grid_for(sk->n));
+  hipStream_t st = (hipStream_t)stream;
+  const size_t lds = e_off + (size_t)(nbins + 1) * 8;
+  if (tab) {
+    SMK_HIP_CHECK(hipFuncSetAttribute((const void*)k_p1d_adjoint<true>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_p1d_adjoint<true>, grid, dim3(FIT_THREADS), lds, st, a);
+    SMK_LAUNCH_CHECK();
+    const int nblue = nmax < BLUE_NMAX ? nmax : BLUE_NMAX;
+    const size_t tail = (size_t)(nbins + 1) * 8;
+    SMK_CALL(launch_adj_blue<1024>(a, grid, tail, st));
+    if (blue_m_of(nblue) >= 2048) SMK_CALL(launch_adj_blue<2048>(a, grid, tail, st));
+    if (blue_m_of(nblue) >= 4096) SMK_CALL(launch_adj_blue<4096>(a, grid, tail, st));
+  } else {
+    SMK_HIP_CHECK(hipFuncSetAttribute((const void*)k_p1d_adjoint<false>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_p1d_adjoint<false>, grid, dim3(FIT_THREADS), lds, st, a);
+    SMK_LAUNCH_CHECK();
+  }
   return SMK_OK;
 }

@@ -603,6 +603,8 @@ struct StepArgs {
   int32_t* queue;
   int32_t max_class;
   double hx, hy, hz;
+  int32_t ctab_cap;      // z cells the per-skewer column table holds (0: no table)
+  int32_t ctab_off;      // its offset in the dynamic LDS, in float2
 };
 
 // Block of the fused step: one wave per skewer (NT = 64) for batches whose
@@ -640,7 +642,13 @@ __device__ __forceinline__ void step_missing_power(const StepArgs& a, int s, int
   const uint64_t id = a.sk.noise_id ? a.sk.noise_id[s] : (uint64_t)s;
   // (1) modes k = 0..M: M/2 Philox calls (see mp_mode)
   const float sr = sqrtf((float)M), s2 = sqrtf(2.0f * M);
-  for (int c = tid; c < M / 2; c += NT) {
+  // a compile-time trip count, so that the filter loads of every pass are
+  // issued together ahead of the Philox rounds
+  constexpr int NPASS = (M / 2 + NT - 1) / NT;
+#pragma unroll
+  for (int it = 0; it < NPASS; ++it) {
+    const int c = tid + it * NT;
+    if (NPASS * NT > M / 2 && c >= M / 2) break;
     const U4 u = mp_call(seed, id, c);
     float n0, n1, n2, n3;
     box_muller_hw(u.x, u.y, n0, n1);
@@ -689,9 +697,8 @@ __device__ __forceinline__ void step_missing_power(const StepArgs& a, int s, int
 
 // MAXC: the longest length class of the batch (classes above it are
 // flagged); NT threads per block
-template <int MAXC, int NT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 64 ? 4 : 1))) void k_forest_step(
-    StepArgs a) {
+template <int MAXC, int NT, int OCC>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(OCC))) void k_forest_step(StepArgs a) {
   extern __shared__ float2 sh[];
   __shared__ int s_next;
   const int tid = threadIdx.x;
@@ -703,13 +710,18 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 64 ? 4
   const float* __restrict__ fd = b.delta;
   const float* __restrict__ fe = b.eta[2];
   const double D = b.cell;
+  // the queue slot of the next skewer is taken while this one is processed
+  // (its atomic's latency overlaps the work); every slot goes to one block
+  int q_next = 0;
+  if (tid == 0) q_next = atomicAdd(a.queue, 1);
   for (;;) {
-    if (tid == 0) s_next = atomicAdd(a.queue, 1);
+    if (tid == 0) s_next = q_next;
     __syncthreads();
     // the same value in every lane: scalar registers for all per-skewer terms
     const int s = __builtin_amdgcn_readfirstlane(s_next);
     __syncthreads();
     if (s >= sk.n) break;
+    if (tid == 0) q_next = atomicAdd(a.queue, 1);
     const int n = sk.npix[s];
     int c = 0;
     for (int nzc = 256; nzc < n + 50 && c < N_CLASS; nzc <<= 1) ++c;
@@ -771,6 +783,56 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 64 ? 4
     float4 prow = make_float4(0.f, 0.f, 0.f, 0.f);
     bool row_ok = per_pixel_row || (srow >= 0 && srow < a.abcg_rows);
     if (!per_pixel_row && row_ok) prow = reinterpret_cast<const float4*>(a.abcg)[srow];
+    // the xy-interpolated delta and eta of every z cell the forest spans,
+    // once per skewer into LDS (the same fused multiply-adds, in the same
+    // order, as the per-pixel sums below): a pixel then reads 4 LDS values
+    // instead of 16 box values.  Cells k = kmin .. kmax, unwrapped.
+    float* ctab = reinterpret_cast<float*>(sh + a.ctab_off);
+    int64_t kmin = 0;
+    bool use_tab = false;
+    if (a.ctab_cap > 0 && !xy_bad && imin < imax) {
+      kmin = (int64_t)floor(fma((double)imin, duz, uz0));
+      const int64_t kmax = (int64_t)floor(fma((double)(imax - 1), duz, uz0)) + 1;
+      const int64_t ncell = kmax - kmin + 1;
+      use_tab = ncell <= a.ctab_cap;
+      if (use_tab) {
+        // two cells per pass: the 16 loads of both are issued together
+        for (int k0 = tid; k0 < ncell; k0 += 2 * NT) {
+          float dv[2][4], evv[2][4];
+          bool ok[2];
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const int k = k0 + h * NT;
+            int64_t jz = kmin + k;
+            ok[h] = k < ncell;
+            if (b.periodic) jz = wrap(jz, nz);
+            else ok[h] = ok[h] && jz >= 0 && jz < nz;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              dv[h][q] = ok[h] ? fd[col[q] + jz] : 0.f;
+              evv[h][q] = (ok[h] && fe) ? fe[col[q] + jz] : 0.f;
+            }
+          }
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const int k = k0 + h * NT;
+            float av = 0.f, ev = 0.f;
+            if (ok[h]) {
+#pragma unroll
+              for (int q = 0; q < 4; ++q) {
+                av = fmaf(wxy[q], dv[h][q], av);
+                if (fe) ev = fmaf(wxy[q], evv[h][q], ev);
+              }
+            }
+            if (k < ncell) {
+              ctab[k] = av;
+              ctab[a.ctab_cap + k] = ev;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
     int flags = 0;
     for (int i = tid; i < n; i += NT) {
       float ds = dsf[i];
@@ -792,6 +854,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 64 ? 4
           dl = __int_as_float(0x7fc00000);
           eta = dl;
           flags |= (outside || zout ? 1 : 0) | (off_slab ? 4 : 0);
+        } else if (use_tab) {
+          const int k0 = (int)((int64_t)fz0 - kmin);
+          const float a0 = ctab[k0], a1 = ctab[k0 + 1];
+          const float e0 = ctab[a.ctab_cap + k0], e1 = ctab[a.ctab_cap + k0 + 1];
+          dl = fmaf(fz, a1 - a0, a0);
+          eta = fmaf(fz, e1 - e0, e0);
         } else {
           float a0 = 0.f, a1 = 0.f, e0 = 0.f, e1 = 0.f;
 #pragma unroll
@@ -873,6 +941,7 @@ struct smk_forest_plan {
   float2* tw[N_CLASS];   // W_M^m, M = 128 << c
   float2* twr[N_CLASS];  // W_{2M}^k
   const uint64_t* seed_dev;  // graph mode: the seed in device memory, no host read-back
+  int step_variant;          // smk_forest_step instance for batches up to class 3
 };
 
 extern "C" int smk_forest_plan_create(smk_forest_plan** plan) {
@@ -898,6 +967,14 @@ extern "C" int smk_forest_plan_create(smk_forest_plan** plan) {
 extern "C" int smk_forest_plan_set_graph_mode(smk_forest_plan* p, const uint64_t* seed_dev) {
   SMK_REQUIRE(p != nullptr, SMK_ERR_ARG, "plan is NULL");
   p->seed_dev = seed_dev;
+  return SMK_OK;
+}
+
+extern "C" int smk_forest_plan_set_step_variant(smk_forest_plan* p, int variant) {
+  SMK_REQUIRE(p != nullptr, SMK_ERR_ARG, "plan is NULL");
+  SMK_REQUIRE(variant >= 0 && variant < SMK_FOREST_STEP_NVAR, SMK_ERR_ARG, "step variant %d not in [0, %d)",
+              variant, SMK_FOREST_STEP_NVAR);
+  p->step_variant = variant;
   return SMK_OK;
 }
 
@@ -1084,23 +1161,34 @@ extern "C" int smk_forest_step(smk_forest_plan* plan, const smk_skewers* sk, con
     SMK_REQUIRE(h == 0, SMK_ERR_RANGE, "a filter set index is outside [0, %d)", (int)n_sets);
   }
   // kernel instance: one wave per skewer for classes up to 3 (nz <= 2048,
-  // n <= 1998), four waves with all six classes; skewers of a class above
-  // max_npix's are flagged (a.max_class)
+  // n <= 1998), at least 4 waves per SIMD (variant 0: some registers spill)
+  // or 3 (variant 1: none spill); four waves per skewer with all six
+  // classes.  Skewers of a class above max_npix's are flagged (a.max_class)
   const bool small = c <= 3;
+  const bool v1 = plan->step_variant == 1;
   constexpr int NT_S = 64, NT_L = 256;
   const int nt = small ? NT_S : NT_L;
-  const size_t lds = (size_t)(small ? step_lds_max(3, NT_S) : step_lds_max(N_CLASS - 1, NT_L)) * sizeof(float2);
+  const int mp_float2 = small ? step_lds_max(3, NT_S) : step_lds_max(N_CLASS - 1, NT_L);
+  // column table: the z cells of the longest forest (n pixels span at most
+  // n dr / D + 2 cells; 2 more for the rounding of the positions), up to
+  // 4096 cells; longer spans read the box per pixel
+  const double span = (double)max_npix * sk->dr / box->cell + 4.0;
+  a.ctab_cap = (sk->dr > 0 && span <= 4096.0) ? (int32_t)span : 0;
+  a.ctab_off = mp_float2;
+  const size_t lds = ((size_t)mp_float2 + (size_t)a.ctab_cap) * sizeof(float2);
   int dev = 0, ncu = 0, per_cu = 0;
   SMK_HIP_CHECK(hipGetDevice(&dev));
   SMK_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  SMK_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &per_cu, small ? k_forest_step<3, NT_S> : k_forest_step<N_CLASS - 1, NT_L>, nt, lds));
+  void (*kern)(StepArgs) = small ? (v1 ? k_forest_step<3, NT_S, 3> : k_forest_step<3, NT_S, 4>)
+                                 : k_forest_step<N_CLASS - 1, NT_L, 1>;
+  SMK_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, nt, lds));
   const int64_t want = (int64_t)ncu * (per_cu > 0 ? per_cu : 1);
   const int nblk = (int)(want < sk->n ? want : sk->n);
   hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, a.queue, 1);
   SMK_LAUNCH_CHECK();
-  if (small) hipLaunchKernelGGL((k_forest_step<3, NT_S>), dim3(nblk), dim3(NT_S), lds, s, a);
-  else hipLaunchKernelGGL((k_forest_step<N_CLASS - 1, NT_L>), dim3(nblk), dim3(NT_L), lds, s, a);
+  if (small && v1) hipLaunchKernelGGL((k_forest_step<3, NT_S, 3>), dim3(nblk), dim3(NT_S), lds, s, a);
+  else if (small) hipLaunchKernelGGL((k_forest_step<3, NT_S, 4>), dim3(nblk), dim3(NT_S), lds, s, a);
+  else hipLaunchKernelGGL((k_forest_step<N_CLASS - 1, NT_L, 1>), dim3(nblk), dim3(NT_L), lds, s, a);
   SMK_LAUNCH_CHECK();
   return SMK_OK;
 }

@@ -30,7 +30,7 @@ class P1DFitModel:
     """Device state of one fit: skewer batch with a parameter set per skewer,
     the fixed fields delta = delta_l + delta_s and eta_par, G per set, k bins."""
 
-    def __init__(self, batch, delta, eta, growth, edges, pixel=0.2, regroup=1, save_modes=True):
+    def __init__(self, batch, delta, eta, growth, edges, pixel=0.2, regroup=1, save_modes=True, use_tables=True):
         if batch.pset is None:
             raise ValueError("the batch needs a parameter set (pset) per skewer")
         self.batch = batch
@@ -67,8 +67,33 @@ class P1DFitModel:
             moff[1:] = np.cumsum(nj[:-1])
         self.modes = torch.empty(2 * int(nj.sum()) + 2, dtype=torch.float64, device=dev) if save_modes else None
         self.modes_off = torch.from_numpy(moff).to(dev) if save_modes else None
+        self.tables, self.tab_off = self._bluestein_tables(nf // self.regroup) if use_tables else (None, None)
         self.n_eval = 0
         self.n_grad = 0
+
+    def _bluestein_tables(self, nq):
+        """The Bluestein tables of every forest length of the batch, made
+        once (smk_p1d_tables); returns (tables, per-skewer offsets)."""
+        L = _lib.lib()
+        lens = np.unique(nq[nq > 0]).astype(np.int64)
+        size = np.array([L.smk_p1d_table_len(int(n)) for n in lens], dtype=np.int64)
+        lens, size = lens[size > 0], size[size > 0]
+        offs = np.zeros(len(lens), dtype=np.int64)
+        if len(lens):
+            offs[1:] = np.cumsum(size[:-1])
+        tab_sk = np.full(len(nq), -1, dtype=np.int64)
+        if len(lens):
+            i = np.minimum(np.searchsorted(lens, nq), len(lens) - 1)
+            hit = lens[i] == nq
+            tab_sk[hit] = offs[i[hit]]
+        tables = torch.empty(2 * max(int(size.sum()), 1), dtype=torch.float64, device=self.dev)
+        if len(lens):
+            lens_d = torch.from_numpy(lens.astype(np.int32)).to(self.dev)
+            offs_d = torch.from_numpy(offs).to(self.dev)
+            _lib.check(L.smk_p1d_tables(_lib.ptr(lens_d), _lib.ptr(offs_d), len(lens), _lib.ptr(tables),
+                                        _lib.stream_ptr()), "smk_p1d_tables")
+            torch.cuda.current_stream().synchronize()
+        return tables, torch.from_numpy(tab_sk).to(self.dev)
 
     def set_edges(self, edges):
         """New k bin edges (h/Mpc, increasing) for the following evaluations."""
@@ -110,7 +135,8 @@ class P1DFitModel:
         _lib.check(L.smk_p1d_binned(ctypes.byref(self.sk), _lib.ptr(flux), _lib.ptr(self.fmean_dev),
                                     self.pixel, self.regroup, _lib.ptr(self.edges), self.nb, self.max_forest,
                                     _lib.ptr(self.partial), _lib.ptr(self.status), _lib.ptr(modes),
-                                    _lib.ptr(self.modes_off if modes is not None else None), s), "smk_p1d_binned")
+                                    _lib.ptr(self.modes_off if modes is not None else None), _lib.ptr(self.tables),
+                                    _lib.ptr(self.tab_off), s), "smk_p1d_binned")
         _lib.check(L.smk_reduce_by_set(ctypes.byref(self.sk), _lib.ptr(self.partial), self.nb * 4, self.n_sets,
                                        _lib.ptr(self.bins_set), s), "smk_reduce_by_set")
         return _sync_host(self.bins_set)
@@ -170,7 +196,8 @@ class P1DFitModel:
                                      _lib.ptr(abcg), abcg.shape[0], _lib.ptr(self.flux), _lib.ptr(self.fmean_dev),
                                      self.pixel, self.regroup, _lib.ptr(self.edges), self.nb, self.max_forest,
                                      _lib.ptr(self.modes), _lib.ptr(self.modes_off), _lib.ptr(gp_dev),
-                                     _lib.ptr(self.adj), s), "smk_p1d_adjoint")
+                                     _lib.ptr(self.adj), _lib.ptr(self.tables), _lib.ptr(self.tab_off), s),
+                   "smk_p1d_adjoint")
         _lib.check(L.smk_reduce_by_set(ctypes.byref(self.sk), _lib.ptr(self.adj), 7, self.n_sets,
                                        _lib.ptr(self.adj_set), s), "smk_reduce_by_set")
         r = _sync_host(self.adj_set)

@@ -169,6 +169,12 @@ class ForestPlan:
                    "smk_forest_plan_set_graph_mode")
         self._seed_dev = seed_dev
 
+    def set_step_variant(self, variant):
+        """Kernel instance of step() for batches of forests up to 1998
+        pixels (smk_forest_plan_set_step_variant): 0 or 1, same results."""
+        _lib.check(_lib.lib().smk_forest_plan_set_step_variant(self._h, int(variant)),
+                   "smk_forest_plan_set_step_variant")
+
     def __del__(self):
         h = getattr(self, "_h", None)
         destroy = getattr(self, "_destroy", None)

@@ -8,14 +8,14 @@ from oracle import fit_oracle as fto
 pytestmark = pytest.mark.gpu
 
 
-def _problem(cuda, seed=0, ns=3, per=40, m=1, lo=200, hi=900, save_modes=True):
+def _problem(cuda, seed=0, ns=3, per=40, m=1, lo=200, hi=900, save_modes=True, use_tables=True, lens=None):
     from saclaymocks_amd import fit, forest
     rng = np.random.default_rng(seed)
-    lens = rng.integers(lo, hi, ns * per)
+    lens = rng.integers(lo, hi, ns * per) if lens is None else np.asarray(lens)
     # smooth-ish Gaussian fields along each skewer
     deltas, etas = [], []
     for n in lens:
-        w = np.convolve(rng.standard_normal(n + 8), np.ones(9) / 3.0, mode="valid")
+        w = np.convolve(rng.standard_normal(n + 8), np.ones(9) / 3.0, mode="valid")[:n]
         deltas.append((0.8 * w).astype(np.float32))
         etas.append((0.25 * rng.standard_normal(n)).astype(np.float32))
     pset = np.repeat(np.arange(ns), per).astype(np.int32)
@@ -26,7 +26,8 @@ def _problem(cuda, seed=0, ns=3, per=40, m=1, lo=200, hi=900, save_modes=True):
                            np.zeros(len(lens)), pset=pset)
     d = torch.from_numpy(np.concatenate(deltas)).to(cuda)
     e = torch.from_numpy(np.concatenate(etas)).to(cuda)
-    model = fit.P1DFitModel(b, d, e, growth, edges, pixel=0.2, regroup=m, save_modes=save_modes)
+    model = fit.P1DFitModel(b, d, e, growth, edges, pixel=0.2, regroup=m, save_modes=save_modes,
+                            use_tables=use_tables)
     theta_true = np.array([[0.2, 1.4, 0.9], [0.3, 1.2, 1.1], [0.15, 1.6, 0.7], [0.25, 1.5, 1.0],
                            [0.35, 1.3, 0.8]])[:ns].reshape(-1)
     d64 = [x.astype(np.float64) for x in deltas]
@@ -39,10 +40,7 @@ def _problem(cuda, seed=0, ns=3, per=40, m=1, lo=200, hi=900, save_modes=True):
     return model, (d64, e64, pset, growth, edges, 0.2, targets, m), theta_true
 
 
-@pytest.mark.parametrize("m, ns", [(1, 3), (2, 3), (1, 5)])
-def test_forward_parity(cuda, m, ns):
-    model, args, theta = _problem(cuda, m=m, ns=ns)
-    th = theta * np.array([1.1, 0.95, 1.05] * ns)
+def _check_forward(model, args, th):
     fmean, bins = model.forward(th)
     d64, e64, pset, growth, edges, pixel, targets, mm = args
     rf, rb, _ = fto.forward(d64, e64, pset, th, growth, edges, pixel, mm)
@@ -54,9 +52,38 @@ def test_forward_parity(cuda, m, ns):
     assert model.status.sum().item() == 0
 
 
+@pytest.mark.parametrize("use_tables", [True, False])
+@pytest.mark.parametrize("m, ns", [(1, 3), (2, 3), (1, 5)])
+def test_forward_parity(cuda, m, ns, use_tables):
+    model, args, theta = _problem(cuda, m=m, ns=ns, use_tables=use_tables)
+    _check_forward(model, args, theta * np.array([1.1, 0.95, 1.05] * ns))
+
+
+# forests at the edges of every path: empty, 1-3 points, the Bluestein size
+# classes' bounds (M = 1024 up to 512 points, 2048 up to 1024, 4096 up to
+# 2048), the switch to the direct DFT at 2049
+EDGE_LENGTHS = [0, 1, 2, 3, 7, 511, 512, 513, 1024, 1025, 1500, 2047, 2048, 2049, 3001, 700, 300, 900]
+
+
+@pytest.mark.parametrize("use_tables", [True, False])
+def test_forward_parity_edge_lengths(cuda, use_tables):
+    lens = EDGE_LENGTHS * 2
+    model, args, theta = _problem(cuda, ns=2, per=len(EDGE_LENGTHS), lens=lens, use_tables=use_tables)
+    if use_tables:
+        # skewers without a table take the direct DFT of the generic kernel
+        off = model.tab_off.clone()
+        assert int((off >= 0).sum()) == sum(1 for n in lens if 1 <= n <= 2048)
+        model.tab_off[::5] = -1
+    _check_forward(model, args, theta * np.array([1.1, 0.95, 1.05] * 2))
+    val, g = model.value_and_grad(theta * np.array([1.2, 0.9, 1.1] * 2))
+    want = fto.adjoint_grad(theta * np.array([1.2, 0.9, 1.1] * 2), *args)
+    assert np.allclose(g, want, rtol=1e-7, atol=1e-9 * np.abs(want).max())
+
+
+@pytest.mark.parametrize("use_tables", [True, False])
 @pytest.mark.parametrize("save_modes", [True, False])
-def test_adjoint_gradient_parity(cuda, save_modes):
-    model, args, theta = _problem(cuda, save_modes=save_modes, lo=100, hi=2500)
+def test_adjoint_gradient_parity(cuda, save_modes, use_tables):
+    model, args, theta = _problem(cuda, save_modes=save_modes, lo=100, hi=2500, use_tables=use_tables)
     th = theta * np.array([1.2, 0.9, 1.1] * 3)
     val, g = model.value_and_grad(th)
     want = fto.adjoint_grad(th, *args)

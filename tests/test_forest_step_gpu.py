@@ -181,3 +181,51 @@ def test_forest_step_slab_edges_and_status(cuda):
     assert o1["status"].cpu().tolist() == [0, 4]
     n0 = 300
     assert torch.equal(o1["delta_l"][:n0], o2["delta_l"][:n0])
+
+
+def test_forest_step_wraps_the_periodic_box_and_variants_agree(cuda):
+    """Skewers that run past the end of a periodic box in z (the per-skewer
+    column table wraps its cells), against the oracle; and the two kernel
+    instances of the step (smk_forest_plan_set_step_variant) agree bit for bit."""
+    from saclaymocks_amd import forest
+    print("[forest step] z wrap, step variants", flush=True)
+    shape, cell, r0 = (32, 24, 256), 2.19, 1000.0
+    rng = np.random.default_rng(21)
+    npix = rng.integers(300, 1501, 40).astype(np.int32)
+    L = [cell * s for s in shape]
+    n = len(npix)
+    rmin = r0 - L[2] / 2 + 0.05
+    maxg = int((L[2] - 0.2) / DR)
+    g0 = (maxg - npix // 2).astype(np.int32)          # half of each skewer beyond the box end
+    qx, qy = rng.uniform(-L[0] / 2, L[0] / 2, n), rng.uniform(-L[1] / 2, L[1] / 2, n)
+    imin = np.zeros(n, np.int32)
+    imax = npix.copy()
+    ids = rng.integers(0, 2**62, n).astype(np.uint64)
+    b = forest.SkewerBatch(forest.GEOM_PLANE, g0, npix, rmin, DR, qx, qy, imin=imin, imax=imax,
+                           pset=np.zeros(n, np.int32), noise_id=ids)
+    c = dict(g0=g0, qx=qx, qy=qy, imin=imin, imax=imax, rmin=rmin, L=L, ids=ids)
+    d, e = _field(shape, 4), _field(shape, 5)
+    box = forest.BoxView.plane(torch.from_numpy(d).to(cuda), torch.from_numpy(e).to(cuda), cell=cell, r0=r0,
+                               periodic=True)
+    rows_h = np.array([[0.2, 1.5, 1.0, 0.6]])
+    rows = torch.from_numpy(rows_h.astype(np.float32)).to(cuda)
+    filt, foff = forest.missing_power_filters(DR, fo.p1d_missing_analytic)
+    want = ("flux", "delta_l", "eta_par", "delta_s")
+    outs = []
+    for v in (0, 1):
+        plan = forest.ForestPlan()
+        plan.set_step_variant(v)
+        outs.append(forest.step(plan, b, box, filt, foff, rows, seed=9, want=want))
+    torch.cuda.synchronize()
+    for k in want + ("status",):
+        assert torch.equal(outs[0][k], outs[1][k]), k
+    assert int(outs[0]["status"].sum().item()) == 0
+    got = {k: b.split(outs[0][k]) for k in want}
+    for s, k in enumerate(npix):
+        rd, re, ds, g = _oracle_skewer(c, s, int(k), [d, e], cell, r0, 9)
+        assert g[-1] * DR > L[2]                       # the skewer does leave the box
+        assert np.abs(got["delta_l"][s] - rd).max() < 2e-6, s
+        assert np.abs(got["eta_par"][s] - re).max() < 2e-6, s
+        assert np.abs(got["delta_s"][s] - ds).max() / np.abs(ds).max() < 1e-5, s
+    with pytest.raises(ValueError):
+        forest.ForestPlan().set_step_variant(2)
