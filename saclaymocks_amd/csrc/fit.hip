@@ -366,7 +366,7 @@ __host__ __device__ constexpr size_t blue_line_bytes_m(int M) { return (((size_t
 // (blue_m_of(N) == M), M / 8 threads of 8 points each: the DFT by
 // bluestein_tab (two FFTs on one register tile), then the bins.
 template <int M>
-__global__ __launch_bounds__(M / 8) void k_p1d_blue(P1dArgs a) {
+__global__ __launch_bounds__(M / 8) __attribute__((amdgpu_waves_per_eu(2))) void k_p1d_blue(P1dArgs a) {
   constexpr int NT = M / 8;
   extern __shared__ double dyn[];
   double2* line = reinterpret_cast<double2*>(dyn);
@@ -453,38 +453,54 @@ struct AdjArgs {
 };
 
 // The FGPA part of the adjoint of one skewer, with A_q (q < nq) in A:
-// (sum A F', sum A_q(n) dF_n/d(a, b, c) / m, sum dF_n/d(a, b, c)) into res
+// (sum A F', sum A_q(n) dF_n/d(a, b, c) / m, sum dF_n/d(a, b, c)) into res.
+// F is the forward's flux at the same parameters (the same expression, so
+// the same bits as recomputing it).  The seven block sums share one pass:
+// a shuffle tree per wave, then the waves' partials in a fixed order (red:
+// 7 NT / 64 doubles).
 template <int NT>
 __device__ __forceinline__ void adj_fgpa(const AdjArgs& a, int s, int set, int i0, int i1, int nq, double saf,
                                          const double* A, double* red, double* res) {
+  constexpr int NW = NT / 64;
   const int m = a.regroup;
   const int64_t o = a.sk.off[s];
   const double4 p = prow(a.abcg, a.rows, set);
   const double pa = p.x, pb = p.y, pc = p.z, pg = p.w;
-  double t1a = 0.0, t1b = 0.0, t1c = 0.0, t2a = 0.0, t2b = 0.0, t2c = 0.0;
+  double v[7] = {saf, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (int i = threadIdx.x; i < i1 - i0; i += NT) {
     const int q = i / m;
     const double d = a.delta[o + i0 + i];
     const double e = a.eta ? (double)a.eta[o + i0 + i] : 0.0;
     const double gf = d + pc * e;
     const double T = exp(pb * pg * gf);
-    const double F = exp(-pa * T);
+    const double F = a.flux[o + i0 + i];
     const double dfa = -T * F;
     const double dfb = -pa * T * F * pg * gf;
     const double dfc = -pa * T * F * pb * pg * e;
-    t2a += dfa; t2b += dfb; t2c += dfc;
+    v[4] += dfa; v[5] += dfb; v[6] += dfc;
     if (q < nq) {
       const double Aq = A[q] / m;
-      t1a += Aq * dfa;
-      t1b += Aq * dfb;
-      t1c += Aq * dfc;
+      v[1] += Aq * dfa;
+      v[2] += Aq * dfb;
+      v[3] += Aq * dfc;
     }
   }
-  const double v[7] = {saf, t1a, t1b, t1c, t2a, t2b, t2c};
-  for (int k = 0; k < 7; ++k) {
-    const double tot = block_reduce_sum<NT>(v[k], red);
-    if (threadIdx.x == 0) res[k] = tot;
+#pragma unroll
+  for (int k = 0; k < 7; ++k)
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) red[k * NW + wv] = v[k];
   }
+  __syncthreads();
+  if (threadIdx.x < 7) {
+    double t = 0.0;
+    for (int i = 0; i < NW; ++i) t += red[threadIdx.x * NW + i];
+    res[threadIdx.x] = t;
+  }
+  __syncthreads();
 }
 
 // Adjoint of the binned P1D through delta_F = F/<F> - 1 (at fixed <F>) and
@@ -498,7 +514,7 @@ __global__ __launch_bounds__(FIT_THREADS) void k_p1d_adjoint(AdjArgs a) {
   double2* cj = reinterpret_cast<double2*>(dyn + ((a.nmax + 1) & ~1));
   double2* bline = reinterpret_cast<double2*>(reinterpret_cast<char*>(dyn) + a.line_off);
   double* ledges = reinterpret_cast<double*>(reinterpret_cast<char*>(dyn) + a.e_off);
-  __shared__ double red[FIT_THREADS / 64];
+  __shared__ double red[7 * (FIT_THREADS / 64)];
   for (int i = threadIdx.x; i <= a.nb; i += FIT_THREADS) ledges[i] = a.edges[i];
   const smk_skewers& sk = a.sk;
   for (int s = blockIdx.x; s < sk.n; s += gridDim.x) {
@@ -579,7 +595,8 @@ __host__ __device__ constexpr size_t adj_blue_x_bytes(int M) { return (size_t)(M
 // The adjoint of the skewers with a table and a forest of M's size class,
 // M / 8 threads of 8 points each.
 template <int M>
-__global__ __launch_bounds__(M / 8) void k_p1d_adjoint_blue(AdjArgs a) {
+__global__ __launch_bounds__(M / 8) __attribute__((amdgpu_waves_per_eu(2))) void k_p1d_adjoint_blue(
+    AdjArgs a) {
   constexpr int NT = M / 8;
   extern __shared__ double dyn[];
   double* x = dyn;
@@ -587,7 +604,7 @@ __global__ __launch_bounds__(M / 8) void k_p1d_adjoint_blue(AdjArgs a) {
   double2* cj = line;
   double* ledges =
       reinterpret_cast<double*>(reinterpret_cast<char*>(dyn) + adj_blue_x_bytes(M) + blue_line_bytes_m(M));
-  __shared__ double red[NT / 64];
+  __shared__ double red[7 * (NT / 64)];
   for (int i = threadIdx.x; i <= a.nb; i += NT) ledges[i] = a.edges[i];
   const smk_skewers& sk = a.sk;
   for (int s = blockIdx.x; s < sk.n; s += gridDim.x) {
