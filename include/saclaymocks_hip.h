@@ -280,10 +280,13 @@ int smk_forest_plan_destroy(smk_forest_plan* plan);
  * checks it.  NULL restores the normal mode. */
 int smk_forest_plan_set_graph_mode(smk_forest_plan* plan, const uint64_t* seed_dev);
 /* Kernel instance of smk_forest_step for batches whose FFTs are at most 2048
- * points (the planner's choice, no counterpart in the reference): 0 (the
- * default) keeps at least 4 waves per SIMD, 1 at least 3 with no register
- * spills.  Same results, bit for bit. */
-#define SMK_FOREST_STEP_NVAR 2
+ * points (the planner's choice, no counterpart in the reference): one wave
+ * per skewer with at least 4 (0, the default) or 3 (1) waves per SIMD, two
+ * waves per skewer with at least 4 (2) or 5 (3).  0 and 1 give the same
+ * bits, as do 2 and 3; the two pairs transform the missing power with
+ * different register tiles, so their delta_s and F differ by float32
+ * rounding (delta_l and eta_par are the same bits). */
+#define SMK_FOREST_STEP_NVAR 4
 int smk_forest_plan_set_step_variant(smk_forest_plan* plan, int variant);
 size_t smk_forest_workspace_bytes(int n_skewers);   /* class bucketing scratch */
 /*   filt_off    HOST int64[6]

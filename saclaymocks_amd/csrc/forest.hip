@@ -1160,15 +1160,18 @@ extern "C" int smk_forest_step(smk_forest_plan* plan, const smk_skewers* sk, con
     SMK_HIP_CHECK(hipStreamSynchronize(s));
     SMK_REQUIRE(h == 0, SMK_ERR_RANGE, "a filter set index is outside [0, %d)", (int)n_sets);
   }
-  // kernel instance: one wave per skewer for classes up to 3 (nz <= 2048,
-  // n <= 1998), at least 4 waves per SIMD (variant 0: some registers spill)
-  // or 3 (variant 1: none spill); four waves per skewer with all six
-  // classes.  Skewers of a class above max_npix's are flagged (a.max_class)
+  // kernel instance for classes up to 3 (nz <= 2048, n <= 1998): one wave
+  // per skewer with at least 4 waves per SIMD (variant 0: some registers
+  // spill) or 3 (variant 1: none spill), two waves per skewer with at least
+  // 4 (variant 2) or 5 (variant 3: some spill); four waves per skewer with
+  // all six classes.  Skewers of a class above max_npix's are flagged
+  // (a.max_class)
   const bool small = c <= 3;
-  const bool v1 = plan->step_variant == 1;
-  constexpr int NT_S = 64, NT_L = 256;
-  const int nt = small ? NT_S : NT_L;
-  const int mp_float2 = small ? step_lds_max(3, NT_S) : step_lds_max(N_CLASS - 1, NT_L);
+  const int var = plan->step_variant;
+  constexpr int NT_L = 256;
+  const int nt = small ? (var >= 2 ? 128 : 64) : NT_L;
+  const int mp_float2 = small ? (nt == 128 ? step_lds_max(3, 128) : step_lds_max(3, 64))
+                              : step_lds_max(N_CLASS - 1, NT_L);
   // column table: the z cells of the longest forest (n pixels span at most
   // n dr / D + 2 cells; 2 more for the rounding of the positions), up to
   // 4096 cells; longer spans read the box per pixel
@@ -1179,16 +1182,15 @@ extern "C" int smk_forest_step(smk_forest_plan* plan, const smk_skewers* sk, con
   int dev = 0, ncu = 0, per_cu = 0;
   SMK_HIP_CHECK(hipGetDevice(&dev));
   SMK_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  void (*kern)(StepArgs) = small ? (v1 ? k_forest_step<3, NT_S, 3> : k_forest_step<3, NT_S, 4>)
-                                 : k_forest_step<N_CLASS - 1, NT_L, 1>;
+  void (*const small_kern[SMK_FOREST_STEP_NVAR])(StepArgs) = {k_forest_step<3, 64, 4>, k_forest_step<3, 64, 3>,
+                                                                k_forest_step<3, 128, 4>, k_forest_step<3, 128, 5>};
+  void (*kern)(StepArgs) = small ? small_kern[var] : k_forest_step<N_CLASS - 1, NT_L, 1>;
   SMK_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, nt, lds));
   const int64_t want = (int64_t)ncu * (per_cu > 0 ? per_cu : 1);
   const int nblk = (int)(want < sk->n ? want : sk->n);
   hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, a.queue, 1);
   SMK_LAUNCH_CHECK();
-  if (small && v1) hipLaunchKernelGGL((k_forest_step<3, NT_S, 3>), dim3(nblk), dim3(NT_S), lds, s, a);
-  else if (small) hipLaunchKernelGGL((k_forest_step<3, NT_S, 4>), dim3(nblk), dim3(NT_S), lds, s, a);
-  else hipLaunchKernelGGL((k_forest_step<N_CLASS - 1, NT_L, 1>), dim3(nblk), dim3(NT_L), lds, s, a);
+  hipLaunchKernelGGL(kern, dim3(nblk), dim3(nt), lds, s, a);
   SMK_LAUNCH_CHECK();
   return SMK_OK;
 }

@@ -171,7 +171,8 @@ class ForestPlan:
 
     def set_step_variant(self, variant):
         """Kernel instance of step() for batches of forests up to 1998
-        pixels (smk_forest_plan_set_step_variant): 0 or 1, same results."""
+        pixels (smk_forest_plan_set_step_variant): 0 to 3; 0 and 1, and 2
+        and 3, give the same bits, the pairs agree to float32 rounding."""
         _lib.check(_lib.lib().smk_forest_plan_set_step_variant(self._h, int(variant)),
                    "smk_forest_plan_set_step_variant")
 
@@ -326,14 +327,16 @@ class CapturedStep:
     Outputs are the same tensors at every call: copy them before the next.
     fused=True captures the one-launch step (smk_forest_step: plane-parallel
     skewers, trilinear, FGPA rows required); otherwise the missing power and
-    the gather are separate launches, with delta_s in between."""
+    the gather are separate launches, with delta_s in between; step_variant
+    picks the fused step's kernel instance (ForestPlan.set_step_variant)."""
 
     def __init__(self, batch, box, filt, filt_off, abcg=None, interp="trilinear",
                  want=("delta_l", "eta_par", "flux"), filt_set=None, set_stride=0, n_sets=1, scale=None,
-                 fused=False):
+                 fused=False, step_variant=0):
         dev = _dev()
         n = batch.npix_total
         plan = ForestPlan()
+        plan.set_step_variant(step_variant)
         self.plan, self.batch = plan, batch
         self.seed = torch.zeros(1, dtype=torch.int64, device=dev)
         self.delta_s = torch.empty(n, dtype=torch.float32, device=dev)

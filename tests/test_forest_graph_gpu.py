@@ -39,19 +39,21 @@ def _eager_fused(forest, plan, batch, box, filt, foff, rows, seed):
     return forest.step(plan, batch, box, filt, foff, rows, seed=seed, want=("flux", "delta_l", "eta_par", "delta_s"))
 
 
-@pytest.mark.parametrize("interp", ["trilinear", "gauss343", "fused"])
+@pytest.mark.parametrize("interp", ["trilinear", "gauss343", "fused", "fused_two_waves"])
 def test_captured_step_replays_match_eager(cuda, interp):
     forest, batch, box, filt, foff, rows = _case(cuda, n=60 if interp == "gauss343" else 300)
     print(f"[forest graph] interp={interp}", flush=True)
     plan = forest.ForestPlan()
-    if interp == "fused":
-        step = forest.CapturedStep(batch, box, filt, foff, abcg=rows, interp="trilinear", fused=True)
+    if interp.startswith("fused"):
+        v = 2 if interp == "fused_two_waves" else 0
+        plan.set_step_variant(v)
+        step = forest.CapturedStep(batch, box, filt, foff, abcg=rows, interp="trilinear", fused=True, step_variant=v)
     else:
         step = forest.CapturedStep(batch, box, filt, foff, abcg=rows, interp=interp)
     seeds = [5, 9, 5, 123456789, -1, 2 ** 63 + 5]
     prev = None
     for s in seeds:
-        if interp == "fused":
+        if interp.startswith("fused"):
             want = _eager_fused(forest, plan, batch, box, filt, foff, rows, s)
         else:
             want = _eager(forest, plan, batch, box, filt, foff, rows, s, interp)   # eager work between replays
