@@ -383,7 +383,9 @@ int smk_reduce_by_set(const smk_skewers* sk, const double* per_skewer, int width
  * then per bin b of `edges` (scipy binned_statistic: [e_b, e_b+1), last bin
  * closed): partial[s][b] = (count, sum k, sum P, sum P^2), FP64.
  *   fmean       device double per set
- *   max_forest  longest forest (imax - imin) in the batch, <= 8192 * m
+ *   max_forest  longest forest (imax - imin) in the batch; regrouped
+ *               forests of more than 8192 points take a direct DFT with the
+ *               input staged through LDS (O(N^2), no length limit)
  *   status      NULL or int32 per skewer, 1 if its forest exceeds max_forest
  *   modes       NULL or device complex double: the rfft modes X_j of every
  *               skewer, (N/2 + 1) of them at modes + 2 * modes_off[s], kept
@@ -419,7 +421,7 @@ int smk_p1d_tables(const int32_t* lens, const int64_t* offs, int n_lens, double*
  *   gp    device double [sets][nbins]: dL/dP for one P_j sample of bin b
  *         (dL/d mean_b divided by the bin count)
  *   modes NULL (recompute) or the modes saved by smk_p1d_binned at the same
- *         parameters
+ *         parameters; required (SMK_ERR_ARG) when max_forest / m > 8192
  *   tables, tab_off  as smk_p1d_binned
  * out[s][7] = (sum_q A_q F'_q, sum_n A_q(n) dF_n/d(a, b, c) / m,
  *              sum_n dF_n/d(a, b, c)) with A_q = dL/d delta_F,q at fixed <F>;

@@ -30,7 +30,9 @@ using namespace smk;
 namespace {
 
 constexpr int FIT_THREADS = 512;
-constexpr int FIT_NMAX = 8192;  // longest (regrouped) forest handled by the P1D kernels
+constexpr int FIT_NLDS = 8192;  // longest (regrouped) forest of the LDS layouts; longer ones take the
+                                // direct DFT with the input staged through LDS in pieces
+constexpr int FIT_XC = 2048;    // input points per staged piece of that path (complex doubles)
 constexpr int BLUE_NMAX = 2048; // longest forest of the Bluestein path (M <= 4096)
 constexpr int BLUE_MMAX = 4096;
 
@@ -56,12 +58,24 @@ size_t p1d_a_bytes(int nmax) {
   const size_t nj = (size_t)(nmax / 2 + 1);
   return align16(max_sz(blue_line_bytes(nmax), (size_t)nmax * 8 + nj * 12));
 }
-size_t p1d_lds_bytes(int nmax, int nb) { return p1d_a_bytes(nmax) + (size_t)(nb + 1) * 8 + (size_t)nb * 8; }
-// adjoint, generic kernel: [x][cj][Bluestein line (TAB = false)][edges]
+// + the edges, the bin runs (2 nb ints) and, for forests longer than the
+// layout, the running bin sums (4 nb doubles)
+size_t p1d_lds_bytes(int nmax, int nb) {
+  return p1d_a_bytes(nmax) + (size_t)(nb + 1) * 8 + (size_t)nb * 8 + (size_t)nb * 32;
+}
+// adjoint, generic kernel: x (nmax doubles) and cj (nmax / 2 + 1 complex)
+// for the Horner forests; a Bluestein forest (TAB = false, N <= BLUE_NMAX)
+// keeps x and cj in the first BLUE_NMAX-point layout and the Bluestein line
+// after them; then the edges
 size_t adj_x_bytes(int nmax) { return align16((size_t)nmax * 8); }
 size_t adj_cj_bytes(int nmax) { return align16((size_t)(nmax / 2 + 1) * 16); }
+size_t adj_line_off(int nmax) {
+  const int nb = nmax < BLUE_NMAX ? nmax : BLUE_NMAX;
+  return adj_x_bytes(nb) + adj_cj_bytes(nb);
+}
 size_t adj_e_off(int nmax, bool tab) {
-  return adj_x_bytes(nmax) + adj_cj_bytes(nmax) + (tab ? 0 : blue_line_bytes(nmax));
+  const size_t horner = adj_x_bytes(nmax) + adj_cj_bytes(nmax);
+  return tab ? horner : max_sz(horner, adj_line_off(nmax) + blue_line_bytes(nmax));
 }
 
 // one transform size per function, so that each gets its own register
@@ -231,6 +245,64 @@ __device__ __forceinline__ void dft_all(const double* yr, const double2* yc, int
   }
 }
 
+// first mode j in [0, nj) with k_j >= e (gt: k_j > e), nj if none (k_j
+// increases with j)
+__device__ __forceinline__ int first_mode(int nj, int n, double k_ny, double e, bool gt) {
+  int lo = 0, hi = nj;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    const double k = kfreq(mid, n, k_ny);
+    if (gt ? k > e : k >= e) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+
+// the modes [lo, hi) of bin b: e_b <= k_j < e_b+1, the last bin closed (the
+// comparisons of find_bin, so the same membership)
+__device__ __forceinline__ void bin_span(const double* edges, int nb, int b, int nj, int n, double k_ny, int& lo,
+                                         int& hi) {
+  lo = first_mode(nj, n, k_ny, edges[b], false);
+  hi = b + 1 < nb ? first_mode(nj, n, k_ny, edges[b + 1], false) : first_mode(nj, n, k_ny, edges[nb], true);
+}
+
+// Horner's rule for the frequencies f = fb + threadIdx.x + r FIT_THREADS
+// (r < 4, f < fe): out_f = sum_{n<N} y_n exp(-2 pi i f n / P), y staged
+// through LDS (buf, FIT_XC points) in pieces, n descending; stage(n0, n1)
+// fills buf[0, n1 - n0) with y_n0 .. y_n1-1 (every thread calls it)
+template <class Stage>
+__device__ __forceinline__ void horner_staged(Stage stage, double2* buf, int N, int P, int fb, int fe,
+                                              double (&re)[4], double (&im)[4]) {
+  double c[4], sn[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int f = fb + (int)threadIdx.x + r * FIT_THREADS;
+    if (f < fe) {
+      sincospi(-2.0 * (double)f / (double)P, &sn[r], &c[r]);
+    } else {
+      sn[r] = 0.0;
+      c[r] = 1.0;
+    }
+    re[r] = 0.0;
+    im[r] = 0.0;
+  }
+  for (int nh = N; nh > 0; nh -= FIT_XC) {
+    const int nl = nh > FIT_XC ? nh - FIT_XC : 0;
+    __syncthreads();
+    stage(nl, nh);
+    __syncthreads();
+    for (int n = nh - 1; n >= nl; --n) {
+      const double2 y = buf[n - nl];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double r2 = fma(re[r], c[r], fma(-im[r], sn[r], y.x));
+        im[r] = fma(re[r], sn[r], fma(im[r], c[r], y.y));
+        re[r] = r2;
+      }
+    }
+  }
+}
+
 struct P1dArgs {
   smk_skewers sk;
   const double* flux;
@@ -248,6 +320,8 @@ struct P1dArgs {
   const double2* tables; // NULL, or the per-length Bluestein tables (bluestein_table layout)
   const int64_t* tab_off;// with tables: per skewer, its table at tables + tab_off[s]; < 0: none
   size_t a_bytes;        // byte offset of the edges in the dynamic LDS
+  int nlds;              // longest forest of the LDS layout (min(nmax, FIT_NLDS))
+  int chunk;             // modes per pass of the long-forest path
 };
 
 // skewer s has a table and a forest in the Bluestein range: the per-size
@@ -299,10 +373,11 @@ __device__ __forceinline__ void bin_runs(const double* pk, const int* bins, int 
 template <bool TAB>
 __global__ __launch_bounds__(FIT_THREADS) void k_p1d(P1dArgs a) {
   extern __shared__ double dyn[];
-  const int njmax = a.nmax / 2 + 1;
+  const int njmax = a.nlds / 2 + 1;
   double* ledges = reinterpret_cast<double*>(reinterpret_cast<char*>(dyn) + a.a_bytes);
   int* bfirst = reinterpret_cast<int*>(ledges + a.nb + 1);
   int* blast = bfirst + a.nb;
+  double* bsum = reinterpret_cast<double*>(blast + a.nb);
   for (int i = threadIdx.x; i <= a.nb; i += FIT_THREADS) ledges[i] = a.edges[i];
   const smk_skewers& sk = a.sk;
   for (int s = blockIdx.x; s < sk.n; s += gridDim.x) {
@@ -321,8 +396,58 @@ __global__ __launch_bounds__(FIT_THREADS) void k_p1d(P1dArgs a) {
     const double k_ny = M_PI / px;
     const int nj = nq > 0 ? nq / 2 + 1 : 0;
     double2* xj = a.xj ? a.xj + a.xoff[s] : nullptr;
+    if (nq > a.nlds) {
+      // longer than the LDS layout: the direct DFT with delta_F staged
+      // through LDS in pieces, the modes in passes of a.chunk, each bin's
+      // running sums in LDS, added in ascending j as a full scan adds them
+      double2* xbuf = reinterpret_cast<double2*>(dyn);
+      double* pkc = dyn + 2 * FIT_XC;
+      for (int b = threadIdx.x; b < a.nb; b += FIT_THREADS) {
+        bin_span(ledges, a.nb, b, nj, nq, k_ny, bfirst[b], blast[b]);
+        bsum[4 * b] = bsum[4 * b + 1] = bsum[4 * b + 2] = bsum[4 * b + 3] = 0.0;
+      }
+      const int64_t o = sk.off[s] + i0;
+      const int m = a.regroup;
+      auto stage = [&](int n0, int n1) {
+        for (int n = n0 + (int)threadIdx.x; n < n1; n += FIT_THREADS) {
+          double acc = 0.0;
+          for (int r = 0; r < m; ++r) acc += a.flux[o + (int64_t)n * m + r];
+          xbuf[n - n0] = make_double2((acc / m) * inv - 1.0, 0.0);
+        }
+      };
+      for (int j0 = 0; j0 < nj; j0 += a.chunk) {
+        const int j1 = j0 + a.chunk < nj ? j0 + a.chunk : nj;
+        for (int fb = j0; fb < j1; fb += 4 * FIT_THREADS) {
+          double re[4], im[4];
+          horner_staged(stage, xbuf, nq, nq, fb, j1, re, im);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int j = fb + (int)threadIdx.x + r * FIT_THREADS;
+            if (j < j1) {
+              pkc[j - j0] = (re[r] * re[r] + im[r] * im[r]) * px / nq;
+              if (xj) xj[j] = make_double2(re[r], im[r]);
+            }
+          }
+        }
+        __syncthreads();
+        for (int b = threadIdx.x; b < a.nb; b += FIT_THREADS) {
+          const int lo = bfirst[b] > j0 ? bfirst[b] : j0, hi = blast[b] < j1 ? blast[b] : j1;
+          double c = bsum[4 * b], sk_ = bsum[4 * b + 1], sp = bsum[4 * b + 2], sp2 = bsum[4 * b + 3];
+          for (int j = lo; j < hi; ++j) {
+            c += 1.0;
+            sk_ += kfreq(j, nq, k_ny);
+            sp += pkc[j - j0];
+            sp2 += pkc[j - j0] * pkc[j - j0];
+          }
+          bsum[4 * b] = c; bsum[4 * b + 1] = sk_; bsum[4 * b + 2] = sp; bsum[4 * b + 3] = sp2;
+        }
+        __syncthreads();
+      }
+      for (int b = threadIdx.x; b < 4 * a.nb; b += FIT_THREADS) part[b] = bsum[b];
+      continue;
+    }
     const bool blue = !TAB && nq > 0 && nq <= BLUE_NMAX;
-    double* pk = blue ? dyn : dyn + a.nmax;
+    double* pk = blue ? dyn : dyn + a.nlds;
     int* bins = reinterpret_cast<int*>(pk + njmax);
     if (blue) {
       if constexpr (!TAB) {
@@ -448,9 +573,45 @@ struct AdjArgs {
   const double2* tw;     // W_4096^k (FP64), for the Bluestein FFTs
   const double2* tables; // NULL, or the per-length Bluestein tables
   const int64_t* tab_off;
-  size_t line_off;       // generic kernel: byte offset of the Bluestein line (after x and cj)
+  size_t line_off;       // generic kernel: byte offset of the Bluestein line
+  size_t cjb_off;        // generic kernel: byte offset of cj for the Bluestein forests
   size_t e_off;          // generic kernel: byte offset of the edges
+  int nlds;              // longest forest of the LDS layout (min(nmax, FIT_NLDS))
 };
+
+// res[k] = the block sum of v[k], k < 7, in one pass: a shuffle tree per
+// wave, then the waves' partials in a fixed order (red: 7 NT / 64 doubles)
+template <int NT>
+__device__ __forceinline__ void reduce7(double (&v)[7], double* red, double* res) {
+  constexpr int NW = NT / 64;
+#pragma unroll
+  for (int k = 0; k < 7; ++k)
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) red[k * NW + wv] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < 7) {
+    double t = 0.0;
+    for (int i = 0; i < NW; ++i) t += red[threadIdx.x * NW + i];
+    res[threadIdx.x] = t;
+  }
+  __syncthreads();
+}
+
+// dF/d(a, b, c) of pixel i (forest offset) of skewer s at parameters p
+struct DF { double a, b, c; };
+__device__ __forceinline__ DF fgpa_df(const AdjArgs& a, int64_t at, double4 p) {
+  const double d = a.delta[at];
+  const double e = a.eta ? (double)a.eta[at] : 0.0;
+  const double gf = d + p.z * e;
+  const double T = exp(p.y * p.w * gf);
+  const double F = a.flux[at];
+  return DF{-T * F, -p.x * T * F * p.w * gf, -p.x * T * F * p.y * p.w * e};
+}
 
 // The FGPA part of the adjoint of one skewer, with A_q (q < nq) in A:
 // (sum A F', sum A_q(n) dF_n/d(a, b, c) / m, sum dF_n/d(a, b, c)) into res.
@@ -461,7 +622,6 @@ struct AdjArgs {
 template <int NT>
 __device__ __forceinline__ void adj_fgpa(const AdjArgs& a, int s, int set, int i0, int i1, int nq, double saf,
                                          const double* A, double* red, double* res) {
-  constexpr int NW = NT / 64;
   const int m = a.regroup;
   const int64_t o = a.sk.off[s];
   const double4 p = prow(a.abcg, a.rows, set);
@@ -485,22 +645,7 @@ __device__ __forceinline__ void adj_fgpa(const AdjArgs& a, int s, int set, int i
       v[3] += Aq * dfc;
     }
   }
-#pragma unroll
-  for (int k = 0; k < 7; ++k)
-    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) {
-#pragma unroll
-    for (int k = 0; k < 7; ++k) red[k * NW + wv] = v[k];
-  }
-  __syncthreads();
-  if (threadIdx.x < 7) {
-    double t = 0.0;
-    for (int i = 0; i < NW; ++i) t += red[threadIdx.x * NW + i];
-    res[threadIdx.x] = t;
-  }
-  __syncthreads();
+  reduce7<NT>(v, red, res);
 }
 
 // Adjoint of the binned P1D through delta_F = F/<F> - 1 (at fixed <F>) and
@@ -511,9 +656,12 @@ template <bool TAB>
 __global__ __launch_bounds__(FIT_THREADS) void k_p1d_adjoint(AdjArgs a) {
   extern __shared__ double dyn[];
   double* x = dyn;
-  double2* cj = reinterpret_cast<double2*>(dyn + ((a.nmax + 1) & ~1));
+  double2* const cjh = reinterpret_cast<double2*>(dyn + ((a.nlds + 1) & ~1));
   double2* bline = reinterpret_cast<double2*>(reinterpret_cast<char*>(dyn) + a.line_off);
   double* ledges = reinterpret_cast<double*>(reinterpret_cast<char*>(dyn) + a.e_off);
+  int* blo = reinterpret_cast<int*>(ledges + a.nb + 1);
+  int* bhi = blo + a.nb;
+  double* gco = reinterpret_cast<double*>(bhi + a.nb);
   __shared__ double red[7 * (FIT_THREADS / 64)];
   for (int i = threadIdx.x; i <= a.nb; i += FIT_THREADS) ledges[i] = a.edges[i];
   const smk_skewers& sk = a.sk;
@@ -531,17 +679,76 @@ __global__ __launch_bounds__(FIT_THREADS) void k_p1d_adjoint(AdjArgs a) {
     }
     const int set = sk.pset[s];
     const double fm = a.fmean[set];
-    load_delta_f(a.flux, o, i0, i1, m, 1.0 / fm, x);
-    __syncthreads();
     const double px = a.pixel * m;
     const double k_ny = M_PI / px;
     const int nj = nq > 0 ? nq / 2 + 1 : 0;
     const double* gps = a.gp + (int64_t)set * a.nb;
+    if (nq > a.nlds) {
+      // longer than the LDS layout (the modes saved by the forward are
+      // required): A_q by Horner's rule over j with c_j = g_b(j) conj(X_j)
+      // staged through LDS in pieces; each thread folds its own A_q into the
+      // FGPA sums of the pixels of q, then all pixels give the dF sums
+      const double2* xs = a.xj + a.xoff[s];
+      double2* cbuf = cjh;
+      for (int b = threadIdx.x; b < a.nb; b += FIT_THREADS) {
+        bin_span(ledges, a.nb, b, nj, nq, k_ny, blo[b], bhi[b]);
+        gco[b] = gps[b] * 2.0 * px / nq;
+      }
+      auto stage = [&](int n0, int n1) {
+        for (int j = n0 + (int)threadIdx.x; j < n1; j += FIT_THREADS) {
+          // the bin of mode j: the last b with blo[b] <= j, if j < bhi[b]
+          int lo = 0, hi = a.nb - 1;
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (blo[mid] <= j) lo = mid; else hi = mid - 1;
+          }
+          double2 c = make_double2(0.0, 0.0);
+          if (blo[lo] <= j && j < bhi[lo]) {
+            const double g = gco[lo];
+            const double2 X = xs[j];
+            c = make_double2(g * X.x, -g * X.y);   // g conj(X_j)
+          }
+          cbuf[j - n0] = c;
+        }
+      };
+      const double4 p = prow(a.abcg, a.rows, set);
+      const double inv = 1.0 / fm;
+      double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      for (int fb = 0; fb < nq; fb += 4 * FIT_THREADS) {
+        double re[4], im[4];
+        horner_staged(stage, cbuf, nj, nq, fb, nq, re, im);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int q = fb + (int)threadIdx.x + r * FIT_THREADS;
+          if (q >= nq) continue;
+          const double A = re[r];
+          double acc = 0.0;
+          for (int k = 0; k < m; ++k) acc += a.flux[o + i0 + (int64_t)q * m + k];
+          const double xq = (acc / m) * inv - 1.0;
+          v[0] += A * (xq + 1.0) * fm;
+          for (int k = 0; k < m; ++k) {
+            const DF df = fgpa_df(a, o + i0 + (int64_t)q * m + k, p);
+            v[1] += A / m * df.a;
+            v[2] += A / m * df.b;
+            v[3] += A / m * df.c;
+          }
+        }
+      }
+      for (int i = threadIdx.x; i < i1 - i0; i += FIT_THREADS) {
+        const DF df = fgpa_df(a, o + i0 + i, p);
+        v[4] += df.a; v[5] += df.b; v[6] += df.c;
+      }
+      reduce7<FIT_THREADS>(v, red, res);
+      continue;
+    }
+    load_delta_f(a.flux, o, i0, i1, m, 1.0 / fm, x);
+    __syncthreads();
     auto coef = [&](int j) -> double {
       const int b = find_bin(ledges, a.nb, kfreq(j, nq, k_ny));
       return b >= 0 ? gps[b] * 2.0 * px / nq : 0.0;
     };
     const bool blue = !TAB && nq > 0 && nq <= BLUE_NMAX;
+    double2* const cj = blue ? reinterpret_cast<double2*>(reinterpret_cast<char*>(dyn) + a.cjb_off) : cjh;
     if (a.xj) {
       const double2* xs = a.xj + a.xoff[s];
       for (int j = threadIdx.x; j < nj; j += FIT_THREADS) {
@@ -764,22 +971,24 @@ extern "C" int smk_p1d_binned(const smk_skewers* sk, const double* flux, const d
               SMK_ERR_ARG, "NULL argument");
   SMK_REQUIRE(sk->pset != nullptr, SMK_ERR_ARG, "the fit kernels need skewers.pset");
   SMK_REQUIRE(regroup >= 1 && nbins >= 1 && pixel > 0, SMK_ERR_ARG, "bad regroup/nbins/pixel");
-  SMK_REQUIRE(max_forest >= 0 && max_forest / regroup <= FIT_NMAX, SMK_ERR_RANGE,
-              "forest of %d points exceeds %d", max_forest / regroup, FIT_NMAX);
+  SMK_REQUIRE(max_forest >= 0, SMK_ERR_ARG, "max_forest < 0");
   SMK_REQUIRE((tables == nullptr) == (tab_off == nullptr), SMK_ERR_ARG, "tables and tab_off go together");
   if (sk->n <= 0) return SMK_OK;
   int nmax = max_forest / regroup;
   nmax += nmax & 1;
   if (nmax < 2) nmax = 2;
+  const int nlds = nmax < FIT_NLDS ? nmax : FIT_NLDS;
   SMK_REQUIRE(modes == nullptr || modes_off != nullptr, SMK_ERR_ARG, "modes need modes_off");
   hipError_t te = hipSuccess;
   const double2* tw = fp64_twiddles(&te);
   SMK_REQUIRE(tw != nullptr, SMK_ERR_HIP, "FP64 twiddle table: %s", hipGetErrorString(te));
+  const size_t a_bytes = p1d_a_bytes(nlds);
+  const int64_t chunk = ((int64_t)a_bytes - 16 * FIT_XC) / 8;   // modes per pass after the staged input
   P1dArgs a{*sk,      flux,      fmean, pixel, regroup, edges, nbins, partial, status, nmax, (double2*)modes,
-            modes_off, tw, (const double2*)tables, tab_off, p1d_a_bytes(nmax)};
+            modes_off, tw, (const double2*)tables, tab_off, a_bytes, nlds, (int)(chunk > 1 ? chunk : 1)};
   const dim3 grid(grid_for(sk->n));
   hipStream_t st = (hipStream_t)stream;
-  const size_t lds = p1d_lds_bytes(nmax, nbins);
+  const size_t lds = p1d_lds_bytes(nlds, nbins);
   if (tables) {
     // the forests without a table (Horner), then one launch per Bluestein
     // size class; each skips the skewers of the others
@@ -811,25 +1020,30 @@ extern "C" int smk_p1d_adjoint(const smk_skewers* sk, const float* delta, const 
               SMK_ERR_ARG, "NULL argument");
   SMK_REQUIRE(sk->pset != nullptr, SMK_ERR_ARG, "the fit kernels need skewers.pset");
   SMK_REQUIRE(regroup >= 1 && nbins >= 1 && pixel > 0, SMK_ERR_ARG, "bad regroup/nbins/pixel");
-  SMK_REQUIRE(max_forest >= 0 && max_forest / regroup <= FIT_NMAX, SMK_ERR_RANGE,
-              "forest of %d points exceeds %d", max_forest / regroup, FIT_NMAX);
+  SMK_REQUIRE(max_forest >= 0, SMK_ERR_ARG, "max_forest < 0");
   SMK_REQUIRE((tables == nullptr) == (tab_off == nullptr), SMK_ERR_ARG, "tables and tab_off go together");
   if (sk->n <= 0) return SMK_OK;
   int nmax = max_forest / regroup;
   nmax += nmax & 1;
   if (nmax < 2) nmax = 2;
+  const int nlds = nmax < FIT_NLDS ? nmax : FIT_NLDS;
   SMK_REQUIRE(modes == nullptr || modes_off != nullptr, SMK_ERR_ARG, "modes need modes_off");
+  SMK_REQUIRE(modes != nullptr || nmax <= FIT_NLDS, SMK_ERR_ARG,
+              "forests longer than %d points need the modes saved by smk_p1d_binned", FIT_NLDS);
   hipError_t te = hipSuccess;
   const double2* tw = fp64_twiddles(&te);
   SMK_REQUIRE(tw != nullptr, SMK_ERR_HIP, "FP64 twiddle table: %s", hipGetErrorString(te));
   const bool tab = tables != nullptr;
-  const size_t line_off = adj_x_bytes(nmax) + adj_cj_bytes(nmax);
-  const size_t e_off = adj_e_off(nmax, tab);
-  AdjArgs a{*sk,   delta, eta,   abcg, abcg_rows, flux,      fmean, pixel, regroup, edges, nbins, gp, out, nmax,
-            (const double2*)modes, modes_off, tw, (const double2*)tables, tab_off, line_off, e_off};
+  const size_t e_off = adj_e_off(nlds, tab);
+  const size_t cjb_off = adj_x_bytes(nlds < BLUE_NMAX ? nlds : BLUE_NMAX);
+  AdjArgs a{*sk,      delta, eta,          abcg,  abcg_rows, flux, fmean, pixel, regroup,
+            edges,    nbins, gp,           out,   nmax,      (const double2*)modes,
+            modes_off, tw,   (const double2*)tables, tab_off, adj_line_off(nlds), cjb_off, e_off, nlds};
   const dim3 grid(grid_for(sk->n));
   hipStream_t st = (hipStream_t)stream;
-  const size_t lds = e_off + (size_t)(nbins + 1) * 8;
+  // + the edges, and for forests longer than the layout the bin spans and
+  // coefficients (2 nb ints, nb doubles)
+  const size_t lds = e_off + (size_t)(nbins + 1) * 8 + (size_t)nbins * 16;
   if (tab) {
     SMK_HIP_CHECK(hipFuncSetAttribute((const void*)k_p1d_adjoint<true>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

@@ -30,6 +30,10 @@ class P1DFitModel:
     """Device state of one fit: skewer batch with a parameter set per skewer,
     the fixed fields delta = delta_l + delta_s and eta_par, G per set, k bins."""
 
+    # longest regrouped forest whose adjoint can recompute its modes in LDS
+    # (the LDS layout of smk_p1d_adjoint); longer ones need the saved modes
+    LONG_FOREST = 8192
+
     def __init__(self, batch, delta, eta, growth, edges, pixel=0.2, regroup=1, save_modes=True, use_tables=True):
         if batch.pset is None:
             raise ValueError("the batch needs a parameter set (pset) per skewer")
@@ -61,10 +65,13 @@ class P1DFitModel:
         self.fmean_dev = torch.empty(self.n_sets, dtype=torch.float64, device=dev)
         self.status = torch.zeros(n, dtype=torch.int32, device=dev)
         # rfft modes saved by the forward for the adjoint: N/2 + 1 per skewer
+        # (always for forests longer than LONG_FOREST regrouped points, whose
+        # adjoint reads them)
         nj = np.where(nf // self.regroup > 0, nf // self.regroup // 2 + 1, 0).astype(np.int64)
         moff = np.zeros(n, dtype=np.int64)
         if n:
             moff[1:] = np.cumsum(nj[:-1])
+        save_modes = save_modes or self.max_forest // self.regroup > self.LONG_FOREST
         self.modes = torch.empty(2 * int(nj.sum()) + 2, dtype=torch.float64, device=dev) if save_modes else None
         self.modes_off = torch.from_numpy(moff).to(dev) if save_modes else None
         self.tables, self.tab_off = self._bluestein_tables(nf // self.regroup) if use_tables else (None, None)

@@ -80,6 +80,40 @@ def test_forward_parity_edge_lengths(cuda, use_tables):
     assert np.allclose(g, want, rtol=1e-7, atol=1e-9 * np.abs(want).max())
 
 
+# forests about the LDS layout's bound of 8192 regrouped points: longer ones
+# take the direct DFT with the input staged through LDS (and their adjoint
+# reads the saved modes, which the model then always keeps); at m = 2 the
+# 16385-pixel forest regroups to 8192 points and stays in the layout
+LONG_LENGTHS = [8191, 8192, 8193, 9000, 12001, 2048, 1000, 16385]
+
+
+@pytest.mark.parametrize("use_tables", [True, False])
+@pytest.mark.parametrize("save_modes", [True, False])
+@pytest.mark.parametrize("m", [1, 2])
+def test_long_forests(cuda, m, save_modes, use_tables):
+    lens = LONG_LENGTHS
+    model, args, theta = _problem(cuda, ns=2, per=len(lens) // 2, lens=lens, m=m, save_modes=save_modes,
+                                  use_tables=use_tables)
+    if m == 1:
+        assert model.modes is not None
+    _check_forward(model, args, theta * np.array([1.1, 0.95, 1.05] * 2))
+    th = theta * np.array([1.2, 0.9, 1.1] * 2)
+    val, g = model.value_and_grad(th)
+    want = fto.adjoint_grad(th, *args)
+    assert np.allclose(g, want, rtol=1e-7, atol=1e-9 * np.abs(want).max())
+    assert abs(val / fto.chi2(th, *args) - 1) < 1e-10
+
+
+def test_long_forest_adjoint_needs_modes(cuda):
+    """A forest past the LDS layout has no in-LDS recomputation of its modes:
+    the adjoint without saved modes is refused, not run."""
+    model, args, theta = _problem(cuda, ns=1, per=2, lens=[9000, 500])
+    model.forward(theta)
+    model.modes = None
+    with pytest.raises(ValueError, match="need the modes saved"):
+        model.grad_last()
+
+
 @pytest.mark.parametrize("use_tables", [True, False])
 @pytest.mark.parametrize("save_modes", [True, False])
 def test_adjoint_gradient_parity(cuda, save_modes, use_tables):
