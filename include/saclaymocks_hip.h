@@ -186,6 +186,11 @@ size_t smk_fft_and_store_workspace_bytes(const smk_box_plan* plan);
 #define SMK_GEOM_RADIAL 1 /* line of sight from the observer, make_spectra.py:443-452 */
 #define SMK_INTERP_TRILINEAR 0
 #define SMK_INTERP_GAUSS343 1 /* 343-cell Gaussian kernel, make_spectra.py:39-68 */
+/* Gaussian kernel over (2d + 1)^3 cells, make_spectra -dmax d
+ * (make_spectra.py:147,385-390), 0 <= d <= SMK_GAUSS_DMAX;
+ * SMK_INTERP_GAUSS(3) is the same kernel as SMK_INTERP_GAUSS343. */
+#define SMK_GAUSS_DMAX 6
+#define SMK_INTERP_GAUSS(d) (16 + (d))
 
 /* Resident density / tidal boxes, float32 [nx][ny][nz].  Box frame of
  * make_spectra.py:196-206,300-305: X in [-LX/2, LX/2), Y likewise, Z in

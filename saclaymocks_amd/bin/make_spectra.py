@@ -11,12 +11,12 @@ As the reference, slice i of N reads the box planes [iXmin, iXmax) (the
 slice plus dmax planes each side, :220-226) from box-<ix>.fits (and the six
 eta_ij-<ix>.fits with -rsd); the planes stay resident on the device and all
 QSOs of the slice are gathered in one launch (smk_forest_gather,
-343-cell Gaussian weights of ComputeWeight/computeRho, :39-139).  A QSO whose
+(2 dmax + 1)^3-cell Gaussian weights of ComputeWeight/computeRho, :39-139).  A QSO whose
 kernel leaves the box is skipped with the reference's warning (:490-494).
 With -rsd and -dla the velocity boxes vx, vy, vz-<j>.fits are read too and a
 VELO_PAR extension follows ETA_PAR: the smoothed line-of-sight velocity
 (:130-134) times (1 + z) h_of_z(z) / h_of_z(0) dD/dz(z) / dD/dz(0) per pixel
-(:509-511).  Not supported: -dmax other than 3 (the kernel's 7^3 stencil).
+(:509-511).  -dmax d takes d in [0, 6] ((2d + 1)^3-cell stencil).
 """
 import argparse
 import os
@@ -81,8 +81,8 @@ def run(args):
     rsd = util.str2bool(args.rsd)
     velo = rsd and util.str2bool(args.dla)      # the reference keeps v_par only with rsd (:505-511)
     dgrowth = dgrowth_table(args.dgrowthfile) if util.str2bool(args.dla) else None
-    if args.dmax != 3:
-        raise NotImplementedError("the gather kernel uses the 7^3-cell stencil of -dmax 3")
+    if not 0 <= args.dmax <= forest.GAUSS_DMAX:
+        raise ValueError(f"-dmax {args.dmax}: the gather kernel takes 0 <= dmax <= {forest.GAUSS_DMAX}")
     h0 = boxfiles.read_header(os.path.join(args.boxdir, "box"))
     cell, nz, ny, nhdu = h0["DX"], h0["NAXIS1"], h0["NAXIS2"], int(h0["NX"])
     if h0.get("NAXIS3", 1) != 1:
@@ -134,7 +134,7 @@ def run(args):
     batch = forest.SkewerBatch(forest.GEOM_RADIAL, rows["g0"], rows["npix"], grid["rmin"], grid["dr"], rows["qx"],
                                rows["qy"], rows["qz"], rows["qr"], imin=rows["imin"], imax=rows["imax"])
     want = ("delta_l", "eta_par", "v_par") if velo else (("delta_l", "eta_par") if rsd else ("delta_l",))
-    out = forest.gather(batch, box, interp="gauss343", want=want)
+    out = forest.gather(batch, box, interp="gauss343", want=want, dmax=dmax)
     status = out["status"].cpu().numpy()
     dl = batch.split(out["delta_l"])
     eta = batch.split(out["eta_par"]) if rsd else None

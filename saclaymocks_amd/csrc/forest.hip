@@ -6,8 +6,9 @@
 //   k_gather          one thread per pixel: the block's skewers by one
 //                     64-wide search over the packed offsets, position and
 //                     containing cell in the reference's float64 evaluation
-//                     order (bit-exact cell indices), trilinear or 343-cell
-//                     Gaussian smoothing of delta and eta, then the FGPA
+//                     order (bit-exact cell indices), trilinear or Gaussian
+//                     smoothing over (2 dmax + 1)^3 cells (343 for the
+//                     default dmax = 3) of delta and eta, then the FGPA
 //   k_gather_plane_tri  the same for plane-parallel skewers and trilinear
 //                     interpolation, with the eight corner loads of both
 //                     fields issued together
@@ -111,8 +112,9 @@ __device__ __forceinline__ void block_skewers(const smk_skewers& sk, int64_t pix
 }
 
 // NF fields: 1 (delta), 2 (delta, eta_zz), 7 (delta and the six eta_ij) or
-// 10 (those and the three velocities)
-template <int NF>
+// 10 (those and the three velocities); DM: the Gaussian stencil's radius in
+// cells (make_spectra -dmax), unused by trilinear interpolation
+template <int NF, int DM>
 __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
   const int64_t pix0 = (int64_t)blockIdx.x * blockDim.x;
   const int64_t pix = pix0 + threadIdx.x;
@@ -172,19 +174,21 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
     double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     const float* fld[10] = {b.delta, radial ? b.eta[0] : b.eta[2], b.eta[1], b.eta[2], b.eta[3], b.eta[4],
                             b.eta[5], b.vel[0], b.vel[1], b.vel[2]};
-    if (a.interp == SMK_INTERP_GAUSS343) {
+    if (a.interp != SMK_INTERP_TRILINEAR) {
       // cell centres (i+0.5)D - L/2 (+R0 along Z), weights exp(-dr^2/(2 D^2)),
-      // taken as the product of the three axes' factors (21 exponentials, not 343)
+      // taken as the product of the three axes' factors (3 W exponentials,
+      // not W^3)
+      constexpr int W = 2 * DM + 1;
       const double cx = sub_rn(mul_rn((double)ix + 0.5, D), a.hx);
       const double cy = sub_rn(mul_rn((double)iy + 0.5, D), a.hy);
       const double cz = add_rn(sub_rn(mul_rn((double)iz + 0.5, D), a.hz), b.r0);
       const double sig2 = 2.0 * D * D;
-      double wy[7], wz[7];
-      int64_t jyv[7], jzv[7];
-      for (int d = 0; d < 7; ++d) {
-        const double ey = sub_rn(add_rn((d - 3) * D, cy), Y);
-        const double ez = sub_rn(add_rn((d - 3) * D, cz), Z);
-        int64_t jy = iy + d - 3, jz = iz + d - 3;
+      double wy[W], wz[W];
+      int64_t jyv[W], jzv[W];
+      for (int d = 0; d < W; ++d) {
+        const double ey = sub_rn(add_rn((d - DM) * D, cy), Y);
+        const double ez = sub_rn(add_rn((d - DM) * D, cz), Z);
+        int64_t jy = iy + d - DM, jz = iz + d - DM;
         if (b.periodic) {
           jy = wrap(jy, ny);
           jz = wrap(jz, nz);
@@ -197,7 +201,7 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
         jzv[d] = jz;
       }
       double wsum = 0.0;
-      for (int da = -3; da <= 3; ++da) {
+      for (int da = -DM; da <= DM; ++da) {
         const double ex = sub_rn(add_rn(da * D, cx), X);
         int64_t jx = ix + da;
         if (b.periodic) jx = wrap(jx, nx); else if (jx < 0 || jx >= nx) outside = true;
@@ -205,11 +209,11 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
         if (jx >= 0 && jx < nx && (jx < x0 || jx >= x0 + nxs)) off_slab = true;
         if (outside || off_slab) continue;
         const double wx = exp(-mul_rn(ex, ex) / sig2);
-        for (int db = 0; db < 7; ++db) {
+        for (int db = 0; db < W; ++db) {
           const double wxy = wx * wy[db];
           const int64_t row = ((jx - x0) * ny + jyv[db]) * nz;
 #pragma unroll
-          for (int dc = 0; dc < 7; ++dc) {
+          for (int dc = 0; dc < W; ++dc) {
             const double w = wxy * wz[dc];
             wsum += w;
             const int64_t flat = row + jzv[dc];
@@ -988,6 +992,26 @@ extern "C" int smk_forest_plan_destroy(smk_forest_plan* p) {
   return SMK_OK;
 }
 
+// stencil radius of a Gaussian interp code, -1 if it is not one
+static int gauss_radius(int interp) {
+  if (interp == SMK_INTERP_GAUSS343) return 3;
+  const int r = interp - SMK_INTERP_GAUSS(0);
+  return r >= 0 && r <= SMK_GAUSS_DMAX ? r : -1;
+}
+
+template <int NF>
+static void launch_gather(int r, dim3 grid, hipStream_t st, const GatherArgs& a) {
+  switch (r) {
+    case 0: hipLaunchKernelGGL((k_gather<NF, 0>), grid, dim3(256), 0, st, a); break;
+    case 1: hipLaunchKernelGGL((k_gather<NF, 1>), grid, dim3(256), 0, st, a); break;
+    case 2: hipLaunchKernelGGL((k_gather<NF, 2>), grid, dim3(256), 0, st, a); break;
+    case 4: hipLaunchKernelGGL((k_gather<NF, 4>), grid, dim3(256), 0, st, a); break;
+    case 5: hipLaunchKernelGGL((k_gather<NF, 5>), grid, dim3(256), 0, st, a); break;
+    case 6: hipLaunchKernelGGL((k_gather<NF, 6>), grid, dim3(256), 0, st, a); break;
+    default: hipLaunchKernelGGL((k_gather<NF, 3>), grid, dim3(256), 0, st, a); break;
+  }
+}
+
 extern "C" size_t smk_forest_workspace_bytes(int n_skewers) {
   return (size_t)(N_CLASS * (size_t)(n_skewers > 0 ? n_skewers : 0) + 16) * sizeof(int32_t);
 }
@@ -998,7 +1022,7 @@ extern "C" int smk_forest_gather(const smk_skewers* sk, const smk_box_view* box,
   SMK_REQUIRE(sk != nullptr && box != nullptr, SMK_ERR_ARG, "NULL skewers/box");
   SMK_REQUIRE(box->delta != nullptr, SMK_ERR_ARG, "box delta is NULL");
   SMK_REQUIRE(box->nx > 0 && box->ny > 0 && box->nz > 0 && box->cell > 0, SMK_ERR_ARG, "bad box geometry");
-  SMK_REQUIRE(interp == SMK_INTERP_TRILINEAR || interp == SMK_INTERP_GAUSS343, SMK_ERR_ARG, "bad interp %d", interp);
+  SMK_REQUIRE(interp == SMK_INTERP_TRILINEAR || gauss_radius(interp) >= 0, SMK_ERR_ARG, "bad interp %d", interp);
   SMK_REQUIRE(box->nxs >= 0 && box->x0 >= 0 && (box->nxs == 0 || box->x0 + box->nxs <= box->nx), SMK_ERR_ARG,
               "bad resident x slab [%d, %d + %d) of a %d-plane box", box->x0, box->x0, box->nxs, box->nx);
   SMK_REQUIRE(sk->geometry == SMK_GEOM_PLANE || sk->geometry == SMK_GEOM_RADIAL, SMK_ERR_ARG, "bad geometry");
@@ -1036,10 +1060,12 @@ extern "C" int smk_forest_gather(const smk_skewers* sk, const smk_box_view* box,
     const uint64_t nblk = (uint64_t)(a.npix_total + 255) / 256;
     const bool radial = sk->geometry == SMK_GEOM_RADIAL;
     const int nf = box->eta[2] == nullptr ? 1 : (radial ? (v_par ? 10 : 7) : 2);
-    if (nf == 1) hipLaunchKernelGGL(k_gather<1>, dim3((uint32_t)nblk), dim3(256), 0, (hipStream_t)stream, a);
-    else if (nf == 2) hipLaunchKernelGGL(k_gather<2>, dim3((uint32_t)nblk), dim3(256), 0, (hipStream_t)stream, a);
-    else if (nf == 10) hipLaunchKernelGGL(k_gather<10>, dim3((uint32_t)nblk), dim3(256), 0, (hipStream_t)stream, a);
-    else hipLaunchKernelGGL(k_gather<7>, dim3((uint32_t)nblk), dim3(256), 0, (hipStream_t)stream, a);
+    const int r = interp == SMK_INTERP_TRILINEAR ? 3 : gauss_radius(interp);
+    const dim3 grid((uint32_t)nblk);
+    if (nf == 1) launch_gather<1>(r, grid, (hipStream_t)stream, a);
+    else if (nf == 2) launch_gather<2>(r, grid, (hipStream_t)stream, a);
+    else if (nf == 10) launch_gather<10>(r, grid, (hipStream_t)stream, a);
+    else launch_gather<7>(r, grid, (hipStream_t)stream, a);
   }
   SMK_LAUNCH_CHECK();
   return SMK_OK;

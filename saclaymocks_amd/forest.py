@@ -19,6 +19,20 @@ from . import util
 GEOM_PLANE = 0
 GEOM_RADIAL = 1
 INTERP = {"trilinear": 0, "gauss343": 1}
+GAUSS_DMAX = 6      # largest -dmax stencil radius of the gather kernel
+
+
+def interp_code(interp, dmax=3):
+    """smk_forest_gather's interp argument: "trilinear", or "gauss343" with the
+    stencil radius dmax (make_spectra -dmax, (2 dmax + 1)^3 cells)."""
+    if interp == "trilinear":
+        return INTERP["trilinear"]
+    if interp != "gauss343":
+        raise ValueError(f"unknown interpolation {interp!r}")
+    dmax = int(dmax)
+    if not 0 <= dmax <= GAUSS_DMAX:
+        raise ValueError(f"-dmax {dmax} outside [0, {GAUSS_DMAX}]")
+    return INTERP["gauss343"] if dmax == 3 else 16 + dmax
 N_CLASS = 6
 
 
@@ -251,9 +265,10 @@ def missing_power(plan, batch, filt, filt_off, seed=0, noise=None, noise_off=Non
 
 
 def gather(batch, box, interp="gauss343", abcg=None, delta_s=None, want=("delta_l", "eta_par", "flux"),
-           cells=False, stream=None):
+           cells=False, stream=None, dmax=3):
     """ReadSpec (+ FGPA when abcg is given).  Returns a dict of packed per-pixel
-    tensors and the per-skewer status (1 = left a non-periodic box)."""
+    tensors and the per-skewer status (1 = left a non-periodic box).  dmax:
+    the Gaussian stencil's radius in cells (make_spectra -dmax)."""
     dev = _dev()
     n = batch.npix_total
     out = {}
@@ -263,18 +278,18 @@ def gather(batch, box, interp="gauss343", abcg=None, delta_s=None, want=("delta_
     if cells:
         out["cells"] = torch.empty((n, 3), dtype=torch.int32, device=dev)
     out["status"] = torch.zeros(batch.n, dtype=torch.int32, device=dev)
-    gather_into(batch, box, out, interp, abcg, delta_s, stream)
+    gather_into(batch, box, out, interp, abcg, delta_s, stream, dmax=dmax)
     return out
 
 
-def gather_into(batch, box, out, interp="gauss343", abcg=None, delta_s=None, stream=None):
+def gather_into(batch, box, out, interp="gauss343", abcg=None, delta_s=None, stream=None, dmax=3):
     """gather() into caller-owned buffers: out maps "delta_l", "eta_par", "v_par",
     "flux", "cells" (any subset) and "status" (int32 per skewer, zeroed by the
     caller) to CUDA tensors.  Allocates nothing."""
     sk = batch.struct()
     bv = box.struct()
     rows = int(abcg.shape[0]) if abcg is not None else 0
-    rc = _lib.lib().smk_forest_gather(ctypes.byref(sk), ctypes.byref(bv), INTERP[interp], _lib.ptr(abcg),
+    rc = _lib.lib().smk_forest_gather(ctypes.byref(sk), ctypes.byref(bv), interp_code(interp, dmax), _lib.ptr(abcg),
                                       rows, _lib.ptr(delta_s), _lib.ptr(out.get("delta_l")),
                                       _lib.ptr(out.get("eta_par")), _lib.ptr(out.get("v_par")),
                                       _lib.ptr(out.get("flux")),

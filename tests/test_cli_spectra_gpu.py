@@ -219,3 +219,52 @@ def test_velocity_skewers_match_oracle(cuda, tmp_path):
             assert np.any(got != 0) == bool(np.any(forest))     # one QSO has no forest pixel
             seen += 1
     assert seen == sum(1 for r in rows if r[5] == 0)
+
+
+@pytest.mark.parametrize("dmax", [1])
+def test_make_spectra_dmax(cuda, tmp_path, dmax):
+    """make_spectra -dmax d: slabs padded by d planes and the (2d + 1)^3-cell
+    Gaussian stencil; the merged DELTA_L against the oracle's stencil of
+    that radius."""
+    from saclaymocks_amd import fitsio_lite
+    from saclaymocks_amd.bin import make_boxes, make_spectra, merge_spectra
+    boxdir, qdir = str(tmp_path / "boxes"), str(tmp_path / "qso")
+    sdir, mdir = str(tmp_path / "spectra"), str(tmp_path / "merged")
+    print(f"[dmax {dmax} make_boxes]", flush=True)
+    assert make_boxes.main(["-NX", str(NX), "-NY", str(NY), "-NZ", str(NZ), "-seed", "5", "-outDir", boxdir]) == 0
+    cos = fo.Cosmo()
+    R0 = fo.H * cos.r_comoving(fo.Z0)
+    rows = _qsos(R0)
+    _write_qsos(qdir, rows)
+    print(f"[dmax {dmax} make_spectra]", flush=True)
+    for i in range(NSLICE):
+        assert make_spectra.main(["-QSOfile", os.path.join(qdir, "QSO-"), "-boxdir", boxdir, "-outDir", sdir,
+                                  "-i", str(i), "-N", str(NSLICE), "-zmin", str(ZMIN), "-zmax", str(ZMAX),
+                                  "-dmax", str(dmax)]) == 0
+    with pytest.raises(ValueError, match="dmax"):
+        make_spectra.main(["-QSOfile", os.path.join(qdir, "QSO-"), "-boxdir", boxdir, "-outDir", sdir,
+                           "-i", "0", "-N", str(NSLICE), "-dmax", "7"])
+    print(f"[dmax {dmax} merge_spectra]", flush=True)
+    assert merge_spectra.main(["-inDir", sdir, "-outDir", mdir, "-i", "0", "-seed", str(SEED),
+                               "--store-g", "True"]) == 0
+    delta = _read_box(boxdir, "box")
+    L, D = (CELL * NX, CELL * NY, CELL * NZ), (CELL,) * 3
+    Rmin, Rmax = fo.H * cos.r_comoving(ZMIN), fo.H * cos.r_comoving(ZMAX)
+    R_all = Rmin + np.arange(int((Rmax - Rmin) / PIXEL + 0.5)) * PIXEL
+    lam_all = fo.LYA * (1 + cos.r_2_z(R_all / fo.H))
+    keep = lam_all > 3476.0
+    R_vec = R_all[keep]
+    seen = 0
+    for path in sorted(glob.glob(os.path.join(mdir, "spectra_merged-*-0.fits.gz"))):
+        hd = {h.name: h for h in fitsio_lite.read_all(path)}
+        meta = hd["METADATA"].data
+        for q, tid in enumerate(meta["THING_ID"]):
+            r = next(r for r in rows if r[4] == tid)
+            R_Q = fo.H * cos.r_comoving(r[2])
+            X, Y, Z = fo.compute_xyz2(np.radians(r[0]), np.radians(r[1]), R_Q, 0.0, 0.0)
+            (v,), _ = fo.gauss343([delta], R_vec * X / R_Q, R_vec * Y / R_Q, R_vec * Z / R_Q, L, D, R0, dmax=dmax)
+            forest = lam_all[keep] < fo.LYA * (1 + r[2])
+            want = np.where(forest, np.float32(v), np.float32(-1e6))
+            assert np.abs(hd["DELTA_L"].data[q] - want).max() < 2e-6, tid
+            seen += 1
+    assert seen == sum(1 for r in rows if r[5] == 0)

@@ -111,15 +111,19 @@ def test_plane_gather_many_short_and_empty_skewers(cuda, interp):
         assert torch.equal(out[k], out2[k]), k
 
 
-def test_radial_gather_parity_reference_geometry(cuda):
+@pytest.mark.parametrize("dmax", [3, 0, 1, 2, 6])
+def test_radial_gather_parity_reference_geometry(cuda, dmax):
+    """make_spectra's gather (-dmax 3 by default: 343 cells; any stencil
+    radius up to 6) against the oracle on radial lines of sight."""
     from saclaymocks_amd import forest, util
-    print("[forest radial gather gauss343]", flush=True)
+    print(f"[forest radial gather, dmax {dmax}]", flush=True)
     shape, cell = (64, 64, 64), 20.0
     cos = util.Cosmo()
     R0 = util.H * cos.r_comoving(util.Z0)
     L = [cell * s for s in shape]
-    zmax = float(cos.r_2_z((R0 + L[2] / 2 - 4 * cell) / util.H))
-    zmin = float(cos.r_2_z((R0 - L[2] / 2 + 4 * cell) / util.H))
+    edge = max(4, dmax + 1) * cell
+    zmax = float(cos.r_2_z((R0 + L[2] / 2 - edge) / util.H))
+    zmin = float(cos.r_2_z((R0 - L[2] / 2 + edge) / util.H))
     rng = np.random.default_rng(7)
     nq = 16
     ang = np.degrees(0.25 * (L[0] / 2) / R0)
@@ -132,7 +136,7 @@ def test_radial_gather_parity_reference_geometry(cuda):
     box = forest.BoxView(tens[0], tens[1:], cell=cell, r0=R0, periodic=False)
     b = forest.SkewerBatch(forest.GEOM_RADIAL, rows["g0"], rows["npix"], grid["rmin"], grid["dr"], rows["qx"],
                            rows["qy"], rows["qz"], rows["qr"], imin=rows["imin"], imax=rows["imax"])
-    out = forest.gather(b, box, interp="gauss343", want=("delta_l", "eta_par"), cells=True)
+    out = forest.gather(b, box, interp="gauss343", want=("delta_l", "eta_par"), cells=True, dmax=dmax)
     # oracle positions: R = rmin + g dr, X = R * X_Q / R_Q (make_spectra.py:312,450-452)
     X, Y, Z, msk = [], [], [], []
     for s in range(b.n):
@@ -148,7 +152,7 @@ def test_radial_gather_parity_reference_geometry(cuda):
     ix, iy, iz = fo.cell_index(X, Y, Z, L, D, R0)
     cells = out["cells"].cpu().numpy()
     assert np.array_equal(cells[:, 0], ix) and np.array_equal(cells[:, 1], iy) and np.array_equal(cells[:, 2], iz)
-    vals, _ = fo.gauss343(fields, X[msk], Y[msk], Z[msk], L, D, R0)
+    vals, _ = fo.gauss343(fields, X[msk], Y[msk], Z[msk], L, D, R0, dmax=dmax)
     eta = fo.eta_par_radial(vals[1:], X[msk], Y[msk], Z[msk])
     gd, ge = out["delta_l"].cpu().numpy(), out["eta_par"].cpu().numpy()
     assert np.abs(gd[msk] - vals[0]).max() < 2e-6

@@ -158,6 +158,30 @@ def test_gauss343_constant_and_bruteforce():
         assert abs(v[p] - num / den) < 1e-12
 
 
+@pytest.mark.parametrize("dmax", [0, 1, 2, 4])
+def test_gauss_stencil_radius_against_every_cell_of_the_box(dmax):
+    """make_spectra -dmax d: the weights over the cells within Chebyshev
+    distance d of the containing cell, here found by scanning every cell of
+    the box; d = 0 returns the containing cell's value."""
+    shape = (14, 12, 16)
+    D, R0 = 2.0, 100.0
+    L = tuple(D * n for n in shape)
+    f = _box(shape)
+    X = np.array([0.3, -1.1, 2.9]); Y = np.array([0.2, 1.7, -3.3]); Z = np.array([R0 + 0.4, R0 - 2.2, R0 + 3.1])
+    (v,), (ix, iy, iz) = fo.gauss343([f], X, Y, Z, L, (D, D, D), R0, dmax=dmax)
+    if dmax == 0:
+        assert np.array_equal(v, f[ix, iy, iz].astype(np.float64))
+    I = np.indices(shape).reshape(3, -1)
+    cc = (I + 0.5) * D - np.array(L)[:, None] / 2 + np.array([0.0, 0.0, R0])[:, None]
+    for p in range(len(X)):
+        near = np.max(np.abs(I - np.array([ix[p], iy[p], iz[p]])[:, None]), axis=0) <= dmax
+        assert near.sum() == (2 * dmax + 1) ** 3
+        r2 = np.sum((cc[:, near] - np.array([X[p], Y[p], Z[p]])[:, None]) ** 2, axis=0)
+        w = np.exp(-r2 / (2 * D * D))
+        want = np.sum(w * f.reshape(-1)[near]) / np.sum(w)
+        assert abs(v[p] - want) < 1e-12
+
+
 def test_v_par_of_constant_velocity_fields():
     """Constant velocity boxes smooth to themselves, so v_par is v . x_hat."""
     shape = (12, 10, 14)
