@@ -412,7 +412,8 @@ def cpu_baseline_p1d(rows=100):
 
 def fit_bench(delta, eta, shape, cell, per_bin, maxfev):
     """15-parameter FGPA fit (5 z bins of 4000 skewers), plain derivative-free fit
-    vs adjoint-gradient fit on the same device forward model."""
+    vs the gradient fits on the same device forward model: L-BFGS-B on adjoint
+    gradients, and Levenberg-Marquardt on the forward-mode Jacobian."""
     from saclaymocks_amd import fit, forest, workloads
     box = forest.BoxView.plane(delta, eta, cell=cell, r0=workloads.box_frame(shape, cell)["R0"], periodic=True)
     edges = workloads.p1d_edges()
@@ -448,13 +449,23 @@ def fit_bench(delta, eta, shape, cell, per_bin, maxfev):
         for _ in range(10):
             mdl.value_and_grad(theta0)
         t2 = time.perf_counter()
-        eval_ms[name] = {"forward": (t1 - t0) * 100.0, "forward_and_adjoint": (t2 - t1) * 100.0}
+        mdl.jacobian(theta0)
+        t3 = time.perf_counter()
+        for _ in range(10):
+            mdl.jacobian(theta0)
+        t4 = time.perf_counter()
+        eval_ms[name] = {"forward": (t1 - t0) * 100.0, "forward_and_adjoint": (t2 - t1) * 100.0,
+                         "forward_and_jacobian": (t4 - t3) * 100.0}
     del nt
     plains = {m: fit.fit_plain(model, theta0, bounds, maxfev=maxfev, method=m) for m in ("powell", "fd-lbfgs")}
     # the gradient fit has to reach the better of the plain fits' chi2
     plain = min(plains.values(), key=lambda r: r["chi2"])
     target = plain["chi2"] + 1e-6 * max(1.0, plain["chi2"])
     grad = fit.fit_gradient(model, theta0, bounds, chi2_target=target)
+    gn = fit.fit_gauss_newton(model, theta0, bounds, chi2_target=target)
+    # the forward model's chi2 at the Gauss-Newton point (its own chi2 comes
+    # from the Jacobian kernels, equal to FP64 rounding)
+    gn_chi2_forward = model.chi2(gn["x"])
     return {"workload": f"{b.n} skewers in {len(zb)} z bins (zfix per bin), 15 params a,b,c(z); "
                         "targets <F>=exp(-0.0025(1+z)^3.7) and the P1D of an independent realisation "
                         "with b, c from etc/params.fits and a solved for the <F> target",
@@ -466,10 +477,16 @@ def fit_bench(delta, eta, shape, cell, per_bin, maxfev):
                               "fd-lbfgs": "L-BFGS-B on 2-point finite-difference gradients (migrad-like)"},
             "gradient": {"method": "L-BFGS-B on adjoint gradients", "chi2": grad["chi2"],
                          "wall_s": grad["wall_s"], "n_eval": grad["n_eval"], "status": grad["message"]},
+            "gauss_newton": {"method": "Levenberg-Marquardt per z bin on the forward-mode Jacobian kernels",
+                             "chi2": gn["chi2"], "chi2_forward_model": gn_chi2_forward, "wall_s": gn["wall_s"],
+                             "n_jacobian": gn["n_jac"], "status": gn["message"],
+                             "x": np.asarray(gn["x"]).reshape(-1, 3).tolist()},
             "eval_ms": eval_ms,
             "target_chi2": target,
             "reached_plain_chi2": bool(grad["chi2"] <= target),
-            "speedup": {m: r["wall_s"] / grad["wall_s"] for m, r in plains.items()}}
+            "gauss_newton_reached_plain_chi2": bool(gn_chi2_forward <= target),
+            "speedup": {m: r["wall_s"] / grad["wall_s"] for m, r in plains.items()},
+            "speedup_gauss_newton": {m: r["wall_s"] / gn["wall_s"] for m, r in plains.items()}}
 
 
 def main():

@@ -438,6 +438,27 @@ int smk_p1d_adjoint(const smk_skewers* sk, const float* delta, const float* eta,
                     const int64_t* modes_off, const double* gp, double* out, const double* tables,
                     const int64_t* tab_off, void* stream);
 
+/* Forward-mode derivatives for Gauss-Newton fits (no counterpart in the
+ * reference, whose fit is derivative-free, fit_az.py:266-279).
+ * smk_fgpa_flux_jac: F per pixel as smk_fgpa_flux (same bits), and per
+ * skewer sums[s][4] = forest sums of (F, dF/da, dF/db, dF/dc) at the
+ * parameters of its set (skewers.pset required).
+ * smk_p1d_jacobian: as smk_p1d_binned at the same flux and <F>, plus the
+ * derivatives of the per-bin P sums: partial[s][b][7] = (count, sum k,
+ * sum P, sum P^2, sum dP/da, sum dP/db, sum dP/dc), where, with F' the
+ * regrouped flux, delta_F = F'/<F> - 1 and t = d delta_F / d theta =
+ * dF'/<F> - F' d<F>/<F>^2, dP_j = 2 px / N Re(conj(X_j) T_j), T = DFT(t).
+ *   dfmean  device double [sets][3]: d<F>/d(a, b, c) (from the sums above)
+ * With tables the forests of a table take two Bluestein DFTs (of delta_F +
+ * i t_c and of t_a + i t_b, split by the symmetry of real inputs); the others
+ * a direct DFT of the four series by Horner's rule, any length. */
+int smk_fgpa_flux_jac(const smk_skewers* sk, const float* delta, const float* eta, const double* abcg,
+                      int64_t abcg_rows, double* flux, double* sums, void* stream);
+int smk_p1d_jacobian(const smk_skewers* sk, const float* delta, const float* eta, const double* abcg,
+                     int64_t abcg_rows, const double* flux, const double* fmean, const double* dfmean,
+                     double pixel, int regroup, const double* edges, int nbins, int max_forest, double* partial,
+                     int32_t* status, const double* tables, const int64_t* tab_off, void* stream);
+
 /* ---------------------------------------------------------- QSO catalogue */
 /* Quasar draw of one x slice of the lognormal boxes, draw_qso.py:186-462:
  * per cell, ptot = c(z)(p12 - p23) + p23 of the three fields exp(p_k)^a_k(z)

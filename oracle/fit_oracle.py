@@ -6,7 +6,10 @@ Pinned by (tests/test_oracle_fit.py):
   - forward: per-set mean flux by direct numpy means; binned P1D sums against
     forest_oracle.profile_histo (scipy binned_statistic, util.py:46-75);
   - adjoint_grad: central finite differences of chi2 in FP64 (relative 1e-6),
-    which fixes every term of the chain rule (P1D, delta_F, <F>, FGPA).
+    which fixes every term of the chain rule (P1D, delta_F, <F>, FGPA);
+  - jacobian: central finite differences of forward()'s mean flux and
+    binned P sums, and 2 J^T r equal to adjoint_grad (the two
+    differentiation modes of the same chain rule).
 """
 import numpy as np
 
@@ -117,3 +120,66 @@ def adjoint_grad(theta, deltas, etas, pset, growth, edges, pixel, targets, m=1):
     dLdF = gF - saf / fmean ** 2
     grad = t1 / fmean[:, None] + (dLdF / count)[:, None] * t2
     return grad.reshape(-1)
+
+
+def fgpa_dtheta(d, e, f, a, b, c, G):
+    """dF/d(a, b, c) per pixel of F = exp(-a exp(b G (d + c e))), [n][3]."""
+    g = d + c * e
+    T = np.exp(b * G * g)
+    return np.stack([-T * f, -a * T * f * G * g, -a * T * f * b * G * e], axis=1)
+
+
+def jacobian(theta, deltas, etas, pset, growth, edges, pixel, m=1):
+    """Forward-mode derivatives of forward(): returns (fmean[set],
+    bins[set][b][4], dfmean[set][3] = d<F>/d(a, b, c), dbins[set][b][3] = d
+    (sum of P_j over the bin's modes)/d(a, b, c)).  Per skewer, with
+    delta_F = F'/<F> - 1 (F' regrouped) and t = d delta_F / d theta =
+    dF'/<F> - F' d<F>/<F>^2: dP_j = 2 px / N Re(conj(X_j) T_j), T = rfft(t)."""
+    th = np.asarray(theta, dtype=np.float64).reshape(-1, 3)
+    ns = len(th)
+    fmean, bins, flux = forward(deltas, etas, pset, theta, growth, edges, pixel, m)
+    dsum = np.zeros((ns, 3))
+    count = np.zeros(ns)
+    dfs = []
+    for d, e, s, f in zip(deltas, etas, pset, flux):
+        a, b, c = th[s]
+        dF = fgpa_dtheta(np.asarray(d, np.float64), np.asarray(e, np.float64), f, a, b, c, growth[s])
+        dfs.append(dF)
+        dsum[s] += dF.sum(0)
+        count[s] += len(f)
+    dfmean = dsum / count[:, None]
+    nb = len(edges) - 1
+    px = pixel * m
+    dbins = np.zeros((ns, nb, 3))
+    for f, dF, s in zip(flux, dfs, pset):
+        fr = regroup(f, m)
+        nq = len(fr)
+        if nq == 0:
+            continue
+        x = fr / fmean[s] - 1
+        X = np.fft.rfft(x)
+        k = np.fft.rfftfreq(nq) * 2 * (np.pi / px)
+        bi = np.searchsorted(edges, k, side="right") - 1
+        bi[k == edges[-1]] = nb - 1
+        ok = (bi >= 0) & (bi < nb)
+        for kk in range(3):
+            t = regroup(dF[:, kk], m) / fmean[s] - fr * dfmean[s, kk] / fmean[s] ** 2
+            T = np.fft.rfft(t)
+            dP = 2 * px / nq * np.real(np.conj(X) * T)
+            dbins[s, :, kk] += np.bincount(bi[ok], weights=dP[ok], minlength=nb)
+    return fmean, bins, dfmean, dbins
+
+
+def residuals_jacobian(fmean, bins, dfmean, dbins, f_target, f_sigma, p_target, p_sigma):
+    """Per set: the residuals of chi2_from (mean flux first, then the P1D
+    bins, zero where masked) and their derivatives d r / d(a, b, c):
+    r[set][1 + nb], J[set][1 + nb][3]."""
+    c = bins[..., 0]
+    mask = np.isfinite(p_target) & np.isfinite(p_sigma) & (p_sigma > 0) & (c > 0)
+    cs = np.where(c > 0, c, 1.0)
+    sig = np.where(mask, p_sigma, 1.0)
+    rF = (fmean - f_target) / f_sigma
+    rP = np.where(mask, (bins[..., 2] / cs - np.nan_to_num(p_target)) / sig, 0.0)
+    JF = dfmean / np.asarray(f_sigma, dtype=np.float64).reshape(-1, 1)
+    JP = np.where(mask[..., None], dbins / (cs * sig)[..., None], 0.0)
+    return np.concatenate([rF[:, None], rP], axis=1), np.concatenate([JF[:, None, :], JP], axis=1)

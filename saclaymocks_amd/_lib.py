@@ -90,6 +90,9 @@ _SIGNATURES = {
     "smk_p1d_tables": (_i, [_vp, _vp, _i, _vp, _vp]),
     "smk_p1d_adjoint": (_i, [_vp, _vp, _vp, _vp, _c.c_int64, _vp, _vp, _d, _i, _vp, _i, _i, _vp, _vp, _vp, _vp,
                              _vp, _vp, _vp]),
+    "smk_fgpa_flux_jac": (_i, [_vp, _vp, _vp, _vp, _c.c_int64, _vp, _vp, _vp]),
+    "smk_p1d_jacobian": (_i, [_vp, _vp, _vp, _vp, _c.c_int64, _vp, _vp, _vp, _d, _i, _vp, _i, _i, _vp, _vp, _vp,
+                              _vp, _vp]),
 }
 
 

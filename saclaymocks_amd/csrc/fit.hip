@@ -579,27 +579,32 @@ struct AdjArgs {
   int nlds;              // longest forest of the LDS layout (min(nmax, FIT_NLDS))
 };
 
-// res[k] = the block sum of v[k], k < 7, in one pass: a shuffle tree per
-// wave, then the waves' partials in a fixed order (red: 7 NT / 64 doubles)
-template <int NT>
-__device__ __forceinline__ void reduce7(double (&v)[7], double* red, double* res) {
+// res[k] = the block sum of v[k], k < K, in one pass: a shuffle tree per
+// wave, then the waves' partials in a fixed order (red: K NT / 64 doubles);
+// the order of block_reduce_sum, so the same bits for each k
+template <int NT, int K>
+__device__ __forceinline__ void reduce_k(double (&v)[K], double* red, double* res) {
   constexpr int NW = NT / 64;
 #pragma unroll
-  for (int k = 0; k < 7; ++k)
+  for (int k = 0; k < K; ++k)
     for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   __syncthreads();
   if (lane == 0) {
 #pragma unroll
-    for (int k = 0; k < 7; ++k) red[k * NW + wv] = v[k];
+    for (int k = 0; k < K; ++k) red[k * NW + wv] = v[k];
   }
   __syncthreads();
-  if (threadIdx.x < 7) {
+  if (threadIdx.x < K) {
     double t = 0.0;
     for (int i = 0; i < NW; ++i) t += red[threadIdx.x * NW + i];
     res[threadIdx.x] = t;
   }
   __syncthreads();
+}
+template <int NT>
+__device__ __forceinline__ void reduce7(double (&v)[7], double* red, double* res) {
+  reduce_k<NT, 7>(v, red, res);
 }
 
 // dF/d(a, b, c) of pixel i (forest offset) of skewer s at parameters p
@@ -863,6 +868,328 @@ __global__ __launch_bounds__(M / 8) __attribute__((amdgpu_waves_per_eu(2))) void
   }
 }
 
+// ------------------------------------------------------------- Jacobian
+// Forward-mode derivatives of the fit's observables, for Gauss-Newton: with
+// theta = (a, b, c) of the skewer's set, F' the regrouped flux and
+// delta_F = F'/<F> - 1, the tangents t_k = d delta_F / d theta_k =
+// dF'_k / <F> - F' d<F>_k / <F>^2 and dP_j / d theta_k =
+// 2 px / N Re(conj(X_j) T_kj) with T_k = DFT(t_k); per bin, the sums of
+// dP_j / d theta_k beside the forward's (count, sum k, sum P, sum P^2).
+
+struct JacArgs {
+  smk_skewers sk;
+  const float* delta;
+  const float* eta;
+  const double* abcg;
+  int64_t rows;
+  const double* flux;
+  const double* fmean;   // per set
+  const double* dfmean;  // [sets][3]: d<F>/d(a, b, c)
+  double pixel;
+  int regroup;
+  const double* edges;
+  int nb;
+  double* partial;       // [n][nb][7]: count, sum k, sum P, sum P^2, sum dP/d(a, b, c)
+  int32_t* status;       // per skewer: 1 if the forest is longer than nmax
+  int nmax;
+  const double2* tw;
+  const double2* tables;
+  const int64_t* tab_off;
+};
+
+// F and dF/d(a, b, c) summed over the m pixels of regrouped point q (the
+// expressions of k_fgpa_flux and fgpa_df, so F is the same bits)
+struct JacPoint { double f, da, db, dc; };
+__device__ __forceinline__ JacPoint jac_point(const JacArgs& a, int64_t o, int q, int m, double4 p) {
+  JacPoint r{0.0, 0.0, 0.0, 0.0};
+  for (int k = 0; k < m; ++k) {
+    const int64_t at = o + (int64_t)q * m + k;
+    const double e = a.eta ? (double)a.eta[at] : 0.0;
+    const double gf = (double)a.delta[at] + p.z * e;
+    const double T = exp(p.y * p.w * gf);
+    const double F = a.flux[at];
+    r.f += F;
+    r.da += -T * F;
+    r.db += -p.x * T * F * p.w * gf;
+    r.dc += -p.x * T * F * p.y * p.w * e;
+  }
+  return r;
+}
+
+// F for every pixel (k_fgpa_flux's expression) and, per skewer, the forest
+// sums of F and of dF/d(a, b, c) at the parameters of its set
+__global__ __launch_bounds__(FIT_THREADS) void k_fgpa_flux_jac(smk_skewers sk, const float* __restrict__ delta,
+                                                               const float* __restrict__ eta,
+                                                               const double* __restrict__ abcg, int64_t rows,
+                                                               double* __restrict__ flux,
+                                                               double* __restrict__ sums) {
+  __shared__ double red[4 * (FIT_THREADS / 64)];
+  for (int s = blockIdx.x; s < sk.n; s += gridDim.x) {
+    const int64_t o = sk.off[s];
+    const int n = sk.npix[s], i0 = sk.imin[s], i1 = sk.imax[s];
+    const double4 p = prow(abcg, rows, sk.pset[s]);
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < n; i += FIT_THREADS) {
+      const double e = eta ? (double)eta[o + i] : 0.0;
+      const double g = (double)delta[o + i] + p.z * e;
+      const double T = exp(p.y * p.w * g);
+      const double f = exp(-p.x * T);
+      flux[o + i] = f;
+      if (i >= i0 && i < i1) {
+        v[0] += f;
+        v[1] += -T * f;
+        v[2] += -p.x * T * f * p.w * g;
+        v[3] += -p.x * T * f * p.y * p.w * e;
+      }
+    }
+    reduce_k<FIT_THREADS, 4>(v, red, sums + (int64_t)s * 4);
+  }
+}
+
+// Z_j and Z_{N-j} of Z = DFT(u + i w), u and w real: U_j and W_j
+__device__ __forceinline__ void split_pair(double2 zj, double2 zn, double2& u, double2& w) {
+  u = make_double2(0.5 * (zj.x + zn.x), 0.5 * (zj.y - zn.y));
+  w = make_double2(0.5 * (zj.y + zn.y), 0.5 * (zn.x - zj.x));
+}
+
+// bin runs as bin_runs, seven sums per bin: count, sum k, sum P, sum P^2 and
+// the sums of the three dP rows (dp + r * ds)
+template <int NT>
+__device__ __forceinline__ void bin_runs_jac(const double* pk, const double* dp, int ds, const int* bins, int nj,
+                                             int nq, double k_ny, int nb, int* bfirst, int* blast, double* part) {
+  for (int b = threadIdx.x; b < nb; b += NT) {
+    bfirst[b] = 0;
+    blast[b] = -1;
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < nj; j += NT) {
+    const int b = bins[j];
+    if (b < 0) continue;
+    if (j == 0 || bins[j - 1] != b) bfirst[b] = j;
+    if (j == nj - 1 || bins[j + 1] != b) blast[b] = j;
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < nb; b += NT) {
+    double c = 0.0, sk_ = 0.0, sp = 0.0, sp2 = 0.0, d0 = 0.0, d1 = 0.0, d2 = 0.0;
+    for (int j = bfirst[b]; j <= blast[b]; ++j) {
+      c += 1.0;
+      sk_ += kfreq(j, nq, k_ny);
+      sp += pk[j];
+      sp2 += pk[j] * pk[j];
+      d0 += dp[j];
+      d1 += dp[ds + j];
+      d2 += dp[2 * ds + j];
+    }
+    double* o = part + 7 * b;
+    o[0] = c; o[1] = sk_; o[2] = sp; o[3] = sp2; o[4] = d0; o[5] = d1; o[6] = d2;
+  }
+}
+
+// LDS of k_p1d_jac_blue<M>: the line, then X_j, pk, the three dP rows (NJ
+// each, NJ = M/4 + 1 >= N/2 + 1), the bins, the edges and the bin runs
+__host__ __device__ constexpr int jac_nj(int M) { return M / 4 + 1; }
+__host__ __device__ constexpr size_t jac_blue_fixed_bytes(int M) {
+  return blue_line_bytes_m(M) + (size_t)jac_nj(M) * 16 + (size_t)jac_nj(M) * 32 +
+         (size_t)((jac_nj(M) + 1) & ~1) * 4;
+}
+
+// The Jacobian of the skewers with a table and a forest of M's size class:
+// two Bluestein DFTs (bluestein_tab), of delta_F + i t_c (X and T_c) and of
+// t_a + i t_b (T_a and T_b), each emitting all N outputs into the line and
+// split by the Hermitian symmetry of the real inputs
+template <int M>
+__global__ __launch_bounds__(M / 8) __attribute__((amdgpu_waves_per_eu(2))) void k_p1d_jac_blue(JacArgs a) {
+  constexpr int NT = M / 8;
+  constexpr int NJ = jac_nj(M);
+  extern __shared__ double dyn[];
+  double2* line = reinterpret_cast<double2*>(dyn);
+  double2* xs = reinterpret_cast<double2*>(reinterpret_cast<char*>(dyn) + blue_line_bytes_m(M));
+  double* pk = reinterpret_cast<double*>(xs + NJ);
+  double* dp = pk + NJ;
+  int* bins = reinterpret_cast<int*>(dp + 3 * NJ);
+  double* ledges = reinterpret_cast<double*>(bins + ((NJ + 1) & ~1));
+  int* bfirst = reinterpret_cast<int*>(ledges + a.nb + 1);
+  int* blast = bfirst + a.nb;
+  for (int i = threadIdx.x; i <= a.nb; i += NT) ledges[i] = a.edges[i];
+  const smk_skewers& sk = a.sk;
+  for (int s = blockIdx.x; s < sk.n; s += gridDim.x) {
+    const int i0 = sk.imin[s], i1 = sk.imax[s];
+    const int m = a.regroup;
+    const int nq = (i1 - i0) / m;
+    if (!uses_table(a.tables, a.tab_off, s, nq, a.nmax) || blue_m_of(nq) != M) continue;
+    __syncthreads();
+    const int set = sk.pset[s];
+    const double inv = 1.0 / a.fmean[set];
+    const double g0 = a.dfmean[3 * set], g1 = a.dfmean[3 * set + 1], g2 = a.dfmean[3 * set + 2];
+    const double4 p = prow(a.abcg, a.rows, set);
+    const double px = a.pixel * m;
+    const double k_ny = M_PI / px;
+    const int nj = nq / 2 + 1;
+    const double c2 = 2.0 * px / nq;
+    const double2* T = a.tables + a.tab_off[s];
+    const int64_t o = sk.off[s] + i0;
+    auto tangent = [&](double dsum, double fsum, double g) { return (dsum / m) * inv - (fsum / m) * g * inv * inv; };
+    auto to_line = [&](int j, double2 Z) { line[j] = Z; };
+    f64::bluestein_tab<M, 8>(
+        nq, nq,
+        [&](int q) {
+          const JacPoint P = jac_point(a, o, q, m, p);
+          return make_double2((P.f / m) * inv - 1.0, tangent(P.dc, P.f, g2));
+        },
+        to_line, line, a.tw, BLUE_MMAX, T, T + nq);
+    __syncthreads();
+    for (int j = threadIdx.x; j < nj; j += NT) {
+      double2 X, Tc;
+      split_pair(line[j], line[j == 0 ? 0 : nq - j], X, Tc);
+      xs[j] = X;
+      pk[j] = (X.x * X.x + X.y * X.y) * px / nq;
+      dp[2 * NJ + j] = c2 * (X.x * Tc.x + X.y * Tc.y);
+      bins[j] = find_bin(ledges, a.nb, kfreq(j, nq, k_ny));
+    }
+    // (bluestein_tab synchronises the block before its first exchange
+    // overwrites the line read above)
+    f64::bluestein_tab<M, 8>(
+        nq, nq,
+        [&](int q) {
+          const JacPoint P = jac_point(a, o, q, m, p);
+          return make_double2(tangent(P.da, P.f, g0), tangent(P.db, P.f, g1));
+        },
+        to_line, line, a.tw, BLUE_MMAX, T, T + nq);
+    __syncthreads();
+    for (int j = threadIdx.x; j < nj; j += NT) {
+      double2 Ta, Tb;
+      split_pair(line[j], line[j == 0 ? 0 : nq - j], Ta, Tb);
+      const double2 X = xs[j];
+      dp[j] = c2 * (X.x * Ta.x + X.y * Ta.y);
+      dp[NJ + j] = c2 * (X.x * Tb.x + X.y * Tb.y);
+    }
+    __syncthreads();
+    bin_runs_jac<NT>(pk, dp, NJ, bins, nj, nq, k_ny, a.nb, bfirst, blast, a.partial + (int64_t)s * a.nb * 7);
+  }
+}
+
+// Generic Jacobian kernel (any N; TAB as k_p1d: with tables, the skewers
+// without a usable one): the four real series delta_F, t_a, t_b, t_c staged
+// through LDS in pieces of JAC_XC points, their DFTs by Horner's rule for
+// 4 FIT_THREADS modes per pass (four accumulators per mode sharing the
+// twiddle), each bin's seven running sums in LDS, added in ascending j.
+constexpr int JAC_XC = 1024;
+constexpr int JAC_CH = 4 * FIT_THREADS;
+size_t jac_generic_lds(int nb) {
+  return (size_t)JAC_XC * 32 + (size_t)JAC_CH * 32 + (size_t)(nb + 1) * 8 + (size_t)nb * 8 + (size_t)nb * 56;
+}
+
+template <bool TAB>
+__global__ __launch_bounds__(FIT_THREADS) void k_p1d_jac(JacArgs a) {
+  extern __shared__ double dyn[];
+  double2* ya = reinterpret_cast<double2*>(dyn);  // (delta_F, t_c)
+  double2* yb = ya + JAC_XC;                      // (t_a, t_b)
+  double* vals = reinterpret_cast<double*>(yb + JAC_XC);  // pk, dP_a, dP_b, dP_c rows of JAC_CH
+  double* ledges = vals + 4 * JAC_CH;
+  int* blo = reinterpret_cast<int*>(ledges + a.nb + 1);
+  int* bhi = blo + a.nb;
+  double* bsum = reinterpret_cast<double*>(bhi + a.nb);
+  for (int i = threadIdx.x; i <= a.nb; i += FIT_THREADS) ledges[i] = a.edges[i];
+  const smk_skewers& sk = a.sk;
+  for (int s = blockIdx.x; s < sk.n; s += gridDim.x) {
+    const int i0 = sk.imin[s], i1 = sk.imax[s];
+    const int m = a.regroup;
+    const int nq = (i1 - i0) / m;
+    if (TAB && uses_table(a.tables, a.tab_off, s, nq, a.nmax)) continue;
+    double* part = a.partial + (int64_t)s * a.nb * 7;
+    __syncthreads();
+    if (nq > a.nmax) {
+      for (int b = threadIdx.x; b < 7 * a.nb; b += FIT_THREADS) part[b] = 0.0;
+      if (threadIdx.x == 0 && a.status) a.status[s] = 1;
+      continue;
+    }
+    const int set = sk.pset[s];
+    const double inv = 1.0 / a.fmean[set];
+    const double g0 = a.dfmean[3 * set], g1 = a.dfmean[3 * set + 1], g2 = a.dfmean[3 * set + 2];
+    const double4 p = prow(a.abcg, a.rows, set);
+    const double px = a.pixel * m;
+    const double k_ny = M_PI / px;
+    const int nj = nq > 0 ? nq / 2 + 1 : 0;
+    const double c2 = nq > 0 ? 2.0 * px / nq : 0.0;
+    const int64_t o = sk.off[s] + i0;
+    for (int b = threadIdx.x; b < a.nb; b += FIT_THREADS) {
+      bin_span(ledges, a.nb, b, nj, nq, k_ny, blo[b], bhi[b]);
+#pragma unroll
+      for (int r = 0; r < 7; ++r) bsum[7 * b + r] = 0.0;
+    }
+    auto tangent = [&](double dsum, double fsum, double g) { return (dsum / m) * inv - (fsum / m) * g * inv * inv; };
+    for (int j0 = 0; j0 < nj; j0 += JAC_CH) {
+      const int j1 = j0 + JAC_CH < nj ? j0 + JAC_CH : nj;
+      double c[4], sn[4], re[4][4], im[4][4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int f = j0 + (int)threadIdx.x + r * FIT_THREADS;
+        if (f < j1) {
+          sincospi(-2.0 * (double)f / (double)nq, &sn[r], &c[r]);
+        } else {
+          sn[r] = 0.0;
+          c[r] = 1.0;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) re[r][k] = im[r][k] = 0.0;
+      }
+      // Horner over n descending, the input staged in pieces
+      for (int nh = nq; nh > 0; nh -= JAC_XC) {
+        const int nl = nh > JAC_XC ? nh - JAC_XC : 0;
+        __syncthreads();
+        for (int q = nl + (int)threadIdx.x; q < nh; q += FIT_THREADS) {
+          const JacPoint P = jac_point(a, o, q, m, p);
+          ya[q - nl] = make_double2((P.f / m) * inv - 1.0, tangent(P.dc, P.f, g2));
+          yb[q - nl] = make_double2(tangent(P.da, P.f, g0), tangent(P.db, P.f, g1));
+        }
+        __syncthreads();
+        for (int n = nh - 1; n >= nl; --n) {
+          const double2 u = ya[n - nl], w = yb[n - nl];
+          const double y[4] = {u.x, w.x, w.y, u.y};   // delta_F, t_a, t_b, t_c
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+              const double r2 = fma(re[r][k], c[r], fma(-im[r][k], sn[r], y[k]));
+              im[r][k] = fma(re[r][k], sn[r], im[r][k] * c[r]);
+              re[r][k] = r2;
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int j = j0 + (int)threadIdx.x + r * FIT_THREADS;
+        if (j < j1) {
+          const double xr = re[r][0], xi = im[r][0];
+          vals[j - j0] = (xr * xr + xi * xi) * px / nq;
+#pragma unroll
+          for (int k = 1; k < 4; ++k) vals[k * JAC_CH + j - j0] = c2 * (xr * re[r][k] + xi * im[r][k]);
+        }
+      }
+      __syncthreads();
+      for (int b = threadIdx.x; b < a.nb; b += FIT_THREADS) {
+        const int lo = blo[b] > j0 ? blo[b] : j0, hi = bhi[b] < j1 ? bhi[b] : j1;
+        double* t = bsum + 7 * b;
+        double cnt = t[0], sk_ = t[1], sp = t[2], sp2 = t[3], d0 = t[4], d1 = t[5], d2 = t[6];
+        for (int j = lo; j < hi; ++j) {
+          const double pj = vals[j - j0];
+          cnt += 1.0;
+          sk_ += kfreq(j, nq, k_ny);
+          sp += pj;
+          sp2 += pj * pj;
+          d0 += vals[JAC_CH + j - j0];
+          d1 += vals[2 * JAC_CH + j - j0];
+          d2 += vals[3 * JAC_CH + j - j0];
+        }
+        t[0] = cnt; t[1] = sk_; t[2] = sp; t[3] = sp2; t[4] = d0; t[5] = d1; t[6] = d2;
+      }
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < 7 * a.nb; b += FIT_THREADS) part[b] = bsum[b];
+  }
+}
+
 int grid_for(int n) { return n < 65536 ? (n > 0 ? n : 1) : 65536; }
 
 // propagate a nonzero SMK_ERR_* return
@@ -888,6 +1215,16 @@ int launch_adj_blue(const AdjArgs& a, dim3 grid, size_t tail, hipStream_t st) {
   SMK_HIP_CHECK(hipFuncSetAttribute((const void*)k_p1d_adjoint_blue<M>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   hipLaunchKernelGGL(k_p1d_adjoint_blue<M>, grid, dim3(M / 8), lds, st, a);
+  SMK_LAUNCH_CHECK();
+  return SMK_OK;
+}
+
+template <int M>
+int launch_p1d_jac_blue(const JacArgs& a, dim3 grid, size_t tail, hipStream_t st) {
+  const size_t lds = jac_blue_fixed_bytes(M) + tail;
+  SMK_HIP_CHECK(hipFuncSetAttribute((const void*)k_p1d_jac_blue<M>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds));
+  hipLaunchKernelGGL(k_p1d_jac_blue<M>, grid, dim3(M / 8), lds, st, a);
   SMK_LAUNCH_CHECK();
   return SMK_OK;
 }
@@ -1058,6 +1395,61 @@ extern "C" int smk_p1d_adjoint(const smk_skewers* sk, const float* delta, const 
     SMK_HIP_CHECK(hipFuncSetAttribute((const void*)k_p1d_adjoint<false>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_p1d_adjoint<false>, grid, dim3(FIT_THREADS), lds, st, a);
+    SMK_LAUNCH_CHECK();
+  }
+  return SMK_OK;
+}
+
+extern "C" int smk_fgpa_flux_jac(const smk_skewers* sk, const float* delta, const float* eta, const double* abcg,
+                                 int64_t abcg_rows, double* flux, double* sums, void* stream) {
+  SMK_REQUIRE(sk != nullptr && delta != nullptr && abcg != nullptr && flux != nullptr && sums != nullptr,
+              SMK_ERR_ARG, "NULL argument");
+  SMK_REQUIRE(sk->pset != nullptr, SMK_ERR_ARG, "the fit kernels need skewers.pset");
+  if (sk->n <= 0) return SMK_OK;
+  hipLaunchKernelGGL(k_fgpa_flux_jac, dim3(grid_for(sk->n)), dim3(FIT_THREADS), 0, (hipStream_t)stream, *sk,
+                     delta, eta, abcg, abcg_rows, flux, sums);
+  SMK_LAUNCH_CHECK();
+  return SMK_OK;
+}
+
+extern "C" int smk_p1d_jacobian(const smk_skewers* sk, const float* delta, const float* eta, const double* abcg,
+                                int64_t abcg_rows, const double* flux, const double* fmean, const double* dfmean,
+                                double pixel, int regroup, const double* edges, int nbins, int max_forest,
+                                double* partial, int32_t* status, const double* tables, const int64_t* tab_off,
+                                void* stream) {
+  SMK_REQUIRE(sk != nullptr && delta != nullptr && abcg != nullptr && flux != nullptr && fmean != nullptr &&
+                  dfmean != nullptr && edges != nullptr && partial != nullptr,
+              SMK_ERR_ARG, "NULL argument");
+  SMK_REQUIRE(sk->pset != nullptr, SMK_ERR_ARG, "the fit kernels need skewers.pset");
+  SMK_REQUIRE(regroup >= 1 && nbins >= 1 && pixel > 0, SMK_ERR_ARG, "bad regroup/nbins/pixel");
+  SMK_REQUIRE(max_forest >= 0, SMK_ERR_ARG, "max_forest < 0");
+  SMK_REQUIRE((tables == nullptr) == (tab_off == nullptr), SMK_ERR_ARG, "tables and tab_off go together");
+  if (sk->n <= 0) return SMK_OK;
+  int nmax = max_forest / regroup;
+  nmax += nmax & 1;
+  if (nmax < 2) nmax = 2;
+  hipError_t te = hipSuccess;
+  const double2* tw = fp64_twiddles(&te);
+  SMK_REQUIRE(tw != nullptr, SMK_ERR_HIP, "FP64 twiddle table: %s", hipGetErrorString(te));
+  JacArgs a{*sk,    delta, eta,    abcg,    abcg_rows, flux, fmean, dfmean, pixel, regroup, edges, nbins,
+            partial, status, nmax, tw,      (const double2*)tables, tab_off};
+  const dim3 grid(grid_for(sk->n));
+  hipStream_t st = (hipStream_t)stream;
+  const size_t lds = jac_generic_lds(nbins);
+  if (tables) {
+    SMK_HIP_CHECK(hipFuncSetAttribute((const void*)k_p1d_jac<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds));
+    hipLaunchKernelGGL(k_p1d_jac<true>, grid, dim3(FIT_THREADS), lds, st, a);
+    SMK_LAUNCH_CHECK();
+    const int nblue = nmax < BLUE_NMAX ? nmax : BLUE_NMAX;
+    const size_t tail = (size_t)(nbins + 1) * 8 + (size_t)nbins * 8;
+    SMK_CALL(launch_p1d_jac_blue<1024>(a, grid, tail, st));
+    if (blue_m_of(nblue) >= 2048) SMK_CALL(launch_p1d_jac_blue<2048>(a, grid, tail, st));
+    if (blue_m_of(nblue) >= 4096) SMK_CALL(launch_p1d_jac_blue<4096>(a, grid, tail, st));
+  } else {
+    SMK_HIP_CHECK(hipFuncSetAttribute((const void*)k_p1d_jac<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds));
+    hipLaunchKernelGGL(k_p1d_jac<false>, grid, dim3(FIT_THREADS), lds, st, a);
     SMK_LAUNCH_CHECK();
   }
   return SMK_OK;

@@ -77,6 +77,7 @@ class P1DFitModel:
         self.tables, self.tab_off = self._bluestein_tables(nf // self.regroup) if use_tables else (None, None)
         self.n_eval = 0
         self.n_grad = 0
+        self.n_jac = 0
 
     def _bluestein_tables(self, nq):
         """The Bluestein tables of every forest length of the batch, made
@@ -111,6 +112,7 @@ class P1DFitModel:
         self.edges = torch.from_numpy(self.edges_host).to(self.dev)
         self.partial = torch.empty((self.batch.n, self.nb, 4), dtype=torch.float64, device=self.dev)
         self.bins_set = torch.empty((self.n_sets, self.nb, 4), dtype=torch.float64, device=self.dev)
+        self._jac_bufs = None   # sized by nb, made on the first jacobian()
 
     # ------------------------------------------------------------ forward
     def _abcg(self, theta):
@@ -166,6 +168,60 @@ class P1DFitModel:
             my2 = bins[..., 3] / c
             err = np.sqrt(np.maximum(my2 - my * my, 0)) / np.sqrt(c)
         return mx, my, err, c
+
+    # ----------------------------------------------------------- Jacobian
+    def jacobian(self, theta):
+        """Mean flux and binned P1D sums at theta with their derivatives by
+        the forward-mode kernels: (fmean[set], bins[set][b][4],
+        dfmean[set][3] = d<F>/d(a, b, c), dbins[set][b][3] = d(sum P)/d(a, b, c))."""
+        L = _lib.lib()
+        s = _lib.stream_ptr()
+        if self._jac_bufs is None:
+            n, dev = self.batch.n, self.dev
+            self._jac_bufs = (torch.empty((n, 4), dtype=torch.float64, device=dev),
+                              torch.empty((self.n_sets, 4), dtype=torch.float64, device=dev),
+                              torch.empty((self.n_sets, 3), dtype=torch.float64, device=dev),
+                              torch.empty((n, self.nb, 7), dtype=torch.float64, device=dev),
+                              torch.empty((self.n_sets, self.nb, 7), dtype=torch.float64, device=dev))
+        sums, sums_set, dfm_dev, part, part_set = self._jac_bufs
+        abcg = self._abcg(theta)
+        _lib.check(L.smk_fgpa_flux_jac(ctypes.byref(self.sk), _lib.ptr(self.delta), _lib.ptr(self.eta),
+                                       _lib.ptr(abcg), abcg.shape[0], _lib.ptr(self.flux), _lib.ptr(sums), s),
+                   "smk_fgpa_flux_jac")
+        _lib.check(L.smk_reduce_by_set(ctypes.byref(self.sk), _lib.ptr(sums), 4, self.n_sets, _lib.ptr(sums_set),
+                                       s), "smk_reduce_by_set")
+        r = _sync_host(sums_set)
+        fmean = r[:, 0] / self.count
+        dfmean = r[:, 1:4] / self.count[:, None]
+        self.fmean_dev.copy_(torch.from_numpy(fmean))
+        dfm_dev.copy_(torch.from_numpy(np.ascontiguousarray(dfmean)))
+        _lib.check(L.smk_p1d_jacobian(ctypes.byref(self.sk), _lib.ptr(self.delta), _lib.ptr(self.eta),
+                                      _lib.ptr(abcg), abcg.shape[0], _lib.ptr(self.flux), _lib.ptr(self.fmean_dev),
+                                      _lib.ptr(dfm_dev), self.pixel, self.regroup, _lib.ptr(self.edges), self.nb,
+                                      self.max_forest, _lib.ptr(part), _lib.ptr(self.status),
+                                      _lib.ptr(self.tables), _lib.ptr(self.tab_off), s), "smk_p1d_jacobian")
+        _lib.check(L.smk_reduce_by_set(ctypes.byref(self.sk), _lib.ptr(part), self.nb * 7, self.n_sets,
+                                       _lib.ptr(part_set), s), "smk_reduce_by_set")
+        q = _sync_host(part_set)
+        self.n_jac += 1
+        return fmean, q[..., :4], dfmean, q[..., 4:7]
+
+    def residuals(self, fmean, bins, dfmean=None, dbins=None):
+        """Per set, the residuals of chi2 (mean flux, then the P1D bins; 0
+        where a bin is masked) and, with the derivatives, d r/d(a, b, c):
+        r[set][1 + nb], J[set][1 + nb][3] (J None without derivatives)."""
+        c = bins[..., 0]
+        m = self.p_mask & (c > 0)
+        cs = np.where(c > 0, c, 1.0)
+        sig = np.where(m, self.p_sigma, 1.0)
+        rF = (fmean - self.f_target) / self.f_sigma
+        rP = np.where(m, (bins[..., 2] / cs - np.nan_to_num(self.p_target)) / sig, 0.0)
+        r = np.concatenate([rF[:, None], rP], axis=1)
+        if dfmean is None:
+            return r, None
+        JF = dfmean / self.f_sigma[:, None]
+        JP = np.where(m[..., None], dbins / (cs * sig)[..., None], 0.0)
+        return r, np.concatenate([JF[:, None, :], JP], axis=1)
 
     # --------------------------------------------------------------- loss
     def set_targets(self, f_target, f_sigma, p_target, p_sigma):
@@ -314,3 +370,79 @@ def fit_gradient(model, theta0, bounds, chi2_target=None, maxiter=500, log_a=Tru
         msg = "reached target chi2"
     return {"x": best["x"], "chi2": best["chi2"], "wall_s": time.perf_counter() - t0,
             "time_to_best_s": best["t"], "n_eval": model.n_eval - n0, "message": msg}
+
+
+def fit_gauss_newton(model, theta0, bounds, chi2_target=None, maxiter=100, log_a=True, lam0=1e-3, rtol=1e-12):
+    """Levenberg-Marquardt on the device Jacobian (P1DFitModel.jacobian: the
+    mean flux and the binned P1D with their derivatives, one forward-mode
+    pass per iteration).  The parameters of different sets touch disjoint
+    residuals, so each set takes its own 3x3 damped step and is accepted or
+    rejected on its own chi2 (the total is their sum).  Optimises (ln a, b, c)
+    per set when log_a; stops as soon as chi2 <= chi2_target when given, or
+    when no set improves by more than rtol relative."""
+    t0 = time.perf_counter()
+    n0 = model.n_jac
+    ns = model.n_sets
+    u = _to_u(theta0, log_a).reshape(ns, 3)
+    ub = _u_bounds(bounds, log_a)
+    lo = np.array([(-np.inf if b[0] is None else b[0]) for b in ub]).reshape(ns, 3) if ub else -np.inf
+    hi = np.array([(np.inf if b[1] is None else b[1]) for b in ub]).reshape(ns, 3) if ub else np.inf
+
+    def evaluate(uu):
+        th = _to_theta(uu.reshape(-1), log_a).reshape(ns, 3)
+        fm, bins, dfm, dbins = model.jacobian(th.reshape(-1))
+        r, J = model.residuals(fm, bins, dfm, dbins)
+        if log_a:
+            J = J.copy()
+            J[:, :, 0] *= th[:, 0][:, None]       # d/d ln a = a d/da
+        return np.sum(r * r, axis=1), r, J
+
+    c2, r, J = evaluate(u)
+    lam = np.full(ns, float(lam0))
+    done = np.zeros(ns, dtype=bool)
+    msg = "maxiter"
+    it = 0
+    history = [float(c2.sum())]
+    for it in range(1, maxiter + 1):
+        if chi2_target is not None and c2.sum() <= chi2_target:
+            msg = "reached target chi2"
+            break
+        if np.all(done):
+            msg = "converged"
+            break
+        step = np.zeros((ns, 3))
+        for k in range(ns):
+            if done[k]:
+                continue
+            A = J[k].T @ J[k]
+            g = J[k].T @ r[k]
+            D = np.diag(np.maximum(np.diag(A), 1e-300))
+            try:
+                step[k] = np.linalg.solve(A + lam[k] * D, -g)
+            except np.linalg.LinAlgError:
+                step[k] = np.zeros(3)
+            # the linear model's decrease |r|^2 - |r + J step|^2: a set whose
+            # step cannot gain rtol of its chi2, or does not move it, is done
+            pred = -(2.0 * g @ step[k] + step[k] @ A @ step[k])
+            if pred <= rtol * c2[k] or np.all(np.abs(step[k]) <= 1e-13 * (np.abs(u[k]) + 1e-8)):
+                done[k] = True
+                step[k] = 0.0
+        if np.all(done):
+            msg = "converged"
+            break
+        ut = np.clip(u + step, lo, hi)
+        c2t, rt, Jt = evaluate(ut)
+        better = (c2t < c2) & ~done
+        gain = np.where(better, (c2 - c2t) / np.maximum(c2, 1e-300), 0.0)
+        u = np.where(better[:, None], ut, u)
+        r = np.where(better[:, None], rt, r)
+        J = np.where(better[:, None, None], Jt, J)
+        c2 = np.where(better, c2t, c2)
+        lam = np.where(better, np.maximum(lam / 10.0, 1e-15), np.maximum(lam * 10.0, 1e-4))
+        # a set is done when an accepted step gains at most rtol, or when
+        # the damping has grown past any useful step
+        done |= (better & (gain <= rtol)) | (lam > 1e10)
+        history.append(float(c2.sum()))
+    th = _to_theta(u.reshape(-1), log_a)
+    return {"x": th, "chi2": float(c2.sum()), "wall_s": time.perf_counter() - t0, "n_jac": model.n_jac - n0,
+            "iterations": it, "message": msg, "history": history}

@@ -148,3 +148,54 @@ def test_gradient_fit_recovers_parameters(cuda, ns):
     assert np.allclose(res["x"], theta, rtol=2e-2, atol=2e-2)
     plain = fit.fit_plain(model, th0, bounds, maxfev=3000, method="fd-lbfgs")
     assert plain["chi2"] < 1e-2
+
+
+def _check_jacobian(model, args, th):
+    fmean, bins, dfm, dbins = model.jacobian(th)
+    d64, e64, pset, growth, edges, pixel, targets, mm = args
+    rf, rb, rdf, rdb = fto.jacobian(th, d64, e64, pset, growth, edges, pixel, mm)
+    assert np.allclose(fmean, rf, rtol=1e-12)
+    assert np.array_equal(bins[..., 0], rb[..., 0])
+    assert np.allclose(bins[..., 1], rb[..., 1], rtol=1e-12)
+    assert np.allclose(bins[..., 2], rb[..., 2], rtol=1e-9, atol=1e-12 * rb[..., 2].max())
+    assert np.allclose(bins[..., 3], rb[..., 3], rtol=1e-9, atol=1e-12 * rb[..., 3].max())
+    assert np.allclose(dfm, rdf, rtol=1e-10, atol=1e-13 * np.abs(rdf).max())
+    for k in range(3):
+        assert np.allclose(dbins[..., k], rdb[..., k], rtol=1e-8, atol=1e-11 * np.abs(rdb[..., k]).max()), k
+    assert model.status.sum().item() == 0
+    # the mean flux is the forward's, bit for bit (same expressions, same reduction order)
+    f2, _ = model.forward(th)
+    assert np.array_equal(f2, fmean)
+    # Gauss-Newton's gradient 2 J^T r is the adjoint's
+    r, J = model.residuals(fmean, bins, dfm, dbins)
+    g = 2 * np.einsum("sik,si->sk", J, r).reshape(-1)
+    want = fto.adjoint_grad(th, *args)
+    assert np.allclose(g, want, rtol=1e-7, atol=1e-9 * np.abs(want).max())
+
+
+@pytest.mark.parametrize("use_tables", [True, False])
+@pytest.mark.parametrize("m", [1, 2])
+def test_jacobian_parity(cuda, m, use_tables):
+    """Forward-mode kernels against the FP64 oracle: forests of every path
+    (Bluestein size classes with tables, the direct DFT without, beyond
+    the LDS layout's 8192 points)."""
+    lens = EDGE_LENGTHS + [9000, 4100]
+    model, args, theta = _problem(cuda, ns=2, per=len(lens) // 2, lens=lens, m=m, use_tables=use_tables)
+    if use_tables:
+        model.tab_off[::5] = -1
+    _check_jacobian(model, args, theta * np.array([1.1, 0.95, 1.05] * 2))
+
+
+@pytest.mark.parametrize("ns", [2, 5])
+def test_gauss_newton_fit_recovers_parameters(cuda, ns):
+    from saclaymocks_amd import fit
+    model, args, theta = _problem(cuda, ns=ns, per=60)
+    _check_jacobian(model, args, theta * np.array([1.2, 0.9, 1.1] * ns))
+    th0 = theta * np.array([1.3, 0.85, 1.2] * ns)
+    bounds = [(1e-3, 3.0), (0.1, 4.0), (0.0, 4.0)] * ns
+    res = fit.fit_gauss_newton(model, th0, bounds)
+    assert res["chi2"] < 1e-6, res
+    assert np.allclose(res["x"], theta, rtol=1e-3, atol=1e-4)
+    assert abs(model.chi2(res["x"]) - res["chi2"]) < 1e-9 * max(1.0, res["chi2"])
+    grad = fit.fit_gradient(model, th0, bounds)
+    assert res["n_jac"] < grad["n_eval"]

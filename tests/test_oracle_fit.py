@@ -42,3 +42,33 @@ def test_adjoint_matches_finite_differences():
         ga = fto.adjoint_grad(theta, *args)
         gf = fto.grad_fd(theta, args)
         assert np.allclose(ga, gf, rtol=1e-5, atol=1e-6 * np.abs(gf).max())
+
+
+def test_jacobian_matches_finite_differences_and_the_adjoint():
+    """Forward-mode derivatives of the mean flux and of the binned P sums
+    against central differences of forward(); 2 J^T r (Gauss-Newton's
+    gradient) against the adjoint chain rule."""
+    for m in (1, 2):
+        deltas, etas, pset, growth, edges, theta, targets = _case(m=m)
+        fm, bins, dfm, dbins = fto.jacobian(theta, deltas, etas, pset, growth, edges, 0.2, m)
+        fm0, bins0, _ = fto.forward(deltas, etas, pset, theta, growth, edges, 0.2, m)
+        assert np.array_equal(fm, fm0) and np.array_equal(bins, bins0)
+        for i in range(len(theta)):
+            s, k = divmod(i, 3)
+            h = 1e-6 * max(1.0, abs(theta[i]))
+            tp, tm = theta.copy(), theta.copy()
+            tp[i] += h
+            tm[i] -= h
+            fp, bp, _ = fto.forward(deltas, etas, pset, tp, growth, edges, 0.2, m)
+            fn, bn, _ = fto.forward(deltas, etas, pset, tm, growth, edges, 0.2, m)
+            assert abs(dfm[s, k] - (fp[s] - fn[s]) / (2 * h)) < 1e-7 * max(1.0, abs(dfm[s, k]))
+            fd = (bp[s, :, 2] - bn[s, :, 2]) / (2 * h)
+            assert np.allclose(dbins[s, :, k], fd, rtol=1e-6, atol=1e-8 * np.abs(fd).max())
+            # other sets do not move
+            other = [q for q in range(len(fm)) if q != s]
+            assert np.all(fp[other] == fn[other])
+        r, J = fto.residuals_jacobian(fm, bins, dfm, dbins, *targets)
+        assert abs(np.sum(r * r) - fto.chi2_from(fm, bins, *targets)) < 1e-9 * np.sum(r * r)
+        g = 2 * np.einsum("sik,si->sk", J, r).reshape(-1)
+        ga = fto.adjoint_grad(theta, deltas, etas, pset, growth, edges, 0.2, targets, m)
+        assert np.allclose(g, ga, rtol=1e-10, atol=1e-12 * np.abs(ga).max())
