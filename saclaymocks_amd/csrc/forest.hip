@@ -838,6 +838,49 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(OCC))) void 
       }
     }
     int flags = 0;
+    if (use_tab && !per_pixel_row && a.scale == nullptr) {
+      // the common case (one parameter row per skewer, the column table, no
+      // per-pixel scale): the loop below with its uniform branches hoisted
+      // and the cell index in 32 bits (a z cell index is below nz); the same
+      // floating-point operations, so the same bits
+      const int kmin32 = (int)kmin;
+      const bool periodic = b.periodic;
+      const float* __restrict__ cd = ctab;
+      const float* __restrict__ ce = ctab + a.ctab_cap;
+      for (int i = tid; i < n; i += NT) {
+        const float ds = dsf[i];
+        float dl = -1000000.0f, eta = 0.0f;
+        if (i >= imin && i < imax) {
+          const double uz = fma((double)i, duz, uz0);
+          const double fz0 = floor(uz);
+          const float fz = (float)(uz - fz0);
+          const int iz = (int)fz0;
+          if (!periodic && (iz < 0 || iz + 1 >= nz)) {
+            dl = __int_as_float(0x7fc00000);
+            eta = dl;
+            flags |= 1;
+          } else {
+            const int k0 = iz - kmin32;
+            const float a0 = cd[k0], a1 = cd[k0 + 1];
+            const float e0 = ce[k0], e1 = ce[k0 + 1];
+            dl = fmaf(fz, a1 - a0, a0);
+            eta = fmaf(fz, e1 - e0, e0);
+          }
+        }
+        if (a.delta_l) a.delta_l[off + i] = dl;
+        if (a.eta_par) a.eta_par[off + i] = eta;
+        if (a.delta_s) a.delta_s[off + i] = ds;
+        if (!row_ok) {
+          a.flux[off + i] = __int_as_float(0x7fc00000);
+          flags |= 2;
+        } else {
+          const float gfield = dl + ds + prow.z * eta;
+          a.flux[off + i] = expf(-prow.x * expf(prow.y * prow.w * gfield));
+        }
+      }
+      if (flags && a.status) atomicOr(&a.status[s], flags);
+      continue;
+    }
     for (int i = tid; i < n; i += NT) {
       float ds = dsf[i];
       if (a.scale) ds *= a.scale[off + i];
