@@ -113,7 +113,8 @@ __device__ __forceinline__ void block_skewers(const smk_skewers& sk, int64_t pix
 
 // NF fields: 1 (delta), 2 (delta, eta_zz), 7 (delta and the six eta_ij) or
 // 10 (those and the three velocities); DM: the Gaussian stencil's radius in
-// cells (make_spectra -dmax), unused by trilinear interpolation
+// cells (make_spectra -dmax), or -1 for trilinear interpolation (one
+// instance per kind, so that neither carries the other's registers)
 template <int NF, int DM>
 __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
   const int64_t pix0 = (int64_t)blockIdx.x * blockDim.x;
@@ -174,11 +175,11 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
     double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     const float* fld[10] = {b.delta, radial ? b.eta[0] : b.eta[2], b.eta[1], b.eta[2], b.eta[3], b.eta[4],
                             b.eta[5], b.vel[0], b.vel[1], b.vel[2]};
-    if (a.interp != SMK_INTERP_TRILINEAR) {
+    if constexpr (DM >= 0) {
       // cell centres (i+0.5)D - L/2 (+R0 along Z), weights exp(-dr^2/(2 D^2)),
       // taken as the product of the three axes' factors (3 W exponentials,
       // not W^3)
-      constexpr int W = 2 * DM + 1;
+      constexpr int W = 2 * (DM >= 0 ? DM : 0) + 1;
       const double cx = sub_rn(mul_rn((double)ix + 0.5, D), a.hx);
       const double cy = sub_rn(mul_rn((double)iy + 0.5, D), a.hy);
       const double cz = add_rn(sub_rn(mul_rn((double)iz + 0.5, D), a.hz), b.r0);
@@ -1045,6 +1046,7 @@ static int gauss_radius(int interp) {
 template <int NF>
 static void launch_gather(int r, dim3 grid, hipStream_t st, const GatherArgs& a) {
   switch (r) {
+    case -1: hipLaunchKernelGGL((k_gather<NF, -1>), grid, dim3(256), 0, st, a); break;
     case 0: hipLaunchKernelGGL((k_gather<NF, 0>), grid, dim3(256), 0, st, a); break;
     case 1: hipLaunchKernelGGL((k_gather<NF, 1>), grid, dim3(256), 0, st, a); break;
     case 2: hipLaunchKernelGGL((k_gather<NF, 2>), grid, dim3(256), 0, st, a); break;
@@ -1103,7 +1105,7 @@ extern "C" int smk_forest_gather(const smk_skewers* sk, const smk_box_view* box,
     const uint64_t nblk = (uint64_t)(a.npix_total + 255) / 256;
     const bool radial = sk->geometry == SMK_GEOM_RADIAL;
     const int nf = box->eta[2] == nullptr ? 1 : (radial ? (v_par ? 10 : 7) : 2);
-    const int r = interp == SMK_INTERP_TRILINEAR ? 3 : gauss_radius(interp);
+    const int r = interp == SMK_INTERP_TRILINEAR ? -1 : gauss_radius(interp);
     const dim3 grid((uint32_t)nblk);
     if (nf == 1) launch_gather<1>(r, grid, (hipStream_t)stream, a);
     else if (nf == 2) launch_gather<2>(r, grid, (hipStream_t)stream, a);
