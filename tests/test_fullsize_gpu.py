@@ -198,3 +198,11 @@ def test_c4_fit_problem_all_sets(cuda):
     want = fto.adjoint_grad(th, *args)
     assert abs(val / fto.chi2(th, *args) - 1) < 1e-10
     assert np.allclose(g, want, rtol=1e-7, atol=1e-9 * np.abs(want).max())
+    # the forward-mode Jacobian of all 15 parameters at the same point
+    jfm, jbins, jdfm, jdbins = model.jacobian(th)
+    rf, rb, rdf, rdb = fto.jacobian(th, d64, e64, pset, growth, edges, 0.2, 1)
+    assert np.array_equal(jfm, gfm)
+    assert np.allclose(jbins[..., 2], rb[..., 2], rtol=1e-9, atol=1e-12 * rb[..., 2].max())
+    assert np.allclose(jdfm, rdf, rtol=1e-10, atol=1e-13 * np.abs(rdf).max())
+    for k in range(3):
+        assert np.allclose(jdbins[..., k], rdb[..., k], rtol=1e-8, atol=1e-11 * np.abs(rdb[..., k]).max()), k
