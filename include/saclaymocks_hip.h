@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define SMK_ABI_VERSION 6
+#define SMK_ABI_VERSION 7
 
 #define SMK_OK 0
 #define SMK_ERR_ARG (-1)         /* bad argument (null pointer, size, enum) */
@@ -400,11 +400,18 @@ int smk_reduce_by_set(const smk_skewers* sk, const double* per_skewer, int width
  *               complex doubles) of skewer s's length table, < 0 for none;
  *               the transform is then two FFTs on one register tile instead
  *               of three, in a kernel per transform size (same results to
- *               FP64 rounding) */
+ *               FP64 rounding)
+ *   pairs       NULL, or (with tables) device int32 [n_pairs][2]: skewers of
+ *               one forest length that share one DFT (of delta_F1 + i
+ *               delta_F2, split by the symmetry of real inputs; second -1 for
+ *               a skewer alone), sorted by transform size; every skewer with a
+ *               table in exactly one pair
+ *   pair_off    with pairs, HOST int64[4]: the pairs of M = 1024, 2048, 4096
+ *               at [pair_off[c], pair_off[c + 1]) */
 int smk_p1d_binned(const smk_skewers* sk, const double* flux, const double* fmean, double pixel, int regroup,
                    const double* edges, int nbins, int max_forest, double* partial, int32_t* status,
                    double* modes, const int64_t* modes_off, const double* tables, const int64_t* tab_off,
-                   void* stream);
+                   const int32_t* pairs, const int64_t* pair_off, void* stream);
 
 /* Length of the Bluestein table of a forest of n (regrouped) points, in
  * complex doubles: n + M (M the smallest power of two >= max(2n - 1, 1024))
@@ -427,7 +434,9 @@ int smk_p1d_tables(const int32_t* lens, const int64_t* offs, int n_lens, double*
  *         (dL/d mean_b divided by the bin count)
  *   modes NULL (recompute) or the modes saved by smk_p1d_binned at the same
  *         parameters; required (SMK_ERR_ARG) when max_forest / m > 8192
- *   tables, tab_off  as smk_p1d_binned
+ *   tables, tab_off, pairs, pair_off  as smk_p1d_binned (a pair's two
+ *         inverse transforms as one DFT of H1 + i H2, H the Hermitian spectra
+ *         whose DFTs are the real A1, A2)
  * out[s][7] = (sum_q A_q F'_q, sum_n A_q(n) dF_n/d(a, b, c) / m,
  *              sum_n dF_n/d(a, b, c)) with A_q = dL/d delta_F,q at fixed <F>;
  * the host adds the <F> chain rule per set.  No counterpart in the
@@ -436,7 +445,7 @@ int smk_p1d_adjoint(const smk_skewers* sk, const float* delta, const float* eta,
                     int64_t abcg_rows, const double* flux, const double* fmean, double pixel, int regroup,
                     const double* edges, int nbins, int max_forest, const double* modes,
                     const int64_t* modes_off, const double* gp, double* out, const double* tables,
-                    const int64_t* tab_off, void* stream);
+                    const int64_t* tab_off, const int32_t* pairs, const int64_t* pair_off, void* stream);
 
 /* Forward-mode derivatives for Gauss-Newton fits (no counterpart in the
  * reference, whose fit is derivative-free, fit_az.py:266-279).

@@ -322,6 +322,8 @@ struct P1dArgs {
   size_t a_bytes;        // byte offset of the edges in the dynamic LDS
   int nlds;              // longest forest of the LDS layout (min(nmax, FIT_NLDS))
   int chunk;             // modes per pass of the long-forest path
+  const int32_t* pairs;  // NULL, or [n_pairs][2] skewers of one length sharing a DFT (second -1: alone)
+  int pair_lo, pair_hi;  // the pairs of the launch's transform size
 };
 
 // skewer s has a table and a forest in the Bluestein range: the per-size
@@ -329,6 +331,12 @@ struct P1dArgs {
 __device__ __forceinline__ bool uses_table(const double2* tables, const int64_t* tab_off, int s, int nq,
                                            int nmax) {
   return tables != nullptr && tab_off[s] >= 0 && nq >= 1 && nq <= BLUE_NMAX && nq <= nmax;
+}
+
+// Z_j and Z_{N-j} of Z = DFT(u + i w), u and w real: U_j and W_j
+__device__ __forceinline__ void split_pair(double2 zj, double2 zn, double2& u, double2& w) {
+  u = make_double2(0.5 * (zj.x + zn.x), 0.5 * (zj.y - zn.y));
+  w = make_double2(0.5 * (zj.y + zn.y), 0.5 * (zn.x - zj.x));
 }
 
 // bin runs of one skewer: k_j increases with j, so each bin holds a run of
@@ -534,6 +542,92 @@ __global__ __launch_bounds__(M / 8) __attribute__((amdgpu_waves_per_eu(2))) void
   }
 }
 
+// LDS of k_p1d_blue_pair<M>: the line, the two forests' pk (NJ each,
+// NJ = M/4 + 1 >= N/2 + 1), the bins, then the edges and the bin runs
+__host__ __device__ constexpr size_t pair_fixed_bytes(int M) {
+  return blue_line_bytes_m(M) + (size_t)(M / 4 + 1) * 16 + (size_t)(((M / 4 + 1) + 1) & ~1) * 4;
+}
+
+// The P1D of two forests of one length with one Bluestein DFT: the DFT of
+// delta_F1 + i delta_F2 (all N outputs into the line), split by the
+// symmetry of real inputs into X1 and X2, then each forest's bins.  The
+// pairs [pair_lo, pair_hi) are those of M's size class.
+template <int M>
+__global__ __launch_bounds__(M / 8) __attribute__((amdgpu_waves_per_eu(2))) void k_p1d_blue_pair(P1dArgs a) {
+  constexpr int NT = M / 8;
+  constexpr int NJ = M / 4 + 1;
+  extern __shared__ double dyn[];
+  double2* line = reinterpret_cast<double2*>(dyn);
+  double* pk1 = reinterpret_cast<double*>(reinterpret_cast<char*>(dyn) + blue_line_bytes_m(M));
+  double* pk2 = pk1 + NJ;
+  int* bins = reinterpret_cast<int*>(pk2 + NJ);
+  double* ledges = reinterpret_cast<double*>(bins + ((NJ + 1) & ~1));
+  int* bfirst = reinterpret_cast<int*>(ledges + a.nb + 1);
+  int* blast = bfirst + a.nb;
+  for (int i = threadIdx.x; i <= a.nb; i += NT) ledges[i] = a.edges[i];
+  const smk_skewers& sk = a.sk;
+  const int m = a.regroup;
+  const double px = a.pixel * m;
+  const double k_ny = M_PI / px;
+  for (int w = a.pair_lo + (int)blockIdx.x; w < a.pair_hi; w += gridDim.x) {
+    // each member must still take this kernel (a table of this size class,
+    // the second of the same length and table); a member that does not is
+    // left to the generic kernel, and its partner goes alone
+    int s1 = a.pairs[2 * w], s2 = a.pairs[2 * w + 1];
+    const int nq1 = (sk.imax[s1] - sk.imin[s1]) / m;
+    const bool v1 = uses_table(a.tables, a.tab_off, s1, nq1, a.nmax) && blue_m_of(nq1) == M;
+    bool v2 = s2 >= 0;
+    if (v2) {
+      const int nq2 = (sk.imax[s2] - sk.imin[s2]) / m;
+      v2 = uses_table(a.tables, a.tab_off, s2, nq2, a.nmax) && blue_m_of(nq2) == M &&
+           (!v1 || (nq2 == nq1 && a.tab_off[s2] == a.tab_off[s1]));
+    }
+    if (!v1) {
+      s1 = v2 ? s2 : -1;
+      s2 = -1;
+    } else if (!v2) {
+      s2 = -1;
+    }
+    if (s1 < 0) continue;
+    const int nq = (sk.imax[s1] - sk.imin[s1]) / m;
+    __syncthreads();
+    const double inv1 = 1.0 / a.fmean[sk.pset[s1]];
+    const double inv2 = s2 >= 0 ? 1.0 / a.fmean[sk.pset[s2]] : 0.0;
+    const int64_t o1 = sk.off[s1] + sk.imin[s1];
+    const int64_t o2 = s2 >= 0 ? sk.off[s2] + sk.imin[s2] : 0;
+    const int nj = nq / 2 + 1;
+    const double2* T = a.tables + a.tab_off[s1];
+    f64::bluestein_tab<M, 8>(
+        nq, nq,
+        [&](int q) {
+          double acc1 = 0.0, acc2 = 0.0;
+          for (int r = 0; r < m; ++r) acc1 += a.flux[o1 + (int64_t)q * m + r];
+          if (s2 >= 0)
+            for (int r = 0; r < m; ++r) acc2 += a.flux[o2 + (int64_t)q * m + r];
+          return make_double2((acc1 / m) * inv1 - 1.0, s2 >= 0 ? (acc2 / m) * inv2 - 1.0 : 0.0);
+        },
+        [&](int j, double2 Z) { line[j] = Z; }, line, a.tw, BLUE_MMAX, T, T + nq);
+    __syncthreads();
+    double2* xj1 = a.xj ? a.xj + a.xoff[s1] : nullptr;
+    double2* xj2 = (a.xj && s2 >= 0) ? a.xj + a.xoff[s2] : nullptr;
+    for (int j = threadIdx.x; j < nj; j += NT) {
+      double2 X1, X2;
+      split_pair(line[j], line[j == 0 ? 0 : nq - j], X1, X2);
+      pk1[j] = (X1.x * X1.x + X1.y * X1.y) * px / nq;
+      pk2[j] = (X2.x * X2.x + X2.y * X2.y) * px / nq;
+      bins[j] = find_bin(ledges, a.nb, kfreq(j, nq, k_ny));
+      if (xj1) xj1[j] = X1;
+      if (xj2) xj2[j] = X2;
+    }
+    __syncthreads();
+    bin_runs<NT>(pk1, bins, nj, nq, k_ny, a.nb, bfirst, blast, a.partial + (int64_t)s1 * a.nb * 4);
+    if (s2 >= 0) {
+      __syncthreads();
+      bin_runs<NT>(pk2, bins, nj, nq, k_ny, a.nb, bfirst, blast, a.partial + (int64_t)s2 * a.nb * 4);
+    }
+  }
+}
+
 // the Bluestein table of every length lens[d] at tables + offs[d]
 // (bluestein_table layout); lengths outside [1, BLUE_NMAX] are skipped
 __global__ __launch_bounds__(FIT_THREADS) void k_p1d_tables(const int32_t* __restrict__ lens,
@@ -577,6 +671,8 @@ struct AdjArgs {
   size_t cjb_off;        // generic kernel: byte offset of cj for the Bluestein forests
   size_t e_off;          // generic kernel: byte offset of the edges
   int nlds;              // longest forest of the LDS layout (min(nmax, FIT_NLDS))
+  const int32_t* pairs;  // NULL, or the pairs of smk_p1d_binned
+  int pair_lo, pair_hi;
 };
 
 // res[k] = the block sum of v[k], k < K, in one pass: a shuffle tree per
@@ -868,6 +964,139 @@ __global__ __launch_bounds__(M / 8) __attribute__((amdgpu_waves_per_eu(2))) void
   }
 }
 
+// The adjoint of two forests of one length with one Bluestein DFT per
+// transform.  A_q = Re sum_{j < N/2+1} c_j W^{jq} is the DFT of the
+// Hermitian spectrum H (H_0 = Re c_0, H_j = c_j / 2 and H_{N-j} = conj(c_j)
+// / 2 for 0 < j < N/2, H_{N/2} = Re c_{N/2} for even N), a real sequence, so
+// the DFT of H1 + i H2 is A1 + i A2.  Without saved modes, X1 and X2 come from
+// one DFT of delta_F1 + i delta_F2 as in k_p1d_blue_pair.  LDS: x1, x2 (M/2
+// doubles each), the line (c1 and c2 over its start), the edges.
+template <int M>
+__global__ __launch_bounds__(M / 8) __attribute__((amdgpu_waves_per_eu(2))) void k_p1d_adjoint_blue_pair(
+    AdjArgs a) {
+  constexpr int NT = M / 8;
+  extern __shared__ double dyn[];
+  double* x1 = dyn;
+  double* x2 = dyn + M / 2;
+  double2* line = reinterpret_cast<double2*>(reinterpret_cast<char*>(dyn) + 2 * adj_blue_x_bytes(M));
+  double* ledges = reinterpret_cast<double*>(reinterpret_cast<char*>(dyn) + 2 * adj_blue_x_bytes(M) +
+                                             blue_line_bytes_m(M));
+  __shared__ double red[7 * (NT / 64)];
+  for (int i = threadIdx.x; i <= a.nb; i += NT) ledges[i] = a.edges[i];
+  const smk_skewers& sk = a.sk;
+  const int m = a.regroup;
+  const double px = a.pixel * m;
+  const double k_ny = M_PI / px;
+  for (int w = a.pair_lo + (int)blockIdx.x; w < a.pair_hi; w += gridDim.x) {
+    // as k_p1d_blue_pair: members that no longer take this kernel are left out
+    int s1 = a.pairs[2 * w], s2 = a.pairs[2 * w + 1];
+    const int nq1 = (sk.imax[s1] - sk.imin[s1]) / m;
+    const bool v1 = uses_table(a.tables, a.tab_off, s1, nq1, a.nmax) && blue_m_of(nq1) == M;
+    bool v2 = s2 >= 0;
+    if (v2) {
+      const int nq2 = (sk.imax[s2] - sk.imin[s2]) / m;
+      v2 = uses_table(a.tables, a.tab_off, s2, nq2, a.nmax) && blue_m_of(nq2) == M &&
+           (!v1 || (nq2 == nq1 && a.tab_off[s2] == a.tab_off[s1]));
+    }
+    if (!v1) {
+      s1 = v2 ? s2 : -1;
+      s2 = -1;
+    } else if (!v2) {
+      s2 = -1;
+    }
+    if (s1 < 0) continue;
+    const bool two = s2 >= 0;
+    const int nq = (sk.imax[s1] - sk.imin[s1]) / m;
+    const int nj = nq / 2 + 1;
+    const int set1 = sk.pset[s1], set2 = two ? sk.pset[s2] : set1;
+    const double fm1 = a.fmean[set1], fm2 = a.fmean[set2];
+    __syncthreads();
+    load_delta_f<NT>(a.flux, sk.off[s1], sk.imin[s1], sk.imax[s1], m, 1.0 / fm1, x1);
+    if (two) load_delta_f<NT>(a.flux, sk.off[s2], sk.imin[s2], sk.imax[s2], m, 1.0 / fm2, x2);
+    __syncthreads();
+    const double* gp1 = a.gp + (int64_t)set1 * a.nb;
+    const double* gp2 = a.gp + (int64_t)set2 * a.nb;
+    const double2* T = a.tables + a.tab_off[s1];
+    double2* c1 = line;        // c_j = g_b(j) 2 px / N conj(X_j), j < nj
+    double2* c2 = line + nj;
+    auto coef = [&](const double* gps, int b) { return b >= 0 ? gps[b] * 2.0 * px / nq : 0.0; };
+    if (a.xj) {
+      const double2* xs1 = a.xj + a.xoff[s1];
+      const double2* xs2 = two ? a.xj + a.xoff[s2] : nullptr;
+      for (int j = threadIdx.x; j < nj; j += NT) {
+        const int b = find_bin(ledges, a.nb, kfreq(j, nq, k_ny));
+        const double g1 = coef(gp1, b);
+        const double2 X1 = xs1[j];
+        c1[j] = make_double2(g1 * X1.x, -g1 * X1.y);
+        double2 cc = make_double2(0.0, 0.0);
+        if (two) {
+          const double g2 = coef(gp2, b);
+          const double2 X2 = xs2[j];
+          cc = make_double2(g2 * X2.x, -g2 * X2.y);
+        }
+        c2[j] = cc;
+      }
+    } else {
+      f64::bluestein_tab<M, 8>(nq, nq, [&](int n) { return make_double2(x1[n], two ? x2[n] : 0.0); },
+                               [&](int j, double2 Z) { line[j] = Z; }, line, a.tw, BLUE_MMAX, T, T + nq);
+      __syncthreads();
+      // the split reads line[j] and line[N - j]: into registers first, then
+      // the block writes c1, c2 over the line's start
+      double2 cr1[(M / 4 + 1 + NT - 1) / NT], cr2[(M / 4 + 1 + NT - 1) / NT];
+#pragma unroll
+      for (int t = 0; t < (M / 4 + 1 + NT - 1) / NT; ++t) {
+        const int j = (int)threadIdx.x + t * NT;
+        cr1[t] = cr2[t] = make_double2(0.0, 0.0);
+        if (j < nj) {
+          double2 X1, X2;
+          split_pair(line[j], line[j == 0 ? 0 : nq - j], X1, X2);
+          const int b = find_bin(ledges, a.nb, kfreq(j, nq, k_ny));
+          const double g1 = coef(gp1, b), g2 = two ? coef(gp2, b) : 0.0;
+          cr1[t] = make_double2(g1 * X1.x, -g1 * X1.y);
+          cr2[t] = make_double2(g2 * X2.x, -g2 * X2.y);
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int t = 0; t < (M / 4 + 1 + NT - 1) / NT; ++t) {
+        const int j = (int)threadIdx.x + t * NT;
+        if (j < nj) {
+          c1[j] = cr1[t];
+          c2[j] = cr2[t];
+        }
+      }
+    }
+    __syncthreads();
+    // u_n = H1_n + i H2_n, n < N (the Hermitian completion above)
+    auto herm = [&](const double2* c, int n) -> double2 {
+      if (n == 0) return make_double2(c[0].x, 0.0);
+      if (2 * n == nq) return make_double2(c[n].x, 0.0);
+      if (2 * n < nq) return make_double2(0.5 * c[n].x, 0.5 * c[n].y);
+      const double2 v = c[nq - n];
+      return make_double2(0.5 * v.x, -0.5 * v.y);
+    };
+    double saf1 = 0.0, saf2 = 0.0;
+    f64::bluestein_tab<M, 8>(
+        nq, nq,
+        [&](int n) {
+          const double2 h1 = herm(c1, n), h2 = herm(c2, n);
+          return make_double2(h1.x - h2.y, h1.y + h2.x);   // H1 + i H2
+        },
+        [&](int q, double2 Z) {
+          saf1 += Z.x * (x1[q] + 1.0) * fm1;
+          x1[q] = Z.x;
+          if (two) {
+            saf2 += Z.y * (x2[q] + 1.0) * fm2;
+            x2[q] = Z.y;
+          }
+        },
+        line, a.tw, BLUE_MMAX, T, T + nq);
+    __syncthreads();
+    adj_fgpa<NT>(a, s1, set1, sk.imin[s1], sk.imax[s1], nq, saf1, x1, red, a.out + (int64_t)s1 * 7);
+    if (two) adj_fgpa<NT>(a, s2, set2, sk.imin[s2], sk.imax[s2], nq, saf2, x2, red, a.out + (int64_t)s2 * 7);
+  }
+}
+
 // ------------------------------------------------------------- Jacobian
 // Forward-mode derivatives of the fit's observables, for Gauss-Newton: with
 // theta = (a, b, c) of the skewer's set, F' the regrouped flux and
@@ -944,12 +1173,6 @@ __global__ __launch_bounds__(FIT_THREADS) void k_fgpa_flux_jac(smk_skewers sk, c
     }
     reduce_k<FIT_THREADS, 4>(v, red, sums + (int64_t)s * 4);
   }
-}
-
-// Z_j and Z_{N-j} of Z = DFT(u + i w), u and w real: U_j and W_j
-__device__ __forceinline__ void split_pair(double2 zj, double2 zn, double2& u, double2& w) {
-  u = make_double2(0.5 * (zj.x + zn.x), 0.5 * (zj.y - zn.y));
-  w = make_double2(0.5 * (zj.y + zn.y), 0.5 * (zn.x - zj.x));
 }
 
 // bin runs as bin_runs, seven sums per bin: count, sum k, sum P, sum P^2 and
@@ -1200,6 +1423,19 @@ int grid_for(int n) { return n < 65536 ? (n > 0 ? n : 1) : 65536; }
   } while (0)
 
 template <int M>
+int launch_p1d_blue_pair(P1dArgs a, int lo, int hi, size_t tail, hipStream_t st) {
+  if (hi <= lo) return SMK_OK;
+  a.pair_lo = lo;
+  a.pair_hi = hi;
+  const size_t lds = pair_fixed_bytes(M) + tail;
+  SMK_HIP_CHECK(hipFuncSetAttribute((const void*)k_p1d_blue_pair<M>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds));
+  hipLaunchKernelGGL(k_p1d_blue_pair<M>, dim3(grid_for(hi - lo)), dim3(M / 8), lds, st, a);
+  SMK_LAUNCH_CHECK();
+  return SMK_OK;
+}
+
+template <int M>
 int launch_p1d_blue(const P1dArgs& a, dim3 grid, size_t tail, hipStream_t st) {
   const size_t lds = blue_line_bytes_m(M) + tail;
   SMK_HIP_CHECK(hipFuncSetAttribute((const void*)k_p1d_blue<M>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1215,6 +1451,19 @@ int launch_adj_blue(const AdjArgs& a, dim3 grid, size_t tail, hipStream_t st) {
   SMK_HIP_CHECK(hipFuncSetAttribute((const void*)k_p1d_adjoint_blue<M>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   hipLaunchKernelGGL(k_p1d_adjoint_blue<M>, grid, dim3(M / 8), lds, st, a);
+  SMK_LAUNCH_CHECK();
+  return SMK_OK;
+}
+
+template <int M>
+int launch_adj_blue_pair(AdjArgs a, int lo, int hi, size_t tail, hipStream_t st) {
+  if (hi <= lo) return SMK_OK;
+  a.pair_lo = lo;
+  a.pair_hi = hi;
+  const size_t lds = 2 * adj_blue_x_bytes(M) + blue_line_bytes_m(M) + tail;
+  SMK_HIP_CHECK(hipFuncSetAttribute((const void*)k_p1d_adjoint_blue_pair<M>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(k_p1d_adjoint_blue_pair<M>, dim3(grid_for(hi - lo)), dim3(M / 8), lds, st, a);
   SMK_LAUNCH_CHECK();
   return SMK_OK;
 }
@@ -1303,7 +1552,8 @@ extern "C" int smk_p1d_tables(const int32_t* lens, const int64_t* offs, int n_le
 extern "C" int smk_p1d_binned(const smk_skewers* sk, const double* flux, const double* fmean, double pixel,
                               int regroup, const double* edges, int nbins, int max_forest, double* partial,
                               int32_t* status, double* modes, const int64_t* modes_off, const double* tables,
-                              const int64_t* tab_off, void* stream) {
+                              const int64_t* tab_off, const int32_t* pairs, const int64_t* pair_off,
+                              void* stream) {
   SMK_REQUIRE(sk != nullptr && flux != nullptr && fmean != nullptr && edges != nullptr && partial != nullptr,
               SMK_ERR_ARG, "NULL argument");
   SMK_REQUIRE(sk->pset != nullptr, SMK_ERR_ARG, "the fit kernels need skewers.pset");
@@ -1321,8 +1571,11 @@ extern "C" int smk_p1d_binned(const smk_skewers* sk, const double* flux, const d
   SMK_REQUIRE(tw != nullptr, SMK_ERR_HIP, "FP64 twiddle table: %s", hipGetErrorString(te));
   const size_t a_bytes = p1d_a_bytes(nlds);
   const int64_t chunk = ((int64_t)a_bytes - 16 * FIT_XC) / 8;   // modes per pass after the staged input
-  P1dArgs a{*sk,      flux,      fmean, pixel, regroup, edges, nbins, partial, status, nmax, (double2*)modes,
-            modes_off, tw, (const double2*)tables, tab_off, a_bytes, nlds, (int)(chunk > 1 ? chunk : 1)};
+  SMK_REQUIRE(pairs == nullptr || (tables != nullptr && pair_off != nullptr), SMK_ERR_ARG,
+              "pairs need tables and pair_off");
+  P1dArgs a{*sk,      flux,      fmean, pixel,   regroup, edges, nbins, partial, status, nmax, (double2*)modes,
+            modes_off, tw,       (const double2*)tables, tab_off, a_bytes, nlds, (int)(chunk > 1 ? chunk : 1),
+            pairs,     0,        0};
   const dim3 grid(grid_for(sk->n));
   hipStream_t st = (hipStream_t)stream;
   const size_t lds = p1d_lds_bytes(nlds, nbins);
@@ -1335,9 +1588,19 @@ extern "C" int smk_p1d_binned(const smk_skewers* sk, const double* flux, const d
     SMK_LAUNCH_CHECK();
     const int nblue = nmax < BLUE_NMAX ? nmax : BLUE_NMAX;
     const size_t tail = (size_t)(nbins + 1) * 8 + (size_t)nbins * 8;
-    SMK_CALL(launch_p1d_blue<1024>(a, grid, tail, st));
-    if (blue_m_of(nblue) >= 2048) SMK_CALL(launch_p1d_blue<2048>(a, grid, tail, st));
-    if (blue_m_of(nblue) >= 4096) SMK_CALL(launch_p1d_blue<4096>(a, grid, tail, st));
+    if (pairs) {
+      // pair_off (host): the pairs of M = 1024, 2048, 4096 at [pair_off[c], pair_off[c + 1])
+      for (int c = 0; c < 3; ++c)
+        SMK_REQUIRE(pair_off[c] >= 0 && pair_off[c] <= pair_off[c + 1] && pair_off[c + 1] <= INT32_MAX,
+                    SMK_ERR_ARG, "bad pair_off");
+      SMK_CALL(launch_p1d_blue_pair<1024>(a, (int)pair_off[0], (int)pair_off[1], tail, st));
+      SMK_CALL(launch_p1d_blue_pair<2048>(a, (int)pair_off[1], (int)pair_off[2], tail, st));
+      SMK_CALL(launch_p1d_blue_pair<4096>(a, (int)pair_off[2], (int)pair_off[3], tail, st));
+    } else {
+      SMK_CALL(launch_p1d_blue<1024>(a, grid, tail, st));
+      if (blue_m_of(nblue) >= 2048) SMK_CALL(launch_p1d_blue<2048>(a, grid, tail, st));
+      if (blue_m_of(nblue) >= 4096) SMK_CALL(launch_p1d_blue<4096>(a, grid, tail, st));
+    }
   } else {
     SMK_HIP_CHECK(hipFuncSetAttribute((const void*)k_p1d<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)lds));
@@ -1351,7 +1614,8 @@ extern "C" int smk_p1d_adjoint(const smk_skewers* sk, const float* delta, const 
                                int64_t abcg_rows, const double* flux, const double* fmean, double pixel,
                                int regroup, const double* edges, int nbins, int max_forest, const double* modes,
                                const int64_t* modes_off, const double* gp, double* out, const double* tables,
-                               const int64_t* tab_off, void* stream) {
+                               const int64_t* tab_off, const int32_t* pairs, const int64_t* pair_off,
+                               void* stream) {
   SMK_REQUIRE(sk != nullptr && delta != nullptr && abcg != nullptr && flux != nullptr && fmean != nullptr &&
                   edges != nullptr && gp != nullptr && out != nullptr,
               SMK_ERR_ARG, "NULL argument");
@@ -1375,7 +1639,10 @@ extern "C" int smk_p1d_adjoint(const smk_skewers* sk, const float* delta, const 
   const size_t cjb_off = adj_x_bytes(nlds < BLUE_NMAX ? nlds : BLUE_NMAX);
   AdjArgs a{*sk,      delta, eta,          abcg,  abcg_rows, flux, fmean, pixel, regroup,
             edges,    nbins, gp,           out,   nmax,      (const double2*)modes,
-            modes_off, tw,   (const double2*)tables, tab_off, adj_line_off(nlds), cjb_off, e_off, nlds};
+            modes_off, tw,   (const double2*)tables, tab_off, adj_line_off(nlds), cjb_off, e_off, nlds,
+            pairs,     0,    0};
+  SMK_REQUIRE(pairs == nullptr || (tables != nullptr && pair_off != nullptr), SMK_ERR_ARG,
+              "pairs need tables and pair_off");
   const dim3 grid(grid_for(sk->n));
   hipStream_t st = (hipStream_t)stream;
   // + the edges, and for forests longer than the layout the bin spans and
@@ -1388,9 +1655,18 @@ extern "C" int smk_p1d_adjoint(const smk_skewers* sk, const float* delta, const 
     SMK_LAUNCH_CHECK();
     const int nblue = nmax < BLUE_NMAX ? nmax : BLUE_NMAX;
     const size_t tail = (size_t)(nbins + 1) * 8;
-    SMK_CALL(launch_adj_blue<1024>(a, grid, tail, st));
-    if (blue_m_of(nblue) >= 2048) SMK_CALL(launch_adj_blue<2048>(a, grid, tail, st));
-    if (blue_m_of(nblue) >= 4096) SMK_CALL(launch_adj_blue<4096>(a, grid, tail, st));
+    if (pairs) {
+      for (int c = 0; c < 3; ++c)
+        SMK_REQUIRE(pair_off[c] >= 0 && pair_off[c] <= pair_off[c + 1] && pair_off[c + 1] <= INT32_MAX,
+                    SMK_ERR_ARG, "bad pair_off");
+      SMK_CALL(launch_adj_blue_pair<1024>(a, (int)pair_off[0], (int)pair_off[1], tail, st));
+      SMK_CALL(launch_adj_blue_pair<2048>(a, (int)pair_off[1], (int)pair_off[2], tail, st));
+      SMK_CALL(launch_adj_blue_pair<4096>(a, (int)pair_off[2], (int)pair_off[3], tail, st));
+    } else {
+      SMK_CALL(launch_adj_blue<1024>(a, grid, tail, st));
+      if (blue_m_of(nblue) >= 2048) SMK_CALL(launch_adj_blue<2048>(a, grid, tail, st));
+      if (blue_m_of(nblue) >= 4096) SMK_CALL(launch_adj_blue<4096>(a, grid, tail, st));
+    }
   } else {
     SMK_HIP_CHECK(hipFuncSetAttribute((const void*)k_p1d_adjoint<false>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

@@ -34,7 +34,8 @@ class P1DFitModel:
     # (the LDS layout of smk_p1d_adjoint); longer ones need the saved modes
     LONG_FOREST = 8192
 
-    def __init__(self, batch, delta, eta, growth, edges, pixel=0.2, regroup=1, save_modes=True, use_tables=True):
+    def __init__(self, batch, delta, eta, growth, edges, pixel=0.2, regroup=1, save_modes=True, use_tables=True,
+                 pair_forests=True):
         if batch.pset is None:
             raise ValueError("the batch needs a parameter set (pset) per skewer")
         self.batch = batch
@@ -75,6 +76,8 @@ class P1DFitModel:
         self.modes = torch.empty(2 * int(nj.sum()) + 2, dtype=torch.float64, device=dev) if save_modes else None
         self.modes_off = torch.from_numpy(moff).to(dev) if save_modes else None
         self.tables, self.tab_off = self._bluestein_tables(nf // self.regroup) if use_tables else (None, None)
+        self.pairs, self.pair_off = (self._forest_pairs(nf // self.regroup, _sync_host(self.tab_off))
+                                     if use_tables and pair_forests else (None, None))
         self.n_eval = 0
         self.n_grad = 0
         self.n_jac = 0
@@ -102,6 +105,33 @@ class P1DFitModel:
                                         _lib.stream_ptr()), "smk_p1d_tables")
             torch.cuda.current_stream().synchronize()
         return tables, torch.from_numpy(tab_sk).to(self.dev)
+
+    @staticmethod
+    def _pair_list(nq, tab_sk):
+        """Skewers with a table paired by forest length (two real DFTs in one
+        complex one); the last of an odd group alone (-1).  Returns the
+        [n_pairs][2] int32 list, sorted by transform size, and the host
+        offsets of the sizes M = 1024, 2048, 4096."""
+        idx = np.nonzero(tab_sk >= 0)[0]
+        idx = idx[np.lexsort((idx, nq[idx]))]
+        out, cls = [], []
+        for g in np.split(idx, np.nonzero(np.diff(nq[idx]))[0] + 1) if len(idx) else []:
+            n = int(nq[g[0]])
+            c = 0 if 2 * n - 1 <= 1024 else (1 if 2 * n - 1 <= 2048 else 2)
+            for i in range(0, len(g), 2):
+                out.append((int(g[i]), int(g[i + 1]) if i + 1 < len(g) else -1))
+                cls.append(c)
+        pairs = np.array(out, dtype=np.int32).reshape(-1, 2)
+        cls = np.array(cls, dtype=np.int64)
+        off = np.zeros(4, dtype=np.int64)
+        for c in range(3):
+            off[c + 1] = off[c] + int(np.sum(cls == c))
+        return pairs, off
+
+    def _forest_pairs(self, nq, tab_sk):
+        pairs, off = self._pair_list(np.asarray(nq), np.asarray(tab_sk))
+        dev_pairs = torch.from_numpy(pairs if len(pairs) else np.full((1, 2), -1, np.int32)).to(self.dev)
+        return dev_pairs, (ctypes.c_int64 * 4)(*[int(x) for x in off])
 
     def set_edges(self, edges):
         """New k bin edges (h/Mpc, increasing) for the following evaluations."""
@@ -145,7 +175,8 @@ class P1DFitModel:
                                     self.pixel, self.regroup, _lib.ptr(self.edges), self.nb, self.max_forest,
                                     _lib.ptr(self.partial), _lib.ptr(self.status), _lib.ptr(modes),
                                     _lib.ptr(self.modes_off if modes is not None else None), _lib.ptr(self.tables),
-                                    _lib.ptr(self.tab_off), s), "smk_p1d_binned")
+                                    _lib.ptr(self.tab_off), _lib.ptr(self.pairs), self.pair_off, s),
+                   "smk_p1d_binned")
         _lib.check(L.smk_reduce_by_set(ctypes.byref(self.sk), _lib.ptr(self.partial), self.nb * 4, self.n_sets,
                                        _lib.ptr(self.bins_set), s), "smk_reduce_by_set")
         return _sync_host(self.bins_set)
@@ -259,7 +290,8 @@ class P1DFitModel:
                                      _lib.ptr(abcg), abcg.shape[0], _lib.ptr(self.flux), _lib.ptr(self.fmean_dev),
                                      self.pixel, self.regroup, _lib.ptr(self.edges), self.nb, self.max_forest,
                                      _lib.ptr(self.modes), _lib.ptr(self.modes_off), _lib.ptr(gp_dev),
-                                     _lib.ptr(self.adj), _lib.ptr(self.tables), _lib.ptr(self.tab_off), s),
+                                     _lib.ptr(self.adj), _lib.ptr(self.tables), _lib.ptr(self.tab_off),
+                                     _lib.ptr(self.pairs), self.pair_off, s),
                    "smk_p1d_adjoint")
         _lib.check(L.smk_reduce_by_set(ctypes.byref(self.sk), _lib.ptr(self.adj), 7, self.n_sets,
                                        _lib.ptr(self.adj_set), s), "smk_reduce_by_set")

@@ -8,7 +8,8 @@ from oracle import fit_oracle as fto
 pytestmark = pytest.mark.gpu
 
 
-def _problem(cuda, seed=0, ns=3, per=40, m=1, lo=200, hi=900, save_modes=True, use_tables=True, lens=None):
+def _problem(cuda, seed=0, ns=3, per=40, m=1, lo=200, hi=900, save_modes=True, use_tables=True, lens=None,
+             pairs=True):
     from saclaymocks_amd import fit, forest
     rng = np.random.default_rng(seed)
     lens = rng.integers(lo, hi, ns * per) if lens is None else np.asarray(lens)
@@ -27,7 +28,7 @@ def _problem(cuda, seed=0, ns=3, per=40, m=1, lo=200, hi=900, save_modes=True, u
     d = torch.from_numpy(np.concatenate(deltas)).to(cuda)
     e = torch.from_numpy(np.concatenate(etas)).to(cuda)
     model = fit.P1DFitModel(b, d, e, growth, edges, pixel=0.2, regroup=m, save_modes=save_modes,
-                            use_tables=use_tables)
+                            use_tables=use_tables, pair_forests=pairs)
     theta_true = np.array([[0.2, 1.4, 0.9], [0.3, 1.2, 1.1], [0.15, 1.6, 0.7], [0.25, 1.5, 1.0],
                            [0.35, 1.3, 0.8]])[:ns].reshape(-1)
     d64 = [x.astype(np.float64) for x in deltas]
@@ -52,10 +53,15 @@ def _check_forward(model, args, th):
     assert model.status.sum().item() == 0
 
 
-@pytest.mark.parametrize("use_tables", [True, False])
+# tables with forests paired per DFT (the default), tables one forest per
+# DFT, no tables
+PATHS = [(True, True), (True, False), (False, False)]
+
+
+@pytest.mark.parametrize("use_tables, pairs", PATHS)
 @pytest.mark.parametrize("m, ns", [(1, 3), (2, 3), (1, 5)])
-def test_forward_parity(cuda, m, ns, use_tables):
-    model, args, theta = _problem(cuda, m=m, ns=ns, use_tables=use_tables)
+def test_forward_parity(cuda, m, ns, use_tables, pairs):
+    model, args, theta = _problem(cuda, m=m, ns=ns, use_tables=use_tables, pairs=pairs)
     _check_forward(model, args, theta * np.array([1.1, 0.95, 1.05] * ns))
 
 
@@ -65,10 +71,11 @@ def test_forward_parity(cuda, m, ns, use_tables):
 EDGE_LENGTHS = [0, 1, 2, 3, 7, 511, 512, 513, 1024, 1025, 1500, 2047, 2048, 2049, 3001, 700, 300, 900]
 
 
-@pytest.mark.parametrize("use_tables", [True, False])
-def test_forward_parity_edge_lengths(cuda, use_tables):
+@pytest.mark.parametrize("use_tables, pairs", PATHS)
+def test_forward_parity_edge_lengths(cuda, use_tables, pairs):
     lens = EDGE_LENGTHS * 2
-    model, args, theta = _problem(cuda, ns=2, per=len(EDGE_LENGTHS), lens=lens, use_tables=use_tables)
+    model, args, theta = _problem(cuda, ns=2, per=len(EDGE_LENGTHS), lens=lens, use_tables=use_tables,
+                                  pairs=pairs)
     if use_tables:
         # skewers without a table take the direct DFT of the generic kernel
         off = model.tab_off.clone()
@@ -114,10 +121,11 @@ def test_long_forest_adjoint_needs_modes(cuda):
         model.grad_last()
 
 
-@pytest.mark.parametrize("use_tables", [True, False])
+@pytest.mark.parametrize("use_tables, pairs", PATHS)
 @pytest.mark.parametrize("save_modes", [True, False])
-def test_adjoint_gradient_parity(cuda, save_modes, use_tables):
-    model, args, theta = _problem(cuda, save_modes=save_modes, lo=100, hi=2500, use_tables=use_tables)
+def test_adjoint_gradient_parity(cuda, save_modes, use_tables, pairs):
+    model, args, theta = _problem(cuda, save_modes=save_modes, lo=100, hi=2500, use_tables=use_tables,
+                                  pairs=pairs)
     th = theta * np.array([1.2, 0.9, 1.1] * 3)
     val, g = model.value_and_grad(th)
     want = fto.adjoint_grad(th, *args)
