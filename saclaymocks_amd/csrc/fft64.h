@@ -137,16 +137,37 @@ __host__ __device__ constexpr int n_stages(int M, int E) {
 __device__ __forceinline__ int slot(int pos) { return pos + (pos >> 5); }
 __host__ __device__ constexpr int line_len(int M) { return M + M / 32 + 1; }
 
-// Stockham stage: butterfly j = p + P b takes points j + r M/R (register
-// slots b + r B), twiddles W_M^(r (j % Ns) M/(Ns R)), outputs to points
-// (j / Ns) Ns R + j % Ns + r Ns (the last stage keeps them in registers:
-// slot b + r B holds point j + r M/R).  twM: W_TW^k, k < TW, TW >= M.
-template <int M, int E, int R, int Ns, bool INV, bool LAST>
-__device__ __forceinline__ void stage(double2 (&v)[E], double2* __restrict__ line, int p,
-                                      const double2* __restrict__ twM, int TW) {
+// Twiddles of a Stockham stage for this thread's butterflies j = p + P b:
+// w[b][r] = W_M^(r (j % Ns) M/(Ns R)), r >= 1 (conjugated for the inverse).
+// twM: W_TW^k, k < TW, TW >= M.
+template <int M, int E, int R, int Ns, bool INV>
+__device__ __forceinline__ void stage_twiddles(double2 (&w)[E / R][R], int p, const double2* __restrict__ twM,
+                                               int TW) {
   constexpr int P = M / E;
   constexpr int B = E / R;
   const int tws = TW / M;  // stride of W_M in the table
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
+    const int jj = (p + P * b) & (Ns - 1);
+    w[b][0] = make_double2(1.0, 0.0);
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      double2 t = twM[(int64_t)r * jj * (M / (Ns * R)) * tws];
+      if (INV) t.y = -t.y;
+      w[b][r] = t;
+    }
+  }
+}
+
+// Stockham stage: butterfly j = p + P b takes points j + r M/R (register
+// slots b + r B), multiplies point r by w[b][r] (stage_twiddles; none for
+// Ns = 1), outputs to points (j / Ns) Ns R + j % Ns + r Ns (the last stage
+// keeps them in registers: slot b + r B holds point j + r M/R).
+template <int M, int E, int R, int Ns, bool INV, bool LAST>
+__device__ __forceinline__ void stage(double2 (&v)[E], double2* __restrict__ line, int p,
+                                      const double2 (&w)[E / R][R]) {
+  constexpr int P = M / E;
+  constexpr int B = E / R;
 #pragma unroll
   for (int b = 0; b < B; ++b) {
     const int j = p + P * b;
@@ -154,13 +175,8 @@ __device__ __forceinline__ void stage(double2 (&v)[E], double2* __restrict__ lin
 #pragma unroll
     for (int r = 0; r < R; ++r) a[r] = v[b + r * B];
     if (Ns > 1) {
-      const int jj = j & (Ns - 1);
 #pragma unroll
-      for (int r = 1; r < R; ++r) {
-        double2 w = twM[(int64_t)r * jj * (M / (Ns * R)) * tws];
-        if (INV) w.y = -w.y;
-        a[r] = cmul(a[r], w);
-      }
+      for (int r = 1; r < R; ++r) a[r] = cmul(a[r], w[b][r]);
     }
     DFT<R, INV>::run(a);
     if (LAST) {
@@ -174,22 +190,29 @@ __device__ __forceinline__ void stage(double2 (&v)[E], double2* __restrict__ lin
   }
 }
 
+// The stages from S on; w holds stage S's twiddles.  The next stage's
+// twiddles are loaded before the exchange's barriers, so that their latency
+// overlaps the exchange.
 template <int M, int E, bool INV, int S, int Ns>
 __device__ __forceinline__ void stages(double2 (&v)[E], double2* __restrict__ line, int p,
-                                       const double2* __restrict__ twM, int TW) {
+                                       const double2* __restrict__ twM, int TW,
+                                       const double2 (&w)[E / stage_radix(M, E, S)][stage_radix(M, E, S)]) {
   constexpr int NS = n_stages(M, E);
   static_assert(NS > 0, "unsupported FP64 FFT size");
   constexpr int P = M / E;
   if constexpr (S < NS) {
     constexpr int R = stage_radix(M, E, S);
     constexpr bool LAST = S + 1 == NS;
-    stage<M, E, R, Ns, INV, LAST>(v, line, p, twM, TW);
+    stage<M, E, R, Ns, INV, LAST>(v, line, p, w);
     if constexpr (!LAST) {
+      constexpr int R1 = stage_radix(M, E, S + 1);
+      double2 w1[E / R1][R1];
+      stage_twiddles<M, E, R1, Ns * R, INV>(w1, p, twM, TW);
       __syncthreads();
 #pragma unroll
       for (int e = 0; e < E; ++e) v[e] = line[slot(p + P * e)];
       __syncthreads();
-      stages<M, E, INV, S + 1, Ns * R>(v, line, p, twM, TW);
+      stages<M, E, INV, S + 1, Ns * R>(v, line, p, twM, TW, w1);
     }
   }
 }
@@ -200,7 +223,9 @@ __device__ __forceinline__ void stages(double2 (&v)[E], double2* __restrict__ li
 template <int M, int E, bool INV>
 __device__ __forceinline__ void fft(double2 (&v)[E], double2* __restrict__ line, int p,
                                     const double2* __restrict__ twM, int TW) {
-  stages<M, E, INV, 0, 1>(v, line, p, twM, TW);
+  constexpr int R0 = stage_radix(M, E, 0);
+  double2 w0[E / R0][R0];   // stage 0 has Ns = 1: no twiddles
+  stages<M, E, INV, 0, 1>(v, line, p, twM, TW, w0);
 }
 
 // w_n = exp(-i pi n^2 / N), n^2 mod 2N exact (n < 2^31 / N)
