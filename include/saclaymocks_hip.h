@@ -515,6 +515,8 @@ typedef struct smk_qso_params {
   const double* dg_v;
   int32_t dg_n;
   double dgrowth0;               /* dD/dz at z = 0 */
+  const double* r2z_slope;       /* NULL, or device [r2z_n - 1]: (z[j+1] - z[j]) / (chi[j+1] - chi[j]) */
+  const double* coef_slope;      /* NULL, or device [coef_n - 1]: the same for c(z) */
 } smk_qso_params;
 
 /* One drawn QSO: key = (mz * nx + ix) * ny + iy (the reference's output

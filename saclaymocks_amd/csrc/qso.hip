@@ -35,15 +35,18 @@ __device__ __forceinline__ double u53(uint32_t a, uint32_t b) {
   return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * 1.1102230246251565e-16;
 }
 
-// numpy.interp on ascending nodes, starting the search at node j
+// numpy.interp on ascending nodes, starting the search at node j; sp: NULL,
+// or the segments' slopes (fp[j+1] - fp[j]) / (xp[j+1] - xp[j]) made on the
+// host (the same correctly rounded operations, so the same bits, without a
+// division per call)
 __device__ __forceinline__ double interp_from(double x, const double* __restrict__ xp, const double* __restrict__ fp,
-                                              int n, int j) {
+                                              const double* __restrict__ sp, int n, int j) {
   if (x <= xp[0]) return fp[0];
   if (x >= xp[n - 1]) return fp[n - 1];
   j = j < 0 ? 0 : (j > n - 2 ? n - 2 : j);
   while (j > 0 && xp[j] > x) --j;
   while (j < n - 2 && xp[j + 1] <= x) ++j;
-  const double slope = (fp[j + 1] - fp[j]) / (xp[j + 1] - xp[j]);
+  const double slope = sp ? sp[j] : (fp[j + 1] - fp[j]) / (xp[j + 1] - xp[j]);
   return slope * (x - xp[j]) + fp[j];
 }
 
@@ -58,20 +61,20 @@ __device__ __forceinline__ int uniform_guess(double x, double x0, double dx) {
 __device__ __forceinline__ double z_of_r(const smk_qso_params& p, double r) {
   int b = uniform_guess(r, p.r2z_lo, p.r2z_step);
   b = b >= p.r2z_ng ? p.r2z_ng - 1 : b;
-  return interp_from(r, p.r2z_chi, p.r2z_z, p.r2z_n, p.r2z_guess[b]);
+  return interp_from(r, p.r2z_chi, p.r2z_z, p.r2z_slope, p.r2z_n, p.r2z_guess[b]);
 }
 
 __device__ __forceinline__ double r_of_z(const smk_qso_params& p, double z) {
   // inverse lookup on the same nodes (z ascending, uniform in the reference's table)
   const int n = p.r2z_n;
   const double z0 = p.r2z_z[0], dz = (p.r2z_z[n - 1] - z0) / (double)(n - 1);
-  return interp_from(z, p.r2z_z, p.r2z_chi, n, uniform_guess(z, z0, dz));
+  return interp_from(z, p.r2z_z, p.r2z_chi, nullptr, n, uniform_guess(z, z0, dz));
 }
 
 __device__ __forceinline__ double coef_of_z(const smk_qso_params& p, double z) {
   // nodes: 0, then a regular grid from coef_z[1], then 10
   const double z1 = p.coef_z[1], dz = p.coef_z[2] - p.coef_z[1];
-  return interp_from(z, p.coef_z, p.coef_c, p.coef_n, 1 + uniform_guess(z, z1, dz));
+  return interp_from(z, p.coef_z, p.coef_c, p.coef_slope, p.coef_n, 1 + uniform_guess(z, z1, dz));
 }
 
 // a_k(z) = b(z)(1+z_k) / (b(z_k)(1+z)) with b(z) = 3.7((1+z)/3.33)^1.7
@@ -163,7 +166,7 @@ __global__ __launch_bounds__(256) void k_qso_draw(smk_qso_params p, const float*
     const double vpar = (x * (double)vx[c] + y * (double)vy[c] + z * (double)vz[c]) / r;
     double rr = p.zfix_on ? r_of_z(p, red) * p.h : r;
     if (red < p.zmax + 1.0) {
-      const double dg = interp_from(red, p.dg_z, p.dg_v, p.dg_n,
+      const double dg = interp_from(red, p.dg_z, p.dg_v, nullptr, p.dg_n,
                                     uniform_guess(red, p.dg_z[0], (p.dg_z[p.dg_n - 1] - p.dg_z[0]) / (p.dg_n - 1)));
       rr += vpar * (1.0 + red) * dg / (p.dgrowth0 * H0_KMS);
     }

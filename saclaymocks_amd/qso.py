@@ -37,7 +37,8 @@ class SmkQsoParams(ctypes.Structure):
                 ("dn_cell", _pd), ("dn_n", _i32), ("dn_z0", _d), ("dn_dz", _d),
                 ("sigma", _d * 3), ("norm", _d), ("density_max", _d), ("zmin", _d), ("zmax", _d),
                 ("ra0", _d), ("dec0", _d), ("dra", _d), ("ddec", _d),
-                ("dg_z", _pd), ("dg_v", _pd), ("dg_n", _i32), ("dgrowth0", _d)]
+                ("dg_z", _pd), ("dg_v", _pd), ("dg_n", _i32), ("dgrowth0", _d),
+                ("r2z_slope", _pd), ("coef_slope", _pd)]
 
 
 RECORD = np.dtype([("key", "<i8"), ("x", "<f8"), ("y", "<f8"), ("z", "<f8"), ("r", "<f8"), ("z_qso", "<f8"),
@@ -201,6 +202,9 @@ class Drawer:
         prm.zmin, prm.zmax = s.z_min, s.z_max
         prm.ra0, prm.dec0, prm.dra, prm.ddec = np.radians(s.ra0), np.radians(s.dec0), s.dra, s.ddec
         prm.dg_z, prm.dg_v, prm.dg_n, prm.dgrowth0 = self._t(s.dg_z), self._t(s.dg_v), len(s.dg_z), s.dgrowth0
+        # segment slopes of the two lookups made per cell, as interp_from forms them
+        prm.r2z_slope = self._t(np.diff(zz) / np.diff(chi))
+        prm.coef_slope = self._t(np.diff(s.coef_c) / np.diff(s.coef_z))
         self.prm = prm
         self.count = torch.zeros(1, dtype=torch.int64, device=self.dev)
         self.capacity = int(capacity) if capacity is not None else max(1024, int(4 * s.n_expected) + 1024)
