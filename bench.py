@@ -466,6 +466,11 @@ def fit_bench(delta, eta, shape, cell, per_bin, maxfev):
     # the forward model's chi2 at the Gauss-Newton point (its own chi2 comes
     # from the Jacobian kernels, equal to FP64 rounding)
     gn_chi2_forward = model.chi2(gn["x"])
+    reached = {"gauss_newton": gn_chi2_forward <= target, "adjoint_lbfgs": grad["chi2"] <= target}
+    walls = {"gauss_newton": gn["wall_s"], "adjoint_lbfgs": grad["wall_s"]}
+    ok = [k for k in walls if reached[k]] or list(walls)
+    best_name = min(ok, key=lambda k: walls[k])
+    best_wall = walls[best_name]
     return {"workload": f"{b.n} skewers in {len(zb)} z bins (zfix per bin), 15 params a,b,c(z); "
                         "targets <F>=exp(-0.0025(1+z)^3.7) and the P1D of an independent realisation "
                         "with b, c from etc/params.fits and a solved for the <F> target",
@@ -485,7 +490,11 @@ def fit_bench(delta, eta, shape, cell, per_bin, maxfev):
             "target_chi2": target,
             "reached_plain_chi2": bool(grad["chi2"] <= target),
             "gauss_newton_reached_plain_chi2": bool(gn_chi2_forward <= target),
-            "speedup": {m: r["wall_s"] / grad["wall_s"] for m, r in plains.items()},
+            # the fastest gradient-based fit that reached the target against
+            # each plain fit, and each gradient method on its own
+            "speedup_method": best_name,
+            "speedup": {m: r["wall_s"] / best_wall for m, r in plains.items()},
+            "speedup_adjoint_lbfgs": {m: r["wall_s"] / grad["wall_s"] for m, r in plains.items()},
             "speedup_gauss_newton": {m: r["wall_s"] / gn["wall_s"] for m, r in plains.items()}}
 
 

@@ -1414,6 +1414,11 @@ __global__ __launch_bounds__(FIT_THREADS) void k_p1d_jac(JacArgs a) {
 }
 
 int grid_for(int n) { return n < 65536 ? (n > 0 ? n : 1) : 65536; }
+// grid of a generic kernel that runs beside the per-size table kernels: its
+// blocks walk the batch and take only the forests without a table, usually
+// few, so a block per forest would spend the launch on blocks that exit at
+// once; 2048 blocks still spread many such forests over every CU
+int grid_for_rest(int n) { const int g = grid_for(n); return g < 2048 ? g : 2048; }
 
 // propagate a nonzero SMK_ERR_* return
 #define SMK_CALL(expr)          \
@@ -1584,7 +1589,7 @@ extern "C" int smk_p1d_binned(const smk_skewers* sk, const double* flux, const d
     // size class; each skips the skewers of the others
     SMK_HIP_CHECK(hipFuncSetAttribute((const void*)k_p1d<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)lds));
-    hipLaunchKernelGGL(k_p1d<true>, grid, dim3(FIT_THREADS), lds, st, a);
+    hipLaunchKernelGGL(k_p1d<true>, dim3(grid_for_rest(sk->n)), dim3(FIT_THREADS), lds, st, a);
     SMK_LAUNCH_CHECK();
     const int nblue = nmax < BLUE_NMAX ? nmax : BLUE_NMAX;
     const size_t tail = (size_t)(nbins + 1) * 8 + (size_t)nbins * 8;
@@ -1651,7 +1656,7 @@ extern "C" int smk_p1d_adjoint(const smk_skewers* sk, const float* delta, const 
   if (tab) {
     SMK_HIP_CHECK(hipFuncSetAttribute((const void*)k_p1d_adjoint<true>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(k_p1d_adjoint<true>, grid, dim3(FIT_THREADS), lds, st, a);
+    hipLaunchKernelGGL(k_p1d_adjoint<true>, dim3(grid_for_rest(sk->n)), dim3(FIT_THREADS), lds, st, a);
     SMK_LAUNCH_CHECK();
     const int nblue = nmax < BLUE_NMAX ? nmax : BLUE_NMAX;
     const size_t tail = (size_t)(nbins + 1) * 8;
@@ -1715,7 +1720,7 @@ extern "C" int smk_p1d_jacobian(const smk_skewers* sk, const float* delta, const
   if (tables) {
     SMK_HIP_CHECK(hipFuncSetAttribute((const void*)k_p1d_jac<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)lds));
-    hipLaunchKernelGGL(k_p1d_jac<true>, grid, dim3(FIT_THREADS), lds, st, a);
+    hipLaunchKernelGGL(k_p1d_jac<true>, dim3(grid_for_rest(sk->n)), dim3(FIT_THREADS), lds, st, a);
     SMK_LAUNCH_CHECK();
     const int nblue = nmax < BLUE_NMAX ? nmax : BLUE_NMAX;
     const size_t tail = (size_t)(nbins + 1) * 8 + (size_t)nbins * 8;
