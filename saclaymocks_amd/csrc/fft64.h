@@ -344,5 +344,89 @@ __device__ __forceinline__ void bluestein_tab(int N, int nout, In in, Emit emit,
   }
 }
 
+// Two independent M-point FFTs (va on line la, vb on line lb) in lockstep:
+// each stage's butterflies of both, then one pair of barriers around both
+// exchanges, so the two transforms share every barrier and every twiddle.
+template <int M, int E, bool INV, int S, int Ns>
+__device__ __forceinline__ void stages2(double2 (&va)[E], double2 (&vb)[E], double2* __restrict__ la,
+                                        double2* __restrict__ lb, int p, const double2* __restrict__ twM, int TW,
+                                        const double2 (&w)[E / stage_radix(M, E, S)][stage_radix(M, E, S)]) {
+  constexpr int NS = n_stages(M, E);
+  static_assert(NS > 0, "unsupported FP64 FFT size");
+  constexpr int P = M / E;
+  if constexpr (S < NS) {
+    constexpr int R = stage_radix(M, E, S);
+    constexpr bool LAST = S + 1 == NS;
+    stage<M, E, R, Ns, INV, LAST>(va, la, p, w);
+    stage<M, E, R, Ns, INV, LAST>(vb, lb, p, w);
+    if constexpr (!LAST) {
+      constexpr int R1 = stage_radix(M, E, S + 1);
+      double2 w1[E / R1][R1];
+      stage_twiddles<M, E, R1, Ns * R, INV>(w1, p, twM, TW);
+      __syncthreads();
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        va[e] = la[slot(p + P * e)];
+        vb[e] = lb[slot(p + P * e)];
+      }
+      __syncthreads();
+      stages2<M, E, INV, S + 1, Ns * R>(va, vb, la, lb, p, twM, TW, w1);
+    }
+  }
+}
+
+template <int M, int E, bool INV>
+__device__ __forceinline__ void fft2(double2 (&va)[E], double2 (&vb)[E], double2* __restrict__ la,
+                                     double2* __restrict__ lb, int p, const double2* __restrict__ twM, int TW) {
+  constexpr int R0 = stage_radix(M, E, 0);
+  double2 w0[E / R0][R0];   // stage 0 has Ns = 1: no twiddles
+  stages2<M, E, INV, 0, 1>(va, vb, la, lb, p, twM, TW, w0);
+}
+
+// bluestein_tab for two inputs of one length N (one table) in lockstep on
+// the lines la and lb: in(n) gives both inputs' points n as (x.re, x.im,
+// y.re, y.im); emit_a(k, X_k) and emit_b(k, Y_k), k < nout.  The emits may
+// write the first N points of either line.
+template <int M, int E, class In, class EmitA, class EmitB>
+__device__ __forceinline__ void bluestein_tab2(int N, int nout, In in, EmitA emit_a, EmitB emit_b,
+                                               double2* __restrict__ la, double2* __restrict__ lb,
+                                               const double2* __restrict__ twM, int TW,
+                                               const double2* __restrict__ wtab,
+                                               const double2* __restrict__ btab) {
+  constexpr int P = M / E;
+  const int p = threadIdx.x;
+  double2 va[E], vb[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int n = p + P * e;
+    if (n < N) {
+      const double2 w = wtab[n];
+      const double4 xy = in(n);
+      va[e] = cmul(make_double2(xy.x, xy.y), w);
+      vb[e] = cmul(make_double2(xy.z, xy.w), w);
+    } else {
+      va[e] = vb[e] = make_double2(0.0, 0.0);
+    }
+  }
+  __syncthreads();
+  fft2<M, E, false>(va, vb, la, lb, p, twM, TW);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const double2 b = btab[p + P * e];
+    va[e] = cmul(va[e], b);
+    vb[e] = cmul(vb[e], b);
+  }
+  fft2<M, E, true>(va, vb, la, lb, p, twM, TW);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int k = p + P * e;
+    if (k < nout) {
+      const double2 w = wtab[k];
+      emit_a(k, cmul(va[e], w));
+      emit_b(k, cmul(vb[e], w));
+    }
+  }
+}
+
 }  // namespace f64
 }  // namespace smk

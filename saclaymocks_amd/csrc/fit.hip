@@ -1208,29 +1208,35 @@ __device__ __forceinline__ void bin_runs_jac(const double* pk, const double* dp,
   }
 }
 
-// LDS of k_p1d_jac_blue<M>: the line, then X_j, pk, the three dP rows (NJ
-// each, NJ = M/4 + 1 >= N/2 + 1), the bins, the edges and the bin runs
+// LDS of k_p1d_jac_blue<M>: two lines; over the second line past its first
+// M/2 points (the emitted outputs need at most N <= M/2) pk, the three dP
+// rows (NJ each, NJ = M/4 + 1 >= N/2 + 1) and the bins; then the edges and
+// the bin runs
 __host__ __device__ constexpr int jac_nj(int M) { return M / 4 + 1; }
+__host__ __device__ constexpr size_t jac_ext_off(int M) { return blue_line_bytes_m(M) + (size_t)(M / 2) * 16; }
 __host__ __device__ constexpr size_t jac_blue_fixed_bytes(int M) {
-  return blue_line_bytes_m(M) + (size_t)jac_nj(M) * 16 + (size_t)jac_nj(M) * 32 +
-         (size_t)((jac_nj(M) + 1) & ~1) * 4;
+  return jac_ext_off(M) +
+         ((size_t)jac_nj(M) * 32 + (size_t)((jac_nj(M) + 1) & ~1) * 4 > blue_line_bytes_m(M) - (size_t)(M / 2) * 16
+              ? (size_t)jac_nj(M) * 32 + (size_t)((jac_nj(M) + 1) & ~1) * 4
+              : blue_line_bytes_m(M) - (size_t)(M / 2) * 16);
 }
 
 // The Jacobian of the skewers with a table and a forest of M's size class:
-// two Bluestein DFTs (bluestein_tab), of delta_F + i t_c (X and T_c) and of
-// t_a + i t_b (T_a and T_b), each emitting all N outputs into the line and
-// split by the Hermitian symmetry of the real inputs
+// the Bluestein DFTs of delta_F + i t_c (X and T_c) and of t_a + i t_b (T_a
+// and T_b) in lockstep (bluestein_tab2: one set of barriers and twiddles for
+// both), each emitting all N outputs into its line, split by the symmetry
+// of real inputs
 template <int M>
 __global__ __launch_bounds__(M / 8) __attribute__((amdgpu_waves_per_eu(2))) void k_p1d_jac_blue(JacArgs a) {
   constexpr int NT = M / 8;
   constexpr int NJ = jac_nj(M);
   extern __shared__ double dyn[];
-  double2* line = reinterpret_cast<double2*>(dyn);
-  double2* xs = reinterpret_cast<double2*>(reinterpret_cast<char*>(dyn) + blue_line_bytes_m(M));
-  double* pk = reinterpret_cast<double*>(xs + NJ);
+  double2* la = reinterpret_cast<double2*>(dyn);
+  double2* lb = reinterpret_cast<double2*>(reinterpret_cast<char*>(dyn) + blue_line_bytes_m(M));
+  double* pk = reinterpret_cast<double*>(reinterpret_cast<char*>(dyn) + jac_ext_off(M));
   double* dp = pk + NJ;
   int* bins = reinterpret_cast<int*>(dp + 3 * NJ);
-  double* ledges = reinterpret_cast<double*>(bins + ((NJ + 1) & ~1));
+  double* ledges = reinterpret_cast<double*>(reinterpret_cast<char*>(dyn) + jac_blue_fixed_bytes(M));
   int* bfirst = reinterpret_cast<int*>(ledges + a.nb + 1);
   int* blast = bfirst + a.nb;
   for (int i = threadIdx.x; i <= a.nb; i += NT) ledges[i] = a.edges[i];
@@ -1252,39 +1258,26 @@ __global__ __launch_bounds__(M / 8) __attribute__((amdgpu_waves_per_eu(2))) void
     const double2* T = a.tables + a.tab_off[s];
     const int64_t o = sk.off[s] + i0;
     auto tangent = [&](double dsum, double fsum, double g) { return (dsum / m) * inv - (fsum / m) * g * inv * inv; };
-    auto to_line = [&](int j, double2 Z) { line[j] = Z; };
-    f64::bluestein_tab<M, 8>(
+    f64::bluestein_tab2<M, 8>(
         nq, nq,
         [&](int q) {
           const JacPoint P = jac_point(a, o, q, m, p);
-          return make_double2((P.f / m) * inv - 1.0, tangent(P.dc, P.f, g2));
+          return make_double4((P.f / m) * inv - 1.0, tangent(P.dc, P.f, g2), tangent(P.da, P.f, g0),
+                              tangent(P.db, P.f, g1));
         },
-        to_line, line, a.tw, BLUE_MMAX, T, T + nq);
+        [&](int j, double2 Z) { la[j] = Z; }, [&](int j, double2 Z) { lb[j] = Z; }, la, lb, a.tw, BLUE_MMAX, T,
+        T + nq);
     __syncthreads();
     for (int j = threadIdx.x; j < nj; j += NT) {
-      double2 X, Tc;
-      split_pair(line[j], line[j == 0 ? 0 : nq - j], X, Tc);
-      xs[j] = X;
+      const int jn = j == 0 ? 0 : nq - j;
+      double2 X, Tc, Ta, Tb;
+      split_pair(la[j], la[jn], X, Tc);
+      split_pair(lb[j], lb[jn], Ta, Tb);
       pk[j] = (X.x * X.x + X.y * X.y) * px / nq;
-      dp[2 * NJ + j] = c2 * (X.x * Tc.x + X.y * Tc.y);
-      bins[j] = find_bin(ledges, a.nb, kfreq(j, nq, k_ny));
-    }
-    // (bluestein_tab synchronises the block before its first exchange
-    // overwrites the line read above)
-    f64::bluestein_tab<M, 8>(
-        nq, nq,
-        [&](int q) {
-          const JacPoint P = jac_point(a, o, q, m, p);
-          return make_double2(tangent(P.da, P.f, g0), tangent(P.db, P.f, g1));
-        },
-        to_line, line, a.tw, BLUE_MMAX, T, T + nq);
-    __syncthreads();
-    for (int j = threadIdx.x; j < nj; j += NT) {
-      double2 Ta, Tb;
-      split_pair(line[j], line[j == 0 ? 0 : nq - j], Ta, Tb);
-      const double2 X = xs[j];
       dp[j] = c2 * (X.x * Ta.x + X.y * Ta.y);
       dp[NJ + j] = c2 * (X.x * Tb.x + X.y * Tb.y);
+      dp[2 * NJ + j] = c2 * (X.x * Tc.x + X.y * Tc.y);
+      bins[j] = find_bin(ledges, a.nb, kfreq(j, nq, k_ny));
     }
     __syncthreads();
     bin_runs_jac<NT>(pk, dp, NJ, bins, nj, nq, k_ny, a.nb, bfirst, blast, a.partial + (int64_t)s * a.nb * 7);
