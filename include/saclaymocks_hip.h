@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define SMK_ABI_VERSION 7
+#define SMK_ABI_VERSION 8
 
 #define SMK_OK 0
 #define SMK_ERR_ARG (-1)         /* bad argument (null pointer, size, enum) */
@@ -377,10 +377,16 @@ int smk_forest_missing_power_sets(smk_forest_plan* plan, const smk_skewers* sk, 
 int smk_fgpa_flux(const smk_skewers* sk, const float* delta, const float* eta, const double* abcg,
                   int64_t abcg_rows, double* flux, double* fsum, void* stream);
 
-/* out[set][c] = sum over the skewers of that set of per_skewer[s][c], FP64,
- * deterministic (mean flux, P1D bins and adjoint terms per set). */
-int smk_reduce_by_set(const smk_skewers* sk, const double* per_skewer, int width, int n_sets, double* out,
-                      void* stream);
+/* out[set][c] = sum over the skewers of that set of per_skewer[s][c] (FP64,
+ * deterministic: a fixed order of additions for given n, width, n_sets),
+ * divided by divisor[set] when divisor (device double [n_sets]) is not
+ * NULL: mean flux, P1D bins and adjoint terms per set (fit_az.py:206-217
+ * takes the means over the forests of one redshift).
+ *   work  device double scratch of at least smk_reduce_work_len(skewers.n,
+ *         width, n_sets) values; calls that share it must not overlap. */
+int64_t smk_reduce_work_len(int n, int width, int n_sets);
+int smk_reduce_by_set(const smk_skewers* sk, const double* per_skewer, int width, int n_sets,
+                      const double* divisor, double* out, double* work, int64_t work_len, void* stream);
 
 /* compute_p1d (fit_az.py:194-217): per skewer, delta_F = regroup(F, m) /
  * <F>_set - 1 over the forest (util.py:29-37), P_j = |rfft(delta_F)_j|^2 px / N

@@ -13,7 +13,7 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "saclaymocks_hip.h")
 
 # layout of the structs passed by pointer (smk_skewers, smk_box_view): must
 # equal SMK_ABI_VERSION of include/saclaymocks_hip.h and the library's
-ABI_VERSION = 7
+ABI_VERSION = 8
 
 SMK_OK = 0
 SMK_ERR_ARG = -1
@@ -84,7 +84,8 @@ _SIGNATURES = {
     "smk_qso_draw": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _vp, _c.c_int64, _vp, _vp]),
     "smk_pk_table": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _c.c_float, _vp, _vp]),
     "smk_p1d_rsd_rows": (_i, [_vp, _vp, _i, _vp, _i, _d, _i, _vp, _d, _d, _vp, _vp]),
-    "smk_reduce_by_set": (_i, [_vp, _vp, _i, _i, _vp, _vp]),
+    "smk_reduce_work_len": (_c.c_int64, [_i, _i, _i]),
+    "smk_reduce_by_set": (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp, _c.c_int64, _vp]),
     "smk_p1d_binned": (_i, [_vp, _vp, _vp, _d, _i, _vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                             _c.POINTER(_c.c_int64), _vp]),
     "smk_p1d_table_len": (_c.c_int64, [_i]),

@@ -383,47 +383,59 @@ __device__ __forceinline__ void fft2(double2 (&va)[E], double2 (&vb)[E], double2
   stages2<M, E, INV, 0, 1>(va, vb, la, lb, p, twM, TW, w0);
 }
 
-// bluestein_tab for two inputs of one length N (one table) in lockstep on
-// the lines la and lb: in(n) gives both inputs' points n as (x.re, x.im,
-// y.re, y.im); emit_a(k, X_k) and emit_b(k, Y_k), k < nout.  The emits may
-// write the first N points of either line.
-template <int M, int E, class In, class EmitA, class EmitB>
-__device__ __forceinline__ void bluestein_tab2(int N, int nout, In in, EmitA emit_a, EmitB emit_b,
+// bluestein_tab for two inputs in lockstep on the lines la and lb, of
+// lengths Na and Nb with the tables (wa, ba) and (wb, bb) (SAME: one length,
+// one table, (wa, ba), read once): in(n), n < max(Na, Nb), gives point n of
+// both inputs as (x.re, x.im, y.re, y.im), the first pair used for n < Na,
+// the second for n < Nb; emit_a(k, X_k) for k < Na and emit_b(k, Y_k) for
+// k < Nb.  The emits may write the first Na (Nb) points of la (lb).
+template <int M, int E, bool SAME, class In, class EmitA, class EmitB>
+__device__ __forceinline__ void bluestein_tab2(int Na, int Nb, In in, EmitA emit_a, EmitB emit_b,
                                                double2* __restrict__ la, double2* __restrict__ lb,
                                                const double2* __restrict__ twM, int TW,
-                                               const double2* __restrict__ wtab,
-                                               const double2* __restrict__ btab) {
+                                               const double2* __restrict__ wa, const double2* __restrict__ ba,
+                                               const double2* __restrict__ wb, const double2* __restrict__ bb) {
   constexpr int P = M / E;
   const int p = threadIdx.x;
+  const int nmax = Na > Nb ? Na : Nb;
   double2 va[E], vb[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int n = p + P * e;
-    if (n < N) {
-      const double2 w = wtab[n];
+    va[e] = vb[e] = make_double2(0.0, 0.0);
+    if (n < nmax) {
       const double4 xy = in(n);
-      va[e] = cmul(make_double2(xy.x, xy.y), w);
-      vb[e] = cmul(make_double2(xy.z, xy.w), w);
-    } else {
-      va[e] = vb[e] = make_double2(0.0, 0.0);
+      if (SAME) {
+        const double2 w = wa[n];
+        va[e] = cmul(make_double2(xy.x, xy.y), w);
+        vb[e] = cmul(make_double2(xy.z, xy.w), w);
+      } else {
+        if (n < Na) va[e] = cmul(make_double2(xy.x, xy.y), wa[n]);
+        if (n < Nb) vb[e] = cmul(make_double2(xy.z, xy.w), wb[n]);
+      }
     }
   }
   __syncthreads();
   fft2<M, E, false>(va, vb, la, lb, p, twM, TW);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    const double2 b = btab[p + P * e];
-    va[e] = cmul(va[e], b);
-    vb[e] = cmul(vb[e], b);
+    const double2 x = ba[p + P * e];
+    va[e] = cmul(va[e], x);
+    vb[e] = cmul(vb[e], SAME ? x : bb[p + P * e]);
   }
   fft2<M, E, true>(va, vb, la, lb, p, twM, TW);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int k = p + P * e;
-    if (k < nout) {
-      const double2 w = wtab[k];
-      emit_a(k, cmul(va[e], w));
-      emit_b(k, cmul(vb[e], w));
+    if (SAME) {
+      if (k < Na) {
+        const double2 w = wa[k];
+        emit_a(k, cmul(va[e], w));
+        emit_b(k, cmul(vb[e], w));
+      }
+    } else {
+      if (k < Na) emit_a(k, cmul(va[e], wa[k]));
+      if (k < Nb) emit_b(k, cmul(vb[e], wb[k]));
     }
   }
 }

@@ -56,10 +56,11 @@ class P1DFitModel:
         self.count = np.bincount(pset, weights=nf, minlength=self.n_sets).astype(np.float64)
         if np.any(self.count <= 0):
             raise ValueError("every parameter set needs forest pixels")
+        self.count_dev = torch.from_numpy(self.count).to(dev)
+        self._red_work = None
         n = batch.n
         self.flux = torch.empty(batch.npix_total, dtype=torch.float64, device=dev)
         self.fsum = torch.empty(n, dtype=torch.float64, device=dev)
-        self.fsum_set = torch.empty(self.n_sets, dtype=torch.float64, device=dev)
         self.set_edges(edges)
         self.adj = torch.empty((n, 7), dtype=torch.float64, device=dev)
         self.adj_set = torch.empty((self.n_sets, 7), dtype=torch.float64, device=dev)
@@ -144,6 +145,17 @@ class P1DFitModel:
         self.bins_set = torch.empty((self.n_sets, self.nb, 4), dtype=torch.float64, device=self.dev)
         self._jac_bufs = None   # sized by nb, made on the first jacobian()
 
+    def _reduce(self, per_skewer, width, out, divisor=None):
+        """out[set][c] = sum over the set's skewers of per_skewer[s][c]
+        (divided by divisor[set] when given), on the device."""
+        L = _lib.lib()
+        need = int(L.smk_reduce_work_len(self.batch.n, width, self.n_sets))
+        if self._red_work is None or self._red_work.numel() < need:
+            self._red_work = torch.empty(max(need, 1), dtype=torch.float64, device=self.dev)
+        _lib.check(L.smk_reduce_by_set(ctypes.byref(self.sk), _lib.ptr(per_skewer), width, self.n_sets,
+                                       _lib.ptr(divisor), _lib.ptr(out), _lib.ptr(self._red_work),
+                                       self._red_work.numel(), _lib.stream_ptr()), "smk_reduce_by_set")
+
     # ------------------------------------------------------------ forward
     def _abcg(self, theta):
         th = np.asarray(theta, dtype=np.float64).reshape(self.n_sets, 3)
@@ -159,26 +171,28 @@ class P1DFitModel:
         _lib.check(L.smk_fgpa_flux(ctypes.byref(self.sk), _lib.ptr(self.delta), _lib.ptr(self.eta),
                                    _lib.ptr(abcg), rows, _lib.ptr(self.flux), _lib.ptr(self.fsum), s),
                    "smk_fgpa_flux")
-        _lib.check(L.smk_reduce_by_set(ctypes.byref(self.sk), _lib.ptr(self.fsum), 1, self.n_sets,
-                                       _lib.ptr(self.fsum_set), s), "smk_reduce_by_set")
-        fmean = _sync_host(self.fsum_set) / self.count
-        bins = self._binned(self.flux, fmean, self.modes)
+        # <F> per set stays on the device for the P1D kernels
+        self._reduce(self.fsum, 1, self.fmean_dev, self.count_dev)
+        bins = self._binned(self.flux, None, self.modes)
+        fmean = _sync_host(self.fmean_dev)
         self.n_eval += 1
         self._last = (np.array(theta, dtype=np.float64), abcg, fmean, bins)
         return fmean, bins
 
     def _binned(self, flux, fmean, modes):
+        """Binned P1D sums per set (host numpy); fmean None: the mean flux
+        already in fmean_dev."""
         L = _lib.lib()
         s = _lib.stream_ptr()
-        self.fmean_dev.copy_(torch.from_numpy(np.asarray(fmean, dtype=np.float64)))
+        if fmean is not None:
+            self.fmean_dev.copy_(torch.from_numpy(np.asarray(fmean, dtype=np.float64)))
         _lib.check(L.smk_p1d_binned(ctypes.byref(self.sk), _lib.ptr(flux), _lib.ptr(self.fmean_dev),
                                     self.pixel, self.regroup, _lib.ptr(self.edges), self.nb, self.max_forest,
                                     _lib.ptr(self.partial), _lib.ptr(self.status), _lib.ptr(modes),
                                     _lib.ptr(self.modes_off if modes is not None else None), _lib.ptr(self.tables),
                                     _lib.ptr(self.tab_off), _lib.ptr(self.pairs), self.pair_off, s),
                    "smk_p1d_binned")
-        _lib.check(L.smk_reduce_by_set(ctypes.byref(self.sk), _lib.ptr(self.partial), self.nb * 4, self.n_sets,
-                                       _lib.ptr(self.bins_set), s), "smk_reduce_by_set")
+        self._reduce(self.partial, self.nb * 4, self.bins_set)
         return _sync_host(self.bins_set)
 
     def binned_flux(self, flux, fmean):
@@ -219,21 +233,19 @@ class P1DFitModel:
         _lib.check(L.smk_fgpa_flux_jac(ctypes.byref(self.sk), _lib.ptr(self.delta), _lib.ptr(self.eta),
                                        _lib.ptr(abcg), abcg.shape[0], _lib.ptr(self.flux), _lib.ptr(sums), s),
                    "smk_fgpa_flux_jac")
-        _lib.check(L.smk_reduce_by_set(ctypes.byref(self.sk), _lib.ptr(sums), 4, self.n_sets, _lib.ptr(sums_set),
-                                       s), "smk_reduce_by_set")
-        r = _sync_host(sums_set)
-        fmean = r[:, 0] / self.count
-        dfmean = r[:, 1:4] / self.count[:, None]
-        self.fmean_dev.copy_(torch.from_numpy(fmean))
-        dfm_dev.copy_(torch.from_numpy(np.ascontiguousarray(dfmean)))
+        # <F> and d<F>/d(a, b, c) per set, on the device
+        self._reduce(sums, 4, sums_set, self.count_dev)
+        self.fmean_dev.copy_(sums_set[:, 0])
+        dfm_dev.copy_(sums_set[:, 1:4])
         _lib.check(L.smk_p1d_jacobian(ctypes.byref(self.sk), _lib.ptr(self.delta), _lib.ptr(self.eta),
                                       _lib.ptr(abcg), abcg.shape[0], _lib.ptr(self.flux), _lib.ptr(self.fmean_dev),
                                       _lib.ptr(dfm_dev), self.pixel, self.regroup, _lib.ptr(self.edges), self.nb,
                                       self.max_forest, _lib.ptr(part), _lib.ptr(self.status),
                                       _lib.ptr(self.tables), _lib.ptr(self.tab_off), s), "smk_p1d_jacobian")
-        _lib.check(L.smk_reduce_by_set(ctypes.byref(self.sk), _lib.ptr(part), self.nb * 7, self.n_sets,
-                                       _lib.ptr(part_set), s), "smk_reduce_by_set")
+        self._reduce(part, self.nb * 7, part_set)
         q = _sync_host(part_set)
+        r = _sync_host(sums_set)
+        fmean, dfmean = r[:, 0], r[:, 1:4]
         self.n_jac += 1
         return fmean, q[..., :4], dfmean, q[..., 4:7]
 
@@ -293,8 +305,7 @@ class P1DFitModel:
                                      _lib.ptr(self.adj), _lib.ptr(self.tables), _lib.ptr(self.tab_off),
                                      _lib.ptr(self.pairs), self.pair_off, s),
                    "smk_p1d_adjoint")
-        _lib.check(L.smk_reduce_by_set(ctypes.byref(self.sk), _lib.ptr(self.adj), 7, self.n_sets,
-                                       _lib.ptr(self.adj_set), s), "smk_reduce_by_set")
+        self._reduce(self.adj, 7, self.adj_set)
         r = _sync_host(self.adj_set)
         saf, t1, t2 = r[:, 0], r[:, 1:4], r[:, 4:7]
         dLdF = gF - saf / fmean ** 2

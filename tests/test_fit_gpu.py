@@ -207,3 +207,40 @@ def test_gauss_newton_fit_recovers_parameters(cuda, ns):
     assert abs(model.chi2(res["x"]) - res["chi2"]) < 1e-9 * max(1.0, res["chi2"])
     grad = fit.fit_gradient(model, th0, bounds)
     assert res["n_jac"] < grad["n_eval"]
+
+
+@pytest.mark.parametrize("n, width", [(1, 1), (255, 7), (256, 64), (257, 65), (3001, 280)])
+def test_reduce_by_set(cuda, n, width):
+    """smk_reduce_by_set against numpy sums: sets in random order with one
+    set empty, ragged chunk and column tiles, the divisor, and the same bits
+    on a second call."""
+    import ctypes
+    from saclaymocks_amd import _lib, forest
+    rng = np.random.default_rng(n + width)
+    ns = 4
+    pset = rng.choice([0, 1, 3], size=n).astype(np.int32)   # set 2 has no skewers
+    b = forest.SkewerBatch(forest.GEOM_PLANE, np.zeros(n), np.full(n, 8), 0.0, 0.2, np.zeros(n), np.zeros(n),
+                           pset=pset)
+    sk = b.struct()
+    x = rng.standard_normal((n, width))
+    div = rng.uniform(0.5, 2.0, ns)
+    want = np.zeros((ns, width))
+    for s in range(n):
+        want[pset[s]] += x[s]
+    L = _lib.lib()
+    work = torch.empty(int(L.smk_reduce_work_len(n, width, ns)), dtype=torch.float64, device=cuda)
+    xd = torch.from_numpy(x).to(cuda)
+    dd = torch.from_numpy(div).to(cuda)
+    outs = []
+    for divisor in (None, dd, None):
+        out = torch.full((ns, width), np.nan, dtype=torch.float64, device=cuda)
+        _lib.check(L.smk_reduce_by_set(ctypes.byref(sk), _lib.ptr(xd), width, ns, _lib.ptr(divisor), _lib.ptr(out),
+                                       _lib.ptr(work), work.numel(), _lib.stream_ptr()), "smk_reduce_by_set")
+        outs.append(out.cpu().numpy())
+    assert np.allclose(outs[0], want, rtol=1e-13, atol=1e-13 * np.abs(x).sum())
+    assert np.all(outs[0][2] == 0.0)
+    assert np.array_equal(outs[1], outs[0] / div[:, None])
+    assert np.array_equal(outs[2], outs[0])
+    # a work buffer too small is refused before any launch
+    assert L.smk_reduce_by_set(ctypes.byref(sk), _lib.ptr(xd), width, ns, None, _lib.ptr(out), _lib.ptr(work),
+                               work.numel() - 1, _lib.stream_ptr()) != 0
