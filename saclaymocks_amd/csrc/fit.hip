@@ -494,6 +494,24 @@ __device__ __forceinline__ void bin_runs(const double* pk, const int* bins, int 
 // usable table only (Horner; those with one go to k_p1d_blue).  Dynamic
 // LDS: region A (the Bluestein line with pk and bins over its start, or the
 // Horner input x, then pk and bins), the edges, the bin runs.
+// the generic kernels with tables take only the skewers without a usable
+// one: block b checks its candidates c0 + t * gridDim.x (t < FIT_THREADS)
+// with one load round, as a bit per candidate in need[wave] (candidates in
+// ascending order); every thread must call it
+__device__ __forceinline__ void rest_masks(const smk_skewers& sk, int m, const double2* tables,
+                                           const int64_t* tab_off, int nmax, int c0, uint64_t* need) {
+  __syncthreads();   // the previous masks are no longer read
+  const int64_t s = c0 + (int64_t)threadIdx.x * gridDim.x;
+  bool take = false;
+  if (s < sk.n) {
+    const int nq = (sk.imax[s] - sk.imin[s]) / m;
+    take = !uses_table(tables, tab_off, (int)s, nq, nmax);
+  }
+  const uint64_t b = __ballot(take);
+  if ((threadIdx.x & 63) == 0) need[threadIdx.x >> 6] = b;
+  __syncthreads();
+}
+
 template <bool TAB>
 __global__ __launch_bounds__(FIT_THREADS) void k_p1d(P1dArgs a) {
   extern __shared__ double dyn[];
@@ -504,16 +522,16 @@ __global__ __launch_bounds__(FIT_THREADS) void k_p1d(P1dArgs a) {
   double* bsum = reinterpret_cast<double*>(blast + a.nb);
   for (int i = threadIdx.x; i <= a.nb; i += FIT_THREADS) ledges[i] = a.edges[i];
   const smk_skewers& sk = a.sk;
-  for (int s = blockIdx.x; s < sk.n; s += gridDim.x) {
+  // one skewer (returns early where the loop over skewers moved on)
+  auto one = [&](const int s) {
     double* part = a.partial + (int64_t)s * a.nb * 4;
     const int i0 = sk.imin[s], i1 = sk.imax[s];
     const int nq = (i1 - i0) / a.regroup;
-    if (TAB && uses_table(a.tables, a.tab_off, s, nq, a.nmax)) continue;
     __syncthreads();
     if (nq > a.nmax) {
       for (int b = threadIdx.x; b < 4 * a.nb; b += FIT_THREADS) part[b] = 0.0;
       if (threadIdx.x == 0 && a.status) a.status[s] = 1;
-      continue;
+      return;
     }
     const double inv = 1.0 / a.fmean[sk.pset[s]];
     const double px = a.pixel * a.regroup;
@@ -568,7 +586,7 @@ __global__ __launch_bounds__(FIT_THREADS) void k_p1d(P1dArgs a) {
         __syncthreads();
       }
       for (int b = threadIdx.x; b < 4 * a.nb; b += FIT_THREADS) part[b] = bsum[b];
-      continue;
+      return;
     }
     const bool blue = !TAB && nq > 0 && nq <= BLUE_NMAX;
     double* pk = blue ? dyn : dyn + a.nlds;
@@ -604,6 +622,16 @@ __global__ __launch_bounds__(FIT_THREADS) void k_p1d(P1dArgs a) {
       __syncthreads();
     }
     bin_runs<FIT_THREADS>(pk, bins, nj, nq, k_ny, a.nb, bfirst, blast, part);
+  };
+  if constexpr (TAB) {
+    __shared__ uint64_t need[FIT_THREADS / 64];
+    for (int c0 = blockIdx.x; c0 < sk.n; c0 += gridDim.x * FIT_THREADS) {
+      rest_masks(sk, a.regroup, a.tables, a.tab_off, a.nmax, c0, need);
+      for (int w = 0; w < FIT_THREADS / 64; ++w)
+        for (uint64_t mk = need[w]; mk; mk &= mk - 1) one(c0 + (w * 64 + __builtin_ctzll(mk)) * (int)gridDim.x);
+    }
+  } else {
+    for (int s = blockIdx.x; s < sk.n; s += gridDim.x) one(s);
   }
 }
 
@@ -995,17 +1023,17 @@ __global__ __launch_bounds__(FIT_THREADS) void k_p1d_adjoint(AdjArgs a) {
   __shared__ double red[7 * (FIT_THREADS / 64)];
   for (int i = threadIdx.x; i <= a.nb; i += FIT_THREADS) ledges[i] = a.edges[i];
   const smk_skewers& sk = a.sk;
-  for (int s = blockIdx.x; s < sk.n; s += gridDim.x) {
+  // one skewer (returns early where the loop over skewers moved on)
+  auto one = [&](const int s) {
     const int i0 = sk.imin[s], i1 = sk.imax[s];
     const int m = a.regroup;
     const int nq = (i1 - i0) / m;
-    if (TAB && uses_table(a.tables, a.tab_off, s, nq, a.nmax)) continue;
     const int64_t o = sk.off[s];
     double* res = a.out + (int64_t)s * 7;
     __syncthreads();
     if (nq > a.nmax) {
       if (threadIdx.x < 7) res[threadIdx.x] = 0.0;
-      continue;
+      return;
     }
     const int set = sk.pset[s];
     const double fm = a.fmean[set];
@@ -1069,7 +1097,7 @@ __global__ __launch_bounds__(FIT_THREADS) void k_p1d_adjoint(AdjArgs a) {
         v[4] += df.a; v[5] += df.b; v[6] += df.c;
       }
       reduce7<FIT_THREADS>(v, red, res);
-      continue;
+      return;
     }
     load_delta_f(a.flux, o, i0, i1, m, 1.0 / fm, x);
     __syncthreads();
@@ -1121,6 +1149,16 @@ __global__ __launch_bounds__(FIT_THREADS) void k_p1d_adjoint(AdjArgs a) {
     }
     __syncthreads();
     adj_fgpa<FIT_THREADS>(a, s, set, i0, i1, nq, saf, x, red, res);
+  };
+  if constexpr (TAB) {
+    __shared__ uint64_t need[FIT_THREADS / 64];
+    for (int c0 = blockIdx.x; c0 < sk.n; c0 += gridDim.x * FIT_THREADS) {
+      rest_masks(sk, a.regroup, a.tables, a.tab_off, a.nmax, c0, need);
+      for (int w = 0; w < FIT_THREADS / 64; ++w)
+        for (uint64_t mk = need[w]; mk; mk &= mk - 1) one(c0 + (w * 64 + __builtin_ctzll(mk)) * (int)gridDim.x);
+    }
+  } else {
+    for (int s = blockIdx.x; s < sk.n; s += gridDim.x) one(s);
   }
 }
 
@@ -1570,17 +1608,17 @@ __global__ __launch_bounds__(FIT_THREADS) void k_p1d_jac(JacArgs a) {
   double* bsum = reinterpret_cast<double*>(bhi + a.nb);
   for (int i = threadIdx.x; i <= a.nb; i += FIT_THREADS) ledges[i] = a.edges[i];
   const smk_skewers& sk = a.sk;
-  for (int s = blockIdx.x; s < sk.n; s += gridDim.x) {
+  // one skewer (returns early where the loop over skewers moved on)
+  auto one = [&](const int s) {
     const int i0 = sk.imin[s], i1 = sk.imax[s];
     const int m = a.regroup;
     const int nq = (i1 - i0) / m;
-    if (TAB && uses_table(a.tables, a.tab_off, s, nq, a.nmax)) continue;
     double* part = a.partial + (int64_t)s * a.nb * 7;
     __syncthreads();
     if (nq > a.nmax) {
       for (int b = threadIdx.x; b < 7 * a.nb; b += FIT_THREADS) part[b] = 0.0;
       if (threadIdx.x == 0 && a.status) a.status[s] = 1;
-      continue;
+      return;
     }
     const int set = sk.pset[s];
     const double inv = 1.0 / a.fmean[set];
@@ -1666,6 +1704,16 @@ __global__ __launch_bounds__(FIT_THREADS) void k_p1d_jac(JacArgs a) {
     }
     __syncthreads();
     for (int b = threadIdx.x; b < 7 * a.nb; b += FIT_THREADS) part[b] = bsum[b];
+  };
+  if constexpr (TAB) {
+    __shared__ uint64_t need[FIT_THREADS / 64];
+    for (int c0 = blockIdx.x; c0 < sk.n; c0 += gridDim.x * FIT_THREADS) {
+      rest_masks(sk, a.regroup, a.tables, a.tab_off, a.nmax, c0, need);
+      for (int w = 0; w < FIT_THREADS / 64; ++w)
+        for (uint64_t mk = need[w]; mk; mk &= mk - 1) one(c0 + (w * 64 + __builtin_ctzll(mk)) * (int)gridDim.x);
+    }
+  } else {
+    for (int s = blockIdx.x; s < sk.n; s += gridDim.x) one(s);
   }
 }
 
