@@ -137,22 +137,38 @@ __host__ __device__ constexpr int n_stages(int M, int E) {
 __device__ __forceinline__ int slot(int pos) { return pos + (pos >> 5); }
 __host__ __device__ constexpr int line_len(int M) { return M + M / 32 + 1; }
 
+// Twiddle tables of the Stockham stages: a stage of radix R after Ns
+// points per sub-transform needs W_(Ns R)^(r jj), r < R, jj < Ns; these are
+// stored per (R, Ns) as [r][jj] (so the lanes of a wave, consecutive jj,
+// read consecutive entries), after the TW-point table W_TW^k.  Block of
+// radix R at stage_tab_base(R): the tables of Ns = 1, 2, 4, ... TW / R,
+// Ns R entries each.  Every value is W_TW^(r jj TW / (Ns R)), the same
+// double as the TW-point table holds.
+__host__ __device__ constexpr int stage_tab_base(int R, int TW) {
+  int base = 0;
+  for (int q = 2; q < R; q *= 2) base += 2 * TW - q;
+  return base;
+}
+__host__ __device__ constexpr int stage_tab_len(int TW) { return stage_tab_base(32, TW); }
+__host__ __device__ constexpr int stage_tab_off(int R, int Ns, int TW) { return stage_tab_base(R, TW) + R * (Ns - 1); }
+
 // Twiddles of a Stockham stage for this thread's butterflies j = p + P b:
 // w[b][r] = W_M^(r (j % Ns) M/(Ns R)), r >= 1 (conjugated for the inverse).
-// twM: W_TW^k, k < TW, TW >= M.
+// twM: W_TW^k, k < TW (TW = 4096 >= M), then the stage tables above.
 template <int M, int E, int R, int Ns, bool INV>
 __device__ __forceinline__ void stage_twiddles(double2 (&w)[E / R][R], int p, const double2* __restrict__ twM,
                                                int TW) {
   constexpr int P = M / E;
   constexpr int B = E / R;
-  const int tws = TW / M;  // stride of W_M in the table
+  static_assert(Ns * R <= 4096, "stage tables hold sub-transforms up to 4096 points");
+  const double2* __restrict__ tab = twM + TW + stage_tab_off(R, Ns, 4096);
 #pragma unroll
   for (int b = 0; b < B; ++b) {
     const int jj = (p + P * b) & (Ns - 1);
     w[b][0] = make_double2(1.0, 0.0);
 #pragma unroll
     for (int r = 1; r < R; ++r) {
-      double2 t = twM[(int64_t)r * jj * (M / (Ns * R)) * tws];
+      double2 t = tab[r * Ns + jj];
       if (INV) t.y = -t.y;
       w[b][r] = t;
     }

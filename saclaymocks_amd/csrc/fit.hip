@@ -1797,7 +1797,9 @@ int launch_p1d_jac_blue(const JacArgs& a, dim3 grid, size_t tail, hipStream_t st
   return SMK_OK;
 }
 
-// W_4096^k, k < 4096, FP64, one table per device (built on first use)
+// W_4096^k, k < 4096, FP64, then the FFT stages' tables (f64::stage_tab_off:
+// W_(Ns R)^(r jj) as W_4096^(r jj 4096 / (Ns R)), the same values); one
+// table per device (built on first use)
 const double2* fp64_twiddles(hipError_t* err) {
   static std::mutex mu;
   static const double2* tabs[64] = {};
@@ -1806,15 +1808,22 @@ const double2* fp64_twiddles(hipError_t* err) {
   if (*err != hipSuccess || dev < 0 || dev >= 64) return nullptr;
   std::lock_guard<std::mutex> lk(mu);
   if (tabs[dev] == nullptr) {
-    std::vector<double2> t(BLUE_MMAX);
+    constexpr int NTAB = BLUE_MMAX + f64::stage_tab_len(BLUE_MMAX);
+    std::vector<double2> t(NTAB);
     for (int k = 0; k < BLUE_MMAX; ++k) {
       const double ang = -2.0 * M_PI * (double)k / (double)BLUE_MMAX;
       t[k] = make_double2(cos(ang), sin(ang));
     }
+    for (int R = 2; R <= 16; R *= 2)
+      for (int Ns = 1; Ns * R <= BLUE_MMAX; Ns *= 2)
+        for (int r = 0; r < R; ++r)
+          for (int jj = 0; jj < Ns; ++jj)
+            t[BLUE_MMAX + f64::stage_tab_off(R, Ns, BLUE_MMAX) + r * Ns + jj] =
+                t[(int64_t)r * jj * (BLUE_MMAX / (Ns * R))];
     void* d = nullptr;
-    *err = hipMalloc(&d, BLUE_MMAX * sizeof(double2));
+    *err = hipMalloc(&d, NTAB * sizeof(double2));
     if (*err != hipSuccess) return nullptr;
-    *err = hipMemcpy(d, t.data(), BLUE_MMAX * sizeof(double2), hipMemcpyHostToDevice);
+    *err = hipMemcpy(d, t.data(), NTAB * sizeof(double2), hipMemcpyHostToDevice);
     if (*err != hipSuccess) {
       (void)hipFree(d);
       return nullptr;
