@@ -718,10 +718,66 @@ __device__ __forceinline__ void pair_members(const P1dArgs& a, int w, int& s1, i
   }
 }
 
+// What a pair kernel reads about the members of one pair before it can
+// start, loaded a pair ahead so that the chain pair list -> the members'
+// bounds and tables -> their flux runs while the previous pair is
+// transformed
+struct PairDesc {
+  int s1, s2;            // as in the pair list (s2 < 0: alone; s1 < 0: no pair)
+  int n1, n2;            // forest lengths imax - imin
+  int64_t o1, o2;        // sk.off + imin
+  int64_t t1, t2;        // tab_off (-1 without tables)
+  int set1, set2;        // pset
+};
+
+__device__ __forceinline__ PairDesc pair_desc(const P1dArgs& a, int s1, int s2) {
+  const smk_skewers& sk = a.sk;
+  PairDesc d{s1, s2, 0, 0, 0, 0, -1, -1, 0, 0};
+  if (s1 >= 0) {
+    d.n1 = sk.imax[s1] - sk.imin[s1];
+    d.o1 = sk.off[s1] + sk.imin[s1];
+    d.t1 = a.tab_off ? a.tab_off[s1] : -1;
+    d.set1 = sk.pset[s1];
+  }
+  if (s2 >= 0) {
+    d.n2 = sk.imax[s2] - sk.imin[s2];
+    d.o2 = sk.off[s2] + sk.imin[s2];
+    d.t2 = a.tab_off ? a.tab_off[s2] : -1;
+    d.set2 = sk.pset[s2];
+  }
+  return d;
+}
+
+// pair_members on a descriptor: the members that take M's size class (s1 <
+// 0: neither); swap: the second member of the list goes alone
+template <int M>
+__device__ __forceinline__ void desc_members(const P1dArgs& a, const PairDesc& d, int& s1, int& s2, bool& swap) {
+  const int m = a.regroup;
+  const int nq1 = d.n1 / m;
+  const bool v1 = d.s1 >= 0 && a.tables != nullptr && d.t1 >= 0 && nq1 >= 1 && nq1 <= BLUE_NMAX &&
+                  nq1 <= a.nmax && blue_m_of(nq1) == M;
+  bool v2 = d.s2 >= 0;
+  if (v2) {
+    const int nq2 = d.n2 / m;
+    v2 = a.tables != nullptr && d.t2 >= 0 && nq2 >= 1 && nq2 <= BLUE_NMAX && nq2 <= a.nmax &&
+         blue_m_of(nq2) == M && (!v1 || (nq2 == nq1 && d.t2 == d.t1));
+  }
+  swap = !v1 && v2;
+  if (!v1) {
+    s1 = v2 ? d.s2 : -1;
+    s2 = -1;
+  } else {
+    s1 = d.s1;
+    s2 = v2 ? d.s2 : -1;
+  }
+}
+
 // The P1D of two forests of one length with one Bluestein DFT: the DFT of
 // delta_F1 + i delta_F2 (all N outputs into the line), split by the
 // symmetry of real inputs into X1 and X2, then each forest's bins.  The
-// pairs [pair_lo, pair_hi) are those of M's size class.
+// pairs [pair_lo, pair_hi) are those of M's size class.  Persistent: block
+// b takes pairs pair_lo + b + k gridDim.x, with the members' descriptors
+// loaded a pair ahead and the pair list entry two pairs ahead.
 template <int M>
 __global__ __launch_bounds__(M / 8) __attribute__((amdgpu_waves_per_eu(2))) void k_p1d_blue_pair(P1dArgs a) {
   constexpr int NT = M / 8;
@@ -735,23 +791,43 @@ __global__ __launch_bounds__(M / 8) __attribute__((amdgpu_waves_per_eu(2))) void
   int* bfirst = reinterpret_cast<int*>(ledges + a.nb + 1);
   int* blast = bfirst + a.nb;
   for (int i = threadIdx.x; i <= a.nb; i += NT) ledges[i] = a.edges[i];
-  const smk_skewers& sk = a.sk;
   const int m = a.regroup;
   const double px = a.pixel * m;
   const double k_ny = M_PI / px;
-  for (int w = a.pair_lo + (int)blockIdx.x; w < a.pair_hi; w += gridDim.x) {
+  const int G = gridDim.x;
+  int w = a.pair_lo + (int)blockIdx.x;
+  PairDesc cur = w < a.pair_hi ? pair_desc(a, a.pairs[2 * w], a.pairs[2 * w + 1]) : pair_desc(a, -1, -1);
+  int q1 = -1, q2 = -1;   // the pair list entry of pair w + G
+  if (w + G < a.pair_hi) {
+    q1 = a.pairs[2 * (w + G)];
+    q2 = a.pairs[2 * (w + G) + 1];
+  }
+  for (; w < a.pair_hi; w += G) {
+    // in flight while this pair is transformed: the next pair's
+    // descriptors and the pair list entry after it
+    const PairDesc nxt = pair_desc(a, q1, q2);
+    q1 = q2 = -1;
+    if (w + 2 * G < a.pair_hi) {
+      q1 = a.pairs[2 * (w + 2 * G)];
+      q2 = a.pairs[2 * (w + 2 * G) + 1];
+    }
     int s1, s2;
-    pair_members<M>(a, w, s1, s2);
-    if (s1 < 0) continue;
-    const int nq = (sk.imax[s1] - sk.imin[s1]) / m;
-    __syncthreads();
-    const double inv1 = 1.0 / a.fmean[sk.pset[s1]];
-    const double inv2 = s2 >= 0 ? 1.0 / a.fmean[sk.pset[s2]] : 0.0;
-    const int64_t o1 = sk.off[s1] + sk.imin[s1];
+    bool swap;
+    desc_members<M>(a, cur, s1, s2, swap);
+    const int64_t o1 = swap ? cur.o2 : cur.o1;
+    const int set1 = swap ? cur.set2 : cur.set1;
+    const int64_t tb = swap ? cur.t2 : cur.t1;
+    const int nq = (swap ? cur.n2 : cur.n1) / m;
     // a lone forest reads its own pixels as the (unused) partner
-    const int64_t o2 = s2 >= 0 ? sk.off[s2] + sk.imin[s2] : o1;
+    const int64_t o2 = s2 >= 0 ? cur.o2 : o1;
+    const int set2 = s2 >= 0 ? cur.set2 : set1;
+    cur = nxt;
+    if (s1 < 0) continue;
+    __syncthreads();
+    const double inv1 = 1.0 / a.fmean[set1];
+    const double inv2 = s2 >= 0 ? 1.0 / a.fmean[set2] : 0.0;
     const int nj = nq / 2 + 1;
-    const double2* T = a.tables + a.tab_off[s1];
+    const double2* T = a.tables + tb;
     f64::bluestein_tab<M, 8>(
         nq, nq,
         [&](int q) {
@@ -1731,6 +1807,19 @@ int grid_for_rest(int n) { const int g = grid_for(n); return g < 2048 ? g : 2048
     if (smk_rc_) return smk_rc_; \
   } while (0)
 
+// blocks of a persistent kernel: as many as fit on the device at once (its
+// occupancy at this block size and LDS), at most n
+int resident_grid(const void* kern, int threads, size_t lds, int n) {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, lds) != hipSuccess || cus < 1 ||
+      per_cu < 1)
+    return grid_for(n);
+  const int64_t g = (int64_t)cus * per_cu;
+  return n < g ? (n > 0 ? n : 1) : (int)g;
+}
+
 template <int M>
 int launch_p1d_blue_pair(P1dArgs a, int lo, int hi, size_t tail, hipStream_t st) {
   if (hi <= lo) return SMK_OK;
@@ -1738,7 +1827,8 @@ int launch_p1d_blue_pair(P1dArgs a, int lo, int hi, size_t tail, hipStream_t st)
   a.pair_hi = hi;
   const size_t lds = pair_fixed_bytes(M) + tail;
   SMK_HIP_CHECK(hipFuncSetAttribute((const void*)k_p1d_blue_pair<M>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(k_p1d_blue_pair<M>, dim3(grid_for(hi - lo)), dim3(M / 8), lds, st, a);
+  const int grid = resident_grid((const void*)k_p1d_blue_pair<M>, M / 8, lds, hi - lo);
+  hipLaunchKernelGGL(k_p1d_blue_pair<M>, dim3(grid), dim3(M / 8), lds, st, a);
   SMK_LAUNCH_CHECK();
   return SMK_OK;
 }
