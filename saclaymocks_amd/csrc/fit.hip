@@ -844,7 +844,7 @@ __global__ __launch_bounds__(M / 8) __attribute__((amdgpu_waves_per_eu(2))) void
     const double inv2 = s2 >= 0 ? 1.0 / a.fmean[set2] : 0.0;
     const int nj = nq / 2 + 1;
     const double2* T = a.tables + tb;
-    f64::bluestein_tab<M, 8>(
+    f64::bluestein_tab<M, 8, (M >= 4096)>(
         nq, nq,
         [&](int q) {
           double acc1 = 0.0, acc2 = 0.0;
