@@ -336,10 +336,12 @@ __device__ __forceinline__ void bluestein_table(int N, double2* __restrict__ T, 
 // two transforms.  in(n) may read LDS that `line` aliases (all of it is read
 // before the first exchange); emit may write there too (the line is free
 // once the last transform's final stage runs in registers).
-// KEEP: the chirp's transform is loaded before the first FFT and the chirp
-// kept from the input for the output (their latency overlaps the first FFT;
-// 4 E more registers, for kernels whose occupancy does not drop by them)
-template <int M, int E, bool KEEP = false, class In, class Emit>
+// KEEP bit 0: the chirp's transform is loaded before the first FFT (its
+// latency overlaps that FFT); bit 1: the chirp is kept from the input for
+// the output.  2 E more registers each, for kernels whose occupancy does not
+// drop by them.
+constexpr int KEEP_B = 1, KEEP_W = 2;
+template <int M, int E, int KEEP = 0, class In, class Emit>
 __device__ __forceinline__ void bluestein_tab(int N, int nout, In in, Emit emit, double2* __restrict__ line,
                                               const double2* __restrict__ twM, int TW,
                                               const double2* __restrict__ wtab, const double2* __restrict__ btab) {
@@ -350,26 +352,26 @@ __device__ __forceinline__ void bluestein_tab(int N, int nout, In in, Emit emit,
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int n = p + P * e;
-    if constexpr (KEEP) {
+    if constexpr ((KEEP & KEEP_W) != 0) {
       wk[e] = n < N ? wtab[n] : make_double2(0.0, 0.0);
       va[e] = n < N ? cmul(in(n), wk[e]) : make_double2(0.0, 0.0);
     } else {
       va[e] = n < N ? cmul(in(n), wtab[n]) : make_double2(0.0, 0.0);
     }
   }
-  if constexpr (KEEP) {
+  if constexpr ((KEEP & KEEP_B) != 0) {
 #pragma unroll
     for (int e = 0; e < E; ++e) bk[e] = btab[p + P * e];
   }
   __syncthreads();
   fft<M, E, false>(va, line, p, twM, TW);
 #pragma unroll
-  for (int e = 0; e < E; ++e) va[e] = cmul(va[e], KEEP ? bk[e] : btab[p + P * e]);
+  for (int e = 0; e < E; ++e) va[e] = cmul(va[e], (KEEP & KEEP_B) ? bk[e] : btab[p + P * e]);
   fft<M, E, true>(va, line, p, twM, TW);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int k = p + P * e;
-    if (k < nout) emit(k, cmul(va[e], KEEP ? wk[e] : wtab[k]));
+    if (k < nout) emit(k, cmul(va[e], (KEEP & KEEP_W) ? wk[e] : wtab[k]));
   }
 }
 
