@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define SMK_ABI_VERSION 8
+#define SMK_ABI_VERSION 9
 
 #define SMK_OK 0
 #define SMK_ERR_ARG (-1)         /* bad argument (null pointer, size, enum) */
@@ -466,13 +466,17 @@ int smk_p1d_adjoint(const smk_skewers* sk, const float* delta, const float* eta,
  *   dfmean  device double [sets][3]: d<F>/d(a, b, c) (from the sums above)
  * With tables the forests of a table take two Bluestein DFTs (of delta_F +
  * i t_c and of t_a + i t_b, split by the symmetry of real inputs); the others
- * a direct DFT of the four series by Horner's rule, any length. */
+ * a direct DFT of the four series by Horner's rule, any length.
+ *   pairs, pair_off  NULL, or the pair list of smk_p1d_binned: the per-size
+ *          kernels then take their forests from it (each forest of a table
+ *          in exactly one pair) instead of scanning the batch. */
 int smk_fgpa_flux_jac(const smk_skewers* sk, const float* delta, const float* eta, const double* abcg,
                       int64_t abcg_rows, double* flux, double* sums, void* stream);
 int smk_p1d_jacobian(const smk_skewers* sk, const float* delta, const float* eta, const double* abcg,
                      int64_t abcg_rows, const double* flux, const double* fmean, const double* dfmean,
                      double pixel, int regroup, const double* edges, int nbins, int max_forest, double* partial,
-                     int32_t* status, const double* tables, const int64_t* tab_off, void* stream);
+                     int32_t* status, const double* tables, const int64_t* tab_off, const int32_t* pairs,
+                     const int64_t* pair_off, void* stream);
 
 /* ---------------------------------------------------------- QSO catalogue */
 /* Quasar draw of one x slice of the lognormal boxes, draw_qso.py:186-462:

@@ -13,7 +13,7 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "saclaymocks_hip.h")
 
 # layout of the structs passed by pointer (smk_skewers, smk_box_view): must
 # equal SMK_ABI_VERSION of include/saclaymocks_hip.h and the library's
-ABI_VERSION = 8
+ABI_VERSION = 9
 
 SMK_OK = 0
 SMK_ERR_ARG = -1
@@ -94,7 +94,7 @@ _SIGNATURES = {
                              _vp, _vp, _vp, _c.POINTER(_c.c_int64), _vp]),
     "smk_fgpa_flux_jac": (_i, [_vp, _vp, _vp, _vp, _c.c_int64, _vp, _vp, _vp]),
     "smk_p1d_jacobian": (_i, [_vp, _vp, _vp, _vp, _c.c_int64, _vp, _vp, _vp, _d, _i, _vp, _i, _i, _vp, _vp, _vp,
-                              _vp, _vp]),
+                              _vp, _vp, _c.POINTER(_c.c_int64), _vp]),
 }
 
 

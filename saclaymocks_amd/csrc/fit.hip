@@ -1493,6 +1493,8 @@ struct JacArgs {
   const double2* tw;
   const double2* tables;
   const int64_t* tab_off;
+  const int32_t* pairs;  // NULL, or the pair list of smk_p1d_binned (the forests of each size class)
+  int pair_lo, pair_hi;  // the pairs of the launch's transform size
 };
 
 // F and dF/d(a, b, c) summed over the m pixels of regrouped point q (the
@@ -1687,16 +1689,25 @@ __global__ __launch_bounds__(M / 8) __attribute__((amdgpu_waves_per_eu(2))) void
     bin_runs_jac<NT>(pk, dp, NJ, bins, nj, nq, k_ny, a.nb, bfirst, blast, a.partial + (int64_t)s * a.nb * 7);
   };
   auto takes = [&](int s, int nq) { return uses_table(a.tables, a.tab_off, s, nq, a.nmax) && blue_m_of(nq) == M; };
-  if constexpr (M >= 4096) {
-    __shared__ uint64_t need[NT / 64];
-    for (int c0 = blockIdx.x; c0 < sk.n; c0 += gridDim.x * NT) {
-      cand_masks<NT>(sk, a.regroup, c0, need, takes);
-      for (int w = 0; w < NT / 64; ++w)
-        for (uint64_t mk = need[w]; mk; mk &= mk - 1) one(c0 + (w * 64 + __builtin_ctzll(mk)) * (int)gridDim.x);
+  if (a.pairs) {
+    // the forests of this size class from the pair list (two entries a
+    // pair, -1: none), each checked as above
+    for (int e = 2 * a.pair_lo + (int)blockIdx.x; e < 2 * a.pair_hi; e += gridDim.x) {
+      const int s = a.pairs[e];
+      if (s >= 0 && takes(s, (sk.imax[s] - sk.imin[s]) / a.regroup)) one(s);
     }
   } else {
-    for (int s = blockIdx.x; s < sk.n; s += gridDim.x)
-      if (takes(s, (sk.imax[s] - sk.imin[s]) / a.regroup)) one(s);
+    if constexpr (M >= 4096) {
+      __shared__ uint64_t need[NT / 64];
+      for (int c0 = blockIdx.x; c0 < sk.n; c0 += gridDim.x * NT) {
+        cand_masks<NT>(sk, a.regroup, c0, need, takes);
+        for (int w = 0; w < NT / 64; ++w)
+          for (uint64_t mk = need[w]; mk; mk &= mk - 1) one(c0 + (w * 64 + __builtin_ctzll(mk)) * (int)gridDim.x);
+      }
+    } else {
+      for (int s = blockIdx.x; s < sk.n; s += gridDim.x)
+        if (takes(s, (sk.imax[s] - sk.imin[s]) / a.regroup)) one(s);
+    }
   }
 }
 
@@ -1919,8 +1930,17 @@ int launch_adj_blue_pair(AdjArgs a, int lo, int hi, size_t tail, hipStream_t st)
 }
 
 template <int M>
-int launch_p1d_jac_blue(const JacArgs& a, dim3 grid, size_t tail, hipStream_t st) {
+int launch_p1d_jac_blue(JacArgs a, dim3 grid, int lo, int hi, size_t tail, hipStream_t st) {
   const size_t lds = jac_blue_fixed_bytes(M) + tail;
+  if (a.pairs) {
+    // a block per pair list entry of the class (20 000 forests: 83, 366 and
+    // 692 us at M = 1024, 2048, 4096, against 102, 423 and 812 us scanning
+    // the batch)
+    if (hi <= lo) return SMK_OK;
+    a.pair_lo = lo;
+    a.pair_hi = hi;
+    grid = dim3(grid_for(2 * (hi - lo)));
+  }
   SMK_HIP_CHECK(hipFuncSetAttribute((const void*)k_p1d_jac_blue<M>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   // M = 4096 runs one block per CU: resident blocks, each checking its
   // candidates at once, save a block dispatch per skewer (976 -> 814 us for
@@ -2179,10 +2199,12 @@ extern "C" int smk_p1d_jacobian(const smk_skewers* sk, const float* delta, const
                                 int64_t abcg_rows, const double* flux, const double* fmean, const double* dfmean,
                                 double pixel, int regroup, const double* edges, int nbins, int max_forest,
                                 double* partial, int32_t* status, const double* tables, const int64_t* tab_off,
-                                void* stream) {
+                                const int32_t* pairs, const int64_t* pair_off, void* stream) {
   SMK_REQUIRE(sk != nullptr && delta != nullptr && abcg != nullptr && flux != nullptr && fmean != nullptr &&
                   dfmean != nullptr && edges != nullptr && partial != nullptr,
               SMK_ERR_ARG, "NULL argument");
+  SMK_REQUIRE(pairs == nullptr || (tables != nullptr && pair_off != nullptr), SMK_ERR_ARG,
+              "pairs need tables and pair_off");
   SMK_REQUIRE(sk->pset != nullptr, SMK_ERR_ARG, "the fit kernels need skewers.pset");
   SMK_REQUIRE(regroup >= 1 && nbins >= 1 && pixel > 0, SMK_ERR_ARG, "bad regroup/nbins/pixel");
   SMK_REQUIRE(max_forest >= 0, SMK_ERR_ARG, "max_forest < 0");
@@ -2195,7 +2217,14 @@ extern "C" int smk_p1d_jacobian(const smk_skewers* sk, const float* delta, const
   const double2* tw = fp64_twiddles(&te);
   SMK_REQUIRE(tw != nullptr, SMK_ERR_HIP, "FP64 twiddle table: %s", hipGetErrorString(te));
   JacArgs a{*sk,    delta, eta,    abcg,    abcg_rows, flux, fmean, dfmean, pixel, regroup, edges, nbins,
-            partial, status, nmax, tw,      (const double2*)tables, tab_off};
+            partial, status, nmax, tw,      (const double2*)tables, tab_off, pairs, 0, 0};
+  int64_t po[4] = {0, 0, 0, 0};
+  if (pairs) {
+    for (int c = 0; c < 3; ++c)
+      SMK_REQUIRE(pair_off[c] >= 0 && pair_off[c] <= pair_off[c + 1] && pair_off[c + 1] <= INT32_MAX / 2,
+                  SMK_ERR_ARG, "bad pair_off");
+    for (int c = 0; c < 4; ++c) po[c] = pair_off[c];
+  }
   const dim3 grid(grid_for(sk->n));
   hipStream_t st = (hipStream_t)stream;
   const size_t lds = jac_generic_lds(nbins);
@@ -2206,9 +2235,9 @@ extern "C" int smk_p1d_jacobian(const smk_skewers* sk, const float* delta, const
     SMK_LAUNCH_CHECK();
     const int nblue = nmax < BLUE_NMAX ? nmax : BLUE_NMAX;
     const size_t tail = (size_t)(nbins + 1) * 8 + (size_t)nbins * 8;
-    SMK_CALL(launch_p1d_jac_blue<1024>(a, grid, tail, st));
-    if (blue_m_of(nblue) >= 2048) SMK_CALL(launch_p1d_jac_blue<2048>(a, grid, tail, st));
-    if (blue_m_of(nblue) >= 4096) SMK_CALL(launch_p1d_jac_blue<4096>(a, grid, tail, st));
+    SMK_CALL(launch_p1d_jac_blue<1024>(a, grid, (int)po[0], (int)po[1], tail, st));
+    if (blue_m_of(nblue) >= 2048) SMK_CALL(launch_p1d_jac_blue<2048>(a, grid, (int)po[1], (int)po[2], tail, st));
+    if (blue_m_of(nblue) >= 4096) SMK_CALL(launch_p1d_jac_blue<4096>(a, grid, (int)po[2], (int)po[3], tail, st));
   } else {
     SMK_HIP_CHECK(hipFuncSetAttribute((const void*)k_p1d_jac<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)lds));

@@ -241,7 +241,8 @@ class P1DFitModel:
                                       _lib.ptr(abcg), abcg.shape[0], _lib.ptr(self.flux), _lib.ptr(self.fmean_dev),
                                       _lib.ptr(dfm_dev), self.pixel, self.regroup, _lib.ptr(self.edges), self.nb,
                                       self.max_forest, _lib.ptr(part), _lib.ptr(self.status),
-                                      _lib.ptr(self.tables), _lib.ptr(self.tab_off), s), "smk_p1d_jacobian")
+                                      _lib.ptr(self.tables), _lib.ptr(self.tab_off), _lib.ptr(self.pairs),
+                                      self.pair_off, s), "smk_p1d_jacobian")
         self._reduce(part, self.nb * 7, part_set)
         q = _sync_host(part_set)
         r = _sync_host(sums_set)

@@ -181,14 +181,16 @@ def _check_jacobian(model, args, th):
     assert np.allclose(g, want, rtol=1e-7, atol=1e-9 * np.abs(want).max())
 
 
-@pytest.mark.parametrize("use_tables", [True, False])
+@pytest.mark.parametrize("use_tables, pairs", PATHS)
 @pytest.mark.parametrize("m", [1, 2])
-def test_jacobian_parity(cuda, m, use_tables):
+def test_jacobian_parity(cuda, m, use_tables, pairs):
     """Forward-mode kernels against the FP64 oracle: forests of every path
-    (Bluestein size classes with tables, the direct DFT without, beyond
-    the LDS layout's 8192 points)."""
+    (Bluestein size classes with tables, taken from the pair list or by a
+    scan of the batch, the direct DFT without, beyond the LDS layout's 8192
+    points)."""
     lens = EDGE_LENGTHS + [9000, 4100]
-    model, args, theta = _problem(cuda, ns=2, per=len(lens) // 2, lens=lens, m=m, use_tables=use_tables)
+    model, args, theta = _problem(cuda, ns=2, per=len(lens) // 2, lens=lens, m=m, use_tables=use_tables,
+                                  pairs=pairs)
     if use_tables:
         model.tab_off[::5] = -1
     _check_jacobian(model, args, theta * np.array([1.1, 0.95, 1.05] * 2))
