@@ -1074,26 +1074,43 @@ __device__ __forceinline__ void adj_fgpa(const AdjArgs& a, int set, int64_t o0, 
                                          const double* A, double* red, double* res) {
   const int m = a.regroup;
   const int64_t o = o0;
-  const int i0 = 0, i1 = nf;
   const double4 p = prow(a.abcg, a.rows, set);
   const double pa = p.x, pb = p.y, pc = p.z, pg = p.w;
   double v[7] = {saf, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  for (int i = threadIdx.x; i < i1 - i0; i += NT) {
-    const int q = i / m;
-    const double d = a.delta[o + i0 + i];
-    const double e = a.eta ? (double)a.eta[o + i0 + i] : 0.0;
-    const double gf = d + pc * e;
-    const double T = exp(pb * pg * gf);
-    const double F = a.flux[o + i0 + i];
-    const double dfa = -T * F;
-    const double dfb = -pa * T * F * pg * gf;
-    const double dfc = -pa * T * F * pb * pg * e;
-    v[4] += dfa; v[5] += dfb; v[6] += dfc;
-    if (q < nq) {
-      const double Aq = A[q] / m;
-      v[1] += Aq * dfa;
-      v[2] += Aq * dfb;
-      v[3] += Aq * dfc;
+  // two pixels per thread and pass with their loads issued together (at a
+  // clamped index past the end, unused; without eta the load reads delta
+  // and is not used), in the order of one pixel per pass
+  const float* ep = a.eta ? a.eta : a.delta;
+  for (int i = threadIdx.x; i < nf; i += 2 * NT) {
+    float dv[2], ev[2];
+    double fv[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int j = i + u * NT < nf ? i + u * NT : nf - 1;
+      dv[u] = a.delta[o + j];
+      ev[u] = ep[o + j];
+      fv[u] = a.flux[o + j];
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int j = i + u * NT;
+      if (j >= nf) break;
+      const int q = j / m;
+      const double d = dv[u];
+      const double e = a.eta ? (double)ev[u] : 0.0;
+      const double gf = d + pc * e;
+      const double T = exp(pb * pg * gf);
+      const double F = fv[u];
+      const double dfa = -T * F;
+      const double dfb = -pa * T * F * pg * gf;
+      const double dfc = -pa * T * F * pb * pg * e;
+      v[4] += dfa; v[5] += dfb; v[6] += dfc;
+      if (q < nq) {
+        const double Aq = A[q] / m;
+        v[1] += Aq * dfa;
+        v[2] += Aq * dfb;
+        v[3] += Aq * dfc;
+      }
     }
   }
   reduce7<NT>(v, red, res);
