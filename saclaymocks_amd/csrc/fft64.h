@@ -420,7 +420,8 @@ __device__ __forceinline__ void fft2(double2 (&va)[E], double2 (&vb)[E], double2
 // both inputs as (x.re, x.im, y.re, y.im), the first pair used for n < Na,
 // the second for n < Nb; emit_a(k, X_k) for k < Na and emit_b(k, Y_k) for
 // k < Nb.  The emits may write the first Na (Nb) points of la (lb).
-template <int M, int E, bool SAME, class In, class EmitA, class EmitB>
+// KEEP_B (SAME only): the chirp's transform loaded before the first FFTs.
+template <int M, int E, bool SAME, int KEEP = 0, class In, class EmitA, class EmitB>
 __device__ __forceinline__ void bluestein_tab2(int Na, int Nb, In in, EmitA emit_a, EmitB emit_b,
                                                double2* __restrict__ la, double2* __restrict__ lb,
                                                const double2* __restrict__ twM, int TW,
@@ -446,11 +447,17 @@ __device__ __forceinline__ void bluestein_tab2(int Na, int Nb, In in, EmitA emit
       }
     }
   }
+  static_assert(KEEP == 0 || SAME, "KEEP_B reads one table");
+  [[maybe_unused]] double2 bk[E];
+  if constexpr ((KEEP & KEEP_B) != 0) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) bk[e] = ba[p + P * e];
+  }
   __syncthreads();
   fft2<M, E, false>(va, vb, la, lb, p, twM, TW);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    const double2 x = ba[p + P * e];
+    const double2 x = (KEEP & KEEP_B) ? bk[e] : ba[p + P * e];
     va[e] = cmul(va[e], x);
     vb[e] = cmul(vb[e], SAME ? x : bb[p + P * e]);
   }

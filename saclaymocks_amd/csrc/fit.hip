@@ -1681,7 +1681,7 @@ __global__ __launch_bounds__(M / 8) __attribute__((amdgpu_waves_per_eu(2))) void
     const double2* T = a.tables + a.tab_off[s];
     const int64_t o = sk.off[s] + i0;
     auto tangent = [&](double dsum, double fsum, double g) { return (dsum / m) * inv - (fsum / m) * g * inv * inv; };
-    f64::bluestein_tab2<M, 8, true>(
+    f64::bluestein_tab2<M, 8, true, (M <= 2048 ? f64::KEEP_B : 0)>(
         nq, nq,
         [&](int q) {
           const JacPoint P = jac_point(a, o, q, m, p);
