@@ -1462,7 +1462,7 @@ __global__ __launch_bounds__(M / 8) __attribute__((amdgpu_waves_per_eu(2))) void
       return make_double2(0.5 * v.x, -0.5 * v.y);
     };
     double saf1 = 0.0, saf2 = 0.0;
-    f64::bluestein_tab<M, 8, (M >= 4096 ? f64::KEEP_B : f64::KEEP_B | f64::KEEP_W)>(
+    f64::bluestein_tab<M, 8, f64::KEEP_B | f64::KEEP_W>(
         nq, nq,
         [&](int n) {
           const double2 h1 = herm(c1, n), h2 = herm(c2, n);
