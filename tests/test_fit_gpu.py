@@ -246,3 +246,19 @@ def test_reduce_by_set(cuda, n, width):
     # a work buffer too small is refused before any launch
     assert L.smk_reduce_by_set(ctypes.byref(sk), _lib.ptr(xd), width, ns, None, _lib.ptr(out), _lib.ptr(work),
                                work.numel() - 1, _lib.stream_ptr()) != 0
+
+
+def test_jacobian_pair_list_routing_is_exact(cuda):
+    """The per-size Jacobian kernels find their forests from the pair list or
+    by a scan of the batch: the same arithmetic per forest, so the same
+    bits, including forests whose table is withdrawn after the pairing."""
+    lens = EDGE_LENGTHS + [9000, 4100, 777, 777, 1500, 1500, 300]
+    th = None
+    outs = []
+    for pairs in (True, False):
+        model, args, theta = _problem(cuda, ns=2, per=len(lens) // 2, lens=lens[:2 * (len(lens) // 2)], pairs=pairs)
+        model.tab_off[::7] = -1
+        th = theta * np.array([1.1, 0.95, 1.05] * 2) if th is None else th
+        outs.append(model.jacobian(th))
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
