@@ -136,7 +136,7 @@ int grid_for_waves(int n) {
 // forest [imin, imax).  Parameter row: pset[s], or with pset NULL the pixel's
 // global grid index g0[s] + i (parameters at the pixel's redshift).
 // pixels per lane whose loads are issued together in the FGPA kernels
-constexpr int FLUX_U = 4;
+constexpr int FLUX_U = 8;
 
 // one skewer's flux and forest sum of F on one wave (lane's share, pixels
 // lane + 64 k in ascending k); PP: parameters per pixel; ETA: eta given
